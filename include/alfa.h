@@ -1,0 +1,238 @@
+/*
+ * alfa.h — C ABI of the MI355X-native JAX-ALFA time-step library (libalfa.so).
+ *
+ * Every entry point is stream-ordered and asynchronous: it enqueues HIP
+ * kernels on `stream` and returns.  No entry point allocates device memory
+ * or synchronises the host after alfa_plan_create(), so a sequence of calls
+ * can be captured into a hipGraph.  All array arguments are device pointers
+ * owned by the caller; the library keeps none of them after return.
+ *
+ * Layout (identical to the reference's JAX arrays):
+ *   real field   (nx, ny, nz)            row-major, z fastest
+ *   spectrum     (nx, ny/2+1, nz)        interleaved complex, z fastest
+ *   padded field (mx, my, nz)            mx = int(1.5 nx), my = int(1.5 ny)
+ *   plane        (nx, ny)
+ *   profile      (nz,)                   always double
+ * Element type of fields/spectra/planes follows the plan precision:
+ *   ALFA_FP64  -> double / complex double everywhere;
+ *   ALFA_MIXED -> float / complex float for fields, spectra and planes;
+ *                 plane-mean accumulators, the beta root solve, the pressure
+ *                 tridiagonal factors and sweeps and the surface-layer
+ *                 scalars are double.
+ *
+ * Return value: 0 on success, otherwise an ALFA_E* code; the message is in
+ * alfa_last_error() (thread-local).  Kernels never abort.
+ *
+ * Citations are to the reference repository (Sukantabasu/jax-alfa).
+ */
+#ifndef ALFA_H
+#define ALFA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct alfa_plan alfa_plan;
+typedef void* alfa_stream; /* a hipStream_t; NULL = the null stream */
+
+enum { ALFA_FP64 = 0, ALFA_MIXED = 1 };
+
+enum {
+  ALFA_OK = 0,
+  ALFA_EINVAL = 1,     /* bad shape, option or argument            */
+  ALFA_ERADIX = 2,     /* FFT length with a prime factor above 5    */
+  ALFA_EHIP = 3,       /* HIP runtime error                         */
+  ALFA_ENOMEM = 4,     /* device allocation failed at plan creation */
+  ALFA_ESTATE = 5      /* call out of order (e.g. step on a bad plan) */
+};
+
+/* Run configuration; the field names and meanings are those of the
+ * reference's Config.py (src/config/ConfigLoader.py:67-157,
+ * src/config/DerivedVars.py:39-119).  Lengths in m, times in s. */
+typedef struct alfa_config {
+  int32_t nx, ny, nz;
+  int32_t optSgs;               /* 0 static SM, 1 LASDD-SM, 3 LAD-SM        */
+  int32_t FGR;                  /* 1 (dealiased, TFR=2) or 2 (TFR=sqrt 2)   */
+  int32_t dynamicSGS_call_time;
+  int32_t optSurfFlux;          /* 0 homogeneous, 1 heterogeneous           */
+  int32_t optSurfBC;            /* 0 const flux, 1 series flux, 2 series T  */
+  int32_t optBuoyancy;          /* 0: /T_0, 1: /<theta_v>                   */
+  int32_t optDamping;           /* 1: Rayleigh sponge                       */
+  int32_t istep;                /* first iteration number                   */
+  int32_t reserved_i[7];
+  double l_x, l_y, l_z;
+  double dt;
+  double Ugal;
+  double z0m, z0T, zTemperature;
+  double SensibleHeatFlux;
+  double Ug2, Vg2;
+  double f_coriolis;
+  double inversion;
+  double T_0;
+  double Cs2, Cs2PrRatio;
+  double z_damping, RelaxTime;
+  double reserved_d[8];
+} alfa_config;
+
+/* Derived constants the plan computed (DerivedVars.py:49-119,
+ * Preprocess.py:44-62, Filtering.py cut-offs). */
+typedef struct alfa_derived {
+  double z_scale, dx, dy, dz, L, TFR, dt_nondim, g_nondim, f_coriolis_nondim;
+  double inversion_nondim, z_damping_nondim, RelaxTime_nondim;
+  int32_t mx, my, nx_rfft, ny_rfft, optDealias;
+  int32_t cut0_x, cut0_y, cut1_x, cut1_y, cut2_x, cut2_y;
+  int32_t precision;
+  int64_t workspace_bytes;
+} alfa_derived;
+
+/* Carried state of one run (src/Main.py:66-111).  Device pointers, caller
+ * owned, element type per plan precision.  u is stored minus Ugal and TH
+ * minus T_0 (src/initialization/Initialization.py:58-100).  The step
+ * updates every field in place. */
+typedef struct alfa_state {
+  void *u, *v, *w, *TH;                          /* (nx,ny,nz)          */
+  void *RHS_u, *RHS_v, *RHS_w, *RHS_TH;          /* previous-step RHS   */
+  void *Cs2_3D, *Cs2PrRatio_3D;                  /* (nx,ny,nz)          */
+  void *psi_m, *psi_m0, *psi_h, *psi_h0, *fi_m, *fi_h; /* (nx,ny) MOST   */
+  /* optional diagnostics written by the step when non-NULL: */
+  double *Cs2_1D_avg1, *Cs2_1D_avg2, *beta1_1D;  /* (nz,) */
+  double *Cs2PrRatio_1D, *beta2_1D;              /* (nz,) */
+  double *scalars;   /* [0] CFL, [1] qz_sfc_avg, [2] mean ustar, [3] M_sfc mean */
+} alfa_state;
+
+/* ---- plan --------------------------------------------------------------- */
+int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan** out);
+int alfa_plan_destroy(alfa_plan* plan);
+int alfa_plan_derived(const alfa_plan* plan, alfa_derived* out);
+const char* alfa_last_error(void);
+/* Initial state values of Main.py:87-111 (RHS_prev = 0, Cs2 fields =
+ * Cs2/Cs2PrRatio, psi = 0, fi = 1) written into caller buffers. */
+int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);
+
+/* ---- whole step (src/Main.py:158-540) ------------------------------------
+ * iteration: the reference's loop counter (first step = cfg.istep).
+ * sfc_val:   SurfaceBC_series[iteration-1] for optSurfBC 1/2 (Main.py:224,233);
+ *            ignored for optSurfBC 0. */
+int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_val,
+              alfa_stream stream);
+
+/* ---- operations (src/operations) ---------------------------------------- */
+/* FFT (src/operations/FFT.py:44): F (nx,ny,nz) -> F_fft (nx,ny/2+1,nz). */
+int alfa_fft(alfa_plan*, const void* F, void* F_fft, alfa_stream);
+/* FFT_pad (FFT.py:68): (mx,my,nz) -> (mx,my/2+1,nz). */
+int alfa_fft_pad(alfa_plan*, const void* F_pad, void* F_pad_fft, alfa_stream);
+/* Filtering_Explicit (Filtering.py:50): returns F and the masked F_fft. */
+int alfa_filtering_explicit(alfa_plan*, const void* F_fft, void* F, void* F_fft_new,
+                            alfa_stream);
+/* Filtering_Level1 (Filtering.py:97) / Filtering_Level2 (Filtering.py:135). */
+int alfa_filtering_level1(alfa_plan*, const void* F_fft, void* F_hat, alfa_stream);
+int alfa_filtering_level2(alfa_plan*, const void* F_fft, void* F_hatd, alfa_stream);
+/* Dealias1 (Dealiasing.py:51): spectrum -> padded field (mx,my,nz). */
+int alfa_dealias1(alfa_plan*, const void* F_fft, void* F_pad, alfa_stream);
+/* Dealias2 (Dealiasing.py:86): padded spectrum (mx,my/2+1,nz) -> field. */
+int alfa_dealias2(alfa_plan*, const void* F_pad_fft, void* F, alfa_stream);
+/* Derivxy (Derivatives.py:161): axis 0 -> kx2 (d/dx), 1 -> ky2 (d/dy). */
+int alfa_derivxy(alfa_plan*, const void* F_fft, int axis, void* dF, alfa_stream);
+/* Derivz_M (Derivatives.py:190). */
+int alfa_derivz_m(alfa_plan*, const void* u, const void* v, const void* w,
+                  const void* ustar, const void* M_sfc_loc, const void* fi_m,
+                  void* dudz, void* dvdz, void* dwdz, alfa_stream);
+/* Derivz_TH (Derivatives.py:243). */
+int alfa_derivz_th(alfa_plan*, const void* TH, const void* ustar, const void* qz_sfc,
+                   const void* fi_h, void* dTHdz, alfa_stream);
+/* Derivz_Generic_uvp (Derivatives.py:310), Derivz_Generic_w (:340). */
+int alfa_derivz_generic_uvp(alfa_plan*, const void* F, void* dFdz, alfa_stream);
+int alfa_derivz_generic_w(alfa_plan*, const void* F, void* dFdz, alfa_stream);
+
+/* ---- utilities (src/utilities/Utilities.py) ------------------------------ */
+/* PlanarMean (:41): (nx,ny,nz) -> (nz,) double. */
+int alfa_planar_mean(alfa_plan*, const void* F, double* mean, alfa_stream);
+/* StagGridAvg (:64): (nx,ny,nz) -> (nx,ny,nz-1). */
+int alfa_stag_grid_avg(alfa_plan*, const void* F, void* out, alfa_stream);
+/* Imfilter (:87) applied to every level of an (nx,ny,nz) field. */
+int alfa_imfilter(alfa_plan*, const void* F, void* out, alfa_stream);
+/* ComputeBeta1/2 (DynamicSGS_LASDD_SM.py:53, ScalarLASDD_SM.py:54) with
+ * Roots (:123): coeffs is (6, n) double [ff,ee,dd,cc,bb,aa]; beta (n,). */
+int alfa_compute_beta(alfa_plan*, const double* coeffs, int n, double* beta,
+                      alfa_stream);
+
+/* ---- pde (src/pde) -------------------------------------------------------- */
+/* Advection (NSE_AdvectionTerms.py:246). */
+int alfa_advection(alfa_plan*, const void* u, const void* v, const void* w,
+                   const void* dudy, const void* dudz, const void* dvdx,
+                   const void* dvdz, const void* dwdx, const void* dwdy,
+                   void* Cx, void* Cy, void* Cz, alfa_stream);
+/* ScalarAdvection (SCL_AdvectionTerms.py:158). */
+int alfa_scalar_advection(alfa_plan*, const void* u, const void* v, const void* w,
+                          const void* dsdx, const void* dsdy, const void* dsdz,
+                          void* out, alfa_stream);
+/* BuoyancyOpt1/2 (NSE_BuoyancyTerms.py:49,104) per cfg.optBuoyancy, dry. */
+int alfa_buoyancy(alfa_plan*, const void* TH, void* buoyancy, alfa_stream);
+/* DivStress (NSE_SGSTerms.py:55). */
+int alfa_div_stress(alfa_plan*, const void* txx, const void* tyy, const void* tzz,
+                    const void* txy, const void* txz, const void* tyz,
+                    void* divtx, void* divty, void* divtz, alfa_stream);
+/* DivFlux (SCL_SGSTerms.py:55). */
+int alfa_div_flux(alfa_plan*, const void* qx, const void* qy, const void* qz,
+                  void* divq, alfa_stream);
+/* RHS_Momentum (NSE_AllTerms.py:48), optAdvection = 0. */
+int alfa_rhs_momentum(alfa_plan*, const void* u, const void* v, const void* w,
+                      const void* Cx, const void* Cy, const void* Cz,
+                      const void* buoyancy, const void* divtx, const void* divty,
+                      const void* divtz, void* RHS_u, void* RHS_v, void* RHS_w,
+                      alfa_stream);
+/* RHS_Scalar (SCL_AllTerms.py:48), optAdvection = 0. */
+int alfa_rhs_scalar(alfa_plan*, const void* TH, const void* adv, const void* divq,
+                    void* RHS_TH, alfa_stream);
+/* PressureRC (NSE_PressureTerms.py:109): RC_real/RC_imag (nx,ny/2+1,nz+1)
+ * and fRz_real (nx,ny/2+1,nz). */
+int alfa_pressure_rc(alfa_plan*, const void* u, const void* v, const void* w,
+                     const void* RHS_u, const void* RHS_v, const void* RHS_w,
+                     const void* RHS_u_prev, const void* RHS_v_prev,
+                     const void* RHS_w_prev, const void* divtz,
+                     void* RC_real, void* RC_imag, void* fRz_real, alfa_stream);
+/* ThomasPressureSolve (NSE_PressureTerms_Thomas.py:218); the plan holds
+ * b_thomas/m_thomas of ThomasPressureInit (:151). */
+int alfa_thomas_pressure_solve(alfa_plan*, const void* RC_real, const void* RC_imag,
+                               const void* fRz_real, void* p, void* dpdx,
+                               void* dpdy, void* dpdz, alfa_stream);
+/* AB2_uvw (NSE_TimeAdvancement.py:45), in place on u, v, w. */
+int alfa_ab2_uvw(alfa_plan*, void* u, void* v, void* w, const void* RHS_u,
+                 const void* RHS_u_prev, const void* RHS_v, const void* RHS_v_prev,
+                 const void* RHS_w, const void* RHS_w_prev, alfa_stream);
+/* AB2_TH (SCL_TimeAdvancement.py:46), in place on TH. */
+int alfa_ab2_th(alfa_plan*, void* TH, const void* RHS_TH, const void* RHS_TH_prev,
+                alfa_stream);
+
+/* ---- subgrid scale (src/subgridscale) ------------------------------------- */
+/* StrainsUVPnodes_NoDealias (StrainRates.py:149): S11,S22,S33,S12,S13,S23,S. */
+int alfa_strains_uvp(alfa_plan*, const void* const grads[9], void* out[7],
+                     alfa_stream);
+/* StrainsWnodes_NoDealias (StrainRates.py:307): S13, S23, S_w. */
+int alfa_strains_w(alfa_plan*, const void* const grads[9], void* out[3], alfa_stream);
+/* LASDD (DynamicSGS_LASDD_SM.py:138) for optSgs 1 (LASDD) or 3 (LAD):
+ * strains = S11,S22,S33,S12,S13,S23,S; writes Cs2_3D and the profiles. */
+int alfa_lasdd_sm(alfa_plan*, const void* u, const void* v, const void* w,
+                  const void* const strains[7], void* Cs2_3D, double* Cs2_1D_avg1,
+                  double* Cs2_1D_avg2, double* beta1_1D, alfa_stream);
+/* Wall (SGSStresses_SM.py:63): planes (nx,ny). */
+int alfa_wall(alfa_plan*, const void* u, const void* v, const void* M_sfc_loc,
+              const void* psi_m, const void* psi_m0, void* txz, void* tyz,
+              alfa_stream);
+
+/* ---- surface (src/surface/SurfaceFlux.py:166-443) -------------------------
+ * The variant follows (cfg.optSurfBC, cfg.optSurfFlux).  MOST planes are
+ * updated in place; outputs are planes (nx,ny) and a scalar qz_sfc_avg. */
+int alfa_surface_flux(alfa_plan*, const void* u, const void* v, const void* TH,
+                      double sfc_val, void* psi_m, void* psi_m0, void* psi_h,
+                      void* psi_h0, void* fi_m, void* fi_h, void* M_sfc_loc,
+                      void* ustar, void* qz_sfc, void* invOB, double* qz_sfc_avg,
+                      alfa_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALFA_H */

@@ -1,0 +1,160 @@
+"""Plan and run objects over libalfa (device memory and streams from torch).
+
+``Plan`` owns one ``alfa_plan`` (twiddles, pressure factors, profiles and the
+device workspace of one grid/configuration).  ``LES`` holds the carried state
+of one run (src/Main.py:66-111) as torch device tensors and advances it with
+``alfa_step``.  torch is used only for device allocations and streams; all
+arithmetic is in libalfa's HIP kernels, and there is no CPU path: without a
+GPU or without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch  # imported first so that libalfa binds to torch's HIP runtime
+    return torch
+
+
+def current_stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+class Plan:
+    """One alfa_plan for a configuration (reference Config.py names)."""
+
+    def __init__(self, cfg: dict, precision: str = "fp64", device: str = "cuda:0"):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("alfa_amd needs a GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.cfg = dict(cfg)
+        self.precision = precision
+        prec = {"fp64": _lib.ALFA_FP64, "mixed": _lib.ALFA_MIXED}[precision]
+        self.device = torch.device(device)
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        self.cdtype = torch.complex128 if precision == "fp64" else torch.complex64
+        with torch.cuda.device(self.device):
+            cs = _lib.config_struct(self.cfg)
+            p = C.c_void_p()
+            _lib.check(self.lib.alfa_plan_create(C.byref(cs), prec, C.byref(p)), "alfa_plan_create")
+        self._p = p
+        d = _lib.AlfaDerived()
+        _lib.check(self.lib.alfa_plan_derived(p, C.byref(d)), "alfa_plan_derived")
+        self.derived = {k: getattr(d, k) for k, _ in d._fields_}
+        self.nx, self.ny, self.nz = int(cfg["nx"]), int(cfg["ny"]), int(cfg["nz"])
+        self.nyc = self.ny // 2 + 1
+        self.mx, self.my = self.derived["mx"], self.derived["my"]
+
+    @property
+    def handle(self):
+        return self._p
+
+    def close(self):
+        if getattr(self, "_p", None) is not None and self._p.value:
+            self.lib.alfa_plan_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- allocation helpers -------------------------------------------------
+    def empty(self, shape, complex_=False, dtype=None):
+        torch = _torch()
+        dt = dtype or (self.cdtype if complex_ else self.dtype)
+        return torch.empty(shape, dtype=dt, device=self.device)
+
+    def zeros(self, shape, complex_=False, dtype=None):
+        torch = _torch()
+        dt = dtype or (self.cdtype if complex_ else self.dtype)
+        return torch.zeros(shape, dtype=dt, device=self.device)
+
+    def field(self):
+        return self.zeros((self.nx, self.ny, self.nz))
+
+    def spectrum(self):
+        return self.zeros((self.nx, self.nyc, self.nz), complex_=True)
+
+    def plane(self):
+        return self.zeros((self.nx, self.ny))
+
+    def asdev(self, a, complex_=False):
+        """numpy or torch array -> contiguous device tensor of the plan dtype."""
+        torch = _torch()
+        dt = self.cdtype if complex_ else self.dtype
+        if isinstance(a, np.ndarray):
+            a = torch.from_numpy(np.ascontiguousarray(a))
+        return a.to(device=self.device, dtype=dt).contiguous()
+
+    def ptr(self, t, complex_=None):
+        """Device pointer of a tensor after checking device, dtype and layout."""
+        torch = _torch()
+        if t is None:
+            return None
+        if not isinstance(t, torch.Tensor) or t.device != self.device:
+            raise TypeError("expected a tensor on the plan's device")
+        if not t.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        ok = (self.dtype, self.cdtype, torch.float64)
+        if t.dtype not in ok:
+            raise TypeError(f"tensor dtype {t.dtype} does not match plan precision {self.precision}")
+        return C.c_void_p(t.data_ptr())
+
+    def call(self, name, *args):
+        f = getattr(self.lib, name)
+        _lib.check(f(self._p, *args), name)
+
+
+class LES:
+    """Device-resident state of one run and its time loop (src/Main.py:156-540)."""
+
+    def __init__(self, plan: Plan, u, v, w, TH, sfc_series=None):
+        self.plan = plan
+        P = plan
+        self.t = {n: P.field() for n in _lib.STATE_FIELDS[:10]}
+        for n in _lib.STATE_FIELDS[10:]:
+            self.t[n] = P.plane()
+        self.prof = {n: P.zeros((P.nz,), dtype=_torch().float64) for n in _lib.STATE_DIAG[:5]}
+        self.scalars = P.zeros((8,), dtype=_torch().float64)
+        for n, a in zip(("u", "v", "w", "TH"), (u, v, w, TH)):
+            self.t[n].copy_(P.asdev(a))
+        self.state = _lib.AlfaState()
+        for n in _lib.STATE_FIELDS:
+            setattr(self.state, n, self.t[n].data_ptr())
+        for n in _lib.STATE_DIAG[:5]:
+            setattr(self.state, n, self.prof[n].data_ptr())
+        self.state.scalars = self.scalars.data_ptr()
+        P.call("alfa_state_init", C.byref(self.state), current_stream())
+        self.iteration = int(P.cfg.get("istep", 1))
+        self.sfc_series = None if sfc_series is None else np.asarray(sfc_series, dtype=float)
+        self.CFLmax = 0.0
+
+    def step(self, n: int = 1):
+        P = self.plan
+        st = current_stream()
+        for _ in range(n):
+            sv = 0.0
+            if self.sfc_series is not None:
+                sv = float(self.sfc_series[self.iteration - 1])
+            P.call("alfa_step", C.byref(self.state), int(self.iteration), C.c_double(sv), st)
+            self.iteration += 1
+
+    def fields(self):
+        """Host copies (float64 numpy) of u, v, w, TH in stored form."""
+        return {k: self.t[k].double().cpu().numpy() for k in ("u", "v", "w", "TH")}
+
+    def host(self, name):
+        return self.t[name].double().cpu().numpy()
+
+    def diag(self):
+        s = self.scalars.cpu().numpy()
+        return dict(CFL=float(s[0]), qz_sfc_avg=float(s[1]), ustar_rms=float(s[2]))

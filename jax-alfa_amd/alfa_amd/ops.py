@@ -1,0 +1,447 @@
+"""Reference-named entry points of the per-step hot path, on torch device
+tensors, each a thin call into libalfa (include/alfa.h).
+
+Names, argument order and return values follow the reference's functions
+(src/operations, src/utilities/Utilities.py, src/pde, src/subgridscale,
+src/surface); arguments the reference needs only for its globals or
+pre-allocated zero arrays (ZeRo3D, ZeRo3D_fft, kx2/ky2 tables, Thomas
+factors, damping profiles) are accepted and ignored, because the plan holds
+them.  Grid constants come from the current plan (``use_plan``), the way the
+reference's modules read them from Config.py.  Results are new tensors
+(the reference is functional), except where noted.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from .core import Plan, current_stream
+
+_cur: Plan | None = None
+
+
+def use_plan(plan: Plan) -> Plan:
+    global _cur
+    _cur = plan
+    return plan
+
+
+def _P() -> Plan:
+    if _cur is None:
+        raise RuntimeError("no current plan: call alfa_amd.ops.use_plan(Plan(cfg))")
+    return _cur
+
+
+class WaveTable:
+    """Stand-in for the reference's kx2 / ky2 tables (Preprocess.py:73-100):
+    Derivxy only needs to know the axis."""
+
+    def __init__(self, axis: int):
+        self.axis = axis
+
+
+def Wavenumber():
+    """src/initialization/Preprocess.py:73."""
+    return WaveTable(0), WaveTable(1)
+
+
+def _axis(k):
+    if isinstance(k, WaveTable):
+        return k.axis
+    if k in (0, "x"):
+        return 0
+    if k in (1, "y"):
+        return 1
+    raise TypeError("kxy2 must be a WaveTable from Wavenumber() or an axis")
+
+
+# ---------------------------------------------------------------- operations
+def FFT(F):
+    """src/operations/FFT.py:44."""
+    P = _P()
+    out = P.spectrum()
+    P.call("alfa_fft", P.ptr(F), P.ptr(out), current_stream())
+    return out
+
+
+def FFT_pad(F_pad):
+    """src/operations/FFT.py:68."""
+    P = _P()
+    out = P.zeros((P.mx, P.my // 2 + 1, P.nz), complex_=True)
+    P.call("alfa_fft_pad", P.ptr(F_pad), P.ptr(out), current_stream())
+    return out
+
+
+def Filtering_Explicit(F_fft):
+    """src/operations/Filtering.py:50 -> (F, F_fft_new)."""
+    P = _P()
+    F = P.field()
+    Fn = P.spectrum()
+    P.call("alfa_filtering_explicit", P.ptr(F_fft), P.ptr(F), P.ptr(Fn), current_stream())
+    return F, Fn
+
+
+def Filtering_Level1(F_fft):
+    """src/operations/Filtering.py:97."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_filtering_level1", P.ptr(F_fft), P.ptr(out), current_stream())
+    return out
+
+
+def Filtering_Level2(F_fft):
+    """src/operations/Filtering.py:135."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_filtering_level2", P.ptr(F_fft), P.ptr(out), current_stream())
+    return out
+
+
+def Dealias1(F_fft, ZeRo3D_pad_fft=None):
+    """src/operations/Dealiasing.py:51."""
+    P = _P()
+    out = P.zeros((P.mx, P.my, P.nz))
+    P.call("alfa_dealias1", P.ptr(F_fft), P.ptr(out), current_stream())
+    return out
+
+
+def Dealias2(F_pad_fft, ZeRo3D_fft=None):
+    """src/operations/Dealiasing.py:86."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_dealias2", P.ptr(F_pad_fft), P.ptr(out), current_stream())
+    return out
+
+
+def Derivxy(F_fft, kxy2):
+    """src/operations/Derivatives.py:161."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_derivxy", P.ptr(F_fft), _axis(kxy2), P.ptr(out), current_stream())
+    return out
+
+
+def Derivz_M(u, v, w, ustar, M_sfc_loc, fi2D_m, ZeRo3D=None):
+    """src/operations/Derivatives.py:190."""
+    P = _P()
+    a, b, c = P.field(), P.field(), P.field()
+    P.call("alfa_derivz_m", P.ptr(u), P.ptr(v), P.ptr(w), P.ptr(ustar), P.ptr(M_sfc_loc),
+           P.ptr(fi2D_m), P.ptr(a), P.ptr(b), P.ptr(c), current_stream())
+    return a, b, c
+
+
+def Derivz_TH(TH, ustar, qz_sfc, fi2D_h, ZeRo3D=None):
+    """src/operations/Derivatives.py:243."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_derivz_th", P.ptr(TH), P.ptr(ustar), P.ptr(qz_sfc), P.ptr(fi2D_h), P.ptr(out),
+           current_stream())
+    return out
+
+
+def Derivz_Generic_uvp(F, ZeRo3D=None):
+    """src/operations/Derivatives.py:310."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_derivz_generic_uvp", P.ptr(F), P.ptr(out), current_stream())
+    return out
+
+
+def Derivz_Generic_w(F, ZeRo3D=None):
+    """src/operations/Derivatives.py:340."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_derivz_generic_w", P.ptr(F), P.ptr(out), current_stream())
+    return out
+
+
+def velocityGradients(u, v, w, u_fft, v_fft, w_fft, kx2, ky2, ustar, M_sfc_loc, MOSTfunctions,
+                      ZeRo3D=None):
+    """src/operations/Derivatives.py:47."""
+    dx = [Derivxy(f, 0) for f in (u_fft, v_fft, w_fft)]
+    dy = [Derivxy(f, 1) for f in (u_fft, v_fft, w_fft)]
+    dz = Derivz_M(u, v, w, ustar, M_sfc_loc, MOSTfunctions[4])
+    return (dx[0], dx[1], dx[2], dy[0], dy[1], dy[2], dz[0], dz[1], dz[2])
+
+
+def potentialTemperatureGradients(TH, kx2, ky2, ustar, qz_sfc, MOSTfunctions, ZeRo3D=None):
+    """src/operations/Derivatives.py:107."""
+    f = FFT(TH)
+    return Derivxy(f, 0), Derivxy(f, 1), Derivz_TH(TH, ustar, qz_sfc, MOSTfunctions[5])
+
+
+# ----------------------------------------------------------------- utilities
+def PlanarMean(F):
+    """src/utilities/Utilities.py:41 (float64 result)."""
+    P = _P()
+    out = P.zeros((P.nz,), dtype=_torch().float64)
+    P.call("alfa_planar_mean", P.ptr(F), P.ptr(out), current_stream())
+    return out
+
+
+def StagGridAvg(F):
+    """src/utilities/Utilities.py:64."""
+    P = _P()
+    out = P.zeros((P.nx, P.ny, P.nz - 1))
+    P.call("alfa_stag_grid_avg", P.ptr(F), P.ptr(out), current_stream())
+    return out
+
+
+def Imfilter3D(F):
+    """Utilities.py:87 applied to every level of an (nx, ny, nz) field (the
+    reference vmaps it over levels, DynamicSGS_LASDD_SM.py:357)."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_imfilter", P.ptr(F), P.ptr(out), current_stream())
+    return out
+
+
+def Imfilter(x):
+    """src/utilities/Utilities.py:87 on one (nx, ny) plane."""
+    P = _P()
+    F = x.reshape(P.nx, P.ny, 1).expand(P.nx, P.ny, P.nz).contiguous()
+    return Imfilter3D(F)[:, :, 0].contiguous()
+
+
+def ComputeBeta(ff, ee, dd, cc, bb, aa):
+    """ComputeBeta1 (DynamicSGS_LASDD_SM.py:53) / ComputeBeta2
+    (DynamicSGS_ScalarLASDD_SM.py:54): Laguerre roots (Utilities.py:123)."""
+    torch = _torch()
+    P = _P()
+    co = torch.stack([torch.as_tensor(a, dtype=torch.float64) for a in (ff, ee, dd, cc, bb, aa)])
+    co = co.to(P.device).contiguous()
+    n = co.shape[1]
+    out = P.zeros((n,), dtype=torch.float64)
+    P.call("alfa_compute_beta", P.ptr(co), int(n), P.ptr(out), current_stream())
+    return out
+
+
+ComputeBeta1 = ComputeBeta
+ComputeBeta2 = ComputeBeta
+
+
+# ----------------------------------------------------------------------- pde
+def Advection(u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, *zeros):
+    """src/pde/NSE_AdvectionTerms.py:246."""
+    P = _P()
+    o = [P.field() for _ in range(3)]
+    P.call("alfa_advection", *[P.ptr(t) for t in (u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy)],
+           *[P.ptr(t) for t in o], current_stream())
+    return tuple(o)
+
+
+def ScalarAdvection(u, v, w, dsdx, dsdy, dsdz, *zeros):
+    """src/pde/SCL_AdvectionTerms.py:158."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_scalar_advection", *[P.ptr(t) for t in (u, v, w, dsdx, dsdy, dsdz, out)],
+           current_stream())
+    return out
+
+
+def _buoy(TH, want):
+    P = _P()
+    if int(P.cfg.get("optBuoyancy", 1)) != want:
+        raise ValueError(f"plan has optBuoyancy={P.cfg.get('optBuoyancy', 1)}")
+    out = P.field()
+    P.call("alfa_buoyancy", P.ptr(TH), P.ptr(out), current_stream())
+    return out
+
+
+def BuoyancyOpt1(TH, H=None, ZeRo3D=None):
+    """src/pde/NSE_BuoyancyTerms.py:49 (dry: H must be zero)."""
+    return _buoy(TH, 0)
+
+
+def BuoyancyOpt2(TH, H=None, ZeRo3D=None):
+    """src/pde/NSE_BuoyancyTerms.py:104 (dry: H must be zero)."""
+    return _buoy(TH, 1)
+
+
+def DivStress(txx, tyy, tzz, txy, txz, tyz, ZeRo3D=None, kx2=None, ky2=None):
+    """src/pde/NSE_SGSTerms.py:55."""
+    P = _P()
+    o = [P.field() for _ in range(3)]
+    P.call("alfa_div_stress", *[P.ptr(t) for t in (txx, tyy, tzz, txy, txz, tyz)],
+           *[P.ptr(t) for t in o], current_stream())
+    return tuple(o)
+
+
+def DivFlux(qx, qy, qz, ZeRo3D=None, kx2=None, ky2=None):
+    """src/pde/SCL_SGSTerms.py:55."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_div_flux", P.ptr(qx), P.ptr(qy), P.ptr(qz), P.ptr(out), current_stream())
+    return out
+
+
+def RHS_Momentum(u, v, w, Ug, Vg, Cx, Cy, Cz, buoyancy, divtx, divty, divtz,
+                 RayleighDampCoeff=None, RayleighDampCoeff_stag=None, Uadv=None, Vadv=None):
+    """src/pde/NSE_AllTerms.py:48 (optAdvection = 0; Ug, Vg and the damping
+    profiles are the plan's)."""
+    P = _P()
+    o = [P.field() for _ in range(3)]
+    P.call("alfa_rhs_momentum",
+           *[P.ptr(t) for t in (u, v, w, Cx, Cy, Cz, buoyancy, divtx, divty, divtz)],
+           *[P.ptr(t) for t in o], current_stream())
+    return tuple(o)
+
+
+def RHS_Scalar(TH, THAdvectionSum, divq, RayleighDampCoeff_stag=None, THadv=None):
+    """src/pde/SCL_AllTerms.py:48 (optAdvection = 0)."""
+    P = _P()
+    out = P.field()
+    P.call("alfa_rhs_scalar", P.ptr(TH), P.ptr(THAdvectionSum), P.ptr(divq), P.ptr(out),
+           current_stream())
+    return out
+
+
+def PressureRC(u, v, w, RHS_u, RHS_v, RHS_w, RHS_u_previous, RHS_v_previous, RHS_w_previous,
+               divtz, kr2_pressure=None, kc2_pressure=None):
+    """src/pde/NSE_PressureTerms.py:109 -> (RC_real, RC_imag, fRz_real)."""
+    P = _P()
+    RCr = P.zeros((P.nx, P.nyc, P.nz + 1))
+    RCi = P.zeros((P.nx, P.nyc, P.nz + 1))
+    fRz = P.zeros((P.nx, P.nyc, P.nz))
+    P.call("alfa_pressure_rc",
+           *[P.ptr(t) for t in (u, v, w, RHS_u, RHS_v, RHS_w, RHS_u_previous, RHS_v_previous,
+                                RHS_w_previous, divtz, RCr, RCi, fRz)], current_stream())
+    return RCr, RCi, fRz
+
+
+def ThomasPressureSolve(RC_real, RC_imag, fRz_real, b_thomas=None, m_thomas=None,
+                        c_pressure=None):
+    """src/pde/NSE_PressureTerms_Thomas.py:218 -> (p, dpdx, dpdy, dpdz)."""
+    P = _P()
+    o = [P.field() for _ in range(4)]
+    P.call("alfa_thomas_pressure_solve", P.ptr(RC_real), P.ptr(RC_imag), P.ptr(fRz_real),
+           *[P.ptr(t) for t in o], current_stream())
+    return tuple(o)
+
+
+def AB2_uvw(u, v, w, RHS_u, RHS_u_previous, RHS_v, RHS_v_previous, RHS_w, RHS_w_previous):
+    """src/pde/NSE_TimeAdvancement.py:45 (returns updated copies)."""
+    P = _P()
+    un, vn, wn = u.clone(), v.clone(), w.clone()
+    P.call("alfa_ab2_uvw", *[P.ptr(t) for t in (un, vn, wn, RHS_u, RHS_u_previous, RHS_v,
+                                                RHS_v_previous, RHS_w, RHS_w_previous)],
+           current_stream())
+    return un, vn, wn
+
+
+def AB2_TH(TH, RHS_TH, RHS_TH_previous):
+    """src/pde/SCL_TimeAdvancement.py:46 (returns an updated copy)."""
+    P = _P()
+    tn = TH.clone()
+    P.call("alfa_ab2_th", P.ptr(tn), P.ptr(RHS_TH), P.ptr(RHS_TH_previous), current_stream())
+    return tn
+
+
+# ---------------------------------------------------------------- subgridscale
+def _ptr_array(P, ts):
+    arr = (C.c_void_p * len(ts))()
+    for i, t in enumerate(ts):
+        arr[i] = P.ptr(t).value
+    return arr
+
+
+def StrainsUVPnodes_NoDealias(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, ZeRo3D=None):
+    """src/subgridscale/StrainRates.py:149 -> S11, S22, S33, S12, S13, S23, S."""
+    P = _P()
+    o = [P.field() for _ in range(7)]
+    P.call("alfa_strains_uvp", _ptr_array(P, [dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz]),
+           _ptr_array(P, o), current_stream())
+    return tuple(o)
+
+
+def StrainsWnodes_NoDealias(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, ZeRo3D=None):
+    """src/subgridscale/StrainRates.py:307 -> S13, S23, S_w."""
+    P = _P()
+    o = [P.field() for _ in range(3)]
+    P.call("alfa_strains_w", _ptr_array(P, [dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz]),
+           _ptr_array(P, o), current_stream())
+    return tuple(o)
+
+
+def LASDD(u, v, w, S11, S22, S33, S12, S13, S23, S, ZeRo3D=None):
+    """src/subgridscale/DynamicSGS_LASDD_SM.py:138 (optSgs 1 or 3 of the plan)
+    -> (Cs2_3D, Cs2_1D_avg1, Cs2_1D_avg2, beta1_1D)."""
+    torch = _torch()
+    P = _P()
+    cs = P.field()
+    pr = [P.zeros((P.nz,), dtype=torch.float64) for _ in range(3)]
+    P.call("alfa_lasdd_sm", P.ptr(u), P.ptr(v), P.ptr(w),
+           _ptr_array(P, [S11, S22, S33, S12, S13, S23, S]), P.ptr(cs),
+           *[P.ptr(t) for t in pr], current_stream())
+    return cs, pr[0], pr[1], pr[2]
+
+
+def Wall(u, v, M_sfc_loc, psi2D_m, psi2D_m0):
+    """src/subgridscale/SGSStresses_SM.py:63 -> (txz, tyz) planes."""
+    P = _P()
+    a, b = P.plane(), P.plane()
+    P.call("alfa_wall", *[P.ptr(t) for t in (u, v, M_sfc_loc, psi2D_m, psi2D_m0, a, b)],
+           current_stream())
+    return a, b
+
+
+# -------------------------------------------------------------------- surface
+def _surface(u, v, TH, sfc_val, MOSTfunctions):
+    torch = _torch()
+    P = _P()
+    most = [m.clone() for m in MOSTfunctions]
+    M, us, qz, iob = P.plane(), P.plane(), P.plane(), P.plane()
+    qavg = P.zeros((1,), dtype=torch.float64)
+    P.call("alfa_surface_flux", P.ptr(u), P.ptr(v), P.ptr(TH), C.c_double(float(sfc_val)),
+           *[P.ptr(t) for t in most], *[P.ptr(t) for t in (M, us, qz, iob, qavg)],
+           current_stream())
+    return M, us, qz, float(qavg.item()), iob, tuple(most)
+
+
+def _check_surface(bc, flux):
+    P = _P()
+    if int(P.cfg.get("optSurfBC", 0)) != bc or int(P.cfg.get("optSurfFlux", 0)) != flux:
+        raise ValueError("surface variant does not match the plan's optSurfBC/optSurfFlux")
+
+
+def SurfaceFlux_HomogeneousConstantFlux(u, v, TH, MOSTfunctions):
+    """src/surface/SurfaceFlux.py:166 -> (M, ustar, qz_sfc_avg, invOB, MOST)."""
+    _check_surface(0, 0)
+    M, us, qz, qa, iob, most = _surface(u, v, TH, 0.0, MOSTfunctions)
+    return M, us, qa, iob, most
+
+
+def SurfaceFlux_HeterogeneousConstantFlux(u, v, TH, MOSTfunctions):
+    """src/surface/SurfaceFlux.py:209."""
+    _check_surface(0, 1)
+    M, us, qz, qa, iob, most = _surface(u, v, TH, 0.0, MOSTfunctions)
+    return M, us, qa, iob, most
+
+
+def SurfaceFlux_HomogeneousVaryingFlux(u, v, TH, qz_sfc_t, MOSTfunctions):
+    """src/surface/SurfaceFlux.py:246."""
+    _check_surface(1, 0)
+    return _surface(u, v, TH, qz_sfc_t, MOSTfunctions)
+
+
+def SurfaceFlux_HeterogeneousVaryingFlux(u, v, TH, qz_sfc_t, MOSTfunctions):
+    """src/surface/SurfaceFlux.py:290."""
+    _check_surface(1, 1)
+    return _surface(u, v, TH, qz_sfc_t, MOSTfunctions)
+
+
+def SurfaceFlux_HomogeneousPrescribedTemperature(u, v, TH, TH_sfc_t, MOSTfunctions):
+    """src/surface/SurfaceFlux.py:335."""
+    _check_surface(2, 0)
+    return _surface(u, v, TH, TH_sfc_t, MOSTfunctions)
+
+
+def SurfaceFlux_HeterogeneousPrescribedTemperature(u, v, TH, TH_sfc_t, MOSTfunctions):
+    """src/surface/SurfaceFlux.py:393."""
+    _check_surface(2, 1)
+    return _surface(u, v, TH, TH_sfc_t, MOSTfunctions)
+
+
+def _torch():
+    import torch
+    return torch

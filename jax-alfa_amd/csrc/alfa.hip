@@ -1,0 +1,824 @@
+// libalfa: C ABI of the MI355X-native JAX-ALFA time step (include/alfa.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "step.hpp"
+
+using namespace alfa;
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" const char* alfa_last_error(void) { return g_err.c_str(); }
+
+static bool factor(int n, Fft1D& f) {
+  f.n = n;
+  f.nrad = 0;
+  int m = n;
+  const int order[4] = {4, 2, 3, 5};
+  for (int r : order) {
+    while (m % r == 0) {
+      if (f.nrad >= FFT_MAXRAD) return false;
+      f.rad[f.nrad++] = r;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+template <class T>
+static void* make_twiddles(int n, std::vector<void*>& allocs, hipError_t& e) {
+  std::vector<c2<T>> h(n);
+  for (int t = 0; t < n; ++t) {
+    const double a = 2.0 * M_PI * (double)t / (double)n;
+    h[t] = mkc<T>((T)std::cos(a), (T)(-std::sin(a)));
+  }
+  void* p = nullptr;
+  e = hipMalloc(&p, n * sizeof(c2<T>));
+  if (e != hipSuccess) return nullptr;
+  allocs.push_back(p);
+  e = hipMemcpy(p, h.data(), n * sizeof(c2<T>), hipMemcpyHostToDevice);
+  return p;
+}
+
+static int check_hip(alfa_plan* P, const char* where) {
+  hipError_t e = P->herr;
+  P->herr = hipSuccess;
+  if (e != hipSuccess) return set_error(ALFA_EHIP, "%s: %s", where, hipGetErrorString(e));
+  if (P->ws.overflow)
+    return set_error(ALFA_ESTATE, "%s: workspace buffer '%s' was not planned", where,
+                     P->ws.overflow_name.c_str());
+  return ALFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-operation implementations (templated on the field type)
+// ---------------------------------------------------------------------------
+namespace ops {
+
+template <class T> int fft(alfa_plan* P, hipStream_t s, const void* F, void* out) {
+  Ctx<T> c(P, s);
+  LoadPtr<T> ld{};
+  ld.p[0] = (const T*)F; ld.ny = c.ny; ld.nz = c.nz;
+  c2<T>* o = (c2<T>*)out;
+  rfft2_fields<T>(c, ld, c.nz, 1, &o);
+  return 0;
+}
+
+template <class T> int fft_pad(alfa_plan* P, hipStream_t s, const void* F, void* out) {
+  Ctx<T> c(P, s);
+  const int myc = c.my / 2 + 1;
+  LoadPtr<T> ld{};
+  ld.p[0] = (const T*)F; ld.ny = c.my; ld.nz = c.nz;
+  void* o[1] = {out};
+  y_r2c<T>(P, s, ld, c.mx, P->fmy, P->tw_my, c.nz, myc, 1, o);
+  XArgs xa{};
+  xa.in[0] = out; xa.dst[0][0] = out;
+  xa.op[0] = xop(myc, 0, BIGCUT, XM_NONE, XMAP_ID, c.mx);
+  xa.nops = 1; xa.ncy_in = myc; xa.nz = c.nz; xa.ny_full = c.my;
+  x_pass<T>(P, s, xa, 1, P->fmx, P->fmx, P->tw_mx, P->tw_mx);
+  return 0;
+}
+
+// masked / derivative inverse of a full-pitch spectrum into a real field
+template <class T>
+void spectral_to_real(Ctx<T>& c, const void* Ffft, int ncy, int cx, int mult, T* out) {
+  alfa_plan* P = c.P;
+  c2<T>* tmp = c.S("op0");
+  XArgs xa{};
+  xa.in[0] = Ffft; xa.dst[0][0] = tmp;
+  xa.op[0] = xop(ncy, 1, cx, mult, XMAP_ID, c.nx);
+  xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = c.nz; xa.in_spectral = 1; xa.ny_full = c.ny;
+  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  YC2RArgs ya{};
+  ya.in[0][0] = tmp; ya.nsum = 1; ya.ncy_in = ncy; ya.nz = c.nz;
+  ya.scale = 1.0 / ((double)c.nx * c.ny);
+  StorePtr<T> st{};
+  st.p[0] = out; st.ny = c.ny; st.nz = c.nz;
+  y_c2r<T>(P, c.s, st, ya, 1, c.nx, P->fy, P->tw_y);
+}
+
+template <class T>
+int filtering_explicit(alfa_plan* P, hipStream_t s, const void* Ffft, void* F, void* Fnew) {
+  Ctx<T> c(P, s);
+  const auto& d = P->d;
+  if (P->ws.dry) return 0;
+  const size_t n = 2 * (size_t)c.nx * c.nyc * c.nz;
+  k_fill<T><<<pw_blocks(n), 256, 0, s>>>(n, (T*)Fnew, T(0));
+  launch_ok(P);
+  // masked spectrum at full pitch: only the first cut0_y columns are
+  // processed (grid), columns >= cut0_y keep the zeros written above
+  XArgs xa{};
+  xa.in[0] = Ffft; xa.dst[0][0] = Fnew;
+  xa.op[0] = xop(c.nyc, 0, d.cut0_x, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = c.nz; xa.in_spectral = 1; xa.ny_full = c.ny;
+  constexpr int L = Lines<T>::L;
+  dim3 grid((c.nz + L - 1) / L, d.cut0_y, 1);
+  const size_t lds = (size_t)c.nx * L * sizeof(c2<T>) + 2 * c.nx * sizeof(c2<T>);
+  k_x<T><<<grid, FFT_BS, lds, s>>>(xa, P->fx, P->fx, (const c2<T>*)P->tw_x, (const c2<T>*)P->tw_x);
+  launch_ok(P);
+  spectral_to_real<T>(c, Fnew, d.cut0_y, BIGCUT, XM_NONE, (T*)F);
+  return 0;
+}
+
+template <class T> int filtering_level(alfa_plan* P, hipStream_t s, const void* Ffft, void* out,
+                                       int level) {
+  Ctx<T> c(P, s);
+  const auto& d = P->d;
+  if (level == 1) spectral_to_real<T>(c, Ffft, d.cut1_y, d.cut1_x, XM_NONE, (T*)out);
+  else spectral_to_real<T>(c, Ffft, d.cut2_y, d.cut2_x, XM_NONE, (T*)out);
+  return 0;
+}
+
+template <class T> int derivxy(alfa_plan* P, hipStream_t s, const void* Ffft, int axis, void* out) {
+  Ctx<T> c(P, s);
+  spectral_to_real<T>(c, Ffft, c.nyc, BIGCUT, axis == 0 ? XM_IKX : XM_IKY, (T*)out);
+  return 0;
+}
+
+template <class T> int dealias1_op(alfa_plan* P, hipStream_t s, const void* Ffft, void* out) {
+  Ctx<T> c(P, s);
+  c2<T>* pp = c.PS("op0");
+  XArgs xa{};
+  xa.in[0] = Ffft; xa.dst[0][0] = pp;
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_D1, c.mx);
+  xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = c.nz; xa.in_spectral = 1; xa.ny_full = c.ny;
+  x_pass<T>(P, s, xa, 1, P->fx, P->fmx, P->tw_x, P->tw_mx);
+  YC2RArgs ya{};
+  ya.in[0][0] = pp; ya.nsum = 1; ya.ncy_in = c.nyc; ya.nz = c.nz;
+  ya.scale = 1.0 / ((double)c.mx * c.my);
+  StorePtr<T> st{};
+  st.p[0] = (T*)out; st.ny = c.my; st.nz = c.nz;
+  y_c2r<T>(P, s, st, ya, 1, c.mx, P->fmy, P->tw_my);
+  return 0;
+}
+
+template <class T> int dealias2_op(alfa_plan* P, hipStream_t s, const void* Fpfft, void* out) {
+  Ctx<T> c(P, s);
+  const int ncy = c.ny / 2;
+  c2<T>* tmp = c.S("op0");
+  XArgs xa{};
+  xa.in[0] = Fpfft; xa.dst[0][0] = tmp;
+  xa.op[0] = xop(ncy, 1, BIGCUT, XM_NONE, XMAP_D2, c.nx);
+  xa.nops = 1; xa.ncy_in = c.my / 2 + 1; xa.nz = c.nz; xa.in_spectral = 1; xa.ny_full = c.ny;
+  x_pass<T>(P, s, xa, 1, P->fmx, P->fx, P->tw_mx, P->tw_x);
+  YC2RArgs ya{};
+  ya.in[0][0] = tmp; ya.nsum = 1; ya.ncy_in = ncy; ya.nz = c.nz;
+  ya.scale = 2.25 / ((double)c.nx * c.ny);
+  StorePtr<T> st{};
+  st.p[0] = (T*)out; st.ny = c.ny; st.nz = c.nz;
+  y_c2r<T>(P, s, st, ya, 1, c.nx, P->fy, P->tw_y);
+  return 0;
+}
+
+template <class T>
+int advection(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* w,
+              const void* dudy, const void* dudz, const void* dvdx, const void* dvdz,
+              const void* dwdx, const void* dwdy, void* Cx, void* Cy, void* Cz) {
+  Ctx<T> c(P, s);
+  if (P->d.optDealias == 0) {
+    PW_LAUNCH(P, s, c.F, k_advection<T>, c.dims(), (const T*)u, (const T*)v, (const T*)w,
+              (const T*)dudy, (const T*)dudz, (const T*)dvdx, (const T*)dvdz, (const T*)dwdx,
+              (const T*)dwdy, (T*)Cx, (T*)Cy, (T*)Cz);
+    return 0;
+  }
+  const T* in9[9] = {(const T*)u, (const T*)v, (const T*)w, (const T*)dudy, (const T*)dudz,
+                     (const T*)dvdx, (const T*)dvdz, (const T*)dwdx, (const T*)dwdy};
+  T* p9[9];
+  for (int b = 0; b < 9; ++b) p9[b] = c.Pad("adv" + std::to_string(b));
+  dealias1<T>(c, 9, in9, p9);
+  T *pcx = c.Pad("cx"), *pcy = c.Pad("cy"), *pcz = c.Pad("cz");
+  PW_LAUNCH(P, s, (size_t)c.mx * c.my * c.nz, k_advection<T>, c.pdims(), p9[0], p9[1], p9[2],
+            p9[3], p9[4], p9[5], p9[6], p9[7], p9[8], pcx, pcy, pcz);
+  const T* pc3[3] = {pcx, pcy, pcz};
+  T* o3[3] = {(T*)Cx, (T*)Cy, (T*)Cz};
+  dealias2<T>(c, 3, pc3, o3);
+  return 0;
+}
+
+template <class T>
+int scalar_advection(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* w,
+                     const void* sx, const void* sy, const void* sz, void* out) {
+  Ctx<T> c(P, s);
+  if (P->d.optDealias == 0) {
+    PW_LAUNCH(P, s, c.F, k_scalar_adv<T>, c.dims(), (const T*)u, (const T*)v, (const T*)w,
+              (const T*)sx, (const T*)sy, (const T*)sz, (T*)out);
+    return 0;
+  }
+  const T* in6[6] = {(const T*)u, (const T*)v, (const T*)w, (const T*)sx, (const T*)sy, (const T*)sz};
+  T* p6[6];
+  for (int b = 0; b < 6; ++b) p6[b] = c.Pad("adv" + std::to_string(b));
+  dealias1<T>(c, 6, in6, p6);
+  T* pcs = c.Pad("cs");
+  PW_LAUNCH(P, s, (size_t)c.mx * c.my * c.nz, k_scalar_adv<T>, c.pdims(), p6[0], p6[1], p6[2],
+            p6[3], p6[4], p6[5], pcs);
+  const T* pc1[1] = {pcs};
+  T* o1[1] = {(T*)out};
+  dealias2<T>(c, 1, pc1, o1);
+  return 0;
+}
+
+template <class T> int buoyancy(alfa_plan* P, hipStream_t s, const void* TH, void* out) {
+  Ctx<T> c(P, s);
+  double* bar = c.D("op_bar", c.nz);
+  FSum1<T> f{(const T*)TH, c.nz, P->cfg.T_0};
+  plane_sums<1>(P, s, c.ncol, c.nz, f, bar, 1.0 / c.ncol, "op1");
+  PW_LAUNCH(P, s, c.F, k_buoyancy<T>, c.dims(), (const T*)TH, bar, (T*)out, P->cfg.T_0,
+            P->d.g_nondim, P->cfg.optBuoyancy);
+  return 0;
+}
+
+template <class T> int planar_mean(alfa_plan* P, hipStream_t s, const void* F, double* out) {
+  Ctx<T> c(P, s);
+  FSum1<T> f{(const T*)F, c.nz, 0.0};
+  plane_sums<1>(P, s, c.ncol, c.nz, f, out, 1.0 / c.ncol, "op1");
+  return 0;
+}
+
+template <class T>
+int rhs_momentum(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* w,
+                 const void* Cx, const void* Cy, const void* Cz, const void* b,
+                 const void* dtx, const void* dty, const void* dtz, void* Ru, void* Rv, void* Rw) {
+  Ctx<T> c(P, s);
+  double* m = c.D("op_uv", 2 * c.nz);
+  FSum3<T> f{(const T*)u, (const T*)v, (const T*)u, c.nz, 0.0};
+  double* m3 = c.D("op_m3", 3 * c.nz);
+  plane_sums<3>(P, s, c.ncol, c.nz, f, m3, 1.0 / c.ncol, "op3");
+  (void)m;
+  const double* pr = P->prof;
+  PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), (const T*)u, (const T*)v, (const T*)w, (const T*)nullptr,
+            (const T*)Cx, (const T*)Cy, (const T*)Cz, (const T*)b, (const T*)dtx, (const T*)dty,
+            (const T*)dtz, (const T*)nullptr, (const T*)nullptr, pr, pr + c.nz, pr + 2 * c.nz,
+            pr + 3 * c.nz, m3, m3 + c.nz, m3 + 2 * c.nz, P->d.f_coriolis_nondim, P->cfg.Ugal,
+            P->cfg.optDamping, (T*)Ru, (T*)Rv, (T*)Rw, (T*)nullptr);
+  return 0;
+}
+
+template <class T>
+int rhs_scalar(alfa_plan* P, hipStream_t s, const void* TH, const void* adv, const void* divq,
+               void* RT) {
+  Ctx<T> c(P, s);
+  double* m = c.D("op_bar", c.nz);
+  FSum1<T> f{(const T*)TH, c.nz, 0.0};
+  plane_sums<1>(P, s, c.ncol, c.nz, f, m, 1.0 / c.ncol, "op1");
+  const double* pr = P->prof;
+  PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), (const T*)nullptr, (const T*)nullptr,
+            (const T*)nullptr, (const T*)TH, (const T*)nullptr, (const T*)nullptr,
+            (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+            (const T*)nullptr, (const T*)adv, (const T*)divq, pr, pr + c.nz, pr + 2 * c.nz,
+            pr + 3 * c.nz, m, m, m, 0.0, 0.0, P->cfg.optDamping, (T*)nullptr, (T*)nullptr,
+            (T*)nullptr, (T*)RT);
+  return 0;
+}
+
+template <class T>
+int pressure_rc(alfa_plan* P, hipStream_t s, const void* const R[3], const void* const Rp[3],
+                const void* const U[3], const void* divtz, void* RCr, void* RCi, void* fRzr) {
+  Ctx<T> c(P, s);
+  T* dp[3] = {c.W("op_dpx"), c.W("op_dpy"), c.W("op_dpz")};
+  const T* r[3] = {(const T*)R[0], (const T*)R[1], (const T*)R[2]};
+  const T* rp[3] = {(const T*)Rp[0], (const T*)Rp[1], (const T*)Rp[2]};
+  const T* u[3] = {(const T*)U[0], (const T*)U[1], (const T*)U[2]};
+  c2<T>* fR[3];
+  stage_pressure<T>(c, r, rp, u, (const T*)divtz, dp[0], dp[1], dp[2], nullptr, (T*)RCr, (T*)RCi,
+                    fR);
+  PW_LAUNCH(P, s, (size_t)c.nx * c.nyc * c.nz, k_real_part<T>, (size_t)c.nx * c.nyc * c.nz,
+            (const c2<T>*)fR[2], (T*)fRzr);
+  return 0;
+}
+
+template <class T>
+int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, const void* fRzr,
+                 void* p, void* dpdx, void* dpdy, void* dpdz) {
+  Ctx<T> c(P, s);
+  const auto& d = P->d;
+  c2<T>* phat = c.S("phat");
+  c2<T>* dpz = c.S("dpzh");
+  c2<T>* dpx = c.S("dpxh");
+  c2<T>* dpy = c.S("dpyh");
+  ThomasArgs<T> ta{};
+  ta.m = P->th_m; ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
+  ta.RCin_r = (const T*)RCr; ta.RCin_i = (const T*)RCi; ta.fRzr_in = (const T*)fRzr;
+  ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
+  ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
+  if (!P->ws.dry) {
+    k_thomas<T><<<c.nx * c.nyc, 64, 0, s>>>(ta);
+    launch_ok(P);
+  }
+  XArgs xa{};
+  xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.in_spectral = 1;
+  xa.in[0] = phat; xa.dst[0][0] = dpx; xa.dst[0][1] = dpy; xa.dst[0][2] = phat;
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
+  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
+  xa.op[2] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = 3;
+  x_pass<T>(P, s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  xa.in[0] = dpz; xa.dst[0][0] = dpz; xa.nops = 1;
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  x_pass<T>(P, s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  YC2RArgs ya{};
+  ya.in[0][0] = dpx; ya.in[1][0] = dpy; ya.in[2][0] = dpz; ya.in[3][0] = phat;
+  ya.nsum = 1; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  StorePtr<T> st{};
+  st.p[0] = (T*)dpdx; st.p[1] = (T*)dpdy; st.p[2] = (T*)dpdz; st.p[3] = (T*)p;
+  st.ny = c.ny; st.nz = c.nz;
+  y_c2r<T>(P, s, st, ya, 4, c.nx, P->fy, P->tw_y);
+  return 0;
+}
+
+template <class T>
+int lasdd(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* w,
+          const void* const str[7], void* Cs2, double* a1, double* a2, double* beta) {
+  Ctx<T> c(P, s);
+  const T* Sij[6];
+  for (int n = 0; n < 6; ++n) Sij[n] = (const T*)str[n];
+  stage_lasdd<T>(c, (const T*)u, (const T*)v, (const T*)w, Sij, (const T*)str[6], nullptr, nullptr,
+                 (T*)Cs2, a1, a2, beta);
+  return 0;
+}
+
+template <class T>
+int surface(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* TH,
+            double sfc_val, void* pm, void* pm0, void* ph, void* ph0, void* fm, void* fh,
+            void* M, void* ustar, void* qz, void* invOB, double* qavg) {
+  Ctx<T> c(P, s);
+  StateT<T> S{};
+  S.u = (T*)u; S.v = (T*)v; S.TH = (T*)TH;
+  S.psim = (T*)pm; S.psim0 = (T*)pm0; S.psih = (T*)ph; S.psih0 = (T*)ph0; S.fim = (T*)fm; S.fih = (T*)fh;
+  Surf<T> sf{(T*)M, (T*)ustar, (T*)qz, (T*)invOB};
+  stage_surface<T>(c, S, sf, sfc_val);
+  if (!P->ws.dry && qavg)
+    (void)hipMemcpyAsync(qavg, P->scal + 1, sizeof(double), hipMemcpyDeviceToDevice, s);
+  return 0;
+}
+
+}  // namespace ops
+
+// Run every operation once in dry mode so that its named buffers are planned.
+template <class T> static void plan_dry_run(alfa_plan* P) {
+  hipStream_t s = nullptr;
+  void* f = (void*)(uintptr_t)0x1000;
+  alfa_state st{};
+  st.u = st.v = st.w = st.TH = f;
+  st.RHS_u = st.RHS_v = st.RHS_w = st.RHS_TH = f;
+  st.Cs2_3D = st.Cs2PrRatio_3D = f;
+  st.psi_m = st.psi_m0 = st.psi_h = st.psi_h0 = st.fi_m = st.fi_h = f;
+  step_t<T>(P, &st, P->cfg.istep, 0.0, s);
+  if (P->cfg.dynamicSGS_call_time > 1) step_t<T>(P, &st, P->cfg.istep + 1, 0.0, s);
+  ops::fft<T>(P, s, f, f);
+  ops::fft_pad<T>(P, s, f, f);
+  ops::filtering_explicit<T>(P, s, f, f, f);
+  ops::filtering_level<T>(P, s, f, f, 1);
+  ops::derivxy<T>(P, s, f, 0, f);
+  ops::dealias1_op<T>(P, s, f, f);
+  ops::dealias2_op<T>(P, s, f, f);
+  ops::advection<T>(P, s, f, f, f, f, f, f, f, f, f, f, f, f);
+  ops::scalar_advection<T>(P, s, f, f, f, f, f, f, f);
+  ops::buoyancy<T>(P, s, f, f);
+  ops::planar_mean<T>(P, s, f, (double*)f);
+  ops::rhs_momentum<T>(P, s, f, f, f, f, f, f, f, f, f, f, f, f, f);
+  ops::rhs_scalar<T>(P, s, f, f, f, f);
+  const void* three[3] = {f, f, f};
+  ops::pressure_rc<T>(P, s, three, three, three, f, f, f, f);
+  ops::thomas_solve<T>(P, s, f, f, f, f, f, f, f);
+  Ctx<T> c(P, s);
+  stage_div_stress<T>(c, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f);
+  stage_div_flux<T>(c, (T*)f, (T*)f, (T*)f, (T*)f);
+  const void* seven[7] = {f, f, f, f, f, f, f};
+  ops::lasdd<T>(P, s, f, f, f, seven, f, nullptr, nullptr, nullptr);
+}
+
+template <class T> static int allocate_workspace(alfa_plan* P) {
+  P->ws.dry = true;
+  plan_dry_run<T>(P);
+  size_t total = 0;
+  for (auto& kv : P->ws.bufs) {
+    const size_t bytes = (kv.second.first + 255) & ~(size_t)255;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+      return set_error(ALFA_ENOMEM, "hipMalloc(%zu) for '%s': %s", bytes, kv.first.c_str(),
+                       hipGetErrorString(e));
+    P->allocs.push_back(p);
+    kv.second.second = p;
+    total += bytes;
+  }
+  P->d.workspace_bytes = (int64_t)total;
+  P->ws.dry = false;
+  return ALFA_OK;
+}
+
+static void free_plan(alfa_plan* P) {
+  if (!P) return;
+  for (void* p : P->allocs) (void)hipFree(p);
+  delete P;
+}
+
+extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan** out) {
+  if (!cfg || !out) return set_error(ALFA_EINVAL, "alfa_plan_create: null argument");
+  *out = nullptr;
+  if (precision != ALFA_FP64 && precision != ALFA_MIXED)
+    return set_error(ALFA_EINVAL, "unknown precision %d", precision);
+  const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+  if (nx < 8 || ny < 8 || nz < 4 || (nx & 1) || (ny & 1) || (nz & 1))
+    return set_error(ALFA_EINVAL, "grid %dx%dx%d: nx, ny >= 8 and nz >= 4, all even", nx, ny, nz);
+  if (nz + 1 > TH_MAXN) return set_error(ALFA_EINVAL, "nz=%d exceeds %d", nz, TH_MAXN - 1);
+  if (cfg->FGR != 1 && cfg->FGR != 2) return set_error(ALFA_EINVAL, "FGR must be 1 or 2");
+  if (!(cfg->optSgs == 0 || cfg->optSgs == 1 || cfg->optSgs == 3))
+    return set_error(ALFA_EINVAL, "optSgs=%d: supported are 0 (static SM), 1 (LASDD-SM), 3 (LAD-SM)",
+                     cfg->optSgs);
+  if (cfg->optSurfBC < 0 || cfg->optSurfBC > 2) return set_error(ALFA_EINVAL, "optSurfBC must be 0..2");
+  alfa_plan* P = new alfa_plan();
+  P->cfg = *cfg;
+  if (P->cfg.istep == 0) P->cfg.istep = 1;
+  P->prec = precision;
+  P->nx = nx; P->ny = ny; P->nz = nz; P->nyc = ny / 2 + 1;
+  P->mx = (int)(1.5 * nx); P->my = (int)(1.5 * ny); P->myc = P->my / 2 + 1;
+  alfa_derived& d = P->d;
+  memset(&d, 0, sizeof d);
+  d.z_scale = cfg->l_x / (2 * M_PI);
+  d.g_nondim = 9.81 * d.z_scale;
+  d.f_coriolis_nondim = cfg->f_coriolis * d.z_scale;
+  d.inversion_nondim = cfg->inversion * d.z_scale;
+  d.dx = 2 * M_PI / nx;
+  d.dy = 2 * M_PI / ny;
+  d.dz = (cfg->l_z / d.z_scale) / (nz - 1);
+  d.L = cfg->FGR * std::pow(d.dx * d.dy * d.dz, 1.0 / 3.0);
+  d.dt_nondim = cfg->dt / d.z_scale;
+  d.TFR = cfg->FGR == 1 ? 2.0 : std::sqrt(2.0);
+  d.optDealias = cfg->FGR == 1 ? 1 : 0;
+  d.z_damping_nondim = cfg->z_damping / d.z_scale;
+  d.RelaxTime_nondim = cfg->RelaxTime / d.z_scale;
+  d.mx = P->mx; d.my = P->my; d.nx_rfft = nx / 2 + 1; d.ny_rfft = ny / 2 + 1;
+  // cut-offs: Python round() (half to even) of Filtering.py:73-74,111-112,149-150
+  auto pyround = [](double v) { return (int)std::nearbyint(v); };
+  d.cut0_x = pyround(nx * (1.0 / (2.0 * cfg->FGR)));
+  d.cut0_y = pyround(ny * (1.0 / (2.0 * cfg->FGR)));
+  d.cut1_x = pyround(nx / (2 * cfg->FGR * d.TFR));
+  d.cut1_y = pyround(ny / (2 * cfg->FGR * d.TFR));
+  d.cut2_x = pyround(nx / (2 * cfg->FGR * d.TFR * d.TFR));
+  d.cut2_y = pyround(ny / (2 * cfg->FGR * d.TFR * d.TFR));
+  d.precision = precision;
+  if (!factor(nx, P->fx) || !factor(ny, P->fy) || !factor(P->mx, P->fmx) || !factor(P->my, P->fmy)) {
+    free_plan(P);
+    return set_error(ALFA_ERADIX, "FFT lengths %d, %d, %d, %d must factor into 2, 3, 5", nx, ny,
+                     P->mx, P->my);
+  }
+  const int maxn = d.optDealias ? std::max(P->mx, P->my) : std::max(nx, ny);
+  if (maxn > FFT_MAXN) {
+    free_plan(P);
+    return set_error(ALFA_EINVAL, "transform length %d exceeds %d", maxn, FFT_MAXN);
+  }
+  hipError_t e = hipSuccess;
+  auto tw = [&](int n) -> void* {
+    return precision == ALFA_FP64 ? make_twiddles<double>(n, P->allocs, e)
+                                  : make_twiddles<float>(n, P->allocs, e);
+  };
+  P->tw_x = tw(nx);
+  if (e == hipSuccess) P->tw_y = tw(ny);
+  if (e == hipSuccess) P->tw_mx = tw(P->mx);
+  if (e == hipSuccess) P->tw_my = tw(P->my);
+  // Thomas factors (ThomasPressureInit, NSE_PressureTerms_Thomas.py:151-214)
+  const int N = nz + 1, nyc = P->nyc;
+  const size_t ncolc = (size_t)nx * nyc;
+  std::vector<double> hm(ncolc * N, 0.0), hb(ncolc * N, 0.0);
+  {
+    const double idz2 = 1.0 / (d.dz * d.dz);
+    std::vector<double> a(N, idz2), c(N, idz2), b(N);
+    a[0] = 0; a[nz] = -1; c[0] = 1; c[nz] = 0;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < nyc; ++j) {
+        const double kr = (i < nx / 2) ? i : i - nx, kc = j;
+        const double bb = -(kr * kr + kc * kc + 2.0 / (d.dz * d.dz));
+        for (int k = 0; k < N; ++k) b[k] = bb;
+        if (i == 0 && j == 0)
+          for (int k = 1; k < nz; ++k) b[k] = -(1.0 + 2.0 / (d.dz * d.dz));
+        b[0] = -1.0; b[nz] = 1.0;
+        double* pm = &hm[((size_t)i * nyc + j) * N];
+        double* pb = &hb[((size_t)i * nyc + j) * N];
+        pb[0] = b[0];
+        for (int k = 1; k < N; ++k) {
+          pm[k] = a[k] / pb[k - 1];
+          pb[k] = b[k] - pm[k] * c[k - 1];
+        }
+      }
+  }
+  auto upload = [&](const std::vector<double>& h) -> double* {
+    void* p = nullptr;
+    if (e != hipSuccess) return nullptr;
+    e = hipMalloc(&p, h.size() * sizeof(double));
+    if (e != hipSuccess) return nullptr;
+    P->allocs.push_back(p);
+    e = hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+    return (double*)p;
+  };
+  P->th_m = upload(hm);
+  P->th_b = upload(hb);
+  // profiles: Ug, Vg (Initialization.py:163-170), Rayleigh full / staggered (:395-460)
+  {
+    std::vector<double> pr(4 * (size_t)nz, 0.0);
+    const double inv = 1.0 / d.RelaxTime_nondim;
+    const double ztop = cfg->l_z / d.z_scale;
+    const double thick = ztop - d.z_damping_nondim;
+    for (int k = 0; k < nz; ++k) {
+      pr[k] = cfg->Ug2;
+      pr[nz + k] = cfg->Vg2;
+      const double zf = k * d.dz, zs = (k + 0.5) * d.dz;
+      pr[2 * nz + k] = (zf >= d.z_damping_nondim && zf <= ztop)
+                           ? 0.5 * inv * (1.0 - std::cos(M_PI * (zf - d.z_damping_nondim) / thick))
+                           : 0.0;
+      pr[3 * nz + k] = (zs >= d.z_damping_nondim && zs <= ztop)
+                           ? 0.5 * inv * (1.0 - std::cos(M_PI * (zs - d.z_damping_nondim) / thick))
+                           : 0.0;
+    }
+    P->prof = upload(pr);
+    std::vector<double> z8(8, 0.0);
+    P->scal = upload(z8);
+  }
+  if (e != hipSuccess) {
+    int r = set_error(ALFA_EHIP, "plan setup: %s", hipGetErrorString(e));
+    free_plan(P);
+    return r;
+  }
+  int r = precision == ALFA_FP64 ? allocate_workspace<double>(P) : allocate_workspace<float>(P);
+  if (r != ALFA_OK) {
+    free_plan(P);
+    return r;
+  }
+  *out = P;
+  return ALFA_OK;
+}
+
+extern "C" int alfa_plan_destroy(alfa_plan* P) {
+  if (P) (void)hipDeviceSynchronize();
+  free_plan(P);
+  return ALFA_OK;
+}
+
+extern "C" int alfa_plan_derived(const alfa_plan* P, alfa_derived* out) {
+  if (!P || !out) return set_error(ALFA_EINVAL, "null argument");
+  *out = P->d;
+  return ALFA_OK;
+}
+
+#define DISPATCH(P, where, ...)                                               \
+  do {                                                                        \
+    if (!(P)) return set_error(ALFA_EINVAL, "%s: null plan", where);         \
+    if ((P)->prec == ALFA_FP64) { using T = double; __VA_ARGS__; }            \
+    else { using T = float; __VA_ARGS__; }                                    \
+    return check_hip(P, where);                                               \
+  } while (0)
+
+#define STREAM(s) ((hipStream_t)(s))
+
+extern "C" int alfa_state_init(alfa_plan* P, const alfa_state* st, alfa_stream stream) {
+  if (!P || !st) return set_error(ALFA_EINVAL, "alfa_state_init: null argument");
+  hipStream_t s = STREAM(stream);
+  const size_t F = (size_t)P->nx * P->ny * P->nz, pl = (size_t)P->nx * P->ny;
+  DISPATCH(P, "alfa_state_init", {
+    void* zero3[4] = {st->RHS_u, st->RHS_v, st->RHS_w, st->RHS_TH};
+    for (void* z : zero3) k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)z, T(0));
+    k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)st->Cs2_3D, (T)P->cfg.Cs2);
+    k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)st->Cs2PrRatio_3D, (T)P->cfg.Cs2PrRatio);
+    void* zero2[4] = {st->psi_m, st->psi_m0, st->psi_h, st->psi_h0};
+    for (void* z : zero2) k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)z, T(0));
+    k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)st->fi_m, T(1));
+    k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)st->fi_h, T(1));
+    launch_ok(P);
+  });
+}
+
+extern "C" int alfa_step(alfa_plan* P, const alfa_state* st, int iteration, double sfc_val,
+                         alfa_stream stream) {
+  if (!st) return set_error(ALFA_EINVAL, "alfa_step: null state");
+  DISPATCH(P, "alfa_step", step_t<T>(P, st, iteration, sfc_val, STREAM(stream)));
+}
+
+extern "C" int alfa_fft(alfa_plan* P, const void* F, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_fft", ops::fft<T>(P, STREAM(s), F, out));
+}
+extern "C" int alfa_fft_pad(alfa_plan* P, const void* F, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_fft_pad", ops::fft_pad<T>(P, STREAM(s), F, out));
+}
+extern "C" int alfa_filtering_explicit(alfa_plan* P, const void* Ffft, void* F, void* Fnew,
+                                       alfa_stream s) {
+  DISPATCH(P, "alfa_filtering_explicit", ops::filtering_explicit<T>(P, STREAM(s), Ffft, F, Fnew));
+}
+extern "C" int alfa_filtering_level1(alfa_plan* P, const void* Ffft, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_filtering_level1", ops::filtering_level<T>(P, STREAM(s), Ffft, out, 1));
+}
+extern "C" int alfa_filtering_level2(alfa_plan* P, const void* Ffft, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_filtering_level2", ops::filtering_level<T>(P, STREAM(s), Ffft, out, 2));
+}
+extern "C" int alfa_dealias1(alfa_plan* P, const void* Ffft, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_dealias1", ops::dealias1_op<T>(P, STREAM(s), Ffft, out));
+}
+extern "C" int alfa_dealias2(alfa_plan* P, const void* Fpfft, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_dealias2", ops::dealias2_op<T>(P, STREAM(s), Fpfft, out));
+}
+extern "C" int alfa_derivxy(alfa_plan* P, const void* Ffft, int axis, void* out, alfa_stream s) {
+  if (axis != 0 && axis != 1) return set_error(ALFA_EINVAL, "alfa_derivxy: axis must be 0 or 1");
+  DISPATCH(P, "alfa_derivxy", ops::derivxy<T>(P, STREAM(s), Ffft, axis, out));
+}
+extern "C" int alfa_derivz_m(alfa_plan* P, const void* u, const void* v, const void* w,
+                             const void* ustar, const void* M, const void* fim, void* dudz,
+                             void* dvdz, void* dwdz, alfa_stream s) {
+  DISPATCH(P, "alfa_derivz_m", {
+    const size_t F = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{F, P->nz};
+    PW_LAUNCH(P, STREAM(s), F, k_derivz<T>, dd, (const T*)u, (const T*)v, (const T*)w,
+              (const T*)nullptr, (const T*)ustar, (const T*)M, (const T*)fim, (const T*)nullptr,
+              (const T*)nullptr, (T*)dudz, (T*)dvdz, (T*)dwdz, (T*)nullptr, (T)P->cfg.Ugal,
+              (T)(1.0 / P->d.dz), (T)(0.4 * 0.5 * P->d.dz), (T)(0.4 * 0.5 * P->d.dz));
+  });
+}
+extern "C" int alfa_derivz_th(alfa_plan* P, const void* TH, const void* ustar, const void* qz,
+                              const void* fih, void* dTHdz, alfa_stream s) {
+  DISPATCH(P, "alfa_derivz_th", {
+    const size_t F = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{F, P->nz};
+    PW_LAUNCH(P, STREAM(s), F, k_derivz<T>, dd, (const T*)nullptr, (const T*)nullptr,
+              (const T*)nullptr, (const T*)TH, (const T*)ustar, (const T*)nullptr,
+              (const T*)nullptr, (const T*)fih, (const T*)qz, (T*)nullptr, (T*)nullptr,
+              (T*)nullptr, (T*)dTHdz, (T)P->cfg.Ugal, (T)(1.0 / P->d.dz), (T)(0.4 * 0.5 * P->d.dz),
+              (T)(0.4 * 0.5 * P->d.dz));
+  });
+}
+extern "C" int alfa_derivz_generic_uvp(alfa_plan* P, const void* F, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_derivz_generic_uvp", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    PW_LAUNCH(P, STREAM(s), n, k_derivz_generic<T>, dd, (const T*)F, (T*)out, (T)(1.0 / P->d.dz), 0);
+  });
+}
+extern "C" int alfa_derivz_generic_w(alfa_plan* P, const void* F, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_derivz_generic_w", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    PW_LAUNCH(P, STREAM(s), n, k_derivz_generic<T>, dd, (const T*)F, (T*)out, (T)(1.0 / P->d.dz), 1);
+  });
+}
+extern "C" int alfa_planar_mean(alfa_plan* P, const void* F, double* mean, alfa_stream s) {
+  DISPATCH(P, "alfa_planar_mean", ops::planar_mean<T>(P, STREAM(s), F, mean));
+}
+extern "C" int alfa_stag_grid_avg(alfa_plan* P, const void* F, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_stag_grid_avg", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    PW_LAUNCH(P, STREAM(s), n, k_stag_avg<T>, dd, (const T*)F, (T*)out);
+  });
+}
+extern "C" int alfa_imfilter(alfa_plan* P, const void* F, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_imfilter", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    PW_LAUNCH(P, STREAM(s), n, k_imfilter<T>, dd, P->nx, P->ny, (const T*)F, (T*)out);
+  });
+}
+extern "C" int alfa_compute_beta(alfa_plan* P, const double* coeffs, int n, double* beta,
+                                 alfa_stream s) {
+  if (!P) return set_error(ALFA_EINVAL, "alfa_compute_beta: null plan");
+  if (n <= 0) return set_error(ALFA_EINVAL, "alfa_compute_beta: n must be positive");
+  k_beta<<<(n + 7) / 8, 64, 0, STREAM(s)>>>(coeffs, n, beta);
+  launch_ok(P);
+  return check_hip(P, "alfa_compute_beta");
+}
+extern "C" int alfa_advection(alfa_plan* P, const void* u, const void* v, const void* w,
+                              const void* dudy, const void* dudz, const void* dvdx,
+                              const void* dvdz, const void* dwdx, const void* dwdy, void* Cx,
+                              void* Cy, void* Cz, alfa_stream s) {
+  DISPATCH(P, "alfa_advection",
+           ops::advection<T>(P, STREAM(s), u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, Cx, Cy, Cz));
+}
+extern "C" int alfa_scalar_advection(alfa_plan* P, const void* u, const void* v, const void* w,
+                                     const void* sx, const void* sy, const void* sz, void* out,
+                                     alfa_stream s) {
+  DISPATCH(P, "alfa_scalar_advection",
+           ops::scalar_advection<T>(P, STREAM(s), u, v, w, sx, sy, sz, out));
+}
+extern "C" int alfa_buoyancy(alfa_plan* P, const void* TH, void* out, alfa_stream s) {
+  DISPATCH(P, "alfa_buoyancy", ops::buoyancy<T>(P, STREAM(s), TH, out));
+}
+extern "C" int alfa_div_stress(alfa_plan* P, const void* txx, const void* tyy, const void* tzz,
+                               const void* txy, const void* txz, const void* tyz, void* dtx,
+                               void* dty, void* dtz, alfa_stream s) {
+  DISPATCH(P, "alfa_div_stress", {
+    Ctx<T> c(P, STREAM(s));
+    stage_div_stress<T>(c, (T*)txx, (T*)tyy, (T*)tzz, (T*)txy, (T*)txz, (T*)tyz, (T*)dtx,
+                        (T*)dty, (T*)dtz);
+  });
+}
+extern "C" int alfa_div_flux(alfa_plan* P, const void* qx, const void* qy, const void* qz,
+                             void* divq, alfa_stream s) {
+  DISPATCH(P, "alfa_div_flux", {
+    Ctx<T> c(P, STREAM(s));
+    stage_div_flux<T>(c, (T*)qx, (T*)qy, (T*)qz, (T*)divq);
+  });
+}
+extern "C" int alfa_rhs_momentum(alfa_plan* P, const void* u, const void* v, const void* w,
+                                 const void* Cx, const void* Cy, const void* Cz, const void* b,
+                                 const void* dtx, const void* dty, const void* dtz, void* Ru,
+                                 void* Rv, void* Rw, alfa_stream s) {
+  DISPATCH(P, "alfa_rhs_momentum",
+           ops::rhs_momentum<T>(P, STREAM(s), u, v, w, Cx, Cy, Cz, b, dtx, dty, dtz, Ru, Rv, Rw));
+}
+extern "C" int alfa_rhs_scalar(alfa_plan* P, const void* TH, const void* adv, const void* divq,
+                               void* RT, alfa_stream s) {
+  DISPATCH(P, "alfa_rhs_scalar", ops::rhs_scalar<T>(P, STREAM(s), TH, adv, divq, RT));
+}
+extern "C" int alfa_pressure_rc(alfa_plan* P, const void* u, const void* v, const void* w,
+                                const void* Ru, const void* Rv, const void* Rw, const void* Rpu,
+                                const void* Rpv, const void* Rpw, const void* divtz, void* RCr,
+                                void* RCi, void* fRzr, alfa_stream s) {
+  const void* R[3] = {Ru, Rv, Rw};
+  const void* Rp[3] = {Rpu, Rpv, Rpw};
+  const void* U[3] = {u, v, w};
+  DISPATCH(P, "alfa_pressure_rc", ops::pressure_rc<T>(P, STREAM(s), R, Rp, U, divtz, RCr, RCi, fRzr));
+}
+extern "C" int alfa_thomas_pressure_solve(alfa_plan* P, const void* RCr, const void* RCi,
+                                          const void* fRzr, void* p, void* dpdx, void* dpdy,
+                                          void* dpdz, alfa_stream s) {
+  DISPATCH(P, "alfa_thomas_pressure_solve",
+           ops::thomas_solve<T>(P, STREAM(s), RCr, RCi, fRzr, p, dpdx, dpdy, dpdz));
+}
+extern "C" int alfa_ab2_uvw(alfa_plan* P, void* u, void* v, void* w, const void* Ru,
+                            const void* Rup, const void* Rv, const void* Rvp, const void* Rw,
+                            const void* Rwp, alfa_stream s) {
+  DISPATCH(P, "alfa_ab2_uvw", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    const T dt = (T)P->d.dt_nondim;
+    PW_LAUNCH(P, STREAM(s), n, k_ab2_plain<T>, dd, (T*)u, (const T*)Ru, (const T*)Rup, dt, 0, T(0));
+    PW_LAUNCH(P, STREAM(s), n, k_ab2_plain<T>, dd, (T*)v, (const T*)Rv, (const T*)Rvp, dt, 0, T(0));
+    PW_LAUNCH(P, STREAM(s), n, k_ab2_plain<T>, dd, (T*)w, (const T*)Rw, (const T*)Rwp, dt, 1, T(0));
+  });
+}
+extern "C" int alfa_ab2_th(alfa_plan* P, void* TH, const void* R, const void* Rp, alfa_stream s) {
+  DISPATCH(P, "alfa_ab2_th", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    PW_LAUNCH(P, STREAM(s), n, k_ab2_plain<T>, dd, (T*)TH, (const T*)R, (const T*)Rp,
+              (T)P->d.dt_nondim, 2, (T)(P->d.inversion_nondim * P->d.dz));
+  });
+}
+extern "C" int alfa_strains_uvp(alfa_plan* P, const void* const g[9], void* out[7], alfa_stream s) {
+  DISPATCH(P, "alfa_strains_uvp", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    (void)hipMemcpyAsync(out[0], g[0], n * sizeof(T), hipMemcpyDeviceToDevice, STREAM(s));
+    (void)hipMemcpyAsync(out[1], g[4], n * sizeof(T), hipMemcpyDeviceToDevice, STREAM(s));
+    (void)hipMemcpyAsync(out[2], g[8], n * sizeof(T), hipMemcpyDeviceToDevice, STREAM(s));
+    PW_LAUNCH(P, STREAM(s), n, k_strains_uvp<T>, dd, (const T*)g[0], (const T*)g[1],
+              (const T*)g[2], (const T*)g[3], (const T*)g[4], (const T*)g[5], (const T*)g[6],
+              (const T*)g[7], (const T*)g[8], (T*)out[3], (T*)out[4], (T*)out[5], (T*)out[6]);
+  });
+}
+extern "C" int alfa_strains_w(alfa_plan* P, const void* const g[9], void* out[3], alfa_stream s) {
+  DISPATCH(P, "alfa_strains_w", {
+    const size_t n = (size_t)P->nx * P->ny * P->nz;
+    PwDims dd{n, P->nz};
+    PW_LAUNCH(P, STREAM(s), n, k_strains_w<T>, dd, (const T*)g[0], (const T*)g[1], (const T*)g[2],
+              (const T*)g[3], (const T*)g[4], (const T*)g[5], (const T*)g[6], (const T*)g[7],
+              (const T*)g[8], (T*)out[0], (T*)out[1], (T*)out[2], (T*)nullptr, (T*)nullptr,
+              (T*)nullptr, (T*)nullptr);
+  });
+}
+extern "C" int alfa_lasdd_sm(alfa_plan* P, const void* u, const void* v, const void* w,
+                             const void* const str[7], void* Cs2, double* a1, double* a2,
+                             double* beta, alfa_stream s) {
+  if (P && P->cfg.optSgs != 1 && P->cfg.optSgs != 3)
+    return set_error(ALFA_EINVAL, "alfa_lasdd_sm: plan optSgs must be 1 or 3");
+  DISPATCH(P, "alfa_lasdd_sm", ops::lasdd<T>(P, STREAM(s), u, v, w, str, Cs2, a1, a2, beta));
+}
+extern "C" int alfa_wall(alfa_plan* P, const void* u, const void* v, const void* M,
+                         const void* pm, const void* pm0, void* txz, void* tyz, alfa_stream s) {
+  DISPATCH(P, "alfa_wall", {
+    const int ncol = P->nx * P->ny;
+    const double logm = std::log(0.5 * P->d.dz * P->d.z_scale / P->cfg.z0m);
+    k_wall_plane<T><<<pw_blocks(ncol), 256, 0, STREAM(s)>>>(ncol, P->nz, (const T*)u, (const T*)v,
+                                                            (const T*)M, (const T*)pm,
+                                                            (const T*)pm0, logm, 0.4, P->cfg.Ugal,
+                                                            (T*)txz, (T*)tyz);
+    launch_ok(P);
+  });
+}
+extern "C" int alfa_surface_flux(alfa_plan* P, const void* u, const void* v, const void* TH,
+                                 double sfc_val, void* pm, void* pm0, void* ph, void* ph0,
+                                 void* fm, void* fh, void* M, void* ustar, void* qz, void* invOB,
+                                 double* qavg, alfa_stream s) {
+  DISPATCH(P, "alfa_surface_flux",
+           ops::surface<T>(P, STREAM(s), u, v, TH, sfc_val, pm, pm0, ph, ph0, fm, fh, M, ustar, qz,
+                           invOB, qavg));
+}

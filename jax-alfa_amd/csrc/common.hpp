@@ -1,0 +1,44 @@
+// Common device/host helpers for libalfa (MI355X / gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+template <class T> struct cvec;
+template <> struct cvec<float> { using type = float2; };
+template <> struct cvec<double> { using type = double2; };
+template <class T> using c2 = typename cvec<T>::type;
+
+template <class T> __host__ __device__ __forceinline__ c2<T> mkc(T x, T y) {
+  c2<T> r; r.x = x; r.y = y; return r;
+}
+template <class T> __device__ __forceinline__ c2<T> cadd(c2<T> a, c2<T> b) { return mkc<T>(a.x + b.x, a.y + b.y); }
+template <class T> __device__ __forceinline__ c2<T> csub(c2<T> a, c2<T> b) { return mkc<T>(a.x - b.x, a.y - b.y); }
+template <class T> __device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
+  return mkc<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class T> __device__ __forceinline__ c2<T> cscale(c2<T> a, T s) { return mkc<T>(a.x * s, a.y * s); }
+// multiply by i*sign (sign = +1 or -1)
+template <class T> __device__ __forceinline__ c2<T> cmul_i(c2<T> a, int sign) {
+  return sign > 0 ? mkc<T>(-a.y, a.x) : mkc<T>(a.y, -a.x);
+}
+
+// Block geometry shared by the FFT kernels: 256 threads; a block holds LPB
+// complex lines (= 128 bytes of z per line element), so one row of a tile is
+// one 128-byte HBM segment.
+constexpr int FFT_BS = 256;
+constexpr int FFT_MAXN = 384;      // longest transform: 1.5 * 256
+constexpr int FFT_MAXRAD = 8;
+template <class T> struct Lines { static constexpr int L = 128 / (int)sizeof(c2<T>); };  // 16 f32, 8 f64
+
+struct Fft1D {
+  int n;
+  int nrad;
+  int rad[FFT_MAXRAD];
+};
+
+// Maximum number of distinct fields one batched launch handles.
+constexpr int MAXB = 24;
+
+__host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }

@@ -1,0 +1,97 @@
+// Fused R2C loaders: the product fields of the dynamic procedure are formed
+// while the y-transform loads its tile, so they never exist in HBM.
+#pragma once
+#include "common.hpp"
+
+// The 21 momentum fields of DynamicSGS_LASDD_SM.py:184-233:
+//   0..2  u_, v_, w_ (w_ = w averaged to uvp nodes, :187-190)
+//   3..8  uu, vv, ww, uv, uw, vw
+//   9..14 S11, S22, S33, S12, S13, S23
+//   15..20 |S| S11 .. |S| S23
+// In mixed precision the velocities are shifted by a per-level constant
+// (sh*, an estimate of the plane mean) before the products are formed; the
+// Leonard terms L_ij = (u_i u_j)^ - u_i^ u_j^ are exactly invariant under such
+// a shift because the cut-off filters keep the (0,0) mode, and the shift
+// removes the cancellation between the two terms in single precision.
+template <class T> struct LoadLasdd {
+  const T* u; const T* v; const T* w;
+  const T* S[6]; const T* Sm;
+  const double* shu; const double* shv;   // may be NULL
+  int ny, nz;
+  __device__ __forceinline__ T wbar(size_t i, int z) const {
+    if (z == 0) return T(0.5) * w[i + 1];
+    if (z == nz - 1) return w[i - 1];
+    return T(0.5) * (w[i] + w[i + 1]);
+  }
+  __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    if (b >= 15) return Sm[i] * S[b - 15][i];
+    if (b >= 9) return S[b - 9][i];
+    const T uu = shu ? (T)((double)u[i] - shu[z]) : u[i];
+    const T vv = shv ? (T)((double)v[i] - shv[z]) : v[i];
+    const T ww = wbar(i, z);
+    switch (b) {
+      case 0: return uu;
+      case 1: return vv;
+      case 2: return ww;
+      case 3: return uu * uu;
+      case 4: return vv * vv;
+      case 5: return ww * ww;
+      case 6: return uu * vv;
+      case 7: return uu * ww;
+      default: return vv * ww;
+    }
+  }
+};
+
+// The 10 scalar fields of DynamicSGS_ScalarLASDD_SM.py:182-221:
+//   0 TH_, 1..3 u_TH, v_TH, w_TH, 4..6 dTHdx, dTHdy, THz, 7..9 |S| x (4..6)
+// THz is dTHdz averaged to uvp nodes (:182-186).  Same shift as above for
+// u_, v_ and TH (the scalar Leonard flux is shift invariant too).
+template <class T> struct LoadScalar {
+  const T* u; const T* v; const T* w; const T* TH;
+  const T* dx; const T* dy; const T* dz; const T* Sm;
+  const double* shu; const double* shv; const double* sht;
+  int ny, nz;
+  __device__ __forceinline__ T thz(size_t i, int z) const {
+    if (z == 0 || z == nz - 1) return dz[i];
+    return T(0.5) * (dz[i] + dz[i + 1]);
+  }
+  __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    switch (b) {
+      case 4: return dx[i];
+      case 5: return dy[i];
+      case 6: return thz(i, z);
+      case 7: return Sm[i] * dx[i];
+      case 8: return Sm[i] * dy[i];
+      case 9: return Sm[i] * thz(i, z);
+      default: break;
+    }
+    const T th = sht ? (T)((double)TH[i] - sht[z]) : TH[i];
+    if (b == 0) return th;
+    if (b == 1) return (shu ? (T)((double)u[i] - shu[z]) : u[i]) * th;
+    if (b == 2) return (shv ? (T)((double)v[i] - shv[z]) : v[i]) * th;
+    T ww;
+    if (z == 0) ww = T(0.5) * w[i + 1];
+    else if (z == nz - 1) ww = w[i - 1];
+    else ww = T(0.5) * (w[i] + w[i + 1]);
+    return ww * th;
+  }
+};
+
+// Stress / flux divergence epilogue of the y-C2R: out = spectral part +
+// the z-derivative of one field (Derivz_Generic_w for txz, tyz, qz;
+// Derivz_Generic_uvp for tzz), NSE_SGSTerms.py:86-102, SCL_SGSTerms.py:82-87.
+template <class T> struct StoreDiv {
+  T* out[4]; const T* fz[4]; int wnode[4];
+  int ny, nz; T idz;
+  __device__ __forceinline__ void operator()(int b, int x, int y, int z, T v) const {
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    const T* f = fz[b];
+    T dzv;
+    if (wnode[b]) dzv = (z < nz - 1) ? (f[i + 1] - f[i]) * idz : T(0);
+    else dzv = (z > 0) ? (f[i] - f[i - 1]) * idz : T(0);
+    out[b][i] = v + dzv;
+  }
+};

@@ -1,0 +1,469 @@
+// Pointwise and z-stencil kernels of the time step.  Fields are (ncol, nz)
+// with z fastest (ncol = nx*ny, or mx*my on the 3/2-padded grid); planes are
+// (ncol,).  One thread per point, consecutive threads on consecutive z, so
+// every load and store is a coalesced 128-byte segment and z+-1 neighbours
+// come from the same cache lines.
+#pragma once
+#include "common.hpp"
+
+#define PW_LOOP(N)                                                             \
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (N);      \
+       i += (size_t)gridDim.x * blockDim.x)
+
+struct PwDims {
+  size_t n;      // ncol * nz
+  int nz;
+};
+
+// Derivz_M + Derivz_TH (src/operations/Derivatives.py:190-273).
+template <class T>
+__global__ void k_derivz(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                         const T* __restrict__ w, const T* __restrict__ TH,
+                         const T* __restrict__ ustar, const T* __restrict__ M,
+                         const T* __restrict__ fim, const T* __restrict__ fih,
+                         const T* __restrict__ qz, T* dudz, T* dvdz, T* dwdz, T* dTHdz,
+                         T Ugal, T idz, T half_dz_vonk, T vonk_half_dz) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const size_t c = i / nz;
+    if (dudz) {
+      T a, b;
+      if (z == 0) {
+        const T den = M[c] * half_dz_vonk;  // M * vonk * 0.5 * dz
+        a = fim[c] * ustar[c] * (u[i] + Ugal) / den;
+        b = fim[c] * ustar[c] * v[i] / den;
+      } else if (z < nz - 1) {
+        a = (u[i] - u[i - 1]) * idz;
+        b = (v[i] - v[i - 1]) * idz;
+      } else {
+        a = T(0); b = T(0);
+      }
+      dudz[i] = a;
+      dvdz[i] = b;
+      dwdz[i] = (z < nz - 1) ? (w[i + 1] - w[i]) * idz : T(0);
+    }
+    if (dTHdz) {
+      dTHdz[i] = (z == 0) ? fih[c] * (-qz[c] / ustar[c]) / vonk_half_dz
+                          : (TH[i] - TH[i - 1]) * idz;
+    }
+  }
+}
+
+// Derivz_Generic_uvp (Derivatives.py:310) / Derivz_Generic_w (:340).
+template <class T>
+__global__ void k_derivz_generic(PwDims d, const T* __restrict__ F, T* out, T idz, int wnode) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (wnode) out[i] = (z < nz - 1) ? (F[i + 1] - F[i]) * idz : T(0);
+    else out[i] = (z > 0) ? (F[i] - F[i - 1]) * idz : T(0);
+  }
+}
+
+// StagGridAvg (Utilities.py:64): out is (ncol, nz-1).
+template <class T>
+__global__ void k_stag_avg(PwDims d, const T* __restrict__ F, T* out) {
+  const int nz = d.nz;
+  const size_t n = d.n / nz * (nz - 1);
+  PW_LOOP(n) {
+    const size_t c = i / (nz - 1);
+    const int z = (int)(i % (nz - 1));
+    const T* f = F + c * nz + z;
+    out[i] = T(0.5) * (f[0] + f[1]);
+  }
+}
+
+// Advection_NoDealias (src/pde/NSE_AdvectionTerms.py:163-238); on the padded
+// grid it is the product stage of Advection_Dealias (:65-155).
+template <class T>
+__global__ void k_advection(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                            const T* __restrict__ w, const T* __restrict__ dudy,
+                            const T* __restrict__ dudz, const T* __restrict__ dvdx,
+                            const T* __restrict__ dvdz, const T* __restrict__ dwdx,
+                            const T* __restrict__ dwdy, T* Cx, T* Cy, T* Cz) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const T a1x = v[i] * (dudy[i] - dvdx[i]);
+    const T a1y = u[i] * (dvdx[i] - dudy[i]);
+    T cx, cy, cz;
+    if (z == 0) {
+      cx = a1x + T(0.5) * (w[i + 1] * (dudz[i + 1] - dwdx[i + 1]));
+      cy = a1y + T(0.5) * (w[i + 1] * (dvdz[i + 1] - dwdy[i + 1]));
+      cz = T(0);
+    } else if (z < nz - 1) {
+      const T a2x0 = w[i] * (dudz[i] - dwdx[i]), a2x1 = w[i + 1] * (dudz[i + 1] - dwdx[i + 1]);
+      const T a2y0 = w[i] * (dvdz[i] - dwdy[i]), a2y1 = w[i + 1] * (dvdz[i + 1] - dwdy[i + 1]);
+      cx = a1x + T(0.5) * (a2x0 + a2x1);
+      cy = a1y + T(0.5) * (a2y0 + a2y1);
+      const T uG = T(0.5) * (u[i - 1] + u[i]);
+      const T vG = T(0.5) * (v[i - 1] + v[i]);
+      cz = uG * (dwdx[i] - dudz[i]) + vG * (dwdy[i] - dvdz[i]);
+    } else {
+      cx = a1x + w[i] * (dudz[i] - dwdx[i]);
+      cy = a1y + w[i] * (dvdz[i] - dwdy[i]);
+      cz = T(0);
+    }
+    Cx[i] = cx; Cy[i] = cy; Cz[i] = cz;
+  }
+}
+
+// ScalarAdvection_NoDealias (src/pde/SCL_AdvectionTerms.py:114-150); product
+// stage of the dealiased variant (:57-106) on the padded grid.
+template <class T>
+__global__ void k_scalar_adv(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                             const T* __restrict__ w, const T* __restrict__ sx,
+                             const T* __restrict__ sy, const T* __restrict__ sz, T* out) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const T cc1 = u[i] * sx[i] + v[i] * sy[i];
+    T r;
+    if (z < nz - 1) r = cc1 + T(0.5) * (w[i] * sz[i] + w[i + 1] * sz[i + 1]);
+    else r = cc1 + w[i] * sz[i];
+    out[i] = r;
+  }
+}
+
+// BuoyancyOpt1/2 (src/pde/NSE_BuoyancyTerms.py:49-150), dry; bar = plane
+// mean of TH + T_0 (double).
+template <class T>
+__global__ void k_buoyancy(PwDims d, const T* __restrict__ TH, const double* __restrict__ bar,
+                           T* out, double T0, double g, int opt) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (z == 0) { out[i] = T(0); continue; }
+    const double t1 = (double)TH[i] + T0, t0 = (double)TH[i - 1] + T0;
+    double n1, n0;
+    if (opt == 0) { n1 = (t1 - bar[z]) / T0; n0 = (t0 - bar[z - 1]) / T0; }
+    else { n1 = (t1 - bar[z]) / bar[z]; n0 = (t0 - bar[z - 1]) / bar[z - 1]; }
+    out[i] = (T)(0.5 * g * (n1 + n0));
+  }
+}
+
+template <class T> __device__ __forceinline__ T smag(T a, T b, T c, T d, T e, T f) {
+  return sqrt(T(2) * (a * a + b * b + c * c + T(2) * d * d + T(2) * e * e + T(2) * f * f));
+}
+
+// StrainsUVPnodes_NoDealias (src/subgridscale/StrainRates.py:149-211): writes
+// S12, S13, S23 and |S| (S11 = dudx, S22 = dvdy, S33 = dwdz are aliases).
+template <class T>
+__global__ void k_strains_uvp(PwDims d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
+                              const T* __restrict__ dwdx, const T* __restrict__ dudy,
+                              const T* __restrict__ dvdy, const T* __restrict__ dwdy,
+                              const T* __restrict__ dudz, const T* __restrict__ dvdz,
+                              const T* __restrict__ dwdz, T* S12, T* S13, T* S23, T* S) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    T uz, vz, wx, wy;
+    if (z == 0) {
+      uz = dudz[i]; vz = dvdz[i]; wx = T(0.5) * dwdx[i + 1]; wy = T(0.5) * dwdy[i + 1];
+    } else if (z < nz - 1) {
+      uz = T(0.5) * (dudz[i] + dudz[i + 1]); vz = T(0.5) * (dvdz[i] + dvdz[i + 1]);
+      wx = T(0.5) * (dwdx[i] + dwdx[i + 1]); wy = T(0.5) * (dwdy[i] + dwdy[i + 1]);
+    } else {
+      uz = dudz[i]; vz = dvdz[i]; wx = dwdx[i]; wy = dwdy[i];
+    }
+    const T s12 = T(0.5) * (dudy[i] + dvdx[i]);
+    const T s13 = T(0.5) * (uz + wx);
+    const T s23 = T(0.5) * (vz + wy);
+    S12[i] = s12; S13[i] = s13; S23[i] = s23;
+    S[i] = smag<T>(dudx[i], dvdy[i], dwdz[i], s12, s13, s23);
+  }
+}
+
+// StrainsWnodes (src/subgridscale/StrainRates.py:219-368).  S13w, S23w and
+// |S_w| always; the diagonal/S12 w-node components only when S11w != NULL
+// (the dealiased branch transforms all six).
+template <class T>
+__global__ void k_strains_w(PwDims d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
+                            const T* __restrict__ dwdx, const T* __restrict__ dudy,
+                            const T* __restrict__ dvdy, const T* __restrict__ dwdy,
+                            const T* __restrict__ dudz, const T* __restrict__ dvdz,
+                            const T* __restrict__ dwdz, T* S13w, T* S23w, T* Sw,
+                            T* S11w, T* S22w, T* S33w, T* S12w) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    T ux, uy, vx, vy, wz, wx, wy;
+    if (z == 0) {
+      ux = dudx[i]; uy = dudy[i]; vx = dvdx[i]; vy = dvdy[i]; wz = dwdz[i];
+      wx = T(0.5) * (dwdx[i] + dwdx[i + 1]); wy = T(0.5) * (dwdy[i] + dwdy[i + 1]);
+    } else if (z < nz - 1) {
+      ux = T(0.5) * (dudx[i - 1] + dudx[i]); uy = T(0.5) * (dudy[i - 1] + dudy[i]);
+      vx = T(0.5) * (dvdx[i - 1] + dvdx[i]); vy = T(0.5) * (dvdy[i - 1] + dvdy[i]);
+      wz = T(0.5) * (dwdz[i - 1] + dwdz[i]);
+      wx = dwdx[i]; wy = dwdy[i];
+    } else {
+      ux = T(0); uy = T(0); vx = T(0); vy = T(0); wz = T(0);
+      wx = dwdx[i]; wy = dwdy[i];
+    }
+    const T s12 = T(0.5) * (uy + vx);
+    const T s13 = T(0.5) * (dudz[i] + wx);
+    const T s23 = T(0.5) * (dvdz[i] + wy);
+    S13w[i] = s13; S23w[i] = s23;
+    Sw[i] = smag<T>(ux, vy, wz, s12, s13, s23);
+    if (S11w) { S11w[i] = ux; S22w[i] = vy; S33w[i] = wz; S12w[i] = s12; }
+  }
+}
+
+// |S| from six components (the padded magnitudes of StrainRates.py:127-133).
+template <class T>
+__global__ void k_smag(PwDims d, const T* a, const T* b, const T* c, const T* e,
+                       const T* f, const T* g, T* out) {
+  PW_LOOP(d.n) out[i] = smag<T>(a[i], b[i], c[i], e[i], f[i], g[i]);
+}
+
+// StressesUVPnodes (src/subgridscale/SGSStresses_SM.py:98-182) before any
+// Dealias2: tij = -2 L^2 Cs2 |S| Sij, top level 0.
+template <class T>
+__global__ void k_stress_uvp(PwDims d, const T* __restrict__ S11, const T* __restrict__ S22,
+                             const T* __restrict__ S33, const T* __restrict__ S12,
+                             const T* __restrict__ S, const T* __restrict__ Cs2, T m2L2,
+                             T* txx, T* tyy, T* tzz, T* txy) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (z == nz - 1) { txx[i] = tyy[i] = tzz[i] = txy[i] = T(0); continue; }
+    const T pre = m2L2 * Cs2[i] * S[i];
+    txx[i] = pre * S11[i]; tyy[i] = pre * S22[i]; tzz[i] = pre * S33[i]; txy[i] = pre * S12[i];
+  }
+}
+
+// StressesWnodes (SGSStresses_SM.py:190-297) interior; level 0 gets the wall
+// stress of Wall() (:63-90) when `wall` is set, else 0 (padded branch).
+template <class T>
+__global__ void k_stress_w(PwDims d, const T* __restrict__ S13w, const T* __restrict__ S23w,
+                           const T* __restrict__ Sw, const T* __restrict__ Cs2, T m2L2,
+                           T* txz, T* tyz, int wall, const T* __restrict__ u,
+                           const T* __restrict__ v, const T* __restrict__ M,
+                           const T* __restrict__ psim, const T* __restrict__ psim0,
+                           double logz1z0, double vonk, double Ugal) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (z == nz - 1) { txz[i] = tyz[i] = T(0); continue; }
+    if (z == 0) {
+      if (wall) {
+        const size_t c = i / nz;
+        const T denom = (T)logz1z0 + psim[c] - psim0[c];
+        const T us = (T)vonk * M[c] / denom;
+        txz[i] = -(us * us) * (u[i] + (T)Ugal) / M[c];
+        tyz[i] = -(us * us) * v[i] / M[c];
+      } else {
+        txz[i] = tyz[i] = T(0);
+      }
+      continue;
+    }
+    const T pre = m2L2 * (T(0.5) * (Cs2[i - 1] + Cs2[i])) * Sw[i];
+    txz[i] = pre * S13w[i];
+    tyz[i] = pre * S23w[i];
+  }
+}
+
+// Wall stress written into level 0 of existing txz/tyz (dealiased branch,
+// SGSStresses_SM.py:240-241), and qz level 0 = surface flux
+// (ScalarSGSFluxes_SM.py:175).
+template <class T>
+__global__ void k_level0(int ncol, int nz, T* txz, T* tyz, const T* __restrict__ u,
+                         const T* __restrict__ v, const T* __restrict__ M,
+                         const T* __restrict__ psim, const T* __restrict__ psim0,
+                         double logz1z0, double vonk, double Ugal, T* qz,
+                         const T* __restrict__ qsfc) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncol; c += gridDim.x * blockDim.x) {
+    const size_t i = (size_t)c * nz;
+    if (txz) {
+      const T denom = (T)logz1z0 + psim[c] - psim0[c];
+      const T us = (T)vonk * M[c] / denom;
+      txz[i] = -(us * us) * (u[i] + (T)Ugal) / M[c];
+      tyz[i] = -(us * us) * v[i] / M[c];
+    }
+    if (qz) qz[i] = qsfc[c];
+  }
+}
+
+// ScalarFluxes (src/subgridscale/ScalarSGSFluxes_SM.py:55-222) before any
+// Dealias2; qz level 0 = qsfc when qsfc != NULL, else 0.
+template <class T>
+__global__ void k_scalar_flux(PwDims d, const T* __restrict__ dx, const T* __restrict__ dy,
+                              const T* __restrict__ dz, const T* __restrict__ S,
+                              const T* __restrict__ Sw, const T* __restrict__ CsPr, T m2L2,
+                              T* qx, T* qy, T* qz, const T* __restrict__ qsfc) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (z == nz - 1) { qx[i] = qy[i] = qz[i] = T(0); continue; }
+    const T pre = m2L2 * CsPr[i] * S[i];
+    qx[i] = pre * dx[i];
+    qy[i] = pre * dy[i];
+    if (z == 0) qz[i] = qsfc ? qsfc[i / nz] : T(0);
+    else qz[i] = m2L2 * (T(0.5) * (CsPr[i - 1] + CsPr[i])) * Sw[i] * dz[i];
+  }
+}
+
+// RHS_Momentum (src/pde/NSE_AllTerms.py:48-104) + RHS_Scalar
+// (src/pde/SCL_AllTerms.py:48-82), optAdvection = 0.  Profiles are double.
+template <class T>
+__global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                      const T* __restrict__ w, const T* __restrict__ TH,
+                      const T* __restrict__ Cx, const T* __restrict__ Cy,
+                      const T* __restrict__ Cz, const T* __restrict__ buoy,
+                      const T* __restrict__ dtx, const T* __restrict__ dty,
+                      const T* __restrict__ dtz, const T* __restrict__ thadv,
+                      const T* __restrict__ divq, const double* __restrict__ Ug,
+                      const double* __restrict__ Vg, const double* __restrict__ rf,
+                      const double* __restrict__ rs, const double* __restrict__ ubar,
+                      const double* __restrict__ vbar, const double* __restrict__ thbar,
+                      double f, double Ugal, int damping, T* Ru, T* Rv, T* Rw, T* RT) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (Ru) {
+      T ru = -Cx[i] - dtx[i] - (T)f * ((T)Vg[z] - v[i]);
+      T rv = -Cy[i] - dty[i] + (T)f * ((T)Ug[z] - (T)Ugal - u[i]);
+      T rw = -Cz[i] - dtz[i] + buoy[i];
+      if (damping) {
+        const double s = rs[z];
+        ru = (T)((double)ru - s * ((double)u[i] - ubar[z]));
+        rv = (T)((double)rv - s * ((double)v[i] - vbar[z]));
+        rw = (T)((double)rw - rf[z] * (double)w[i]);
+      }
+      Ru[i] = ru; Rv[i] = rv; Rw[i] = rw;
+    }
+    if (RT) {
+      T rt = -thadv[i] - divq[i];
+      if (damping) rt = (T)((double)rt - rs[z] * ((double)TH[i] - thbar[z]));
+      RT[i] = rt;
+    }
+  }
+}
+
+// AB2_uvw (NSE_TimeAdvancement.py:45-76) / AB2_TH (SCL_TimeAdvancement.py:
+// 46-71) alone, in place; the thread of level nz-2 writes the top level.
+template <class T>
+__global__ void k_ab2_plain(PwDims d, T* F, const T* __restrict__ R, const T* __restrict__ Rp,
+                            T dt, int kind, T th_top) {
+  // kind 0: u/v (top copies nz-2), 1: w (0 at bottom and top), 2: TH
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (kind == 1) {
+      F[i] = (z == 0 || z == nz - 1) ? T(0) : F[i] + dt * (T(1.5) * R[i] - T(0.5) * Rp[i]);
+      continue;
+    }
+    if (z == nz - 1) continue;
+    const T fn = F[i] + dt * (T(1.5) * R[i] - T(0.5) * Rp[i]);
+    F[i] = fn;
+    if (z == nz - 2) F[i + 1] = (kind == 2) ? fn + th_top : fn;
+  }
+}
+
+// 3x3 periodic box mean of every level (Utilities.py:87-115).
+template <class T>
+__global__ void k_imfilter(PwDims d, int nx, int ny, const T* __restrict__ F, T* out) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const size_t c = i / nz;
+    const int x = (int)(c / ny), y = (int)(c % ny);
+    const int xs[3] = {(x + nx - 1) % nx, x, (x + 1) % nx};
+    const int ys[3] = {(y + ny - 1) % ny, y, (y + 1) % ny};
+    const T inv9 = T(1.0 / 9.0);
+    T a = T(0);
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) a += F[((size_t)xs[p] * ny + ys[q]) * nz + z] * inv9;
+    out[i] = a;
+  }
+}
+
+// Wall stress planes (SGSStresses_SM.py:63-90).
+template <class T>
+__global__ void k_wall_plane(int ncol, int nz, const T* __restrict__ u, const T* __restrict__ v,
+                             const T* __restrict__ M, const T* __restrict__ psim,
+                             const T* __restrict__ psim0, double logz1z0, double vonk,
+                             double Ugal, T* txz, T* tyz) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncol; c += gridDim.x * blockDim.x) {
+    const size_t i = (size_t)c * nz;
+    const T denom = (T)logz1z0 + psim[c] - psim0[c];
+    const T us = (T)vonk * M[c] / denom;
+    txz[c] = -(us * us) * (u[i] + (T)Ugal) / M[c];
+    tyz[c] = -(us * us) * v[i] / M[c];
+  }
+}
+
+// Real part of a spectrum (fRz_real of NSE_PressureTerms.py:182).
+template <class T>
+__global__ void k_real_part(size_t n, const c2<T>* __restrict__ a, T* out) {
+  PW_LOOP(n) out[i] = a[i].x;
+}
+
+// RHS -= grad p (src/Main.py:498-500), then AB2_uvw and AB2_TH
+// (NSE_TimeAdvancement.py:45-76, SCL_TimeAdvancement.py:46-71) in place.
+// Rn* hold this step's RHS before the pressure correction; Rp* hold the
+// previous step's RHS on entry and this step's corrected RHS on exit (the
+// next step's RHS_previous).  On the first step the previous RHS is the
+// current one (Main.py:506-512).  Each element is read and written by the
+// same thread, except that the thread of level nz-2 also writes the top
+// level (u, v copy; TH + inversion*dz), which no other thread touches.
+// Per-block max |u|,|v|,|w| of the new fields go to cflpart[3*block + k].
+template <class T>
+__global__ void k_ab2(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ Rnu,
+                      const T* __restrict__ Rnv, const T* __restrict__ Rnw,
+                      const T* __restrict__ RnT, T* Rpu, T* Rpv, T* Rpw, T* RpT,
+                      const T* __restrict__ dpdx, const T* __restrict__ dpdy,
+                      const T* __restrict__ dpdz, T dt, T th_top, int first,
+                      double* cflpart) {
+  const int nz = d.nz;
+  float mu = 0.f, mv = 0.f, mw = 0.f;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const T ru = Rnu[i] - dpdx[i];
+    const T rv = Rnv[i] - dpdy[i];
+    const T rw = Rnw[i] - dpdz[i];
+    const T rt = RnT[i];
+    const T rpu = first ? ru : Rpu[i];
+    const T rpv = first ? rv : Rpv[i];
+    const T rpw = first ? rw : Rpw[i];
+    const T rpt = first ? rt : RpT[i];
+    Rpu[i] = ru; Rpv[i] = rv; Rpw[i] = rw; RpT[i] = rt;
+    if (z < nz - 1) {
+      const T un = u[i] + dt * (T(1.5) * ru - T(0.5) * rpu);
+      const T vn = v[i] + dt * (T(1.5) * rv - T(0.5) * rpv);
+      const T tn = TH[i] + dt * (T(1.5) * rt - T(0.5) * rpt);
+      u[i] = un; v[i] = vn; TH[i] = tn;
+      if (z == nz - 2) { u[i + 1] = un; v[i + 1] = vn; TH[i + 1] = tn + th_top; }
+      mu = fmaxf(mu, (float)fabs((double)un));
+      mv = fmaxf(mv, (float)fabs((double)vn));
+    }
+    if (z == 0 || z == nz - 1) {
+      w[i] = T(0);
+    } else {
+      const T wn = w[i] + dt * (T(1.5) * rw - T(0.5) * rpw);
+      w[i] = wn;
+      mw = fmaxf(mw, (float)fabs((double)wn));
+    }
+  }
+  __shared__ float sm[3][256];
+  sm[0][threadIdx.x] = mu; sm[1][threadIdx.x] = mv; sm[2][threadIdx.x] = mw;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmaxf(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) cflpart[3 * blockIdx.x + k] = sm[k][0];
+}
+
+template <class T>
+__global__ void k_copy(size_t n, const T* __restrict__ a, T* b) {
+  PW_LOOP(n) b[i] = a[i];
+}
+
+template <class T>
+__global__ void k_fill(size_t n, T* a, T val) {
+  PW_LOOP(n) a[i] = val;
+}

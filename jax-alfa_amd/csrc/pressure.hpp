@@ -1,0 +1,179 @@
+// Pressure Poisson solve (src/pde/NSE_PressureTerms.py:109-182 PressureRC,
+// src/pde/NSE_PressureTerms_Thomas.py:99-292 ThomasPressureSolve).
+//
+// One 64-lane wavefront owns one (kx, ky) column.  It forms the right-hand
+// side RC of the column on the fly from the transformed Rx, Ry, Rz and the
+// transformed divtz boundary planes, and solves the pre-factored tridiagonal
+// system (nz+1 unknowns) with both Thomas sweeps written as first-order
+// linear recurrences y_k = a_k y_{k-1} + b_k: each lane composes the affine
+// maps of a contiguous chunk of levels, a 6-step shuffle scan combines the
+// lanes, and each lane then replays its chunk.  All solve arithmetic is
+// double.  The column is contiguous in memory (z fastest), so the loads and
+// stores of a wavefront are fully coalesced.  Outputs are p_hat and the
+// spectrum of dp/dz (Derivz_Generic_uvp of p is linear in p_hat).
+#pragma once
+#include "common.hpp"
+
+template <class T> struct ThomasArgs {
+  const c2<T>* fRx; const c2<T>* fRy; const c2<T>* fRz;   // (nx, nyc, nz)
+  const c2<T>* fbt;                                        // (nx, nyc, 2)
+  const double* m; const double* b;                        // (ncol, nz+1)
+  c2<T>* phat; c2<T>* dpz;                                 // (nx, nyc, nz)
+  T* RCr; T* RCi;             // optional out (ncol, nz+1): RC before masking
+  const T* RCin_r; const T* RCin_i;   // optional in (ncol, nz+1): RC given
+  const T* fRzr_in;                   // with RCin: Re fRz (ncol, nz) for the zero mode
+  int nx, ny, nyc, nz;
+  double dz, idz, c_in;       // c_in = 1/dz^2
+};
+
+constexpr int TH_MAXN = 520;   // nz + 1 <= 520
+
+__device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
+__device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
+
+template <class T>
+__global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
+  __shared__ double dr[TH_MAXN], di[TH_MAXN], sb[TH_MAXN], smm[TH_MAXN];
+  const int col = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int nz = a.nz, N = nz + 1;
+  const int i = col / a.nyc, j = col % a.nyc;
+  const double kr = (i < a.nx / 2) ? (double)i : (double)(i - a.nx);
+  const double kc = (double)j;
+  const bool special = (i == 0 && j == 0) || (2 * i == a.nx) || (2 * j == a.ny);
+  const size_t cb = (size_t)col * nz;
+  // right-hand side of the column (NSE_PressureTerms.py:150-180)
+  for (int k = lane; k < N; k += 64) {
+    double rr, ri;
+    if (a.RCin_r) {
+      rr = (double)a.RCin_r[(size_t)col * N + k];
+      ri = (double)a.RCin_i[(size_t)col * N + k];
+    } else if (k == 0 || k == nz) {
+      const c2<T> f = a.fbt[(size_t)col * 2 + (k == 0 ? 0 : 1)];
+      rr = -a.dz * (double)f.x;
+      ri = -a.dz * (double)f.y;
+    } else {
+      const c2<T> X = a.fRx[cb + k - 1], Y = a.fRy[cb + k - 1];
+      const c2<T> Z1 = a.fRz[cb + k], Z0 = a.fRz[cb + k - 1];
+      double hr = -kr * (double)X.y;
+      hr -= kc * (double)Y.y;
+      double hi = kr * (double)X.x;
+      hi += kc * (double)Y.x;
+      rr = hr + ((double)Z1.x - (double)Z0.x) / a.dz;
+      ri = hi + ((double)Z1.y - (double)Z0.y) / a.dz;
+    }
+    if (a.RCr) { a.RCr[(size_t)col * N + k] = (T)rr; a.RCi[(size_t)col * N + k] = (T)ri; }
+    dr[k] = special ? 0.0 : rr;
+    di[k] = special ? 0.0 : ri;
+    smm[k] = a.m[(size_t)col * N + k];
+    sb[k] = a.b[(size_t)col * N + k];
+  }
+  __syncthreads();
+  if (special) {
+    if (col == 0) {
+      // zero mode by vertical integration (NSE_PressureTerms_Thomas.py:271-275)
+      for (int k = lane; k < nz; k += 64)
+        smm[k] = a.RCin_r ? (double)a.fRzr_in[cb + k] : (double)a.fRz[cb + k].x;
+      __syncthreads();
+      if (lane == 0) {
+        const double zf = a.RCin_r ? (double)a.RCin_r[0] : -a.dz * (double)a.fbt[0].x;
+        double S = 0.0, prev = zf;
+        a.phat[cb] = mkc<T>((T)zf, T(0));
+        a.dpz[cb] = mkc<T>(T(0), T(0));
+        for (int k = 1; k < nz; ++k) {
+          S += smm[k] * a.dz;
+          const double p = zf + S;
+          a.phat[cb + k] = mkc<T>((T)p, T(0));
+          a.dpz[cb + k] = mkc<T>((T)((p - prev) * a.idz), T(0));
+          prev = p;
+        }
+      }
+    } else {
+      for (int k = lane; k < nz; k += 64) {
+        a.phat[cb + k] = mkc<T>(T(0), T(0));
+        a.dpz[cb + k] = mkc<T>(T(0), T(0));
+      }
+    }
+    return;
+  }
+  const int C = (N + 63) / 64;
+  const int k0 = lane * C, k1 = imin(N, k0 + C);
+  // forward sweep: y_k = d_k - m_k y_{k-1}   (_thomas_solve :125-131)
+  double A = 1.0, Br = 0.0, Bi = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const double al = (k == 0) ? 0.0 : -smm[k];
+    A *= al;
+    Br = al * Br + dr[k];
+    Bi = al * Bi + di[k];
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    const double A2 = shfl_up_d(A, off), R2 = shfl_up_d(Br, off), I2 = shfl_up_d(Bi, off);
+    if (lane >= off) {
+      Br = A * R2 + Br;
+      Bi = A * I2 + Bi;
+      A = A * A2;
+    }
+  }
+  double yr = shfl_up_d(Br, 1), yi = shfl_up_d(Bi, 1);
+  if (lane == 0) { yr = 0.0; yi = 0.0; }
+  for (int k = k0; k < k1; ++k) {
+    const double al = (k == 0) ? 0.0 : -smm[k];
+    yr = al * yr + dr[k];
+    yi = al * yi + di[k];
+    dr[k] = yr;
+    di[k] = yi;
+  }
+  // back substitution: x_k = (y_k - c_k x_{k+1}) / b_k   (:135-142)
+  A = 1.0; Br = 0.0; Bi = 0.0;
+  for (int k = k1 - 1; k >= k0; --k) {
+    const double ck = (k == 0) ? 1.0 : (k == nz ? 0.0 : a.c_in);
+    const double al = (k == N - 1) ? 0.0 : -ck / sb[k];
+    A *= al;
+    Br = al * Br + dr[k] / sb[k];
+    Bi = al * Bi + di[k] / sb[k];
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    const double A2 = shfl_down_d(A, off), R2 = shfl_down_d(Br, off), I2 = shfl_down_d(Bi, off);
+    if (lane + off < 64) {
+      Br = A * R2 + Br;
+      Bi = A * I2 + Bi;
+      A = A * A2;
+    }
+  }
+  double xr = shfl_down_d(Br, 1), xi = shfl_down_d(Bi, 1);
+  if (lane == 63) { xr = 0.0; xi = 0.0; }
+  for (int k = k1 - 1; k >= k0; --k) {
+    const double ck = (k == 0) ? 1.0 : (k == nz ? 0.0 : a.c_in);
+    const double al = (k == N - 1) ? 0.0 : -ck / sb[k];
+    xr = al * xr + dr[k] / sb[k];
+    xi = al * xi + di[k] / sb[k];
+    dr[k] = xr;
+    di[k] = xi;
+  }
+  __syncthreads();
+  // p_hat[k] = x[k+1]; dpz_hat[k] = (p_hat[k] - p_hat[k-1]) / dz, 0 at k = 0
+  for (int k = lane; k < nz; k += 64) {
+    a.phat[cb + k] = mkc<T>((T)dr[k + 1], (T)di[k + 1]);
+    if (k == 0) a.dpz[cb] = mkc<T>(T(0), T(0));
+    else a.dpz[cb + k] = mkc<T>((T)((dr[k + 1] - dr[k]) * a.idz), (T)((di[k + 1] - di[k]) * a.idz));
+  }
+}
+
+// R = RHS - RHS_prev/3 + 2u/(3 dt) (NSE_PressureTerms.py:140-142) as an
+// R2C loader: field b in {0,1,2} = x, y, z.
+template <class T> struct LoadPressureR {
+  const T* R[3]; const T* Rp[3]; const T* U[3];
+  int ny, nz; T third, c2dt;
+  __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    return R[b][i] - third * Rp[b][i] + c2dt * U[b][i];
+  }
+};
+
+// Two horizontal planes of one field (levels 0 and nz-1) as a 2-level field.
+template <class T> struct LoadPlanes2 {
+  const T* F; int ny, nz;
+  __device__ __forceinline__ T operator()(int, int x, int y, int z) const {
+    return F[((size_t)x * ny + y) * nz + (z == 0 ? 0 : nz - 1)];
+  }
+};

@@ -1,0 +1,403 @@
+// Plane reductions (PlanarMean, src/utilities/Utilities.py:41) and the
+// dynamic-SGS contraction / box-filter / beta kernels
+// (src/subgridscale/DynamicSGS_LASDD_SM.py, DynamicSGS_ScalarLASDD_SM.py).
+//
+// Reduction layout: a block of 256 threads is ZB z-lanes x (256/ZB) column
+// lanes; block (bz, bc) owns levels [bz*ZB, bz*ZB+ZB) and a fixed range of
+// columns.  Each thread accumulates in double over its columns (coalesced:
+// the z-lanes of one column are contiguous), the column lanes are combined
+// in LDS in a fixed order and each block writes one partial per (term,
+// level).  k_plane_final adds the partials in block order, so every plane
+// mean is bitwise reproducible run to run and independent of what else the
+// device is doing.
+#pragma once
+#include "common.hpp"
+#include "pointwise.hpp"
+
+constexpr int RED_BS = 256;
+constexpr int RED_MAXT = 10;
+
+struct RedGeom {
+  int ncol, nz, zb, lanes, cpb, nbz, nbc;
+};
+
+inline RedGeom red_geom(int ncol, int nz) {
+  RedGeom g;
+  g.ncol = ncol;
+  g.nz = nz;
+  g.zb = nz <= 32 ? 32 : 64;
+  g.lanes = RED_BS / g.zb;
+  g.nbz = (nz + g.zb - 1) / g.zb;
+  g.cpb = g.lanes * 64;
+  g.nbc = (ncol + g.cpb - 1) / g.cpb;
+  return g;
+}
+
+// f(c, z, acc) adds NT term values of point (c, z) into acc[0..NT).
+template <int NT, class F>
+__global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double* part) {
+  __shared__ double sm[NT][RED_BS];
+  const int zl = threadIdx.x % g.zb, cl = threadIdx.x / g.zb;
+  const int z = blockIdx.x * g.zb + zl;
+  double acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+  if (z < g.nz) {
+    const int c0 = blockIdx.y * g.cpb;
+    const int c1 = imin(c0 + g.cpb, g.ncol);
+    for (int c = c0 + cl; c < c1; c += g.lanes) f(c, z, acc);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) sm[t][threadIdx.x] = acc[t];
+  __syncthreads();
+  if (cl == 0 && z < g.nz) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double s = 0.0;
+      for (int l = 0; l < g.lanes; ++l) s += sm[t][l * g.zb + zl];
+      part[((size_t)blockIdx.y * NT + t) * g.nz + z] = s;
+    }
+  }
+}
+
+// out[t*nz + z] = sum_b part[b][t][z] * scale
+__global__ void k_plane_final(int nbc, int nt, int nz, const double* __restrict__ part,
+                              double* out, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt * nz) return;
+  const int t = i / nz, z = i % nz;
+  double s = 0.0;
+  for (int b = 0; b < nbc; ++b) s += part[((size_t)b * nt + t) * nz + z];
+  out[i] = s * scale;
+}
+
+// ---------------------------------------------------------------------------
+// plane-sum functors
+// ---------------------------------------------------------------------------
+template <class T> struct FSum1 {   // sum of F + add
+  const T* F; int nz; double add;
+  __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
+    acc[0] += (double)F[(size_t)c * nz + z] + add;
+  }
+};
+template <class T> struct FSum3 {   // u, v, TH + T0 (RHS damping and buoyancy means)
+  const T* a; const T* b; const T* c3; int nz; double add3;
+  __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    acc[0] += (double)a[i];
+    acc[1] += (double)b[i];
+    acc[2] += (double)c3[i] + add3;
+  }
+};
+
+template <class T> __device__ __forceinline__ double con6(const double* A, const double* B) {
+  return A[0] * B[0] + A[1] * B[1] + A[2] * B[2] + 2.0 * (A[3] * B[3] + A[4] * B[4] + A[5] * B[5]);
+}
+
+// Filtered momentum fields of one test-filter level: u_,v_,w_ hats (3), the
+// six u_i u_j products (uu,vv,ww,uv,uw,vw), S_ij (6) and |S| S_ij (6).
+template <class T> struct MomLevel {
+  const T* uh[3];
+  const T* ph[6];
+  const T* Sh[6];
+  const T* SSh[6];
+};
+
+template <class T>
+__device__ __forceinline__ void mom_load(const MomLevel<T>& m, size_t i, double* Lij,
+                                         double* Sij, double* SSij, double& Sm) {
+  const double u0 = m.uh[0][i], u1 = m.uh[1][i], u2 = m.uh[2][i];
+  Lij[0] = (double)m.ph[0][i] - u0 * u0;
+  Lij[1] = (double)m.ph[1][i] - u1 * u1;
+  Lij[2] = (double)m.ph[2][i] - u2 * u2;
+  Lij[3] = (double)m.ph[3][i] - u0 * u1;
+  Lij[4] = (double)m.ph[4][i] - u0 * u2;
+  Lij[5] = (double)m.ph[5][i] - u1 * u2;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) { Sij[n] = m.Sh[n][i]; SSij[n] = m.SSh[n][i]; }
+  Sm = sqrt(2.0 * (Sij[0] * Sij[0] + Sij[1] * Sij[1] + Sij[2] * Sij[2] + 2.0 * Sij[3] * Sij[3] +
+                   2.0 * Sij[4] * Sij[4] + 2.0 * Sij[5] * Sij[5]));
+}
+
+// The ten plane means a1..e2 of DynamicSGS_LASDD_SM.py:266-312 (sums; the
+// caller scales by 1/(nx*ny)).  Arithmetic in double.
+template <class T> struct FLasddSums {
+  MomLevel<T> l1, l2;
+  int nz;
+  double L, TFR;
+  __device__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    double L1[6], S1[6], SS1[6], Sh, L2[6], S2[6], SS2[6], Shd;
+    mom_load<T>(l1, i, L1, S1, SS1, Sh);
+    mom_load<T>(l2, i, L2, S2, SS2, Shd);
+    const double L2v = L * L, T2 = TFR * TFR, T4 = T2 * T2, T8 = T4 * T4;
+    const double k2 = 2.0 * L2v;
+    acc[0] += k2 * con6<T>(L1, SS1);
+    acc[1] += k2 * con6<T>(L2, SS2);
+    acc[2] += k2 * T2 * Sh * con6<T>(L1, S1);
+    acc[3] += k2 * T4 * Shd * con6<T>(L2, S2);
+    acc[4] += (k2 * k2) * con6<T>(SS1, SS1);
+    acc[5] += (k2 * k2) * con6<T>(SS2, SS2);
+    acc[6] += (4.0 * L2v * L2v) * T4 * (Sh * Sh) * con6<T>(S1, S1);
+    acc[7] += (4.0 * L2v * L2v) * T8 * (Shd * Shd) * con6<T>(S2, S2);
+    acc[8] += (8.0 * L2v * L2v) * T2 * Sh * con6<T>(S1, SS1);
+    acc[9] += (8.0 * L2v * L2v) * T4 * Shd * con6<T>(S2, SS2);
+  }
+};
+
+// M_ij, LM, MM (DynamicSGS_LASDD_SM.py:327-355) and |S_hat| (:235).
+template <class T>
+__global__ void k_lasdd_lm(PwDims d, MomLevel<T> l1, const double* __restrict__ beta, double L,
+                           double TFR, T* LM, T* MM, T* Shat) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    double Lij[6], Sij[6], SSij[6], Sm;
+    mom_load<T>(l1, i, Lij, Sij, SSij, Sm);
+    const double T1 = 2.0 * L * L, T2 = 2.0 * (TFR * L) * (TFR * L);
+    const double b = beta[z];
+    double M[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) M[n] = T1 * SSij[n] - T2 * b * Sm * Sij[n];
+    const double lm = (Lij[0] * M[0] + Lij[1] * M[1] + Lij[2] * M[2]) +
+                      2.0 * (Lij[3] * M[3] + Lij[4] * M[4] + Lij[5] * M[5]);
+    const double mm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] +
+                      2.0 * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
+    LM[i] = (T)lm;
+    MM[i] = (T)mm;
+    if (Shat) Shat[i] = (T)Sm;
+  }
+}
+
+// |S| of the level-2 filtered strains (DynamicSGS_LASDD_SM.py:240), kept for
+// the scalar model.
+template <class T>
+__global__ void k_smag_level(PwDims d, MomLevel<T> l, T* out) {
+  PW_LOOP(d.n) {
+    double s[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) s[n] = l.Sh[n][i];
+    out[i] = (T)sqrt(2.0 * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2.0 * s[3] * s[3] +
+                            2.0 * s[4] * s[4] + 2.0 * s[5] * s[5]));
+  }
+}
+
+// Filtered scalar fields of one level: TH_hat, (u,v,w)TH_hat, grad hats (3),
+// |S| grad hats (3); plus the velocity hats and |S_hat| of the same level.
+template <class T> struct ScaLevel {
+  const T* th;
+  const T* uth[3];
+  const T* dh[3];
+  const T* sdh[3];
+  const T* uh[3];
+  const T* Sm;
+};
+
+template <class T>
+__device__ __forceinline__ void sca_terms(const ScaLevel<T>& s, size_t i, double* t5) {
+  const double th = s.th[i], Sm = s.Sm[i];
+  double a = 0, b = 0, c = 0, d = 0, e = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double LT = (double)s.uth[k][i] - (double)s.uh[k][i] * th;
+    const double sd = s.sdh[k][i], dh = s.dh[k][i];
+    a += LT * sd;
+    b += LT * Sm * dh;
+    c += sd * sd;
+    d += sd * Sm * dh;
+    e += (Sm * dh) * (Sm * dh);
+  }
+  t5[0] = a; t5[1] = b; t5[2] = c; t5[3] = d; t5[4] = e;
+}
+
+// a2..e2, a4..e4 of DynamicSGS_ScalarLASDD_SM.py:241-302 (sums).
+template <class T> struct FScalarSums {
+  ScaLevel<T> l1, l2;
+  int nz;
+  double L, TFR;
+  __device__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    double p[5], q[5];
+    sca_terms<T>(l1, i, p);
+    sca_terms<T>(l2, i, q);
+    const double L2 = L * L, L4 = L2 * L2, T2 = TFR * TFR, T4 = T2 * T2, T8 = T4 * T4;
+    acc[0] += L2 * p[0];
+    acc[1] += (-L2 * T2) * p[1];
+    acc[2] += L4 * p[2];
+    acc[3] += (-2.0 * L4 * T2) * p[3];
+    acc[4] += (L4 * T4) * p[4];
+    acc[5] += L2 * q[0];
+    acc[6] += (-L2 * T4) * q[1];
+    acc[7] += L4 * q[2];
+    acc[8] += (-2.0 * L4 * T4) * q[3];
+    acc[9] += (L4 * T8) * q[4];
+  }
+};
+
+// T_up, T_dn (DynamicSGS_ScalarLASDD_SM.py:304-312).
+template <class T>
+__global__ void k_scalar_updn(PwDims d, ScaLevel<T> l1, const double* __restrict__ beta,
+                              double L, double TFR, T* Tup, T* Tdn) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    double t[5];
+    sca_terms<T>(l1, i, t);
+    const double b = beta[z];
+    const double L2 = L * L, L4 = L2 * L2, T2 = TFR * TFR, T4 = T2 * T2;
+    Tup[i] = (T)(L2 * t[0] + (-L2 * T2) * t[1] * b);
+    Tdn[i] = (T)(L4 * t[2] + (-2.0 * L4 * T2) * t[3] * b + (L4 * T4) * t[4] * b * b);
+  }
+}
+
+// Imfilter (Utilities.py:87-115) of both inputs at each level, the clipped
+// ratio (DynamicSGS_LASDD_SM.py:103-130, ScalarLASDD_SM.py:104-131), and the
+// plane sums of sqrt(C) and C for the profiles (LASDD_SM.py:359-360).
+template <class T> struct FBoxClip {
+  const T* num; const T* den; T* out; int nx, ny, nz;
+  __device__ void operator()(int c, int z, double* acc) const {
+    const int x = c / ny, y = c - x * ny;
+    const int xm = (x + nx - 1) % nx, xp = (x + 1) % nx;
+    const int ym = (y + ny - 1) % ny, yp = (y + 1) % ny;
+    const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp};
+    const T inv9 = T(1.0 / 9.0);
+    T a = T(0), b = T(0);
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const size_t j = ((size_t)xs[p] * ny + ys[q]) * nz + z;
+        a += num[j] * inv9;
+        b += den[j] * inv9;
+      }
+    T r = a / b;
+    const bool mask = (fabs((double)b) < 1e-10) || (r < T(0)) || (r > T(1));
+    if (mask) r = T(0);
+    out[(size_t)c * nz + z] = r;
+    acc[0] += sqrt((double)r);
+    acc[1] += (double)r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// beta: ComputeBeta1/2 (LASDD_SM.py:53-95, ScalarLASDD_SM.py:54-96) with the
+// Laguerre iteration of Roots (Utilities.py:123-241), in double complex.
+// coef layout (6, n): ff, ee, dd, cc, bb, aa.  One thread per (level, start).
+// ---------------------------------------------------------------------------
+struct dcx { double x, y; };
+__device__ __forceinline__ dcx dadd(dcx a, dcx b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ dcx dsub(dcx a, dcx b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ dcx dmul(dcx a, dcx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ double dabs(dcx a) { return hypot(a.x, a.y); }
+__device__ __forceinline__ dcx ddiv(dcx a, dcx b) {
+  // Smith's algorithm
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, den = b.x + b.y * r;
+    return {(a.x + a.y * r) / den, (a.y - a.x * r) / den};
+  } else {
+    const double r = b.x / b.y, den = b.x * r + b.y;
+    return {(a.x * r + a.y) / den, (a.y * r - a.x) / den};
+  }
+}
+__device__ __forceinline__ dcx dsqrt(dcx z) {
+  // principal branch; the sign of a zero imaginary part selects the side
+  if (z.x == 0.0 && z.y == 0.0) return {0.0, z.y};
+  const double r = hypot(z.x, z.y);
+  const double s = sqrt(0.5 * (r + fabs(z.x)));
+  if (z.x >= 0.0) return {s, z.y / (2.0 * s)};
+  return {fabs(z.y) / (2.0 * s), copysign(s, z.y)};
+}
+
+__device__ dcx laguerre(const double* p, double guess, bool* ok) {
+  const int n = 5;
+  const double eps = 1e-16, tol = 1e-6;
+  dcx x = {guess, 0.0};
+  bool conv = false;
+  for (int it = 0; it < 20 && !conv; ++it) {
+    dcx f = {0, 0}, df = {0, 0}, d2f = {0, 0};
+    for (int k = 0; k <= n; ++k) f = dadd(dmul(f, x), dcx{p[k], 0.0});
+    for (int k = 0; k < n; ++k) df = dadd(dmul(df, x), dcx{p[k] * (n - k), 0.0});
+    for (int k = 0; k < n - 1; ++k) d2f = dadd(dmul(d2f, x), dcx{p[k] * (n - k) * (n - 1 - k), 0.0});
+    const dcx fs = dabs(f) < eps ? dcx{eps, 0.0} : f;
+    const dcx G = ddiv(df, fs);
+    const dcx H = dsub(dmul(G, G), ddiv(d2f, fs));
+    const dcx disc = dmul(dcx{(double)(n - 1), 0.0}, dsub(dmul(dcx{(double)n, 0.0}, H), dmul(G, G)));
+    const dcx sq = dsqrt(disc);
+    const dcx d1 = dadd(G, sq), d2 = dsub(G, sq);
+    dcx den = dabs(d1) > dabs(d2) ? d1 : d2;
+    if (dabs(den) < eps) den = dcx{eps, 0.0};
+    const dcx xn = dsub(x, ddiv(dcx{(double)n, 0.0}, den));
+    conv = (dabs(dsub(xn, x)) < tol * (1.0 + dabs(x))) || (dabs(f) < tol);
+    x = xn;
+  }
+  *ok = conv;
+  return x;
+}
+
+__global__ void k_beta(const double* __restrict__ coef, int n, double* beta) {
+  const double guesses[8] = {0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0, 4.5};
+  __shared__ double best[64];
+  const int lv = threadIdx.x / 8, g = threadIdx.x % 8;
+  const int k = blockIdx.x * 8 + lv;
+  double val = -1.0;   // "no valid root"
+  if (k < n) {
+    double p[6];
+    for (int j = 0; j < 6; ++j) p[j] = coef[(size_t)j * n + k];
+    bool ok;
+    const dcx r = laguerre(p, guesses[g], &ok);
+    if (ok && fabs(r.y) < 1e-6 && r.x > 0.0 && r.x < 5.0) val = r.x;
+  }
+  best[threadIdx.x] = val;
+  __syncthreads();
+  if (g == 0 && k < n) {
+    double m = -1.0;
+    for (int j = 0; j < 8; ++j) m = fmax(m, best[lv * 8 + j]);
+    beta[k] = (m > 0.0) ? m : 1.0;
+  }
+}
+
+// Quintic coefficients (LASDD_SM.py:314-319 momentum, ScalarLASDD_SM.py:
+// 296-302 scalar) from the plane means m[10][nz]; coef (6, nz).
+__global__ void k_quintic(const double* __restrict__ m, int nz, int scalar, double* coef) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  double ff, ee, dd, cc, bb, aa;
+  if (!scalar) {
+    const double a1 = m[0 * nz + k], a2 = m[1 * nz + k], b1 = m[2 * nz + k], b2 = m[3 * nz + k];
+    const double c1 = m[4 * nz + k], c2 = m[5 * nz + k], d1 = m[6 * nz + k], d2 = m[7 * nz + k];
+    const double e1 = m[8 * nz + k], e2 = m[9 * nz + k];
+    aa = a1 * c2 - a2 * c1;
+    bb = a2 * e1 - b1 * c2;
+    cc = b2 * c1 - a1 * e2 - a2 * d1;
+    dd = b1 * e2 - b2 * e1;
+    ee = a1 * d2 + b2 * d1;
+    ff = -b1 * d2;
+  } else {
+    const double a2 = m[0 * nz + k], b2 = m[1 * nz + k], c2 = m[2 * nz + k], d2 = m[3 * nz + k];
+    const double e2 = m[4 * nz + k], a4 = m[5 * nz + k], b4 = m[6 * nz + k], c4 = m[7 * nz + k];
+    const double d4 = m[8 * nz + k], e4 = m[9 * nz + k];
+    aa = a2 * c4 - a4 * c2;
+    bb = -a4 * d2 + b2 * c4;
+    cc = -c2 * b4 + a2 * d4 - a4 * e2;
+    dd = b2 * d4 - b4 * d2;
+    ee = a2 * e4 - b4 * e2;
+    ff = b2 * e4;
+  }
+  coef[0 * nz + k] = ff; coef[1 * nz + k] = ee; coef[2 * nz + k] = dd;
+  coef[3 * nz + k] = cc; coef[4 * nz + k] = bb; coef[5 * nz + k] = aa;
+}
+
+// Profiles: avg1 = mean(sqrt C)^2, avg2 = mean(C) from the box-clip sums.
+__global__ void k_profiles(const double* __restrict__ s, int nz, double inv_n, double* avg1,
+                           double* avg2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  const double m1 = s[k] * inv_n;
+  if (avg1) avg1[k] = m1 * m1;
+  if (avg2) avg2[k] = s[nz + k] * inv_n;
+}
+
+__global__ void k_fill_d(int n, double* a, double v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = v;
+}

@@ -1,0 +1,678 @@
+// The composed time step (src/Main.py:158-540) and its stages.  Each stage
+// function serves both the step and the matching per-operation entry point.
+#pragma once
+#include <cmath>
+#include "loaders.hpp"
+#include "plan.hpp"
+#include "pressure.hpp"
+#include "surface.hpp"
+
+namespace alfa {
+
+constexpr int BIGCUT = 1 << 28;
+
+template <class T> struct Ctx {
+  alfa_plan* P;
+  hipStream_t s;
+  int nx, ny, nz, nyc, mx, my, ncol;
+  size_t F, FB, SPB, PADB, PSPB;
+  explicit Ctx(alfa_plan* p, hipStream_t st) : P(p), s(st) {
+    nx = p->nx; ny = p->ny; nz = p->nz; nyc = p->nyc; mx = p->mx; my = p->my;
+    ncol = nx * ny;
+    F = (size_t)ncol * nz;
+    FB = F * sizeof(T);
+    SPB = (size_t)nx * nyc * nz * sizeof(c2<T>);
+    PADB = (size_t)mx * my * nz * sizeof(T);
+    PSPB = (size_t)mx * (my / 2 + 1) * nz * sizeof(c2<T>);
+  }
+  T* W(const std::string& n) { return (T*)P->ws.get("f_" + n, FB); }
+  c2<T>* S(const std::string& n) { return (c2<T>*)P->ws.get("s_" + n, SPB); }
+  T* Pad(const std::string& n) { return (T*)P->ws.get("p_" + n, PADB); }
+  c2<T>* PS(const std::string& n) { return (c2<T>*)P->ws.get("ps_" + n, PSPB); }
+  double* D(const std::string& n, size_t cnt) { return (double*)P->ws.get("d_" + n, cnt * sizeof(double)); }
+  PwDims dims() const { PwDims d; d.n = F; d.nz = nz; return d; }
+  PwDims pdims() const { PwDims d; d.n = (size_t)mx * my * nz; d.nz = nz; return d; }
+};
+
+inline XOp xop(int ncy, int inverse, int cx, int mult, int map, int nout, double scale = 1.0) {
+  XOp o;
+  o.ncy = ncy; o.inverse = inverse; o.cx = cx; o.mult = mult; o.map = map; o.nout = nout;
+  o.scale = scale;
+  return o;
+}
+
+// forward rfft2 of nb real fields (through a loader) into full-pitch spectra
+template <class T, class Loader>
+void rfft2_fields(Ctx<T>& c, const Loader& ld, int nz, int nb, c2<T>* const* out) {
+  alfa_plan* P = c.P;
+  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, nz, c.nyc, nb, (void* const*)out);
+  XArgs xa{};
+  for (int b = 0; b < nb; ++b) { xa.in[b] = out[b]; xa.dst[b][0] = out[b]; }
+  xa.op[0] = xop(c.nyc, 0, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = nz; xa.in_spectral = 0; xa.ny_full = c.ny;
+  x_pass<T>(P, c.s, xa, nb, P->fx, P->fx, P->tw_x, P->tw_x);
+}
+
+// Dealias1(FFT(F)) for nb fields (Dealiasing.py:51-78 after FFT.py:44).
+template <class T>
+void dealias1(Ctx<T>& c, int nb, const T* const* in, T* const* out) {
+  alfa_plan* P = c.P;
+  c2<T>* sp[MAXB];
+  c2<T>* pp[MAXB];
+  for (int b = 0; b < nb; ++b) {
+    sp[b] = c.S("d1_" + std::to_string(b));
+    pp[b] = c.PS("d1_" + std::to_string(b));
+  }
+  LoadPtr<T> ld{};
+  for (int b = 0; b < nb; ++b) ld.p[b] = in[b];
+  ld.ny = c.ny; ld.nz = c.nz;
+  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, c.nyc, nb, (void* const*)sp);
+  XArgs xa{};
+  for (int b = 0; b < nb; ++b) { xa.in[b] = sp[b]; xa.dst[b][0] = pp[b]; }
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_D1, c.mx);
+  xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny;
+  x_pass<T>(P, c.s, xa, nb, P->fx, P->fmx, P->tw_x, P->tw_mx);
+  YC2RArgs ya{};
+  for (int b = 0; b < nb; ++b) ya.in[b][0] = pp[b];
+  ya.nsum = 1; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.mx * c.my);
+  StorePtr<T> st{};
+  for (int b = 0; b < nb; ++b) st.p[b] = out[b];
+  st.ny = c.my; st.nz = c.nz;
+  y_c2r<T>(P, c.s, st, ya, nb, c.mx, P->fmy, P->tw_my);
+}
+
+// Dealias2(FFT_pad(F_pad)) for nb padded fields (Dealiasing.py:86-114).
+template <class T>
+void dealias2(Ctx<T>& c, int nb, const T* const* in, T* const* out) {
+  alfa_plan* P = c.P;
+  const int ncy = c.ny / 2;
+  c2<T>* sp[MAXB];
+  c2<T>* op[MAXB];
+  for (int b = 0; b < nb; ++b) {
+    sp[b] = c.PS("d2_" + std::to_string(b));
+    op[b] = c.S("d2_" + std::to_string(b));
+  }
+  LoadPtr<T> ld{};
+  for (int b = 0; b < nb; ++b) ld.p[b] = in[b];
+  ld.ny = c.my; ld.nz = c.nz;
+  y_r2c<T>(P, c.s, ld, c.mx, P->fmy, P->tw_my, c.nz, ncy, nb, (void* const*)sp);
+  XArgs xa{};
+  for (int b = 0; b < nb; ++b) { xa.in[b] = sp[b]; xa.dst[b][0] = op[b]; }
+  xa.op[0] = xop(ncy, 1, BIGCUT, XM_NONE, XMAP_D2, c.nx);
+  xa.nops = 1; xa.ncy_in = ncy; xa.nz = c.nz; xa.ny_full = c.ny;
+  x_pass<T>(P, c.s, xa, nb, P->fmx, P->fx, P->tw_mx, P->tw_x);
+  YC2RArgs ya{};
+  for (int b = 0; b < nb; ++b) ya.in[b][0] = op[b];
+  ya.nsum = 1; ya.ncy_in = ncy; ya.nz = c.nz; ya.scale = 2.25 / ((double)c.nx * c.ny);
+  StorePtr<T> st{};
+  for (int b = 0; b < nb; ++b) st.p[b] = out[b];
+  st.ny = c.ny; st.nz = c.nz;
+  y_c2r<T>(P, c.s, st, ya, nb, c.nx, P->fy, P->tw_y);
+}
+
+// Test filters of nb virtual fields at one or two levels
+// (Filtering_Level1/2 of FFT(field), Filtering.py:97-165).
+template <class T, class Loader>
+void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level2, const std::string& tag,
+                 T* const* out1, T* const* out2) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  c2<T>* s1[MAXB];
+  c2<T>* s2[MAXB];
+  for (int b = 0; b < nb; ++b) {
+    s1[b] = c.S(tag + "1_" + std::to_string(b));
+    if (level2) s2[b] = c.S(tag + "2_" + std::to_string(b));
+  }
+  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, d.cut1_y, nb, (void* const*)s1);
+  XArgs xa{};
+  for (int b = 0; b < nb; ++b) {
+    xa.in[b] = s1[b];
+    xa.dst[b][0] = s1[b];
+    if (level2) xa.dst[b][1] = s2[b];
+  }
+  xa.op[0] = xop(d.cut1_y, 1, d.cut1_x, XM_NONE, XMAP_ID, c.nx);
+  xa.op[1] = xop(d.cut2_y, 1, d.cut2_x, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = level2 ? 2 : 1; xa.ncy_in = d.cut1_y; xa.nz = c.nz; xa.ny_full = c.ny;
+  x_pass<T>(P, c.s, xa, nb, P->fx, P->fx, P->tw_x, P->tw_x);
+  const double sc = 1.0 / ((double)c.nx * c.ny);
+  {
+    YC2RArgs ya{};
+    for (int b = 0; b < nb; ++b) ya.in[b][0] = s1[b];
+    ya.nsum = 1; ya.ncy_in = d.cut1_y; ya.nz = c.nz; ya.scale = sc;
+    StorePtr<T> st{};
+    for (int b = 0; b < nb; ++b) st.p[b] = out1[b];
+    st.ny = c.ny; st.nz = c.nz;
+    y_c2r<T>(P, c.s, st, ya, nb, c.nx, P->fy, P->tw_y);
+  }
+  if (level2) {
+    YC2RArgs ya{};
+    for (int b = 0; b < nb; ++b) ya.in[b][0] = s2[b];
+    ya.nsum = 1; ya.ncy_in = d.cut2_y; ya.nz = c.nz; ya.scale = sc;
+    StorePtr<T> st{};
+    for (int b = 0; b < nb; ++b) st.p[b] = out2[b];
+    st.ny = c.ny; st.nz = c.nz;
+    y_c2r<T>(P, c.s, st, ya, nb, c.nx, P->fy, P->tw_y);
+  }
+}
+
+// Pointers of the carried state of one run.
+template <class T> struct StateT {
+  T *u, *v, *w, *TH, *Ru, *Rv, *Rw, *RT, *Cs2, *CsPr;
+  T *psim, *psim0, *psih, *psih0, *fim, *fih;
+};
+
+// Per-step derived planes.
+template <class T> struct Surf {
+  T *M, *ustar, *qz, *invOB;
+};
+
+template <class T>
+void stage_surface(Ctx<T>& c, const StateT<T>& S, const Surf<T>& sf, double sfc_val) {
+  alfa_plan* P = c.P;
+  const auto& g = P->cfg;
+  const auto& d = P->d;
+  SurfArgs<T> a;
+  a.u = S.u; a.v = S.v; a.TH = S.TH;
+  a.psim = S.psim; a.psim0 = S.psim0; a.psih = S.psih; a.psih0 = S.psih0; a.fim = S.fim; a.fih = S.fih;
+  a.M = sf.M; a.ustar = sf.ustar; a.qz = sf.qz; a.invOB = sf.invOB;
+  a.scal = P->scal;
+  a.ncol = c.ncol; a.nz = c.nz; a.bc = g.optSurfBC; a.hetero = g.optSurfFlux != 0;
+  a.sfc_val = sfc_val; a.SHF = g.SensibleHeatFlux; a.Ugal = g.Ugal; a.T0 = g.T_0;
+  a.vonk = 0.4; a.g = d.g_nondim; a.dz_half = 0.5 * d.dz; a.z1 = 0.0;
+  const double zref = g.zTemperature > 0.0 ? g.zTemperature : g.z0T;
+  a.z0m_nd = g.z0m / d.z_scale; a.zT_nd = zref / d.z_scale;
+  a.logm = std::log(0.5 * d.dz * d.z_scale / g.z0m);
+  a.logh = std::log(0.5 * d.dz * d.z_scale / zref);
+  if (P->ws.dry) return;
+  k_surface<T><<<1, SURF_BS, 0, c.s>>>(a);
+  launch_ok(P);
+}
+
+// DivStress (NSE_SGSTerms.py:55-105): 5 transforms, 6 spectral derivatives
+// summed pairwise in the y-C2R, plus the z-derivatives in its epilogue.
+template <class T>
+void stage_div_stress(Ctx<T>& c, T* txx, T* tyy, T* tzz, T* txy, T* txz, T* tyz, T* dtx,
+                      T* dty, T* dtz) {
+  alfa_plan* P = c.P;
+  c2<T>* sp[5] = {c.S("ds0"), c.S("ds1"), c.S("ds2"), c.S("ds3"), c.S("ds4")};
+  c2<T>* sxy_y = c.S("ds5");
+  // order: txx, txz | tyy, tyz | txy
+  LoadPtr<T> ld{};
+  ld.p[0] = txx; ld.p[1] = txz; ld.p[2] = tyy; ld.p[3] = tyz; ld.p[4] = txy;
+  ld.ny = c.ny; ld.nz = c.nz;
+  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, c.nyc, 5, (void* const*)sp);
+  XArgs xa{};
+  xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.nops = 1;
+  xa.in[0] = sp[0]; xa.dst[0][0] = sp[0];
+  xa.in[1] = sp[1]; xa.dst[1][0] = sp[1];
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
+  x_pass<T>(P, c.s, xa, 2, P->fx, P->fx, P->tw_x, P->tw_x);
+  xa.in[0] = sp[2]; xa.dst[0][0] = sp[2];
+  xa.in[1] = sp[3]; xa.dst[1][0] = sp[3];
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
+  x_pass<T>(P, c.s, xa, 2, P->fx, P->fx, P->tw_x, P->tw_x);
+  xa.in[0] = sp[4]; xa.dst[0][0] = sp[4]; xa.dst[0][1] = sxy_y;
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
+  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
+  xa.nops = 2;
+  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  YC2RArgs ya{};
+  ya.in[0][0] = sp[0]; ya.in[0][1] = sxy_y;   // divtx = d/dx txx + d/dy txy
+  ya.in[1][0] = sp[4]; ya.in[1][1] = sp[2];   // divty = d/dx txy + d/dy tyy
+  ya.in[2][0] = sp[1]; ya.in[2][1] = sp[3];   // divtz = d/dx txz + d/dy tyz
+  ya.nsum = 2; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  StoreDiv<T> st{};
+  st.out[0] = dtx; st.out[1] = dty; st.out[2] = dtz;
+  st.fz[0] = txz; st.fz[1] = tyz; st.fz[2] = tzz;
+  st.wnode[0] = 1; st.wnode[1] = 1; st.wnode[2] = 0;
+  st.ny = c.ny; st.nz = c.nz; st.idz = (T)(1.0 / P->d.dz);
+  y_c2r<T>(P, c.s, st, ya, 3, c.nx, P->fy, P->tw_y);
+}
+
+// DivFlux (SCL_SGSTerms.py:55-89).
+template <class T>
+void stage_div_flux(Ctx<T>& c, T* qx, T* qy, T* qz, T* divq) {
+  alfa_plan* P = c.P;
+  c2<T>* sp[2] = {c.S("dq0"), c.S("dq1")};
+  LoadPtr<T> ld{};
+  ld.p[0] = qx; ld.p[1] = qy; ld.ny = c.ny; ld.nz = c.nz;
+  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, c.nyc, 2, (void* const*)sp);
+  XArgs xa{};
+  xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.nops = 1;
+  xa.in[0] = sp[0]; xa.dst[0][0] = sp[0];
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
+  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  xa.in[0] = sp[1]; xa.dst[0][0] = sp[1];
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
+  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  YC2RArgs ya{};
+  ya.in[0][0] = sp[0]; ya.in[0][1] = sp[1];
+  ya.nsum = 2; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  StoreDiv<T> st{};
+  st.out[0] = divq; st.fz[0] = qz; st.wnode[0] = 1;
+  st.ny = c.ny; st.nz = c.nz; st.idz = (T)(1.0 / P->d.dz);
+  y_c2r<T>(P, c.s, st, ya, 1, c.nx, P->fy, P->tw_y);
+}
+
+// PressureRC + ThomasPressureSolve + the gradient (NSE_PressureTerms.py:109,
+// NSE_PressureTerms_Thomas.py:218).  Writes dpdx, dpdy, dpdz; optionally the
+// unmasked RC and p itself (for the per-operation entry points).
+template <class T>
+void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T* dpdy, T* dpdz,
+                                 T* p, T* RCr, T* RCi) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  c2<T>* phat = c.S("phat");
+  c2<T>* dpz = c.S("dpzh");
+  c2<T>* dpx = c.S("dpxh");
+  c2<T>* dpy = c.S("dpyh");
+  ThomasArgs<T> ta{};
+  ta.fRx = fR[0]; ta.fRy = fR[1]; ta.fRz = fR[2]; ta.fbt = fbt;
+  ta.m = P->th_m; ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
+  ta.RCr = RCr; ta.RCi = RCi;
+  ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
+  ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
+  if (!P->ws.dry) {
+    k_thomas<T><<<c.nx * c.nyc, 64, 0, c.s>>>(ta);
+    launch_ok(P);
+  }
+  XArgs xa{};
+  xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.in_spectral = 1;
+  xa.in[0] = phat; xa.dst[0][0] = dpx; xa.dst[0][1] = dpy; xa.dst[0][2] = phat;
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
+  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
+  xa.op[2] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = p ? 3 : 2;
+  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  xa.in[0] = dpz; xa.dst[0][0] = dpz; xa.nops = 1;
+  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  YC2RArgs ya{};
+  ya.in[0][0] = dpx; ya.in[1][0] = dpy; ya.in[2][0] = dpz; ya.in[3][0] = phat;
+  ya.nsum = 1; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  StorePtr<T> st{};
+  st.p[0] = dpdx; st.p[1] = dpdy; st.p[2] = dpdz; st.p[3] = p;
+  st.ny = c.ny; st.nz = c.nz;
+  y_c2r<T>(P, c.s, st, ya, p ? 4 : 3, c.nx, P->fy, P->tw_y);
+}
+
+template <class T>
+void stage_pressure(Ctx<T>& c, const T* R[3], const T* Rp[3], const T* U[3], const T* divtz,
+                    T* dpdx, T* dpdy, T* dpdz, T* p, T* RCr, T* RCi, c2<T>** fR_out) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  c2<T>* fR[3] = {c.S("fRx"), c.S("fRy"), c.S("fRz")};
+  c2<T>* fbt = (c2<T>*)P->ws.get("s_fbt", (size_t)c.nx * c.nyc * 2 * sizeof(c2<T>));
+  LoadPressureR<T> ld{};
+  for (int k = 0; k < 3; ++k) { ld.R[k] = R[k]; ld.Rp[k] = Rp[k]; ld.U[k] = U[k]; }
+  ld.ny = c.ny; ld.nz = c.nz;
+  ld.third = (T)(1.0 / 3.0);
+  ld.c2dt = (T)(2.0 / (3.0 * d.dt_nondim));
+  rfft2_fields<T>(c, ld, c.nz, 3, fR);
+  LoadPlanes2<T> lp{divtz, c.ny, c.nz};
+  rfft2_fields<T>(c, lp, 2, 1, &fbt);
+  if (fR_out) for (int k = 0; k < 3; ++k) fR_out[k] = fR[k];
+  stage_pressure_from_spectra<T>(c, fR, fbt, dpdx, dpdy, dpdz, p, RCr, RCi);
+}
+
+// Dynamic procedure of the momentum model (DynamicSGS_LASDD_SM.py:138-367).
+template <class T>
+void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const Sij[6],
+                 const T* Sm, const double* shu, const double* shv, T* Cs2, double* avg1,
+                 double* avg2, double* beta_out) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  const bool lasdd = P->cfg.optSgs == 1;
+  LoadLasdd<T> ld{};
+  ld.u = u; ld.v = v; ld.w = w;
+  for (int n = 0; n < 6; ++n) ld.S[n] = Sij[n];
+  ld.Sm = Sm; ld.shu = shu; ld.shv = shv; ld.ny = c.ny; ld.nz = c.nz;
+  T* f1[21];
+  T* f2[21];
+  for (int b = 0; b < 21; ++b) {
+    f1[b] = c.W("lf1_" + std::to_string(b));
+    f2[b] = lasdd ? c.W("lf2_" + std::to_string(b)) : nullptr;
+  }
+  test_filter<T>(c, ld, 21, lasdd, "ls", f1, f2);
+  MomLevel<T> l1{}, l2{};
+  for (int k = 0; k < 3; ++k) { l1.uh[k] = f1[k]; l2.uh[k] = f2[k]; }
+  for (int n = 0; n < 6; ++n) {
+    l1.ph[n] = f1[3 + n]; l1.Sh[n] = f1[9 + n]; l1.SSh[n] = f1[15 + n];
+    l2.ph[n] = f2[3 + n]; l2.Sh[n] = f2[9 + n]; l2.SSh[n] = f2[15 + n];
+  }
+  double* beta = c.D("beta1", c.nz);
+  if (lasdd) {
+    double* sums = c.D("lsums", 10 * c.nz);
+    double* coef = c.D("lcoef", 6 * c.nz);
+    FLasddSums<T> f{l1, l2, c.nz, d.L, d.TFR};
+    plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasdd");
+    if (!P->ws.dry) {
+      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 0, coef);
+      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_ok(P);
+    }
+  } else if (!P->ws.dry) {
+    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_ok(P);
+  }
+  T* LM = c.W("LM");
+  T* MM = c.W("MM");
+  T* Shat = c.W("Shat");
+  PW_LAUNCH(P, c.s, c.F, k_lasdd_lm<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
+  if (lasdd) {
+    T* Shatd = c.W("Shatd");
+    PW_LAUNCH(P, c.s, c.F, k_smag_level<T>, c.dims(), l2, Shatd);
+  }
+  double* s2 = c.D("bsums", 2 * c.nz);
+  FBoxClip<T> fb{LM, MM, Cs2, c.nx, c.ny, c.nz};
+  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "box");
+  if (!P->ws.dry) {
+    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
+    launch_ok(P);
+    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+  }
+}
+
+// Dynamic procedure of the scalar model (DynamicSGS_ScalarLASDD_SM.py:139-316).
+template <class T>
+void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* TH,
+                        const T* dx, const T* dy, const T* dz, const T* Sm, const double* shu,
+                        const double* shv, const double* sht, T* CsPr, double* prof,
+                        double* beta_out) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  const bool lasdd = P->cfg.optSgs == 1;
+  LoadScalar<T> ld{};
+  ld.u = u; ld.v = v; ld.w = w; ld.TH = TH; ld.dx = dx; ld.dy = dy; ld.dz = dz; ld.Sm = Sm;
+  ld.shu = shu; ld.shv = shv; ld.sht = sht; ld.ny = c.ny; ld.nz = c.nz;
+  T* f1[10];
+  T* f2[10];
+  for (int b = 0; b < 10; ++b) {
+    f1[b] = c.W("sf1_" + std::to_string(b));
+    f2[b] = lasdd ? c.W("sf2_" + std::to_string(b)) : nullptr;
+  }
+  test_filter<T>(c, ld, 10, lasdd, "ss", f1, f2);
+  ScaLevel<T> l1{}, l2{};
+  l1.th = f1[0]; l2.th = f2[0];
+  for (int k = 0; k < 3; ++k) {
+    l1.uth[k] = f1[1 + k]; l1.dh[k] = f1[4 + k]; l1.sdh[k] = f1[7 + k];
+    l2.uth[k] = f2[1 + k]; l2.dh[k] = f2[4 + k]; l2.sdh[k] = f2[7 + k];
+    l1.uh[k] = c.W("lf1_" + std::to_string(k));
+    l2.uh[k] = lasdd ? c.W("lf2_" + std::to_string(k)) : nullptr;
+  }
+  l1.Sm = c.W("Shat");
+  l2.Sm = lasdd ? c.W("Shatd") : nullptr;
+  double* beta = c.D("beta2", c.nz);
+  if (lasdd) {
+    double* sums = c.D("ssums", 10 * c.nz);
+    double* coef = c.D("scoef", 6 * c.nz);
+    FScalarSums<T> f{l1, l2, c.nz, d.L, d.TFR};
+    plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasdd");
+    if (!P->ws.dry) {
+      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 1, coef);
+      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_ok(P);
+    }
+  } else if (!P->ws.dry) {
+    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_ok(P);
+  }
+  T* Tup = c.W("Tup");
+  T* Tdn = c.W("Tdn");
+  PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
+  double* s2 = c.D("ssum2", 2 * c.nz);
+  FBoxClip<T> fb{Tup, Tdn, CsPr, c.nx, c.ny, c.nz};
+  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "sbox");
+  if (!P->ws.dry) {
+    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
+    launch_ok(P);
+    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+  }
+}
+
+template <class T> struct FSum4 {
+  const T* u; const T* v; const T* TH; int nz; double T0;
+  __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    const double th = TH[i];
+    acc[0] += (double)u[i];
+    acc[1] += (double)v[i];
+    acc[2] += th;
+    acc[3] += th + T0;
+  }
+};
+
+__global__ void k_cfl_final(const double* __restrict__ part, int nb, double cx, double cy,
+                            double cz, double* scal, double* out) {
+  __shared__ double sm[3][256];
+  double m[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 256)
+    for (int k = 0; k < 3; ++k) m[k] = fmax(m[k], part[3 * b + k]);
+  for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = m[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double cfl = fmax(fmax(sm[0][0] * cx, sm[1][0] * cy), sm[2][0] * cz);
+    scal[0] = cfl;
+    if (out)
+      for (int k = 0; k < 4; ++k) out[k] = scal[k];
+  }
+}
+
+constexpr int AB2_BLOCKS = 2048;
+
+// One time step.
+template <class T>
+int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream_t s) {
+  Ctx<T> c(P, s);
+  const auto& g = P->cfg;
+  const auto& d = P->d;
+  StateT<T> S;
+  S.u = (T*)st->u; S.v = (T*)st->v; S.w = (T*)st->w; S.TH = (T*)st->TH;
+  S.Ru = (T*)st->RHS_u; S.Rv = (T*)st->RHS_v; S.Rw = (T*)st->RHS_w; S.RT = (T*)st->RHS_TH;
+  S.Cs2 = (T*)st->Cs2_3D; S.CsPr = (T*)st->Cs2PrRatio_3D;
+  S.psim = (T*)st->psi_m; S.psim0 = (T*)st->psi_m0; S.psih = (T*)st->psi_h;
+  S.psih0 = (T*)st->psi_h0; S.fim = (T*)st->fi_m; S.fih = (T*)st->fi_h;
+  const bool first = it == g.istep;
+  const bool dealias = d.optDealias == 1;
+  const bool mixed = P->prec == ALFA_MIXED;
+  const size_t PB = (size_t)c.ncol * sizeof(T);
+  Surf<T> sf;
+  sf.M = (T*)P->ws.get("pl_M", PB); sf.ustar = (T*)P->ws.get("pl_ustar", PB);
+  sf.qz = (T*)P->ws.get("pl_qz", PB); sf.invOB = (T*)P->ws.get("pl_invOB", PB);
+  T *dudx = c.W("dudx"), *dvdx = c.W("dvdx"), *dwdx = c.W("dwdx");
+  T *dudy = c.W("dudy"), *dvdy = c.W("dvdy"), *dwdy = c.W("dwdy");
+  T *dudz = c.W("dudz"), *dvdz = c.W("dvdz"), *dwdz = c.W("dwdz");
+  T *dTx = c.W("dTHdx"), *dTy = c.W("dTHdy"), *dTz = c.W("dTHdz");
+
+  // 1. explicit filter of u, v, w, TH and the spectral gradients
+  //    (Main.py:193-197, Derivatives.py:47-153)
+  {
+    c2<T>* ef[4] = {c.S("ef0"), c.S("ef1"), c.S("ef2"), c.S("ef3")};
+    c2<T>* ex[4] = {c.S("ex0"), c.S("ex1"), c.S("ex2"), c.S("ex3")};
+    c2<T>* ey[4] = {c.S("ey0"), c.S("ey1"), c.S("ey2"), c.S("ey3")};
+    LoadPtr<T> ld{};
+    ld.p[0] = S.u; ld.p[1] = S.v; ld.p[2] = S.w; ld.p[3] = S.TH; ld.ny = c.ny; ld.nz = c.nz;
+    y_r2c<T>(P, s, ld, c.nx, P->fy, P->tw_y, c.nz, d.cut0_y, 4, (void* const*)ef);
+    XArgs xa{};
+    for (int b = 0; b < 4; ++b) {
+      xa.in[b] = ef[b]; xa.dst[b][0] = ef[b]; xa.dst[b][1] = ex[b]; xa.dst[b][2] = ey[b];
+    }
+    xa.op[0] = xop(d.cut0_y, 1, d.cut0_x, XM_NONE, XMAP_ID, c.nx);
+    xa.op[1] = xop(d.cut0_y, 1, d.cut0_x, XM_IKX, XMAP_ID, c.nx);
+    xa.op[2] = xop(d.cut0_y, 1, d.cut0_x, XM_IKY, XMAP_ID, c.nx);
+    xa.nops = 3; xa.ncy_in = d.cut0_y; xa.nz = c.nz; xa.ny_full = c.ny;
+    x_pass<T>(P, s, xa, 4, P->fx, P->fx, P->tw_x, P->tw_x);
+    YC2RArgs ya{};
+    T* outs[12] = {S.u, S.v, S.w, S.TH, dudx, dvdx, dwdx, dTx, dudy, dvdy, dwdy, dTy};
+    for (int b = 0; b < 4; ++b) { ya.in[b][0] = ef[b]; ya.in[4 + b][0] = ex[b]; ya.in[8 + b][0] = ey[b]; }
+    ya.nsum = 1; ya.ncy_in = d.cut0_y; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+    StorePtr<T> so{};
+    for (int b = 0; b < 12; ++b) so.p[b] = outs[b];
+    so.ny = c.ny; so.nz = c.nz;
+    y_c2r<T>(P, s, so, ya, 12, c.nx, P->fy, P->tw_y);
+  }
+  // 2. surface fluxes (Main.py:213-243)
+  stage_surface<T>(c, S, sf, sfc_val);
+  // 3. vertical gradients (Derivz_M, Derivz_TH)
+  PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim, S.fih,
+            sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz), (T)(0.4 * 0.5 * d.dz),
+            (T)(0.4 * 0.5 * d.dz));
+  // plane means of u, v, TH, TH + T_0
+  double* means = c.D("means", 4 * c.nz);
+  {
+    FSum4<T> f{S.u, S.v, S.TH, c.nz, g.T_0};
+    plane_sums<4>(P, s, c.ncol, c.nz, f, means, 1.0 / c.ncol, "means");
+  }
+  // 4. advection (Main.py:301-311)
+  T *Cx = c.W("Cx"), *Cy = c.W("Cy"), *Cz = c.W("Cz"), *THadv = c.W("THadv");
+  if (!dealias) {
+    PW_LAUNCH(P, s, c.F, k_advection<T>, c.dims(), S.u, S.v, S.w, dudy, dudz, dvdx, dvdz, dwdx,
+              dwdy, Cx, Cy, Cz);
+    PW_LAUNCH(P, s, c.F, k_scalar_adv<T>, c.dims(), S.u, S.v, S.w, dTx, dTy, dTz, THadv);
+  } else {
+    const T* in9[9] = {S.u, S.v, S.w, dudy, dudz, dvdx, dvdz, dwdx, dwdy};
+    T* p9[9];
+    for (int b = 0; b < 9; ++b) p9[b] = c.Pad("adv" + std::to_string(b));
+    dealias1<T>(c, 9, in9, p9);
+    T *pcx = c.Pad("cx"), *pcy = c.Pad("cy"), *pcz = c.Pad("cz");
+    PW_LAUNCH(P, s, (size_t)c.mx * c.my * c.nz, k_advection<T>, c.pdims(), p9[0], p9[1], p9[2],
+              p9[3], p9[4], p9[5], p9[6], p9[7], p9[8], pcx, pcy, pcz);
+    const T* pc3[3] = {pcx, pcy, pcz};
+    T* o3[3] = {Cx, Cy, Cz};
+    dealias2<T>(c, 3, pc3, o3);
+    const T* in3[3] = {dTx, dTy, dTz};
+    T* q3[3] = {c.Pad("sadv0"), c.Pad("sadv1"), c.Pad("sadv2")};
+    dealias1<T>(c, 3, in3, q3);
+    T* pcs = c.Pad("cs");
+    PW_LAUNCH(P, s, (size_t)c.mx * c.my * c.nz, k_scalar_adv<T>, c.pdims(), p9[0], p9[1], p9[2],
+              q3[0], q3[1], q3[2], pcs);
+    const T* pc1[1] = {pcs};
+    T* o1[1] = {THadv};
+    dealias2<T>(c, 1, pc1, o1);
+  }
+  // 5. buoyancy (Main.py:322-327)
+  T* buoy = c.W("buoy");
+  PW_LAUNCH(P, s, c.F, k_buoyancy<T>, c.dims(), S.TH, means + 3 * c.nz, buoy, g.T_0, d.g_nondim,
+            g.optBuoyancy);
+  // 6. SGS (Main.py:333-443)
+  const bool dynamic = (g.optSgs >= 1 && g.optSgs <= 4) &&
+                       (first || (g.dynamicSGS_call_time > 0 && it % g.dynamicSGS_call_time == 0));
+  T *S12 = c.W("S12"), *S13 = c.W("S13"), *S23 = c.W("S23"), *Sm = c.W("S");
+  PW_LAUNCH(P, s, c.F, k_strains_uvp<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
+            dwdz, S12, S13, S23, Sm);
+  const T* Sij[6] = {dudx, dvdy, dwdz, S12, S13, S23};
+  const double* shu = mixed ? means : nullptr;
+  const double* shv = mixed ? means + c.nz : nullptr;
+  const double* sht = mixed ? means + 2 * c.nz : nullptr;
+  if (dynamic)
+    stage_lasdd<T>(c, S.u, S.v, S.w, Sij, Sm, shu, shv, S.Cs2, st->Cs2_1D_avg1, st->Cs2_1D_avg2,
+                   st->beta1_1D);
+  T *txx = c.W("txx"), *tyy = c.W("tyy"), *tzz = c.W("tzz"), *txy = c.W("txy");
+  T *txz = c.W("txz"), *tyz = c.W("tyz");
+  T *S13w = c.W("S13w"), *S23w = c.W("S23w"), *Sw = c.W("Sw");
+  const double logm = std::log(0.5 * d.dz * d.z_scale / g.z0m);
+  const T m2L2 = (T)(-2.0 * d.L * d.L);
+  T *pS = nullptr, *pSw = nullptr;
+  if (!dealias) {
+    PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
+              dwdz, S13w, S23w, Sw, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
+    PW_LAUNCH(P, s, c.F, k_stress_uvp<T>, c.dims(), dudx, dvdy, dwdz, S12, Sm, S.Cs2, m2L2, txx,
+              tyy, tzz, txy);
+    PW_LAUNCH(P, s, c.F, k_stress_w<T>, c.dims(), S13w, S23w, Sw, S.Cs2, m2L2, txz, tyz, 1, S.u,
+              S.v, sf.M, S.psim, S.psim0, logm, 0.4, g.Ugal);
+  } else {
+    const size_t PF = (size_t)c.mx * c.my * c.nz;
+    T* ps[6];
+    for (int n = 0; n < 6; ++n) ps[n] = c.Pad("su" + std::to_string(n));
+    dealias1<T>(c, 6, Sij, ps);
+    pS = c.Pad("S");
+    PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), ps[0], ps[1], ps[2], ps[3], ps[4], ps[5], pS);
+    T* pc = c.Pad("cs2");
+    const T* cs[1] = {S.Cs2};
+    dealias1<T>(c, 1, cs, &pc);
+    T *ptxx = c.Pad("txx"), *ptyy = c.Pad("tyy"), *ptzz = c.Pad("tzz"), *ptxy = c.Pad("txy");
+    PW_LAUNCH(P, s, PF, k_stress_uvp<T>, c.pdims(), ps[0], ps[1], ps[2], ps[3], pS, pc, m2L2, ptxx,
+              ptyy, ptzz, ptxy);
+    const T* pt4[4] = {ptxx, ptyy, ptzz, ptxy};
+    T* t4[4] = {txx, tyy, tzz, txy};
+    dealias2<T>(c, 4, pt4, t4);
+    T *S11w = c.W("S11w"), *S22w = c.W("S22w"), *S33w = c.W("S33w"), *S12w = c.W("S12w");
+    PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
+              dwdz, S13w, S23w, Sw, S11w, S22w, S33w, S12w);
+    const T* w6[6] = {S11w, S22w, S33w, S12w, S13w, S23w};
+    T* pw[6];
+    for (int n = 0; n < 6; ++n) pw[n] = c.Pad("sw" + std::to_string(n));
+    dealias1<T>(c, 6, w6, pw);
+    pSw = c.Pad("Sw");
+    PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pSw);
+    T *ptxz = c.Pad("txz"), *ptyz = c.Pad("tyz");
+    PW_LAUNCH(P, s, PF, k_stress_w<T>, c.pdims(), pw[4], pw[5], pSw, pc, m2L2, ptxz, ptyz, 0,
+              (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+              (const T*)nullptr, logm, 0.4, g.Ugal);
+    const T* pt2[2] = {ptxz, ptyz};
+    T* t2[2] = {txz, tyz};
+    dealias2<T>(c, 2, pt2, t2);
+    PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, txz, tyz, S.u, S.v, sf.M, S.psim,
+              S.psim0, logm, 0.4, g.Ugal, (T*)nullptr, (const T*)nullptr);
+  }
+  if (dynamic)
+    stage_scalar_lasdd<T>(c, S.u, S.v, S.w, S.TH, dTx, dTy, dTz, Sm, shu, shv, sht, S.CsPr,
+                          st->Cs2PrRatio_1D, st->beta2_1D);
+  T *qx = c.W("qx"), *qy = c.W("qy"), *qzf = c.W("qz");
+  if (!dealias) {
+    PW_LAUNCH(P, s, c.F, k_scalar_flux<T>, c.dims(), dTx, dTy, dTz, Sm, Sw, S.CsPr, m2L2, qx, qy,
+              qzf, (const T*)sf.qz);
+  } else {
+    const size_t PF = (size_t)c.mx * c.my * c.nz;
+    const T* in4[4] = {dTx, dTy, dTz, S.CsPr};
+    T* p4[4] = {c.Pad("q0"), c.Pad("q1"), c.Pad("q2"), c.Pad("q3")};
+    dealias1<T>(c, 4, in4, p4);
+    T *pqx = c.Pad("qx"), *pqy = c.Pad("qy"), *pqz = c.Pad("qz");
+    PW_LAUNCH(P, s, PF, k_scalar_flux<T>, c.pdims(), p4[0], p4[1], p4[2], pS, pSw, p4[3], m2L2, pqx,
+              pqy, pqz, (const T*)nullptr);
+    const T* pq3[3] = {pqx, pqy, pqz};
+    T* q3[3] = {qx, qy, qzf};
+    dealias2<T>(c, 3, pq3, q3);
+    PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, (T*)nullptr, (T*)nullptr,
+              (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+              (const T*)nullptr, 0.0, 0.4, 0.0, qzf, (const T*)sf.qz);
+  }
+  T *dtx = c.W("divtx"), *dty = c.W("divty"), *dtz = c.W("divtz"), *divq = c.W("divq");
+  stage_div_stress<T>(c, txx, tyy, tzz, txy, txz, tyz, dtx, dty, dtz);
+  stage_div_flux<T>(c, qx, qy, qzf, divq);
+  // 7. right-hand sides (Main.py:464-473)
+  T *Rnu = c.W("Rnu"), *Rnv = c.W("Rnv"), *Rnw = c.W("Rnw"), *RnT = c.W("RnT");
+  {
+    const double* pr = P->prof;
+    PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, buoy, dtx, dty, dtz,
+              THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means, means + c.nz,
+              means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu, Rnv, Rnw, RnT);
+  }
+  // 8. pressure (Main.py:481-500)
+  T *dpdx = c.W("dpdx"), *dpdy = c.W("dpdy"), *dpdz = c.W("dpdz");
+  {
+    const T* R[3] = {Rnu, Rnv, Rnw};
+    const T* Rp[3] = {S.Ru, S.Rv, S.Rw};
+    const T* U[3] = {S.u, S.v, S.w};
+    stage_pressure<T>(c, R, Rp, U, dtz, dpdx, dpdy, dpdz, nullptr, nullptr, nullptr, nullptr);
+  }
+  // 9. AB2 + CFL (Main.py:506-540)
+  double* cflpart = c.D("cflpart", 3 * AB2_BLOCKS);
+  if (!P->ws.dry) {
+    k_ab2<T><<<AB2_BLOCKS, 256, 0, s>>>(c.dims(), S.u, S.v, S.w, S.TH, Rnu, Rnv, Rnw, RnT, S.Ru,
+                                         S.Rv, S.Rw, S.RT, dpdx, dpdy, dpdz, (T)d.dt_nondim,
+                                         (T)(d.inversion_nondim * d.dz), first ? 1 : 0, cflpart);
+    launch_ok(P);
+    k_cfl_final<<<1, 256, 0, s>>>(cflpart, AB2_BLOCKS, d.dt_nondim / d.dx, d.dt_nondim / d.dy,
+                                  d.dt_nondim / d.dz, P->scal, st->scalars);
+    launch_ok(P);
+  }
+  return 0;
+}
+
+}  // namespace alfa

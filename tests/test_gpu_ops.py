@@ -1,0 +1,284 @@
+"""Per-operation parity of the HIP path against the FP64 oracle.
+
+Every reference operation of the step is called through the C ABI (via the
+reference-named wrappers in alfa_amd.ops) on the same inputs as the oracle.
+Bars: FP64 plan <= 1e-12 relative to the largest reference magnitude
+(1e-10 where a division or a long recurrence is involved); mixed plan
+(float32 fields) <= 2e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import alfa_oracle as O  # noqa: E402
+from alfa_amd import cases  # noqa: E402
+from alfa_amd import core, ops  # noqa: E402
+
+TOL = {"fp64": 1e-12, "mixed": 2e-5}
+
+GRIDS = {
+    # (case, overrides): FGR=1 neutral at 32^3 and FGR=2 cases on
+    # radix-3 / radix-5 horizontal sizes
+    "neutral32": ("neutral", dict()),
+    "conv48": ("convective", dict(nx=48, ny=48, nz=32)),
+    "stable40": ("stable", dict(nx=40, ny=40, nz=24, optSgs=1)),
+}
+
+_plans = {}
+
+
+def get_plan(key, prec):
+    if (key, prec) not in _plans:
+        name, over = GRIDS[key]
+        cfg, *_ = cases.initial_fields(name, **over)
+        _plans[(key, prec)] = (core.Plan(cfg, prec), O.Grid(cfg), cfg)
+    P, g, cfg = _plans[(key, prec)]
+    ops.use_plan(P)
+    return P, g, cfg
+
+
+def smooth_field(g, seed, kmax=None):
+    rng = np.random.default_rng(seed)
+    F = rng.standard_normal((g.nx, g.ny, g.nz))
+    if kmax is not None:
+        Fh = np.fft.rfft2(F, axes=(0, 1))
+        kx = np.abs(np.fft.fftfreq(g.nx, 1 / g.nx))[:, None, None]
+        ky = np.fft.rfftfreq(g.ny, 1 / g.ny)[None, :, None]
+        Fh *= np.exp(-((kx / kmax) ** 2 + (ky / kmax) ** 2))
+        F = np.fft.irfft2(Fh, s=(g.nx, g.ny), axes=(0, 1))
+    return F
+
+
+def host(t):
+    return t.detach().to(torch.complex128 if t.is_complex() else torch.float64).cpu().numpy()
+
+
+@pytest.fixture(params=["fp64", "mixed"])
+def prec(request):
+    return request.param
+
+
+@pytest.fixture(params=list(GRIDS))
+def grid(request):
+    return request.param
+
+
+def test_fft_and_filters(grid, prec):
+    P, g, _ = get_plan(grid, prec)
+    tol = TOL[prec]
+    F = smooth_field(g, 1)
+    Fd = P.asdev(F)
+    Fh = ops.FFT(Fd)
+    ref = O.FFT(F)
+    assert rel_err(host(Fh), ref) < tol
+    Fhd = P.asdev(ref, complex_=True)
+    a, an = ops.Filtering_Explicit(Fhd)
+    ra, ran = O.Filtering_Explicit(g, ref)
+    assert rel_err(host(a), ra) < tol
+    assert rel_err(host(an), ran) < tol
+    assert rel_err(host(ops.Filtering_Level1(Fhd)), O.Filtering_Level1(g, ref)) < tol
+    assert rel_err(host(ops.Filtering_Level2(Fhd)), O.Filtering_Level2(g, ref)) < tol
+    assert rel_err(host(ops.Derivxy(Fhd, 0)), O.Derivxy(g, ref, g.kx2)) < tol
+    assert rel_err(host(ops.Derivxy(Fhd, 1)), O.Derivxy(g, ref, g.ky2)) < tol
+
+
+def test_dealias(grid, prec):
+    P, g, _ = get_plan(grid, prec)
+    tol = TOL[prec]
+    F = smooth_field(g, 2)
+    ref = O.FFT(F)
+    # a spectrum with a non-zero kx-Nyquist row exercises the duplication
+    # of Dealiasing.py:70,73
+    pad = ops.Dealias1(P.asdev(ref, complex_=True))
+    rpad = O.Dealias1(g, ref)
+    assert rel_err(host(pad), rpad) < tol
+    rng = np.random.default_rng(3)
+    Fp = rng.standard_normal((g.mx, g.my, g.nz))
+    pf = ops.FFT_pad(P.asdev(Fp))
+    rpf = O.FFT_pad(Fp)
+    assert rel_err(host(pf), rpf) < tol
+    out = ops.Dealias2(P.asdev(rpf, complex_=True))
+    assert rel_err(host(out), O.Dealias2(g, rpf)) < tol
+
+
+def test_vertical_and_planar(grid, prec):
+    P, g, _ = get_plan(grid, prec)
+    tol = TOL[prec]
+    rng = np.random.default_rng(4)
+    u, v, w, TH = (smooth_field(g, s) for s in (5, 6, 7, 8))
+    ustar = 0.3 + 0.1 * rng.random((g.nx, g.ny))
+    M = 2.0 + rng.random((g.nx, g.ny))
+    fim = 1.0 + 0.2 * rng.random((g.nx, g.ny))
+    fih = 1.0 + 0.2 * rng.random((g.nx, g.ny))
+    qz = 0.1 * rng.standard_normal((g.nx, g.ny))
+    d = [P.asdev(a) for a in (u, v, w, ustar, M, fim)]
+    got = ops.Derivz_M(*d)
+    ref = O.Derivz_M(g, u, v, w, ustar, M, fim)
+    for a, b in zip(got, ref):
+        assert rel_err(host(a), b) < tol
+    assert rel_err(host(ops.Derivz_TH(P.asdev(TH), d[3], P.asdev(qz), P.asdev(fih))),
+                   O.Derivz_TH(g, TH, ustar, qz, fih)) < tol
+    assert rel_err(host(ops.Derivz_Generic_uvp(P.asdev(TH))), O.Derivz_Generic_uvp(g, TH)) < tol
+    assert rel_err(host(ops.Derivz_Generic_w(P.asdev(TH))), O.Derivz_Generic_w(g, TH)) < tol
+    assert rel_err(host(ops.PlanarMean(P.asdev(TH))), O.PlanarMean(TH)) < 1e-12 + (
+        1e-6 if prec == "mixed" else 0)
+    assert rel_err(host(ops.StagGridAvg(P.asdev(TH))), O.StagGridAvg(TH)) < tol
+    box = host(ops.Imfilter3D(P.asdev(TH)))
+    rbox = np.stack([O.Imfilter(TH[:, :, k]) for k in range(g.nz)], axis=2)
+    assert rel_err(box, rbox) < tol
+
+
+def test_advection_buoyancy(grid, prec):
+    P, g, _ = get_plan(grid, prec)
+    tol = TOL[prec] * (10 if g.optDealias else 1)
+    fs = [smooth_field(g, 10 + i, kmax=g.nx / 6) for i in range(9)]
+    got = ops.Advection(*[P.asdev(f) for f in fs])
+    ref = O.Advection(g, *fs)
+    for a, b in zip(got, ref):
+        assert rel_err(host(a), b) < tol
+    s6 = fs[:6]
+    assert rel_err(host(ops.ScalarAdvection(*[P.asdev(f) for f in s6])),
+                   O.ScalarAdvection(g, *s6)) < tol
+    TH = 2.0 * smooth_field(g, 30)
+    fn = ops.BuoyancyOpt2 if g.optBuoyancy == 1 else ops.BuoyancyOpt1
+    assert rel_err(host(fn(P.asdev(TH))), O.Buoyancy(g, TH)) < tol
+
+
+def test_div_rhs_ab2(grid, prec):
+    P, g, _ = get_plan(grid, prec)
+    tol = TOL[prec]
+    t = [smooth_field(g, 40 + i) for i in range(6)]
+    got = ops.DivStress(*[P.asdev(a) for a in t])
+    ref = O.DivStress(g, *t)
+    for a, b in zip(got, ref):
+        assert rel_err(host(a), b) < tol
+    assert rel_err(host(ops.DivFlux(*[P.asdev(a) for a in t[:3]])), O.DivFlux(g, *t[:3])) < tol
+    f = [smooth_field(g, 60 + i) for i in range(10)]
+    got = ops.RHS_Momentum(P.asdev(f[0]), P.asdev(f[1]), P.asdev(f[2]), None, None,
+                           *[P.asdev(a) for a in f[3:10]])
+    ref = O.RHS_Momentum(g, *f)
+    for a, b in zip(got, ref):
+        assert rel_err(host(a), b) < tol
+    got = ops.RHS_Scalar(*[P.asdev(a) for a in f[:3]])
+    assert rel_err(host(got), O.RHS_Scalar(g, *f[:3])) < tol
+    un = ops.AB2_uvw(*[P.asdev(a) for a in f[:9]])
+    rn = O.AB2_uvw(g, *[a.copy() for a in f[:9]])
+    for a, b in zip(un, rn):
+        assert rel_err(host(a), b) < tol
+    assert rel_err(host(ops.AB2_TH(*[P.asdev(a) for a in f[:3]])),
+                   O.AB2_TH(g, *[a.copy() for a in f[:3]])) < tol
+
+
+def test_pressure(grid, prec):
+    P, g, _ = get_plan(grid, prec)
+    tol = 1e-10 if prec == "fp64" else 1e-4
+    f = [smooth_field(g, 80 + i) for i in range(10)]
+    RCr, RCi, fRz = ops.PressureRC(*[P.asdev(a) for a in f])
+    rr, ri, rz = O.PressureRC(g, *f)
+    assert rel_err(host(RCr), rr) < TOL[prec] * 10
+    assert rel_err(host(RCi), ri) < TOL[prec] * 10
+    assert rel_err(host(fRz), rz) < TOL[prec] * 10
+    th = O.ThomasPressureInit(g)
+    got = ops.ThomasPressureSolve(P.asdev(rr), P.asdev(ri), P.asdev(rz))
+    ref = O.ThomasPressureSolve(g, rr, ri, rz, th)
+    for a, b in zip(got, ref):
+        assert rel_err(host(a), b) < tol
+
+
+def test_strains_wall(grid, prec):
+    P, g, cfg = get_plan(grid, prec)
+    tol = TOL[prec]
+    name, over = GRIDS[grid]
+    cfg0, u, v, w, TH = cases.initial_fields(name, **over)
+    s = O.initial_state(g, u, v, w, TH)
+    ser = cases.surface_series(cfg0, 10)
+    th = O.ThomasPressureInit(g)
+    s = O.step(g, s, th, None if ser is None else ser[s.iteration - 1])
+    s2 = O.step(g, s, th, None if ser is None else ser[s.iteration - 1], keep=True)
+    dg = s2.diag
+    grads = dg["grads"]
+    got = ops.StrainsUVPnodes_NoDealias(*[P.asdev(a) for a in grads])
+    ref = O.StrainsUVPnodes(g, *grads)
+    for a, b in zip(got, ref):
+        assert rel_err(host(a), b) < tol
+    got = ops.StrainsWnodes_NoDealias(*[P.asdev(a) for a in grads])
+    rw = O.StrainsWnodes(g, *grads)
+    for a, b in zip(got, (rw[4], rw[5], rw[6])):
+        assert rel_err(host(a), b) < tol
+    uf, vf = dg["u_f"], dg["v_f"]
+    M, most = dg["M_sfc_loc"], s2.MOST
+    txz, tyz = ops.Wall(P.asdev(uf), P.asdev(vf), P.asdev(M), P.asdev(most[0]), P.asdev(most[1]))
+    rx, ry = O.Wall(g, uf, vf, M, most[0], most[1])
+    assert rel_err(host(txz), rx) < tol
+    assert rel_err(host(tyz), ry) < tol
+
+
+def test_lasdd(grid, prec):
+    """LASDD / LAD on O(1) smooth random inputs (a well-conditioned beta
+    quintic; beta is the largest real root in (0, 5) and jumps between
+    roots when the quintic is ill-conditioned)."""
+    P, g, cfg = get_plan(grid, prec)
+    u, v, w = (smooth_field(g, 90 + i, kmax=g.nx / 8) for i in range(3))
+    Sij = [0.5 * smooth_field(g, 95 + i, kmax=g.nx / 6) for i in range(6)]
+    S = O._Smag(*Sij)
+    ref = O.LASDD_SM(g, u, v, w, *Sij, S)
+    cs, a1, a2, beta = ops.LASDD(*[P.asdev(a) for a in (u, v, w, *Sij, S)])
+    if prec == "fp64":
+        assert rel_err(host(beta), ref["beta1_1D"]) < 1e-9
+        assert rel_err(host(cs), ref["Cs2_3D"]) < 1e-9
+        assert rel_err(host(a1), ref["Cs2_1D_avg1"]) < 1e-9
+        assert rel_err(host(a2), ref["Cs2_1D_avg2"]) < 1e-9
+    else:
+        # float32 filtered fields: report, bound the profile of Cs2
+        eb = rel_err(host(beta), ref["beta1_1D"])
+        e2 = rel_err(host(a2), ref["Cs2_1D_avg2"])
+        print(f"{grid} mixed: beta rel err {eb:.2e}, Cs2 profile rel err {e2:.2e}")
+        assert e2 < 1e-2
+
+
+def test_surface(grid, prec):
+    P, g, cfg = get_plan(grid, prec)
+    name, over = GRIDS[grid]
+    cfg0, u, v, w, TH = cases.initial_fields(name, **over)
+    rng = np.random.default_rng(9)
+    most = tuple(0.05 * rng.standard_normal((g.nx, g.ny)) for _ in range(4)) + (
+        1 + 0.1 * rng.random((g.nx, g.ny)), 1 + 0.1 * rng.random((g.nx, g.ny)))
+    sv = -0.3 if cfg["optSurfBC"] == 2 else 0.1
+    ref = O.SurfaceFlux(g, u, v, TH, most, sv)
+    args = (P.asdev(u), P.asdev(v), P.asdev(TH))
+    md = tuple(P.asdev(m) for m in most)
+    fn = {(0, 0): ops.SurfaceFlux_HomogeneousConstantFlux,
+          (0, 1): ops.SurfaceFlux_HeterogeneousConstantFlux,
+          (2, 0): ops.SurfaceFlux_HomogeneousPrescribedTemperature,
+          (2, 1): ops.SurfaceFlux_HeterogeneousPrescribedTemperature}[(cfg["optSurfBC"],
+                                                                       cfg["optSurfFlux"])]
+    if cfg["optSurfBC"] == 0:
+        M, us, qa, iob, mo = fn(*args, md)
+    else:
+        M, us, qz, qa, iob, mo = fn(*args, sv, md)
+        assert rel_err(host(qz), ref[2]) < (1e-12 if prec == "fp64" else 1e-5)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert rel_err(host(M), ref[0]) < tol
+    assert rel_err(host(us), ref[1]) < tol
+    assert abs(qa - ref[3]) <= tol * max(1.0, abs(ref[3]))
+    assert rel_err(host(iob), ref[4]) < tol
+    for a, b in zip(mo, ref[5]):
+        assert rel_err(host(a), b) < tol
+
+
+def test_compute_beta():
+    P, g, _ = get_plan("neutral32", "fp64")
+    rng = np.random.default_rng(11)
+    co = rng.standard_normal((6, 40))
+    co[:, 0] = [0.1, 0.3, 0.6, 0.1, 2.0, 1.0]      # TEST_Root.py:169 quintic
+    co[:, 1] = 0.0                                  # zero polynomial -> 1.0
+    got = host(ops.ComputeBeta(*co))
+    ref = O.ComputeBeta(*co)
+    assert np.max(np.abs(got - ref)) < 1e-9
