@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Throughput of the composed LES time step on one MI355X.
+
+Workload: the stable boundary layer at 256^3 (400^3 m, LAD-SM dynamic
+Smagorinsky, FGR=2, Thomas pressure solve, prescribed cooling surface,
+dt = 0.05 s), mixed FP32/FP64, fields resident in HBM.  The same workload in
+plain FP64 is timed beside it.  One JSON line goes to stdout:
+  value       = whole-job point-steps/s of the mixed run, in Mpoint-steps/s
+  roofline    = the dominant kernel's algorithmic bytes / its HIP-event time,
+                measured on its launches inside the timed region
+  cpu_baseline= the FP64 oracle (NumPy, one thread) on a bounded sample
+Diagnostics (per-kernel table) go to stderr and gpurun_out/bench_kernels.json.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "jax-alfa_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per point-step of the reference data flow at FGR=2
+# (one read + one write per batched FFT pass, distinct inputs + outputs per
+# pointwise pass): 567 field passes x 4 B (FP32) or x 8 B (FP64)
+B_ALG = {"mixed": 2270.0, "fp64": 4540.0}
+# A100 80 GB, the reference's own run of this grid size (CBL_N91 256^3 LAD-SM SP),
+# SimulationLog.ipynb:661 / examples/CBL_N91/runs/256x256x256_LASDD_SM_SP/Config.py:71-72
+A100_REF = {"Mpoint_steps_per_s": 202.1, "ms_per_step": 83.0,
+            "run": "CBL_N91 256^3 LAD-SM SP (A100 80 GB)"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None):
+    import torch
+    from alfa_amd import cases, core
+    cfg, u, v, w, TH = cases.initial_fields("stable", nx=n, ny=n, nz=n)
+    ser = cases.surface_series(cfg, warmup + steps + 8)
+    P = core.Plan(cfg, prec)
+    les = core.LES(P, u, v, w, TH, sfc_series=ser)
+    del u, v, w, TH
+    log(f"[bench] {prec} {n}^3 plan ready, workspace {P.derived['workspace_bytes'] / 2**30:.2f} GiB")
+    les.step(warmup)
+    torch.cuda.synchronize()
+    table = None
+    if find_dominant:
+        P.profile_begin(None, 16384)
+        les.step(2)
+        table = P.profile_end()
+        table.sort(key=lambda r: -r["total_ms"])
+        dominant = table[0]["name"]
+    P.profile_begin(dominant, 8192)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    les.step(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    live = P.profile_end()
+    d = les.diag()
+    cfl = d["CFL"]
+    P.close()
+    del les
+    torch.cuda.empty_cache()
+    ms = (t1 - t0) * 1e3 / steps
+    return dict(ms=ms, mpts=(n ** 3) / (ms * 1e-3) / 1e6, table=table, dominant=dominant,
+                live=live, cfl=cfl)
+
+
+def cpu_baseline(budget_s=15.0, n=64):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import alfa_oracle as O
+    from alfa_amd import cases
+    cfg, u, v, w, TH = cases.initial_fields("stable", nx=n, ny=n, nz=n)
+    g = O.Grid(cfg)
+    s = O.initial_state(g, u, v, w, TH)
+    ser = cases.surface_series(cfg, 400)
+    th = O.ThomasPressureInit(g)
+    s = O.step(g, s, th, ser[0])          # first step (startup costs) untimed
+    k = 0
+    t0 = time.perf_counter()
+    while True:
+        s = O.step(g, s, th, ser[s.iteration - 1])
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 200:
+            break
+    return dict(value=n ** 3 * k / el / 1e6, unit="Mpoint-steps/s", cores=1, kind="port",
+                sample=f"FP64 NumPy oracle, stable case at {n}^3 (LAD-SM, FGR=2), {k} steps "
+                       f"in {el:.1f} s after one untimed step")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the per-member step does not shard across GPUs in this "
+              "build (ensembles shard by member; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n = args.grid
+    mixed = run_gpu("mixed", n, args.steps, args.warmup)
+    log(f"[bench] mixed: {mixed['ms']:.3f} ms/step, {mixed['mpts']:.1f} Mpoint-steps/s, "
+        f"CFL {mixed['cfl']:.3f}")
+    for r in (mixed["table"] or [])[:25]:
+        bw = r["bytes"] / (r["total_ms"] * 1e-3) / 1e9 if r["total_ms"] > 0 else 0
+        log(f"  {r['name']:40s} {r['launches']:5d} launches {r['total_ms'] / 2:9.3f} ms/step "
+            f"{bw:8.0f} GB/s")
+    fp64 = None
+    if not args.no_fp64:
+        fs = args.fp64_steps or max(3, args.steps // 2)
+        fp64 = run_gpu("fp64", n, fs, max(2, args.warmup // 2), find_dominant=True)
+        log(f"[bench] fp64: {fp64['ms']:.3f} ms/step, {fp64['mpts']:.1f} Mpoint-steps/s")
+    live = {r["name"]: r for r in mixed["live"]}
+    dom = live.get(mixed["dominant"])
+    roof = None
+    if dom and dom["total_ms"] > 0:
+        ach = dom["bytes"] / (dom["total_ms"] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=mixed["dominant"],
+                    launches=dom["launches"],
+                    avg_launch_us=round(dom["total_ms"] * 1e3 / dom["launches"], 2),
+                    bytes_per_launch=dom["bytes"] / dom["launches"])
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": f"Mpoint-steps/s at {n}^3 (FP32-mixed vs FP64)",
+        "value": round(mixed["mpts"], 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, f64 reductions/pressure/beta (mixed)",
+        "data": "synthetic: seeded GABLS1-like stable layer (cases.initial_fields('stable'))",
+        "config": {"workload": f"stable boundary layer {n}^3, 400^3 m, LAD-SM, FGR=2, "
+                               f"Thomas pressure, dt=0.05 s, mixed precision",
+                   "grid": [n, n, n], "optSgs": 3, "FGR": 2, "precision": "mixed"},
+        "hbm_frac_step": round(mixed["mpts"] * 1e6 * B_ALG["mixed"] / (HBM_PEAK_GBS * 1e9), 4),
+        "b_alg_bytes_per_point_step": B_ALG["mixed"],
+        "fp64": None if fp64 is None else {
+            "value": round(fp64["mpts"], 2), "ms_per_step": round(fp64["ms"], 4),
+            "hbm_frac_step": round(fp64["mpts"] * 1e6 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9), 4)},
+        "mixed_over_fp64": None if fp64 is None else round(mixed["mpts"] / fp64["mpts"], 3),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "reference_a100": A100_REF,
+    }
+    try:
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        with open(os.path.join(ROOT, "gpurun_out", "bench_kernels.json"), "w") as f:
+            json.dump(dict(mixed=mixed["table"], fp64=None if fp64 is None else fp64["table"],
+                           mixed_live=mixed["live"]), f, indent=1)
+    except OSError:
+        pass
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

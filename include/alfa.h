@@ -232,6 +232,24 @@ int alfa_surface_flux(alfa_plan*, const void* u, const void* v, const void* TH,
                       void* ustar, void* qz_sfc, void* invOB, double* qz_sfc_avg,
                       alfa_stream);
 
+/* ---- measurement ------------------------------------------------------------
+ * No counterpart in the reference (which only prints wall time,
+ * src/Main.py:562-564,637).  Between begin and end every launch whose kernel
+ * name equals `name_filter` (all launches when it is NULL or "") is bracketed
+ * by HIP events on its stream; end synchronises on the last event and
+ * returns, per kernel name, the launch count, the summed event time and the
+ * summed algorithmic bytes (distinct inputs + outputs of each launch).
+ * begin creates the events (at most max_launches records), so it belongs
+ * outside any captured region. */
+typedef struct alfa_kernel_stat {
+  char name[96];
+  int64_t launches;
+  double total_ms;
+  double bytes;
+} alfa_kernel_stat;
+int alfa_profile_begin(alfa_plan* plan, const char* name_filter, int max_launches);
+int alfa_profile_end(alfa_plan* plan, alfa_kernel_stat* out, int max_out, int* n_out);
+
 #ifdef __cplusplus
 }
 #endif

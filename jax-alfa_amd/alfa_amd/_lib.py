@@ -39,6 +39,11 @@ class AlfaDerived(C.Structure):
         + [("workspace_bytes", C.c_int64)])
 
 
+class AlfaKernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 96), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("bytes", C.c_double)]
+
+
 STATE_FIELDS = ["u", "v", "w", "TH", "RHS_u", "RHS_v", "RHS_w", "RHS_TH", "Cs2_3D",
                 "Cs2PrRatio_3D", "psi_m", "psi_m0", "psi_h", "psi_h0", "fi_m", "fi_h"]
 STATE_DIAG = ["Cs2_1D_avg1", "Cs2_1D_avg2", "beta1_1D", "Cs2PrRatio_1D", "beta2_1D", "scalars"]
@@ -102,6 +107,8 @@ _SIGS = {
     "alfa_lasdd_sm": [_P, _P, _P, _P, C.POINTER(C.c_void_p), _P, _P, _P, _P, _P],
     "alfa_wall": [_P] * 9,
     "alfa_surface_flux": [_P, _P, _P, _P, C.c_double] + [_P] * 12,
+    "alfa_profile_begin": [_P, C.c_char_p, C.c_int],
+    "alfa_profile_end": [_P, C.POINTER(AlfaKernelStat), C.c_int, C.POINTER(C.c_int)],
 }
 
 _lib = None

@@ -109,6 +109,20 @@ class Plan:
             raise TypeError(f"tensor dtype {t.dtype} does not match plan precision {self.precision}")
         return C.c_void_p(t.data_ptr())
 
+    def profile_begin(self, name_filter: Optional[str] = None, max_launches: int = 8192):
+        """Bracket matching launches with HIP events (see include/alfa.h)."""
+        f = None if not name_filter else name_filter.encode()
+        _lib.check(self.lib.alfa_profile_begin(self._p, f, int(max_launches)), "alfa_profile_begin")
+
+    def profile_end(self) -> list:
+        """-> [dict(name, launches, total_ms, bytes)] per kernel name."""
+        arr = (_lib.AlfaKernelStat * 256)()
+        n = C.c_int(0)
+        _lib.check(self.lib.alfa_profile_end(self._p, arr, 256, C.byref(n)), "alfa_profile_end")
+        return [dict(name=arr[i].name.decode(), launches=int(arr[i].launches),
+                     total_ms=float(arr[i].total_ms), bytes=float(arr[i].bytes))
+                for i in range(min(n.value, 256))]
+
     def call(self, name, *args):
         f = getattr(self.lib, name)
         _lib.check(f(self._p, *args), name)

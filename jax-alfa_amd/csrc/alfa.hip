@@ -125,9 +125,14 @@ int filtering_explicit(alfa_plan* P, hipStream_t s, const void* Ffft, void* F, v
   xa.op[0] = xop(c.nyc, 0, d.cut0_x, XM_NONE, XMAP_ID, c.nx);
   xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = c.nz; xa.in_spectral = 1; xa.ny_full = c.ny;
   constexpr int L = Lines<T>::L;
-  dim3 grid((c.nz + L - 1) / L, d.cut0_y, 1);
+  TileGrid tg{(c.nz + L - 1) / L, d.cut0_y, 1, 0};
+  const unsigned grid = (unsigned)tg.ntz * tg.rows;
   const size_t lds = (size_t)c.nx * L * sizeof(c2<T>) + 2 * c.nx * sizeof(c2<T>);
-  k_x<T><<<grid, FFT_BS, lds, s>>>(xa, P->fx, P->fx, (const c2<T>*)P->tw_x, (const c2<T>*)P->tw_x);
+  if (c.nx <= 256)
+    k_x<T, 256><<<grid, FFT_BS, lds, s>>>(xa, P->fx, P->fx, (const c2<T>*)P->tw_x, (const c2<T>*)P->tw_x, tg);
+  else
+    k_x<T, FFT_MAXN><<<grid, FFT_BS, lds, s>>>(xa, P->fx, P->fx, (const c2<T>*)P->tw_x,
+                                               (const c2<T>*)P->tw_x, tg);
   launch_ok(P);
   spectral_to_real<T>(c, Fnew, d.cut0_y, BIGCUT, XM_NONE, (T*)F);
   return 0;
@@ -420,8 +425,14 @@ template <class T> static int allocate_workspace(alfa_plan* P) {
   return ALFA_OK;
 }
 
+static void free_prof(alfa_plan* P) {
+  for (hipEvent_t e : P->pf.pool) (void)hipEventDestroy(e);
+  P->pf = Prof();
+}
+
 static void free_plan(alfa_plan* P) {
   if (!P) return;
+  free_prof(P);
   for (void* p : P->allocs) (void)hipFree(p);
   delete P;
 }
@@ -821,4 +832,53 @@ extern "C" int alfa_surface_flux(alfa_plan* P, const void* u, const void* v, con
   DISPATCH(P, "alfa_surface_flux",
            ops::surface<T>(P, STREAM(s), u, v, TH, sfc_val, pm, pm0, ph, ph0, fm, fh, M, ustar, qz,
                            invOB, qavg));
+}
+
+extern "C" int alfa_profile_begin(alfa_plan* P, const char* filter, int max_launches) {
+  if (!P) return set_error(ALFA_EINVAL, "alfa_profile_begin: null plan");
+  if (max_launches <= 0) return set_error(ALFA_EINVAL, "alfa_profile_begin: max_launches <= 0");
+  free_prof(P);
+  P->pf.pool.resize(2 * (size_t)max_launches);
+  for (auto& e : P->pf.pool) {
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) {
+      P->pf.pool.clear();
+      return set_error(ALFA_EHIP, "hipEventCreate: %s", hipGetErrorString(r));
+    }
+  }
+  P->pf.filter = filter ? filter : "";
+  P->pf.on = true;
+  return ALFA_OK;
+}
+
+extern "C" int alfa_profile_end(alfa_plan* P, alfa_kernel_stat* out, int max_out, int* n_out) {
+  if (!P) return set_error(ALFA_EINVAL, "alfa_profile_end: null plan");
+  P->pf.on = false;
+  std::vector<std::string> names;
+  std::vector<alfa_kernel_stat> acc;
+  if (!P->pf.recs.empty()) {
+    hipError_t r = hipEventSynchronize(P->pf.recs.back().b);
+    if (r != hipSuccess) return set_error(ALFA_EHIP, "hipEventSynchronize: %s", hipGetErrorString(r));
+  }
+  for (const ProfRec& rec : P->pf.recs) {
+    float ms = 0.f;
+    hipError_t r = hipEventElapsedTime(&ms, rec.a, rec.b);
+    if (r != hipSuccess) return set_error(ALFA_EHIP, "hipEventElapsedTime: %s", hipGetErrorString(r));
+    size_t k = 0;
+    while (k < names.size() && names[k] != rec.name) ++k;
+    if (k == names.size()) {
+      names.push_back(rec.name);
+      alfa_kernel_stat z{};
+      snprintf(z.name, sizeof z.name, "%s", rec.name.c_str());
+      acc.push_back(z);
+    }
+    acc[k].launches += 1;
+    acc[k].total_ms += ms;
+    acc[k].bytes += rec.bytes;
+  }
+  const int n = (int)acc.size() < max_out ? (int)acc.size() : max_out;
+  for (int i = 0; i < n; ++i) out[i] = acc[i];
+  if (n_out) *n_out = (int)acc.size();
+  free_prof(P);
+  return ALFA_OK;
 }

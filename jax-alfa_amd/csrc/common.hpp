@@ -37,6 +37,18 @@ struct Fft1D {
   int rad[FFT_MAXRAD];
 };
 
+// MI355X dispatches workgroup i to XCD (i mod 8), and each XCD has its own
+// L2.  xcd_logical() renumbers the launch so that every XCD receives one
+// contiguous range of logical workgroups: tiles that share HBM rows (the z
+// tiles of one transform line set, or the fields of one tile) then run on
+// the same L2 at about the same time.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ unsigned xcd_logical(unsigned lin, unsigned total) {
+  const unsigned xcd = lin % NUM_XCD, idx = lin / NUM_XCD;
+  const unsigned per = total / NUM_XCD, rem = total % NUM_XCD;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
 // Maximum number of distinct fields one batched launch handles.
 constexpr int MAXB = 24;
 

@@ -77,9 +77,9 @@ template <class T> struct Dft<T, 5> {
 // One Stockham pass of radix R over LPB interleaved lines held in LDS
 // (element (j, line p) at buf[j*LPB + p]).  All inputs of the pass are read
 // into registers before any output is written, so the pass runs in place.
-template <class T, int R, int LPB>
+template <class T, int R, int LPB, int MAXN>
 __device__ __forceinline__ void fft_pass(c2<T>* buf, const c2<T>* tw, int n, int Ns, int sign) {
-  constexpr int MAXIT = (FFT_MAXN / R * LPB + FFT_BS - 1) / FFT_BS;
+  constexpr int MAXIT = (MAXN / R * LPB + FFT_BS - 1) / FFT_BS;
   c2<T> v[MAXIT][R];
   const int nR = n / R;
   const int total = nR * LPB;
@@ -118,16 +118,18 @@ __device__ __forceinline__ void fft_pass(c2<T>* buf, const c2<T>* tw, int n, int
   __syncthreads();
 }
 
-// Full transform of LPB lines of length f.n (tw: n-entry table exp(-2 pi i t/n)).
-template <class T, int LPB>
+// Full transform of LPB lines of length f.n <= MAXN (tw: n-entry table
+// exp(-2 pi i t/n)).  MAXN sizes the per-thread register arrays, so kernels
+// are instantiated per length tier (<= 256 and <= 384).
+template <class T, int LPB, int MAXN>
 __device__ void fft_lines(c2<T>* buf, const c2<T>* tw, const Fft1D& f, int sign) {
   int Ns = 1;
   for (int s = 0; s < f.nrad; ++s) {
     const int R = f.rad[s];
-    if (R == 4) fft_pass<T, 4, LPB>(buf, tw, f.n, Ns, sign);
-    else if (R == 2) fft_pass<T, 2, LPB>(buf, tw, f.n, Ns, sign);
-    else if (R == 3) fft_pass<T, 3, LPB>(buf, tw, f.n, Ns, sign);
-    else fft_pass<T, 5, LPB>(buf, tw, f.n, Ns, sign);
+    if (R == 4) fft_pass<T, 4, LPB, MAXN>(buf, tw, f.n, Ns, sign);
+    else if (R == 2) fft_pass<T, 2, LPB, MAXN>(buf, tw, f.n, Ns, sign);
+    else if (R == 3) fft_pass<T, 3, LPB, MAXN>(buf, tw, f.n, Ns, sign);
+    else fft_pass<T, 5, LPB, MAXN>(buf, tw, f.n, Ns, sign);
     Ns *= R;
   }
 }
@@ -140,16 +142,38 @@ struct PtrBatch {
 // y-direction R2C.  grid = (ceil(nz/ZT), rows, batch); row x of field b is
 // read through ld(b, x, y, z); output (rows, ncy_out, nz) complex.
 // ----------------------------------------------------------------------------
-template <class T, class Loader>
+// Launch geometry of the tiled kernels: a 1-D grid of ntz * rows * nb
+// workgroups, renumbered XCD-contiguously; with batch_fast the fields of one
+// (row, z tile) are adjacent (they share input tiles through L2), otherwise
+// the z tiles of one row are (they share HBM rows).
+struct TileGrid {
+  int ntz, rows, nb, batch_fast;
+  __device__ __forceinline__ void decode(int& zt, int& row, int& b) const {
+    const unsigned total = (unsigned)ntz * rows * nb;
+    unsigned l = xcd_logical(blockIdx.x, total);
+    if (batch_fast) {
+      b = l % nb; l /= nb;
+      zt = l % ntz; row = l / ntz;
+    } else {
+      zt = l % ntz; l /= ntz;
+      row = l % rows; b = l / rows;
+    }
+  }
+};
+
+template <class T, class Loader, int MAXN>
 __global__ void __launch_bounds__(FFT_BS)
-k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb) {
+k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
+        TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = fy.n;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
-  const int z0 = blockIdx.x * ZT, x = blockIdx.y, b = blockIdx.z;
+  int zt, x, b;
+  tg.decode(zt, x, b);
+  const int z0 = zt * ZT;
   for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
   T* bufr = reinterpret_cast<T*>(buf);
   for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
@@ -158,7 +182,7 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
     bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
   }
   __syncthreads();
-  fft_lines<T, L>(buf, stw, fy, -1);
+  fft_lines<T, L, MAXN>(buf, stw, fy, -1);
   c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
   for (int i = threadIdx.x; i < ncy_out * ZT; i += FFT_BS) {
     const int k = i / ZT, zl = i - k * ZT;
@@ -201,19 +225,36 @@ struct XArgs {
   int ny_full;       // regular-grid ny (ky Nyquist column = ny_full/2)
 };
 
+// value of op `op` at input mode k (element i of line p) of a forward spectrum
 template <class T>
+__device__ __forceinline__ c2<T> x_apply(const XOp& op, c2<T> v, int k, int nin, int kyv) {
+  const bool keep = (k < op.cx) || (k >= nin - op.cx + 1);
+  if (!keep) return mkc<T>(0, 0);
+  if (op.mult == XM_IKX) {
+    int kx = (k < nin / 2) ? k : k - nin;
+    if (2 * k == nin) kx = 0;
+    v = cscale<T>(cmul_i<T>(v, 1), (T)kx);
+  } else if (op.mult == XM_IKY) {
+    v = cscale<T>(cmul_i<T>(v, 1), (T)kyv);
+  }
+  return cscale<T>(v, (T)op.scale);
+}
+
+template <class T, int MAXN>
 __global__ void __launch_bounds__(FFT_BS)
 k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
-    const c2<T>* __restrict__ tw_out) {
+    const c2<T>* __restrict__ tw_out, TileGrid tg) {
   constexpr int L = Lines<T>::L;
-  constexpr int MAXE = FFT_MAXN * L / FFT_BS;
+  constexpr int MAXE = MAXN * L / FFT_BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nin = fin.n;
   const int nbuf = imax(nin, fout.n);
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw_in = buf + nbuf * L;
   c2<T>* stw_out = stw_in + nin;
-  const int z0 = blockIdx.x * L, ky = blockIdx.y, b = blockIdx.z;
+  int zt, ky, b;
+  tg.decode(zt, ky, b);
+  const int z0 = zt * L;
   const int nz = a.nz;
   for (int i = threadIdx.x; i < nin; i += FFT_BS) stw_in[i] = tw_in[i];
   for (int i = threadIdx.x; i < fout.n; i += FFT_BS) stw_out[i] = tw_out[i];
@@ -224,15 +265,30 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     buf[i] = (z < nz) ? in[((size_t)k * a.ncy_in + ky) * nz + z] : mkc<T>(0, 0);
   }
   __syncthreads();
-  if (!a.in_spectral) fft_lines<T, L>(buf, stw_in, fin, -1);
-  c2<T> spec[MAXE];
+  if (!a.in_spectral) fft_lines<T, L, MAXN>(buf, stw_in, fin, -1);
+  const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
   const int nEl = nin * L;
+  if (a.nops == 1 && a.op[0].map == XMAP_ID) {
+    // one output: apply the op in place, no register copy of the spectrum
+    const XOp op = a.op[0];
+    if (ky >= op.ncy) return;
+    for (int i = threadIdx.x; i < nEl; i += FFT_BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
+    __syncthreads();
+    if (op.inverse) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
+    c2<T>* dst = reinterpret_cast<c2<T>*>(a.dst[b][0]);
+    for (int i = threadIdx.x; i < nin * L; i += FFT_BS) {
+      const int k = i / L, p = i - k * L;
+      const int z = z0 + p;
+      if (z < nz) dst[((size_t)k * op.ncy + ky) * nz + z] = buf[i];
+    }
+    return;
+  }
+  c2<T> spec[MAXE];
 #pragma unroll
   for (int q = 0; q < MAXE; ++q) {
     const int i = threadIdx.x + q * FFT_BS;
     spec[q] = (i < nEl) ? buf[i] : mkc<T>(0, 0);
   }
-  const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
   for (int o = 0; o < a.nops; ++o) {
     const XOp op = a.op[o];
     if (ky >= op.ncy) continue;
@@ -242,23 +298,12 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
       for (int i = threadIdx.x; i < nout * L; i += FFT_BS) buf[i] = mkc<T>(0, 0);
       __syncthreads();
     }
-    const T sc = (T)op.scale;
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
       const int i = threadIdx.x + q * FFT_BS;
       if (i < nEl) {
         const int k = i / L, p = i - k * L;
-        c2<T> v = spec[q];
-        const bool keep = (k < op.cx) || (k >= nin - op.cx + 1);
-        if (!keep) v = mkc<T>(0, 0);
-        if (op.mult == XM_IKX) {
-          int kx = (k < nin / 2) ? k : k - nin;
-          if (2 * k == nin) kx = 0;
-          v = cscale<T>(cmul_i<T>(v, 1), (T)kx);
-        } else if (op.mult == XM_IKY) {
-          v = cscale<T>(cmul_i<T>(v, 1), (T)kyv);
-        }
-        v = cscale<T>(v, sc);
+        const c2<T> v = x_apply<T>(op, spec[q], k, nin, kyv);
         if (op.map == XMAP_ID) {
           buf[i] = v;
         } else if (op.map == XMAP_D1) {
@@ -273,8 +318,8 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     }
     __syncthreads();
     if (op.inverse) {
-      if (op.map == XMAP_ID) fft_lines<T, L>(buf, stw_in, fin, +1);
-      else fft_lines<T, L>(buf, stw_out, fout, +1);
+      if (op.map == XMAP_ID) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
+      else fft_lines<T, L, MAXN>(buf, stw_out, fout, +1);
     }
     c2<T>* dst = reinterpret_cast<c2<T>*>(a.dst[b][o]);
     for (int i = threadIdx.x; i < nout * L; i += FFT_BS) {
@@ -298,16 +343,18 @@ struct YC2RArgs {
   double scale;
 };
 
-template <class T, class Storer>
+template <class T, class Storer, int MAXN>
 __global__ void __launch_bounds__(FFT_BS)
-k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw) {
+k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = fy.n, half = n / 2;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
-  const int z0 = blockIdx.x * ZT, x = blockIdx.y, b = blockIdx.z;
+  int zt, x, b;
+  tg.decode(zt, x, b);
+  const int z0 = zt * ZT;
   const int nz = a.nz;
   for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
   for (int i = threadIdx.x; i < n * L; i += FFT_BS) buf[i] = mkc<T>(0, 0);
@@ -328,7 +375,7 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw) {
     if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
   }
   __syncthreads();
-  fft_lines<T, L>(buf, stw, fy, +1);
+  fft_lines<T, L, MAXN>(buf, stw, fy, +1);
   const T* bufr = reinterpret_cast<const T*>(buf);
   const T sc = (T)a.scale;
   for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
@@ -342,6 +389,9 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw) {
 // plain load / store functors
 // ----------------------------------------------------------------------------
 template <class T> struct LoadPtr {
+  static constexpr const char* NAME = "LoadPtr";
+  static constexpr int BATCH_FAST = 0;
+  static int nin(int nb) { return nb; }
   const T* p[MAXB];
   int ny, nz;
   __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
@@ -350,6 +400,8 @@ template <class T> struct LoadPtr {
 };
 
 template <class T> struct StorePtr {
+  static constexpr const char* NAME = "StorePtr";
+  static constexpr int NOUT = 1;
   T* p[MAXB];
   int ny, nz;
   __device__ __forceinline__ void operator()(int b, int x, int y, int z, T v) const {

@@ -14,6 +14,9 @@
 // a shift because the cut-off filters keep the (0,0) mode, and the shift
 // removes the cancellation between the two terms in single precision.
 template <class T> struct LoadLasdd {
+  static constexpr const char* NAME = "LoadLasdd";
+  static constexpr int BATCH_FAST = 1;
+  static int nin(int) { return 10; }
   const T* u; const T* v; const T* w;
   const T* S[6]; const T* Sm;
   const double* shu; const double* shv;   // may be NULL
@@ -49,6 +52,9 @@ template <class T> struct LoadLasdd {
 // THz is dTHdz averaged to uvp nodes (:182-186).  Same shift as above for
 // u_, v_ and TH (the scalar Leonard flux is shift invariant too).
 template <class T> struct LoadScalar {
+  static constexpr const char* NAME = "LoadScalar";
+  static constexpr int BATCH_FAST = 1;
+  static int nin(int) { return 8; }
   const T* u; const T* v; const T* w; const T* TH;
   const T* dx; const T* dy; const T* dz; const T* Sm;
   const double* shu; const double* shv; const double* sht;
@@ -84,6 +90,8 @@ template <class T> struct LoadScalar {
 // the z-derivative of one field (Derivz_Generic_w for txz, tyz, qz;
 // Derivz_Generic_uvp for tzz), NSE_SGSTerms.py:86-102, SCL_SGSTerms.py:82-87.
 template <class T> struct StoreDiv {
+  static constexpr const char* NAME = "StoreDiv";
+  static constexpr int NOUT = 2;   // the output and the z-differenced field
   T* out[4]; const T* fz[4]; int wnode[4];
   int ny, nz; T idz;
   __device__ __forceinline__ void operator()(int b, int x, int y, int z, T v) const {

@@ -1,6 +1,7 @@
 // Plan object and launch helpers shared by the step driver and the
 // per-operation entry points.
 #pragma once
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -34,6 +35,21 @@ struct Workspace {
   }
 };
 
+// Optional per-launch timing with HIP events (alfa_profile_begin/end).  A
+// record holds the launch's algorithmic bytes (distinct inputs + outputs).
+struct ProfRec {
+  std::string name;
+  double bytes;
+  hipEvent_t a, b;
+};
+struct Prof {
+  bool on = false;
+  std::string filter;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<ProfRec> recs;
+};
+
 struct alfa_plan {
   alfa_config cfg;
   alfa_derived d;
@@ -45,6 +61,7 @@ struct alfa_plan {
   double* prof = nullptr;                     // Ug, Vg, rf, rs: 4*nz
   double* scal = nullptr;                     // 8 scalars
   Workspace ws;
+  Prof pf;
   std::vector<void*> allocs;
   hipError_t herr = hipSuccess;
 };
@@ -56,6 +73,43 @@ inline bool launch_ok(alfa_plan* P) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess && P->herr == hipSuccess) P->herr = e;
   return e == hipSuccess;
+}
+
+struct ProfScope {
+  alfa_plan* P;
+  hipStream_t s;
+  long idx = -1;
+  ProfScope(alfa_plan* P_, hipStream_t s_, const std::string& name, double bytes) : P(P_), s(s_) {
+    Prof& f = P->pf;
+    if (!f.on || P->ws.dry) return;
+    if (!f.filter.empty() && f.filter != name) return;
+    if (f.used + 2 > f.pool.size()) return;
+    ProfRec r{name, bytes, f.pool[f.used], f.pool[f.used + 1]};
+    f.used += 2;
+    (void)hipEventRecord(r.a, s);
+    idx = (long)f.recs.size();
+    f.recs.push_back(r);
+  }
+  ~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(P->pf.recs[idx].b, s);
+  }
+};
+
+inline size_t esize(const alfa_plan* P) { return P->prec == ALFA_FP64 ? 8 : 4; }
+
+// full-field passes (reads + writes) of each pointwise kernel of the step
+inline int pw_fields(const char* name) {
+  static const struct { const char* n; int f; } tab[] = {
+      {"k_derivz", 8}, {"k_derivz_generic", 2}, {"k_stag_avg", 2}, {"k_advection", 12},
+      {"k_scalar_adv", 7}, {"k_buoyancy", 2}, {"k_strains_uvp", 13}, {"k_strains_w", 12},
+      {"k_smag", 7}, {"k_stress_uvp", 10}, {"k_stress_w", 6}, {"k_scalar_flux", 9},
+      {"k_rhs", 17}, {"k_ab2_plain", 4}, {"k_imfilter", 2}, {"k_lasdd_lm", 24},
+      {"k_smag_level", 7}, {"k_scalar_updn", 16}, {"k_real_part", 3}, {"k_fill", 1}};
+  for (const auto& e : tab) {
+    const size_t l = strlen(e.n);
+    if (strncmp(name, e.n, l) == 0 && (name[l] == '<' || name[l] == 0)) return e.f;
+  }
+  return 0;
 }
 
 inline unsigned pw_blocks(size_t n) {
@@ -75,8 +129,16 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   PtrBatch pb{};
   for (int b = 0; b < nb; ++b) pb.p[b] = outs[b];
   constexpr int ZT = 2 * Lines<T>::L;
-  dim3 grid((nz + ZT - 1) / ZT, rows, nb);
-  k_y_r2c<T, Loader><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb);
+  TileGrid tg{(nz + ZT - 1) / ZT, rows, nb, Loader::BATCH_FAST};
+  const unsigned grid = (unsigned)tg.ntz * rows * nb;
+  const double cells = (double)rows * fyy.n * nz;
+  ProfScope ps(P, s, std::string("k_y_r2c<") + Loader::NAME + ">",
+               Loader::nin(nb) * cells * sizeof(T) +
+                   (double)nb * rows * ncy * nz * sizeof(c2<T>));
+  if (fyy.n <= 256)
+    k_y_r2c<T, Loader, 256><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg);
+  else
+    k_y_r2c<T, Loader, FFT_MAXN><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg);
   launch_ok(P);
 }
 
@@ -85,10 +147,18 @@ void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a, int nb, const Fft1D& fi
             const Fft1D& fout, const void* twin, const void* twout) {
   if (P->ws.dry) return;
   constexpr int L = Lines<T>::L;
-  dim3 grid((a.nz + L - 1) / L, a.ncy_in, nb);
+  TileGrid tg{(a.nz + L - 1) / L, a.ncy_in, nb, 0};
+  const unsigned grid = (unsigned)tg.ntz * tg.rows * nb;
   const int nbuf = imax(fin.n, fout.n);
   const size_t lds = (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
-  k_x<T><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout);
+  double bytes = (double)nb * fin.n * a.ncy_in * a.nz * sizeof(c2<T>);
+  for (int o = 0; o < a.nops; ++o)
+    bytes += (double)nb * a.op[o].nout * imin(a.op[o].ncy, a.ncy_in) * a.nz * sizeof(c2<T>);
+  ProfScope ps(P, s, "k_x", bytes);
+  if (nbuf <= 256)
+    k_x<T, 256><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+  else
+    k_x<T, FFT_MAXN><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
   launch_ok(P);
 }
 
@@ -97,8 +167,15 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a, int
            const Fft1D& fyy, const void* tw) {
   if (P->ws.dry) return;
   constexpr int ZT = 2 * Lines<T>::L;
-  dim3 grid((a.nz + ZT - 1) / ZT, rows, nb);
-  k_y_c2r<T, Storer><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw);
+  TileGrid tg{(a.nz + ZT - 1) / ZT, rows, nb, 0};
+  const unsigned grid = (unsigned)tg.ntz * rows * nb;
+  ProfScope ps(P, s, std::string("k_y_c2r<") + Storer::NAME + ">",
+               (double)nb * a.nsum * rows * a.ncy_in * a.nz * sizeof(c2<T>) +
+                   (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T));
+  if (fyy.n <= 256)
+    k_y_c2r<T, Storer, 256><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+  else
+    k_y_c2r<T, Storer, FFT_MAXN><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw, tg);
   launch_ok(P);
 }
 
@@ -110,16 +187,35 @@ void plane_sums(alfa_plan* P, hipStream_t s, int ncol, int nz, const F& f, doubl
   double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * NT * nz * sizeof(double));
   if (P->ws.dry) return;
   dim3 grid(g.nbz, g.nbc);
-  k_plane_partial<NT, F><<<grid, RED_BS, 0, s>>>(g, f, part);
+  {
+    ProfScope ps(P, s, std::string("k_plane_partial<") + F::NAME + ">",
+                 (double)F::NFIELDS * ncol * nz * esize(P));
+    k_plane_partial<NT, F><<<grid, RED_BS, 0, s>>>(g, f, part);
+  }
   launch_ok(P);
-  k_plane_final<<<(NT * nz + 255) / 256, 256, 0, s>>>(g.nbc, NT, nz, part, out, scale);
+  k_plane_final<<<(NT * nz + 3) / 4, 256, 0, s>>>(g.nbc, NT, nz, part, out, scale);
   launch_ok(P);
 }
 
 #define PW_LAUNCH(P, s, n, kern, ...)                                     \
   do {                                                                    \
     if (!(P)->ws.dry) {                                                   \
-      kern<<<pw_blocks(n), 256, 0, s>>>(__VA_ARGS__);                     \
+      {                                                                   \
+        ProfScope ps_(P, s, #kern, (double)pw_fields(#kern) * (n) * esize(P)); \
+        kern<<<pw_blocks(n), 256, 0, s>>>(__VA_ARGS__);                   \
+      }                                                                   \
+      launch_ok(P);                                                       \
+    }                                                                     \
+  } while (0)
+
+// Single kernel launch with a profiling record.
+#define KLAUNCH(P, s, name, bytes, kern, grid, block, ...)                \
+  do {                                                                    \
+    if (!(P)->ws.dry) {                                                   \
+      {                                                                   \
+        ProfScope ps_(P, s, name, bytes);                                 \
+        kern<<<grid, block, 0, s>>>(__VA_ARGS__);                         \
+      }                                                                   \
       launch_ok(P);                                                       \
     }                                                                     \
   } while (0)

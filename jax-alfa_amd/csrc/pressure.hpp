@@ -162,6 +162,9 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
 // R = RHS - RHS_prev/3 + 2u/(3 dt) (NSE_PressureTerms.py:140-142) as an
 // R2C loader: field b in {0,1,2} = x, y, z.
 template <class T> struct LoadPressureR {
+  static constexpr const char* NAME = "LoadPressureR";
+  static constexpr int BATCH_FAST = 1;
+  static int nin(int nb) { return 3 * nb; }
   const T* R[3]; const T* Rp[3]; const T* U[3];
   int ny, nz; T third, c2dt;
   __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
@@ -172,6 +175,9 @@ template <class T> struct LoadPressureR {
 
 // Two horizontal planes of one field (levels 0 and nz-1) as a 2-level field.
 template <class T> struct LoadPlanes2 {
+  static constexpr const char* NAME = "LoadPlanes2";
+  static constexpr int BATCH_FAST = 0;
+  static int nin(int nb) { return nb; }   // 2 of nz levels, counted over the 2-level grid
   const T* F; int ny, nz;
   __device__ __forceinline__ T operator()(int, int x, int y, int z) const {
     return F[((size_t)x * ny + y) * nz + (z == 0 ? 0 : nz - 1)];

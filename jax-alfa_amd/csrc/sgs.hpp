@@ -60,27 +60,34 @@ __global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double
   }
 }
 
-// out[t*nz + z] = sum_b part[b][t][z] * scale
+// out[t*nz + z] = sum_b part[b][t][z] * scale; one wavefront per output,
+// lane-strided partial sums and a fixed shuffle tree (deterministic).
 __global__ void k_plane_final(int nbc, int nt, int nz, const double* __restrict__ part,
                               double* out, double scale) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   if (i >= nt * nz) return;
   const int t = i / nz, z = i % nz;
   double s = 0.0;
-  for (int b = 0; b < nbc; ++b) s += part[((size_t)b * nt + t) * nz + z];
-  out[i] = s * scale;
+  for (int b = lane; b < nbc; b += 64) s += part[((size_t)b * nt + t) * nz + z];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) out[i] = s * scale;
 }
 
 // ---------------------------------------------------------------------------
 // plane-sum functors
 // ---------------------------------------------------------------------------
 template <class T> struct FSum1 {   // sum of F + add
+  static constexpr const char* NAME = "FSum1";
+  static constexpr int NFIELDS = 1;
   const T* F; int nz; double add;
   __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
     acc[0] += (double)F[(size_t)c * nz + z] + add;
   }
 };
 template <class T> struct FSum3 {   // u, v, TH + T0 (RHS damping and buoyancy means)
+  static constexpr const char* NAME = "FSum3";
+  static constexpr int NFIELDS = 3;
   const T* a; const T* b; const T* c3; int nz; double add3;
   __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
     const size_t i = (size_t)c * nz + z;
@@ -122,6 +129,8 @@ __device__ __forceinline__ void mom_load(const MomLevel<T>& m, size_t i, double*
 // The ten plane means a1..e2 of DynamicSGS_LASDD_SM.py:266-312 (sums; the
 // caller scales by 1/(nx*ny)).  Arithmetic in double.
 template <class T> struct FLasddSums {
+  static constexpr const char* NAME = "FLasddSums";
+  static constexpr int NFIELDS = 42;
   MomLevel<T> l1, l2;
   int nz;
   double L, TFR;
@@ -212,6 +221,8 @@ __device__ __forceinline__ void sca_terms(const ScaLevel<T>& s, size_t i, double
 
 // a2..e2, a4..e4 of DynamicSGS_ScalarLASDD_SM.py:241-302 (sums).
 template <class T> struct FScalarSums {
+  static constexpr const char* NAME = "FScalarSums";
+  static constexpr int NFIELDS = 28;
   ScaLevel<T> l1, l2;
   int nz;
   double L, TFR;
@@ -254,6 +265,8 @@ __global__ void k_scalar_updn(PwDims d, ScaLevel<T> l1, const double* __restrict
 // ratio (DynamicSGS_LASDD_SM.py:103-130, ScalarLASDD_SM.py:104-131), and the
 // plane sums of sqrt(C) and C for the profiles (LASDD_SM.py:359-360).
 template <class T> struct FBoxClip {
+  static constexpr const char* NAME = "FBoxClip";
+  static constexpr int NFIELDS = 3;
   const T* num; const T* den; T* out; int nx, ny, nz;
   __device__ void operator()(int c, int z, double* acc) const {
     const int x = c / ny, y = c - x * ny;

@@ -183,9 +183,15 @@ void stage_surface(Ctx<T>& c, const StateT<T>& S, const Surf<T>& sf, double sfc_
   a.z0m_nd = g.z0m / d.z_scale; a.zT_nd = zref / d.z_scale;
   a.logm = std::log(0.5 * d.dz * d.z_scale / g.z0m);
   a.logh = std::log(0.5 * d.dz * d.z_scale / zref);
-  if (P->ws.dry) return;
-  k_surface<T><<<1, SURF_BS, 0, c.s>>>(a);
-  launch_ok(P);
+  const int nb = imin(SURF_NB_MAX, (c.ncol + SURF_T - 1) / SURF_T);
+  double* part = c.D("surf_part", 2 * SURF_NB_MAX);
+  double* part2 = c.D("surf_part2", SURF_NB_MAX);
+  double* part3 = c.D("surf_part3", 3 * SURF_NB_MAX);
+  const double pb = (double)c.ncol * sizeof(T);
+  KLAUNCH(P, c.s, "k_surf_sums", 3 * pb, k_surf_sums<T>, nb, SURF_T, a, part);
+  if (g.optSurfBC == 2) KLAUNCH(P, c.s, "k_surf_q", 7 * pb, k_surf_q<T>, nb, SURF_T, a, part, part2);
+  KLAUNCH(P, c.s, "k_surf_final", 17 * pb, k_surf_final<T>, nb, SURF_T, a, part, part2, part3);
+  KLAUNCH(P, c.s, "k_surf_scal", 0.0, k_surf_scal, 1, SURF_T, nb, c.ncol, part3, P->scal);
 }
 
 // DivStress (NSE_SGSTerms.py:55-105): 5 transforms, 6 spectral derivatives
@@ -272,9 +278,11 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   ta.RCr = RCr; ta.RCi = RCi;
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
-  if (!P->ws.dry) {
-    k_thomas<T><<<c.nx * c.nyc, 64, 0, c.s>>>(ta);
-    launch_ok(P);
+  {
+    // 3 spectra in, 2 spectra out, 2 double factor tables
+    const double spb = (double)c.nx * c.nyc * c.nz * sizeof(c2<T>);
+    KLAUNCH(P, c.s, "k_thomas", 5.0 * spb + 2.0 * c.nx * c.nyc * (c.nz + 1) * 8.0, k_thomas<T>,
+            c.nx * c.nyc, 64, ta);
   }
   XArgs xa{};
   xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.in_spectral = 1;
@@ -431,6 +439,8 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* 
 }
 
 template <class T> struct FSum4 {
+  static constexpr const char* NAME = "FSum4";
+  static constexpr int NFIELDS = 3;
   const T* u; const T* v; const T* TH; int nz; double T0;
   __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
     const size_t i = (size_t)c * nz + z;
@@ -663,11 +673,10 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   }
   // 9. AB2 + CFL (Main.py:506-540)
   double* cflpart = c.D("cflpart", 3 * AB2_BLOCKS);
+  KLAUNCH(P, s, "k_ab2", 23.0 * c.FB, k_ab2<T>, AB2_BLOCKS, 256, c.dims(), S.u, S.v, S.w, S.TH,
+          Rnu, Rnv, Rnw, RnT, S.Ru, S.Rv, S.Rw, S.RT, dpdx, dpdy, dpdz, (T)d.dt_nondim,
+          (T)(d.inversion_nondim * d.dz), first ? 1 : 0, cflpart);
   if (!P->ws.dry) {
-    k_ab2<T><<<AB2_BLOCKS, 256, 0, s>>>(c.dims(), S.u, S.v, S.w, S.TH, Rnu, Rnv, Rnw, RnT, S.Ru,
-                                         S.Rv, S.Rw, S.RT, dpdx, dpdy, dpdz, (T)d.dt_nondim,
-                                         (T)(d.inversion_nondim * d.dz), first ? 1 : 0, cflpart);
-    launch_ok(P);
     k_cfl_final<<<1, 256, 0, s>>>(cflpart, AB2_BLOCKS, d.dt_nondim / d.dx, d.dt_nondim / d.dy,
                                   d.dt_nondim / d.dz, P->scal, st->scalars);
     launch_ok(P);

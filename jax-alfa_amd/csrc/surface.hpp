@@ -1,7 +1,6 @@
 // Monin-Obukhov surface layer (src/surface/SurfaceFlux.py:53-443): the six
 // heat-flux variants selected by (optSurfBC, optSurfFlux).  The work is a
-// handful of reductions over one (nx, ny) plane, so one 1024-thread block
-// does all of it in double with fixed-order block reductions.
+// handful of reductions over one (nx, ny) plane, done in double.
 #pragma once
 #include "common.hpp"
 
@@ -14,19 +13,6 @@ template <class T> struct SurfArgs {
   double sfc_val, SHF, Ugal, T0, vonk, g, dz_half, z1, z0m_nd, zT_nd, logm, logh;
 };
 
-constexpr int SURF_BS = 1024;
-
-__device__ double block_sum_1024(double v, double* sm) {
-  sm[threadIdx.x] = v;
-  __syncthreads();
-  for (int s = SURF_BS / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
-    __syncthreads();
-  }
-  const double r = sm[0];
-  __syncthreads();
-  return r;
-}
 
 // psi_m, psi_h, fi_m, fi_h (MOSTstable :53-77 / MOSTunstable :83-113).
 __device__ __forceinline__ void most(double zL, bool stable, double& pm, double& ph, double& fm,
@@ -42,55 +28,107 @@ __device__ __forceinline__ void most(double zL, bool stable, double& pm, double&
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(SURF_BS) k_surface(SurfArgs<T> a) {
-  __shared__ double sm[SURF_BS];
-  const int n = a.ncol, nz = a.nz;
-  const double invn = 1.0 / n;
-  double sM = 0, sT = 0;
-  for (int c = threadIdx.x; c < n; c += SURF_BS) {
-    const size_t i = (size_t)c * nz;
-    const double uu = (double)a.u[i] + a.Ugal, vv = a.v[i];
-    sM += sqrt(uu * uu + vv * vv);
-    sT += (double)a.TH[i];
+// The plane work is spread over SURF_NB blocks of 256 threads (one column
+// per thread at 256^2).  The three plane means are two-stage reductions:
+// every block writes one partial, and every block that needs a mean adds
+// the partials itself in the same fixed order (so all blocks see the
+// bitwise same mean, run to run).
+constexpr int SURF_NB_MAX = 256;
+constexpr int SURF_T = 256;
+
+__device__ double block_reduce_256(double v, double* sm) {
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = SURF_T / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
   }
-  const double Mavg = block_sum_1024(sM, sm) * invn;
-  const double THavg = block_sum_1024(sT, sm) * invn;
-  // heat flux average
+  const double r = sm[0];
+  __syncthreads();
+  return r;
+}
+
+// sum of part[k*stride] for k < n, same order in every block
+__device__ double sum_parts(const double* part, int n, int stride, double* sm) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += SURF_T) v += part[(size_t)k * stride];
+  return block_reduce_256(v, sm);
+}
+
+template <class T> __device__ __forceinline__ void surf_col(const SurfArgs<T>& a, int c, double& uu,
+                                                            double& vv, double& th) {
+  const size_t i = (size_t)c * a.nz;
+  uu = (double)a.u[i] + a.Ugal;
+  vv = (double)a.v[i];
+  th = (double)a.TH[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(SURF_T) k_surf_sums(SurfArgs<T> a, double* part) {
+  __shared__ double sm[SURF_T];
+  double sM = 0, sT = 0;
+  for (int c = blockIdx.x * SURF_T + threadIdx.x; c < a.ncol; c += gridDim.x * SURF_T) {
+    double uu, vv, th;
+    surf_col(a, c, uu, vv, th);
+    sM += sqrt(uu * uu + vv * vv);
+    sT += th;
+  }
+  const double m = block_reduce_256(sM, sm);
+  const double t = block_reduce_256(sT, sm);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = m; part[2 * blockIdx.x + 1] = t; }
+}
+
+// prescribed-temperature flux partials (optSurfBC = 2)
+template <class T>
+__global__ void __launch_bounds__(SURF_T) k_surf_q(SurfArgs<T> a, const double* part,
+                                                   double* part2) {
+  __shared__ double sm[SURF_T];
+  const double invn = 1.0 / a.ncol;
+  const double Mavg = sum_parts(part, gridDim.x, 2, sm) * invn;
+  const double THavg = sum_parts(part + 1, gridDim.x, 2, sm) * invn;
+  double sq = 0;
+  for (int c = blockIdx.x * SURF_T + threadIdx.x; c < a.ncol; c += gridDim.x * SURF_T) {
+    double uu, vv, th;
+    surf_col(a, c, uu, vv, th);
+    const double M = a.hetero ? sqrt(uu * uu + vv * vv) : Mavg;
+    const double dm = a.logm + (double)a.psim[c] - (double)a.psim0[c];
+    const double us = fmax(a.vonk * M / dm, 1e-3);
+    const double dh = a.logh + (double)a.psih[c] - (double)a.psih0[c];
+    const double tha = a.hetero ? th : THavg;
+    sq += us * a.vonk * (a.sfc_val - tha) / dh;
+  }
+  const double q = block_reduce_256(sq, sm);
+  if (threadIdx.x == 0) part2[blockIdx.x] = q;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SURF_T) k_surf_final(SurfArgs<T> a, const double* part,
+                                                       const double* part2, double* part3) {
+  __shared__ double sm[SURF_T];
+  const double invn = 1.0 / a.ncol;
+  const double Mavg = sum_parts(part, gridDim.x, 2, sm) * invn;
+  const double THavg = sum_parts(part + 1, gridDim.x, 2, sm) * invn;
   double qavg;
   if (a.bc == 0) qavg = a.SHF;
   else if (a.bc == 1) qavg = a.sfc_val;
-  else {
-    double sq = 0;
-    for (int c = threadIdx.x; c < n; c += SURF_BS) {
-      const size_t i = (size_t)c * nz;
-      const double uu = (double)a.u[i] + a.Ugal, vv = a.v[i];
-      const double M = a.hetero ? sqrt(uu * uu + vv * vv) : Mavg;
-      const double dm = a.logm + (double)a.psim[c] - (double)a.psim0[c];
-      const double us = fmax(a.vonk * M / dm, 1e-3);
-      const double dh = a.logh + (double)a.psih[c] - (double)a.psih0[c];
-      const double tha = a.hetero ? (double)a.TH[i] : THavg;
-      sq += us * a.vonk * (a.sfc_val - tha) / dh;
-    }
-    qavg = block_sum_1024(sq, sm) * invn;
-  }
+  else qavg = sum_parts(part2, gridDim.x, 1, sm) * invn;
   const bool stable = qavg <= 0.0;
   double sus = 0;
-  for (int c = threadIdx.x; c < n; c += SURF_BS) {
-    const size_t i = (size_t)c * nz;
-    const double uu = (double)a.u[i] + a.Ugal, vv = a.v[i];
+  for (int c = blockIdx.x * SURF_T + threadIdx.x; c < a.ncol; c += gridDim.x * SURF_T) {
+    double uu, vv, th;
+    surf_col(a, c, uu, vv, th);
     const double M = a.hetero ? sqrt(uu * uu + vv * vv) : Mavg;
     const double dm = a.logm + (double)a.psim[c] - (double)a.psim0[c];
     const double us = fmax(a.vonk * M / dm, 1e-3);
     double q, thref;
     if (a.bc == 2) {
       const double dh = a.logh + (double)a.psih[c] - (double)a.psih0[c];
-      const double tha = a.hetero ? (double)a.TH[i] : THavg;
+      const double tha = a.hetero ? th : THavg;
       q = us * a.vonk * (a.sfc_val - tha) / dh;
       thref = THavg + a.T0;
     } else {
       q = (a.bc == 0) ? a.SHF : a.sfc_val;
-      thref = a.hetero ? (double)a.TH[i] + a.T0 : THavg + a.T0;
+      thref = a.hetero ? th + a.T0 : THavg + a.T0;
     }
     const double iob = -(a.vonk * a.g * qavg) / ((us * us * us) * thref);
     double pm, ph, fm, fh, pm0, ph0, d1, d2;
@@ -105,10 +143,21 @@ __global__ void __launch_bounds__(SURF_BS) k_surface(SurfArgs<T> a) {
     a.fim[c] = (T)fm; a.fih[c] = (T)fh;
     sus += us * us;
   }
-  const double su2 = block_sum_1024(sus, sm) * invn;
-  if (threadIdx.x == 0 && a.scal) {
-    a.scal[1] = qavg;
-    a.scal[2] = sqrt(su2);
-    a.scal[3] = a.hetero ? -1.0 : Mavg;
+  const double su = block_reduce_256(sus, sm);
+  if (threadIdx.x == 0) {
+    part3[3 * blockIdx.x] = su;
+    part3[3 * blockIdx.x + 1] = qavg;
+    part3[3 * blockIdx.x + 2] = a.hetero ? -1.0 : Mavg;
+  }
+}
+
+__global__ void __launch_bounds__(SURF_T) k_surf_scal(int nb, int ncol, const double* part3,
+                                                      double* scal) {
+  __shared__ double sm[SURF_T];
+  const double su = sum_parts(part3, nb, 3, sm);
+  if (threadIdx.x == 0 && scal) {
+    scal[1] = part3[1];
+    scal[2] = sqrt(su / ncol);
+    scal[3] = part3[2];
   }
 }
