@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-MiB device buffer 3 times (a plain copy "
+                         "kernel of known size, to scale HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the per-member step does not shard across GPUs in this "
@@ -135,6 +138,15 @@ def main():
                     launches=dom["launches"],
                     avg_launch_us=round(dom["total_ms"] * 1e3 / dom["launches"], 2),
                     bytes_per_launch=dom["bytes"] / dom["launches"])
+    if args.copy_probe > 0:
+        nel = args.copy_probe * 2 ** 20 // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 copies of {args.copy_probe} MiB")
+        del src, dst
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": f"Mpoint-steps/s at {n}^3 (FP32-mixed vs FP64)",

@@ -27,8 +27,15 @@ static bool factor(int n, Fft1D& f) {
   f.n = n;
   f.nrad = 0;
   int m = n;
-  const int order[4] = {4, 2, 3, 5};
+#if ALFA_RADIX_MAX >= 16
+  const int order[6] = {16, 8, 4, 2, 3, 5};
+#elif ALFA_RADIX_MAX >= 8
+  const int order[6] = {8, 4, 2, 3, 5, 1};
+#else
+  const int order[6] = {4, 2, 3, 5, 1, 1};
+#endif
   for (int r : order) {
+    if (r == 1) continue;
     while (m % r == 0) {
       if (f.nrad >= FFT_MAXRAD) return false;
       f.rad[f.nrad++] = r;
@@ -124,16 +131,7 @@ int filtering_explicit(alfa_plan* P, hipStream_t s, const void* Ffft, void* F, v
   xa.in[0] = Ffft; xa.dst[0][0] = Fnew;
   xa.op[0] = xop(c.nyc, 0, d.cut0_x, XM_NONE, XMAP_ID, c.nx);
   xa.nops = 1; xa.ncy_in = c.nyc; xa.nz = c.nz; xa.in_spectral = 1; xa.ny_full = c.ny;
-  constexpr int L = Lines<T>::L;
-  TileGrid tg{(c.nz + L - 1) / L, d.cut0_y, 1, 0};
-  const unsigned grid = (unsigned)tg.ntz * tg.rows;
-  const size_t lds = (size_t)c.nx * L * sizeof(c2<T>) + 2 * c.nx * sizeof(c2<T>);
-  if (c.nx <= 256)
-    k_x<T, 256><<<grid, FFT_BS, lds, s>>>(xa, P->fx, P->fx, (const c2<T>*)P->tw_x, (const c2<T>*)P->tw_x, tg);
-  else
-    k_x<T, FFT_MAXN><<<grid, FFT_BS, lds, s>>>(xa, P->fx, P->fx, (const c2<T>*)P->tw_x,
-                                               (const c2<T>*)P->tw_x, tg);
-  launch_ok(P);
+  x_pass<T>(P, s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x, d.cut0_y);
   spectral_to_real<T>(c, Fnew, d.cut0_y, BIGCUT, XM_NONE, (T*)F);
   return 0;
 }
@@ -312,7 +310,6 @@ int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, 
   c2<T>* phat = c.S("phat");
   c2<T>* dpz = c.S("dpzh");
   c2<T>* dpx = c.S("dpxh");
-  c2<T>* dpy = c.S("dpyh");
   ThomasArgs<T> ta{};
   ta.m = P->th_m; ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
   ta.RCin_r = (const T*)RCr; ta.RCin_i = (const T*)RCi; ta.fRzr_in = (const T*)fRzr;
@@ -324,18 +321,18 @@ int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, 
   }
   XArgs xa{};
   xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.in_spectral = 1;
-  xa.in[0] = phat; xa.dst[0][0] = dpx; xa.dst[0][1] = dpy; xa.dst[0][2] = phat;
+  xa.in[0] = phat; xa.dst[0][0] = dpx; xa.dst[0][1] = phat;
   xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
-  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
-  xa.op[2] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
-  xa.nops = 3;
+  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = 2;
   x_pass<T>(P, s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
   xa.in[0] = dpz; xa.dst[0][0] = dpz; xa.nops = 1;
   xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
   x_pass<T>(P, s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
   YC2RArgs ya{};
-  ya.in[0][0] = dpx; ya.in[1][0] = dpy; ya.in[2][0] = dpz; ya.in[3][0] = phat;
+  ya.in[0][0] = dpx; ya.in[1][0] = phat; ya.dy[1] = 1; ya.in[2][0] = dpz; ya.in[3][0] = phat;
   ya.nsum = 1; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  ya.dyscale = 1.0;
   StorePtr<T> st{};
   st.p[0] = (T*)dpdx; st.p[1] = (T*)dpdy; st.p[2] = (T*)dpdz; st.p[3] = (T*)p;
   st.ny = c.ny; st.nz = c.nz;
@@ -493,6 +490,13 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
     return set_error(ALFA_EINVAL, "transform length %d exceeds %d", maxn, FFT_MAXN);
   }
   hipError_t e = hipSuccess;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      P->ncu = prop.multiProcessorCount;
+  }
   auto tw = [&](int n) -> void* {
     return precision == ALFA_FP64 ? make_twiddles<double>(n, P->allocs, e)
                                   : make_twiddles<float>(n, P->allocs, e);

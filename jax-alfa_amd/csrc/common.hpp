@@ -23,13 +23,25 @@ template <class T> __device__ __forceinline__ c2<T> cmul_i(c2<T> a, int sign) {
   return sign > 0 ? mkc<T>(-a.y, a.x) : mkc<T>(a.y, -a.x);
 }
 
-// Block geometry shared by the FFT kernels: 256 threads; a block holds LPB
-// complex lines (= 128 bytes of z per line element), so one row of a tile is
-// one 128-byte HBM segment.
+// Block geometry shared by the FFT kernels: 256 threads; a block transforms
+// L interleaved complex lines (L = 8 float or 4 double lines: 64 bytes of z
+// per transform element, half of a 128-byte L2 line whose other half belongs
+// to the neighbouring z tile, which runs on the same XCD next to it).
+// Measured at 256^3 on MI355X, 64-byte rows beat 128-byte rows (higher
+// occupancy) and 32-byte rows (more HBM transactions).
 constexpr int FFT_BS = 256;
 constexpr int FFT_MAXN = 384;      // longest transform: 1.5 * 256
 constexpr int FFT_MAXRAD = 8;
-template <class T> struct Lines { static constexpr int L = 128 / (int)sizeof(c2<T>); };  // 16 f32, 8 f64
+// Build-time tuning of the FFT kernels (defaults are the measured best):
+//   ALFA_LPB       complex lines per tile for float (double uses half)
+//   ALFA_RADIX_MAX largest radix of the register butterflies (4, 8 or 16)
+#ifndef ALFA_LPB
+#define ALFA_LPB 8
+#endif
+#ifndef ALFA_RADIX_MAX
+#define ALFA_RADIX_MAX 8
+#endif
+template <class T> struct Lines { static constexpr int L = ALFA_LPB * 8 / (int)sizeof(c2<T>); };
 
 struct Fft1D {
   int n;

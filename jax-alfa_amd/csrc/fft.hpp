@@ -74,6 +74,61 @@ template <class T> struct Dft<T, 5> {
   }
 };
 
+// radix 8 = 2 x 4 and radix 16 = 4 x 4 in registers: y[k1 + Q k2] =
+// sum_j2 w_R^(j2 k1) w_P^(j2 k2) sum_j1 x[P j1 + j2] w_Q^(j1 k1)  (R = P Q)
+template <class T> __device__ __forceinline__ c2<T> twc(c2<T> a, T c, T s, int sign) {
+  // a * (c + i sign s)
+  const T ss = sign > 0 ? s : -s;
+  return mkc<T>(a.x * c - a.y * ss, a.x * ss + a.y * c);
+}
+template <class T> struct Dft<T, 8> {
+  static __device__ __forceinline__ void run(c2<T>* v, int sign) {
+    const T h = T(0.70710678118654752440);
+    c2<T> a[4] = {v[0], v[2], v[4], v[6]};     // j2 = 0: x[2 j1]
+    c2<T> b[4] = {v[1], v[3], v[5], v[7]};     // j2 = 1: x[2 j1 + 1]
+    Dft<T, 4>::run(a, sign);
+    Dft<T, 4>::run(b, sign);
+    // twiddle w8^(k1) on the j2 = 1 branch
+    b[1] = twc<T>(b[1], h, h, sign);
+    b[2] = cmul_i<T>(b[2], sign);
+    b[3] = twc<T>(b[3], -h, h, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      v[k1] = cadd<T>(a[k1], b[k1]);
+      v[k1 + 4] = csub<T>(a[k1], b[k1]);
+    }
+  }
+};
+template <class T> struct Dft<T, 16> {
+  static __device__ __forceinline__ void run(c2<T>* v, int sign) {
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
+    const T h = T(0.70710678118654752440);
+    c2<T> t[4][4];
+#pragma unroll
+    for (int j2 = 0; j2 < 4; ++j2) {
+      t[j2][0] = v[j2]; t[j2][1] = v[4 + j2]; t[j2][2] = v[8 + j2]; t[j2][3] = v[12 + j2];
+      Dft<T, 4>::run(t[j2], sign);
+    }
+    // twiddles w16^(j2 k1)
+    t[1][1] = twc<T>(t[1][1], c1, s1, sign);
+    t[1][2] = twc<T>(t[1][2], h, h, sign);
+    t[1][3] = twc<T>(t[1][3], s1, c1, sign);
+    t[2][1] = twc<T>(t[2][1], h, h, sign);
+    t[2][2] = cmul_i<T>(t[2][2], sign);
+    t[2][3] = twc<T>(t[2][3], -h, h, sign);
+    t[3][1] = twc<T>(t[3][1], s1, c1, sign);
+    t[3][2] = twc<T>(t[3][2], -h, h, sign);
+    t[3][3] = twc<T>(t[3][3], -c1, -s1, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      c2<T> u[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+      Dft<T, 4>::run(u, sign);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = u[k2];
+    }
+  }
+};
+
 // One Stockham pass of radix R over LPB interleaved lines held in LDS
 // (element (j, line p) at buf[j*LPB + p]).  All inputs of the pass are read
 // into registers before any output is written, so the pass runs in place.
@@ -126,6 +181,14 @@ __device__ void fft_lines(c2<T>* buf, const c2<T>* tw, const Fft1D& f, int sign)
   int Ns = 1;
   for (int s = 0; s < f.nrad; ++s) {
     const int R = f.rad[s];
+#if ALFA_RADIX_MAX >= 16
+    if (R == 16) fft_pass<T, 16, LPB, MAXN>(buf, tw, f.n, Ns, sign);
+    else
+#endif
+#if ALFA_RADIX_MAX >= 8
+    if (R == 8) fft_pass<T, 8, LPB, MAXN>(buf, tw, f.n, Ns, sign);
+    else
+#endif
     if (R == 4) fft_pass<T, 4, LPB, MAXN>(buf, tw, f.n, Ns, sign);
     else if (R == 2) fft_pass<T, 2, LPB, MAXN>(buf, tw, f.n, Ns, sign);
     else if (R == 3) fft_pass<T, 3, LPB, MAXN>(buf, tw, f.n, Ns, sign);
@@ -139,18 +202,26 @@ struct PtrBatch {
 };
 
 // ----------------------------------------------------------------------------
-// y-direction R2C.  grid = (ceil(nz/ZT), rows, batch); row x of field b is
-// read through ld(b, x, y, z); output (rows, ncy_out, nz) complex.
+// Tile geometry.  A launch covers ntz * rows * nb tiles (z tile, transform
+// row, field), one tile per workgroup; the grid is rounded up to a multiple
+// of the 8 XCDs.  The tiles are numbered so that each XCD (workgroup i runs
+// on XCD i mod 8) owns one contiguous range of them: tiles that share HBM
+// rows or input data run on the same L2 at about the same time.  With
+// batch_fast the fields of one (row, z tile) are adjacent (fused loaders
+// read the same inputs for all of them), otherwise the z tiles of one row
+// are.  tile(i, ...) is the i-th tile of a workgroup (a grid smaller than
+// the tile count walks its range gridDim.x/8 tiles at a time).
 // ----------------------------------------------------------------------------
-// Launch geometry of the tiled kernels: a 1-D grid of ntz * rows * nb
-// workgroups, renumbered XCD-contiguously; with batch_fast the fields of one
-// (row, z tile) are adjacent (they share input tiles through L2), otherwise
-// the z tiles of one row are (they share HBM rows).
 struct TileGrid {
   int ntz, rows, nb, batch_fast;
-  __device__ __forceinline__ void decode(int& zt, int& row, int& b) const {
-    const unsigned total = (unsigned)ntz * rows * nb;
-    unsigned l = xcd_logical(blockIdx.x, total);
+  __device__ __forceinline__ bool tile(int i, int& zt, int& row, int& b) const {
+    const unsigned tot = (unsigned)ntz * rows * nb;
+    const unsigned x = blockIdx.x % NUM_XCD, j = blockIdx.x / NUM_XCD;
+    const unsigned per = tot / NUM_XCD, rem = tot % NUM_XCD;
+    const unsigned lo = x * per + (x < rem ? x : rem), cnt = per + (x < rem ? 1u : 0u);
+    const unsigned k = (unsigned)i * (gridDim.x / NUM_XCD) + j;
+    if (k >= cnt) return false;
+    unsigned l = lo + k;
     if (batch_fast) {
       b = l % nb; l /= nb;
       zt = l % ntz; row = l / ntz;
@@ -158,43 +229,54 @@ struct TileGrid {
       zt = l % ntz; l /= ntz;
       row = l % rows; b = l / rows;
     }
+    return true;
   }
 };
 
+// ----------------------------------------------------------------------------
+// y-direction R2C: row x of field b (read through ld(b, x, y, z)) -> the
+// first ncy_out columns of its half spectrum, (rows, ncy_out, nz) complex.
+// ----------------------------------------------------------------------------
 template <class T, class Loader, int MAXN>
 __global__ void __launch_bounds__(FFT_BS)
 k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
         TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
+  constexpr int PRE = (MAXN * ZT + FFT_BS - 1) / FFT_BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = fy.n;
+  const int n = fy.n, nEl = n * ZT;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
-  int zt, x, b;
-  tg.decode(zt, x, b);
-  const int z0 = zt * ZT;
-  for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
   T* bufr = reinterpret_cast<T*>(buf);
-  for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
-    const int y = i / ZT, zl = i - y * ZT;
-    const int z = z0 + zl;
-    bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
-  }
-  __syncthreads();
-  fft_lines<T, L, MAXN>(buf, stw, fy, -1);
-  c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
-  for (int i = threadIdx.x; i < ncy_out * ZT; i += FFT_BS) {
-    const int k = i / ZT, zl = i - k * ZT;
-    const int z = z0 + zl;
-    if (z >= nz) continue;
-    const int p = zl >> 1;
-    const c2<T> Zk = buf[k * L + p];
-    const c2<T> Zn = buf[((n - k) % n) * L + p];
-    c2<T> r;
-    if ((zl & 1) == 0) r = mkc<T>(T(0.5) * (Zk.x + Zn.x), T(0.5) * (Zk.y - Zn.y));
-    else r = mkc<T>(T(0.5) * (Zk.y + Zn.y), T(-0.5) * (Zk.x - Zn.x));
-    out[((size_t)x * ncy_out + k) * nz + z] = r;
+  for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
+  int zt, x, b, it = 0;
+  bool have = tg.tile(0, zt, x, b);
+  while (have) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
+      const int y = i / ZT, z = zt * ZT + (i - y * ZT);
+      bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
+    }
+    __syncthreads();
+    int zn, xn, bn;
+    const bool hn = tg.tile(++it, zn, xn, bn);
+    fft_lines<T, L, MAXN>(buf, stw, fy, -1);
+    c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
+    const int z0 = zt * ZT;
+    for (int i = threadIdx.x; i < ncy_out * ZT; i += FFT_BS) {
+      const int k = i / ZT, zl = i - k * ZT;
+      const int z = z0 + zl;
+      if (z >= nz) continue;
+      const int p = zl >> 1;
+      const c2<T> Zk = buf[k * L + p];
+      const c2<T> Zn = buf[((n - k) % n) * L + p];
+      c2<T> r;
+      if ((zl & 1) == 0) r = mkc<T>(T(0.5) * (Zk.x + Zn.x), T(0.5) * (Zk.y - Zn.y));
+      else r = mkc<T>(T(0.5) * (Zk.y + Zn.y), T(-0.5) * (Zk.x - Zn.x));
+      out[((size_t)x * ncy_out + k) * nz + z] = r;
+    }
+    zt = zn; x = xn; b = bn; have = hn;
   }
 }
 
@@ -225,7 +307,7 @@ struct XArgs {
   int ny_full;       // regular-grid ny (ky Nyquist column = ny_full/2)
 };
 
-// value of op `op` at input mode k (element i of line p) of a forward spectrum
+// value of op `op` at input mode k of a forward spectrum
 template <class T>
 __device__ __forceinline__ c2<T> x_apply(const XOp& op, c2<T> v, int k, int nin, int kyv) {
   const bool keep = (k < op.cx) || (k >= nin - op.cx + 1);
@@ -240,7 +322,34 @@ __device__ __forceinline__ c2<T> x_apply(const XOp& op, c2<T> v, int k, int nin,
   return cscale<T>(v, (T)op.scale);
 }
 
-template <class T, int MAXN>
+// load tile (zt, ky) of field b into LDS, (k, line) at buf[k*L + line]
+template <class T, int L>
+__device__ __forceinline__ void x_load(c2<T>* buf, const XArgs& a, int nEl, int zt, int ky, int b) {
+  const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b]);
+  const int nz = a.nz;
+  for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
+    const int k = i / L, z = zt * L + (i - k * L);
+    buf[i] = (z < nz) ? in[((size_t)k * a.ncy_in + ky) * nz + z] : mkc<T>(0, 0);
+  }
+}
+
+template <class T, int L>
+__device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, int nz, int nout,
+                                        int zt, int ky) {
+  c2<T>* dst = reinterpret_cast<c2<T>*>(dstp);
+  const int z0 = zt * L;
+  for (int i = threadIdx.x; i < nout * L; i += FFT_BS) {
+    const int k = i / L, z = z0 + (i - k * L);
+    if (z < nz) dst[((size_t)k * ncy + ky) * nz + z] = buf[i];
+  }
+}
+
+// One ky column tile (L z lines, full x length) per iteration of a
+// workgroup.  MULTI = false: one op with the identity map, applied in place in
+// LDS (the common case; small register footprint).  MULTI = true: the
+// forward spectrum is kept in registers and every op (mask / derivative /
+// dealias map, optional inverse) is built from it in turn.
+template <class T, int MAXN, bool MULTI>
 __global__ void __launch_bounds__(FFT_BS)
 k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     const c2<T>* __restrict__ tw_out, TileGrid tg) {
@@ -252,136 +361,145 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw_in = buf + nbuf * L;
   c2<T>* stw_out = stw_in + nin;
-  int zt, ky, b;
-  tg.decode(zt, ky, b);
-  const int z0 = zt * L;
   const int nz = a.nz;
-  for (int i = threadIdx.x; i < nin; i += FFT_BS) stw_in[i] = tw_in[i];
-  for (int i = threadIdx.x; i < fout.n; i += FFT_BS) stw_out[i] = tw_out[i];
-  const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b]);
-  for (int i = threadIdx.x; i < nin * L; i += FFT_BS) {
-    const int k = i / L, p = i - k * L;
-    const int z = z0 + p;
-    buf[i] = (z < nz) ? in[((size_t)k * a.ncy_in + ky) * nz + z] : mkc<T>(0, 0);
-  }
-  __syncthreads();
-  if (!a.in_spectral) fft_lines<T, L, MAXN>(buf, stw_in, fin, -1);
-  const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
   const int nEl = nin * L;
-  if (a.nops == 1 && a.op[0].map == XMAP_ID) {
-    // one output: apply the op in place, no register copy of the spectrum
-    const XOp op = a.op[0];
-    if (ky >= op.ncy) return;
-    for (int i = threadIdx.x; i < nEl; i += FFT_BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
+  for (int i = threadIdx.x; i < nin; i += FFT_BS) stw_in[i] = tw_in[i];
+  if (MULTI)
+    for (int i = threadIdx.x; i < fout.n; i += FFT_BS) stw_out[i] = tw_out[i];
+  int zt, ky, b;
+  for (int it = 0; tg.tile(it, zt, ky, b); ++it) {
+    if (!MULTI && ky >= a.op[0].ncy) continue;
     __syncthreads();
-    if (op.inverse) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
-    c2<T>* dst = reinterpret_cast<c2<T>*>(a.dst[b][0]);
-    for (int i = threadIdx.x; i < nin * L; i += FFT_BS) {
-      const int k = i / L, p = i - k * L;
-      const int z = z0 + p;
-      if (z < nz) dst[((size_t)k * op.ncy + ky) * nz + z] = buf[i];
-    }
-    return;
-  }
-  c2<T> spec[MAXE];
-#pragma unroll
-  for (int q = 0; q < MAXE; ++q) {
-    const int i = threadIdx.x + q * FFT_BS;
-    spec[q] = (i < nEl) ? buf[i] : mkc<T>(0, 0);
-  }
-  for (int o = 0; o < a.nops; ++o) {
-    const XOp op = a.op[o];
-    if (ky >= op.ncy) continue;
-    const int nout = op.nout;
+    x_load<T, L>(buf, a, nEl, zt, ky, b);
     __syncthreads();
-    if (op.map != XMAP_ID) {
-      for (int i = threadIdx.x; i < nout * L; i += FFT_BS) buf[i] = mkc<T>(0, 0);
+    if (!a.in_spectral) fft_lines<T, L, MAXN>(buf, stw_in, fin, -1);
+    const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
+    if (!MULTI) {
+      const XOp op = a.op[0];
+      for (int i = threadIdx.x; i < nEl; i += FFT_BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
       __syncthreads();
-    }
+      if (op.inverse) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
+      x_store<T, L>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky);
+    } else {
+      c2<T> spec[MAXE];
 #pragma unroll
-    for (int q = 0; q < MAXE; ++q) {
-      const int i = threadIdx.x + q * FFT_BS;
-      if (i < nEl) {
-        const int k = i / L, p = i - k * L;
-        const c2<T> v = x_apply<T>(op, spec[q], k, nin, kyv);
-        if (op.map == XMAP_ID) {
-          buf[i] = v;
-        } else if (op.map == XMAP_D1) {
-          if (k <= nin / 2) buf[k * L + p] = v;
-          if (k >= nin / 2) buf[(nout - nin + k) * L + p] = v;
-        } else {
-          const int h = nout / 2;
-          if (k < h) buf[k * L + p] = v;
-          else if (k >= nin - (h - 1)) buf[(k - nin + nout) * L + p] = v;
-        }
+      for (int q = 0; q < MAXE; ++q) {
+        const int i = threadIdx.x + q * FFT_BS;
+        spec[q] = (i < nEl) ? buf[i] : mkc<T>(0, 0);
       }
-    }
-    __syncthreads();
-    if (op.inverse) {
-      if (op.map == XMAP_ID) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
-      else fft_lines<T, L, MAXN>(buf, stw_out, fout, +1);
-    }
-    c2<T>* dst = reinterpret_cast<c2<T>*>(a.dst[b][o]);
-    for (int i = threadIdx.x; i < nout * L; i += FFT_BS) {
-      const int k = i / L, p = i - k * L;
-      const int z = z0 + p;
-      if (z < nz) dst[((size_t)k * op.ncy + ky) * nz + z] = buf[i];
+      for (int o = 0; o < a.nops; ++o) {
+        const XOp op = a.op[o];
+        if (ky >= op.ncy) continue;
+        const int nout = op.nout;
+        __syncthreads();
+        if (op.map != XMAP_ID) {
+          for (int i = threadIdx.x; i < nout * L; i += FFT_BS) buf[i] = mkc<T>(0, 0);
+          __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < MAXE; ++q) {
+          const int i = threadIdx.x + q * FFT_BS;
+          if (i < nEl) {
+            const int k = i / L, p = i - k * L;
+            const c2<T> v = x_apply<T>(op, spec[q], k, nin, kyv);
+            if (op.map == XMAP_ID) {
+              buf[i] = v;
+            } else if (op.map == XMAP_D1) {
+              if (k <= nin / 2) buf[k * L + p] = v;
+              if (k >= nin / 2) buf[(nout - nin + k) * L + p] = v;
+            } else {
+              const int h = nout / 2;
+              if (k < h) buf[k * L + p] = v;
+              else if (k >= nin - (h - 1)) buf[(k - nin + nout) * L + p] = v;
+            }
+          }
+        }
+        __syncthreads();
+        if (op.inverse) {
+          if (op.map == XMAP_ID) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
+          else fft_lines<T, L, MAXN>(buf, stw_out, fout, +1);
+        }
+        x_store<T, L>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky);
+      }
     }
   }
 }
 
 // ----------------------------------------------------------------------------
-// y-direction C2R.  grid = (ceil(nz/ZT), rows, batch).  Output b is the sum
-// of nsum pruned half spectra (rows, ncy_in, nz); values go to
-// st(b, x, y, z, value*scale).
+// y-direction C2R.  Output b of row x is the sum of nsum pruned half
+// spectra (rows, ncy_in, nz); values go to st(b, x, y, z, value*scale).
 // ----------------------------------------------------------------------------
 struct YC2RArgs {
   const void* in[MAXB][3];
+  int dy[MAXB];     // bit s set: input s of output b is multiplied by i*ky*dyscale
   int nsum;
   int ncy_in;
   int nz;
   double scale;
+  double dyscale;
 };
+
+// sum of the nsum inputs of output b at (x, ky = k, z); the y wavenumber of
+// the Nyquist column (2k == n) is taken as 0, as in the x pass.
+template <class T>
+__device__ __forceinline__ c2<T> yc2r_in(const YC2RArgs& a, int b, int x, int k, int z, int n) {
+  c2<T> X = mkc<T>(0, 0);
+  const size_t base = ((size_t)x * a.ncy_in + k) * a.nz + z;
+  const int kyv = (2 * k == n) ? 0 : k;
+  for (int s = 0; s < a.nsum; ++s) {
+    c2<T> v = reinterpret_cast<const c2<T>*>(a.in[b][s])[base];
+    if ((a.dy[b] >> s) & 1) v = cscale<T>(cmul_i<T>(v, 1), (T)(a.dyscale * kyv));
+    X = cadd<T>(X, v);
+  }
+  return X;
+}
 
 template <class T, class Storer, int MAXN>
 __global__ void __launch_bounds__(FFT_BS)
 k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
+  constexpr int PRE = ((MAXN / 2 + 1) * L + FFT_BS - 1) / FFT_BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = fy.n, half = n / 2;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
-  int zt, x, b;
-  tg.decode(zt, x, b);
-  const int z0 = zt * ZT;
   const int nz = a.nz;
-  for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
-  for (int i = threadIdx.x; i < n * L; i += FFT_BS) buf[i] = mkc<T>(0, 0);
-  __syncthreads();
   const int kmax = imin(a.ncy_in, half + 1);
-  for (int i = threadIdx.x; i < kmax * L; i += FFT_BS) {
-    const int k = i / L, p = i - k * L;
-    const int z = z0 + 2 * p;
-    c2<T> X1 = mkc<T>(0, 0), X2 = mkc<T>(0, 0);
-    for (int s = 0; s < a.nsum; ++s) {
-      const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b][s]);
-      const size_t base = ((size_t)x * a.ncy_in + k) * nz;
-      if (z < nz) X1 = cadd<T>(X1, in[base + z]);
-      if (z + 1 < nz) X2 = cadd<T>(X2, in[base + z + 1]);
+  const int nIn = kmax * L;
+  for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
+  int zt, x, b, it = 0;
+  bool have = tg.tile(0, zt, x, b);
+  while (have) {
+    __syncthreads();
+    // the modes kmax..n-kmax are zero
+    for (int i = threadIdx.x; i < n * L; i += FFT_BS) {
+      const int k = i / L;
+      if (k >= kmax && k <= n - kmax) buf[i] = mkc<T>(0, 0);
     }
-    if (k == 0 || k == half) { X1.y = T(0); X2.y = T(0); }
-    buf[k * L + p] = mkc<T>(X1.x - X2.y, X1.y + X2.x);
-    if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
-  }
-  __syncthreads();
-  fft_lines<T, L, MAXN>(buf, stw, fy, +1);
-  const T* bufr = reinterpret_cast<const T*>(buf);
-  const T sc = (T)a.scale;
-  for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
-    const int y = i / ZT, zl = i - y * ZT;
-    const int z = z0 + zl;
-    if (z < nz) st(b, x, y, z, bufr[i] * sc);
+    for (int i = threadIdx.x; i < nIn; i += FFT_BS) {
+      const int k = i / L, p = i - k * L;
+      const int z = zt * ZT + 2 * p;
+      c2<T> X1 = mkc<T>(0, 0), X2 = mkc<T>(0, 0);
+      if (z < nz) X1 = yc2r_in<T>(a, b, x, k, z, n);
+      if (z + 1 < nz) X2 = yc2r_in<T>(a, b, x, k, z + 1, n);
+      if (k == 0 || k == half) { X1.y = T(0); X2.y = T(0); }
+      buf[k * L + p] = mkc<T>(X1.x - X2.y, X1.y + X2.x);
+      if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
+    }
+    __syncthreads();
+    int zn, xn, bn;
+    const bool hn = tg.tile(++it, zn, xn, bn);
+    fft_lines<T, L, MAXN>(buf, stw, fy, +1);
+    const T* bufr = reinterpret_cast<const T*>(buf);
+    const T sc = (T)a.scale;
+    const int z0 = zt * ZT;
+    for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
+      const int y = i / ZT, zl = i - y * ZT;
+      const int z = z0 + zl;
+      if (z < nz) st(b, x, y, z, bufr[i] * sc);
+    }
+    zt = zn; x = xn; b = bn; have = hn;
   }
 }
 

@@ -64,7 +64,16 @@ struct alfa_plan {
   Prof pf;
   std::vector<void*> allocs;
   hipError_t herr = hipSuccess;
+  int ncu = 256;   // compute units of the plan's device
 };
+
+// Persistent grid of the tiled FFT kernels: as many workgroups as stay
+// resident (LDS-limited, at most 4 per CU), a multiple of the 8 XCDs, and
+// no more than the tiles rounded up to 8.
+// one workgroup per tile, rounded up to whole XCD rounds
+inline unsigned tile_grid(size_t tiles) {
+  return (unsigned)((tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
+}
 
 int set_error(int code, const char* fmt, ...);
 
@@ -130,7 +139,7 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   for (int b = 0; b < nb; ++b) pb.p[b] = outs[b];
   constexpr int ZT = 2 * Lines<T>::L;
   TileGrid tg{(nz + ZT - 1) / ZT, rows, nb, Loader::BATCH_FAST};
-  const unsigned grid = (unsigned)tg.ntz * rows * nb;
+  const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
   const double cells = (double)rows * fyy.n * nz;
   ProfScope ps(P, s, std::string("k_y_r2c<") + Loader::NAME + ">",
                Loader::nin(nb) * cells * sizeof(T) +
@@ -142,23 +151,38 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   launch_ok(P);
 }
 
+template <class T, int MAXN, bool MULTI>
+void launch_kx(alfa_plan* P, hipStream_t s, unsigned grid, size_t lds, const XArgs& a,
+               const Fft1D& fin, const Fft1D& fout, const void* twin, const void* twout,
+               const TileGrid& tg) {
+  k_x<T, MAXN, MULTI><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin,
+                                                (const c2<T>*)twout, tg);
+}
+
+// x pass over nb spectra; the tiles cover ky in [0, rows) (rows = ncy_in by
+// default).
 template <class T>
 void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a, int nb, const Fft1D& fin,
-            const Fft1D& fout, const void* twin, const void* twout) {
+            const Fft1D& fout, const void* twin, const void* twout, int rows = -1) {
   if (P->ws.dry) return;
   constexpr int L = Lines<T>::L;
-  TileGrid tg{(a.nz + L - 1) / L, a.ncy_in, nb, 0};
-  const unsigned grid = (unsigned)tg.ntz * tg.rows * nb;
+  if (rows < 0) rows = a.ncy_in;
+  TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
   const int nbuf = imax(fin.n, fout.n);
   const size_t lds = (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
-  double bytes = (double)nb * fin.n * a.ncy_in * a.nz * sizeof(c2<T>);
+  const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
+  const bool multi = a.nops > 1 || a.op[0].map != XMAP_ID;
+  double bytes = (double)nb * fin.n * imin(rows, a.ncy_in) * a.nz * sizeof(c2<T>);
   for (int o = 0; o < a.nops; ++o)
-    bytes += (double)nb * a.op[o].nout * imin(a.op[o].ncy, a.ncy_in) * a.nz * sizeof(c2<T>);
-  ProfScope ps(P, s, "k_x", bytes);
-  if (nbuf <= 256)
-    k_x<T, 256><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
-  else
-    k_x<T, FFT_MAXN><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+    bytes += (double)nb * a.op[o].nout * imin(a.op[o].ncy, rows) * a.nz * sizeof(c2<T>);
+  ProfScope ps(P, s, multi ? "k_x<multi>" : "k_x<single>", bytes);
+  if (nbuf <= 256) {
+    if (multi) launch_kx<T, 256, true>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
+    else launch_kx<T, 256, false>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
+  } else {
+    if (multi) launch_kx<T, FFT_MAXN, true>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
+    else launch_kx<T, FFT_MAXN, false>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
+  }
   launch_ok(P);
 }
 
@@ -168,7 +192,7 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a, int
   if (P->ws.dry) return;
   constexpr int ZT = 2 * Lines<T>::L;
   TileGrid tg{(a.nz + ZT - 1) / ZT, rows, nb, 0};
-  const unsigned grid = (unsigned)tg.ntz * rows * nb;
+  const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
   ProfScope ps(P, s, std::string("k_y_c2r<") + Storer::NAME + ">",
                (double)nb * a.nsum * rows * a.ncy_in * a.nz * sizeof(c2<T>) +
                    (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T));

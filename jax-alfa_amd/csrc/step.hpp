@@ -201,32 +201,29 @@ void stage_div_stress(Ctx<T>& c, T* txx, T* tyy, T* tzz, T* txy, T* txz, T* tyz,
                       T* dty, T* dtz) {
   alfa_plan* P = c.P;
   c2<T>* sp[5] = {c.S("ds0"), c.S("ds1"), c.S("ds2"), c.S("ds3"), c.S("ds4")};
-  c2<T>* sxy_y = c.S("ds5");
+  c2<T>* sxy_x = c.S("ds5");
   // order: txx, txz | tyy, tyz | txy
   LoadPtr<T> ld{};
   ld.p[0] = txx; ld.p[1] = txz; ld.p[2] = tyy; ld.p[3] = tyz; ld.p[4] = txy;
   ld.ny = c.ny; ld.nz = c.nz;
   y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, c.nyc, 5, (void* const*)sp);
+  // x derivatives: one x pass over txx, txz, txy (txy into its own buffer,
+  // its y spectrum is still needed).  The y derivatives need no x pass: the
+  // x round trip of an unmasked spectrum is the identity times nx, so i*ky
+  // is applied to the y spectra directly in the y C2R with that factor.
   XArgs xa{};
   xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.nops = 1;
   xa.in[0] = sp[0]; xa.dst[0][0] = sp[0];
   xa.in[1] = sp[1]; xa.dst[1][0] = sp[1];
+  xa.in[2] = sp[4]; xa.dst[2][0] = sxy_x;
   xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
-  x_pass<T>(P, c.s, xa, 2, P->fx, P->fx, P->tw_x, P->tw_x);
-  xa.in[0] = sp[2]; xa.dst[0][0] = sp[2];
-  xa.in[1] = sp[3]; xa.dst[1][0] = sp[3];
-  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
-  x_pass<T>(P, c.s, xa, 2, P->fx, P->fx, P->tw_x, P->tw_x);
-  xa.in[0] = sp[4]; xa.dst[0][0] = sp[4]; xa.dst[0][1] = sxy_y;
-  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
-  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
-  xa.nops = 2;
-  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
+  x_pass<T>(P, c.s, xa, 3, P->fx, P->fx, P->tw_x, P->tw_x);
   YC2RArgs ya{};
-  ya.in[0][0] = sp[0]; ya.in[0][1] = sxy_y;   // divtx = d/dx txx + d/dy txy
-  ya.in[1][0] = sp[4]; ya.in[1][1] = sp[2];   // divty = d/dx txy + d/dy tyy
-  ya.in[2][0] = sp[1]; ya.in[2][1] = sp[3];   // divtz = d/dx txz + d/dy tyz
+  ya.in[0][0] = sp[0]; ya.in[0][1] = sp[4]; ya.dy[0] = 2;   // divtx = d/dx txx + d/dy txy
+  ya.in[1][0] = sxy_x; ya.in[1][1] = sp[2]; ya.dy[1] = 2;   // divty = d/dx txy + d/dy tyy
+  ya.in[2][0] = sp[1]; ya.in[2][1] = sp[3]; ya.dy[2] = 2;   // divtz = d/dx txz + d/dy tyz
   ya.nsum = 2; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  ya.dyscale = c.nx;
   StoreDiv<T> st{};
   st.out[0] = dtx; st.out[1] = dty; st.out[2] = dtz;
   st.fz[0] = txz; st.fz[1] = tyz; st.fz[2] = tzz;
@@ -248,12 +245,10 @@ void stage_div_flux(Ctx<T>& c, T* qx, T* qy, T* qz, T* divq) {
   xa.in[0] = sp[0]; xa.dst[0][0] = sp[0];
   xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
   x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
-  xa.in[0] = sp[1]; xa.dst[0][0] = sp[1];
-  xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
-  x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
   YC2RArgs ya{};
-  ya.in[0][0] = sp[0]; ya.in[0][1] = sp[1];
+  ya.in[0][0] = sp[0]; ya.in[0][1] = sp[1]; ya.dy[0] = 2;   // d/dy qy: y spectrum times nx
   ya.nsum = 2; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  ya.dyscale = c.nx;
   StoreDiv<T> st{};
   st.out[0] = divq; st.fz[0] = qz; st.wnode[0] = 1;
   st.ny = c.ny; st.nz = c.nz; st.idz = (T)(1.0 / P->d.dz);
@@ -271,7 +266,6 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   c2<T>* phat = c.S("phat");
   c2<T>* dpz = c.S("dpzh");
   c2<T>* dpx = c.S("dpxh");
-  c2<T>* dpy = c.S("dpyh");
   ThomasArgs<T> ta{};
   ta.fRx = fR[0]; ta.fRy = fR[1]; ta.fRz = fR[2]; ta.fbt = fbt;
   ta.m = P->th_m; ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
@@ -286,18 +280,20 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   }
   XArgs xa{};
   xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.in_spectral = 1;
-  xa.in[0] = phat; xa.dst[0][0] = dpx; xa.dst[0][1] = dpy; xa.dst[0][2] = phat;
+  // dp/dx from i*kx p^; p^ itself is x-inverted in place and gives dp/dy
+  // (i*ky in the y C2R) and p
+  xa.in[0] = phat; xa.dst[0][0] = dpx; xa.dst[0][1] = phat;
   xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_IKX, XMAP_ID, c.nx);
-  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_IKY, XMAP_ID, c.nx);
-  xa.op[2] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
-  xa.nops = p ? 3 : 2;
+  xa.op[1] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
+  xa.nops = 2;
   x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
   xa.in[0] = dpz; xa.dst[0][0] = dpz; xa.nops = 1;
   xa.op[0] = xop(c.nyc, 1, BIGCUT, XM_NONE, XMAP_ID, c.nx);
   x_pass<T>(P, c.s, xa, 1, P->fx, P->fx, P->tw_x, P->tw_x);
   YC2RArgs ya{};
-  ya.in[0][0] = dpx; ya.in[1][0] = dpy; ya.in[2][0] = dpz; ya.in[3][0] = phat;
+  ya.in[0][0] = dpx; ya.in[1][0] = phat; ya.dy[1] = 1; ya.in[2][0] = dpz; ya.in[3][0] = phat;
   ya.nsum = 1; ya.ncy_in = c.nyc; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+  ya.dyscale = 1.0;
   StorePtr<T> st{};
   st.p[0] = dpdx; st.p[1] = dpdy; st.p[2] = dpdz; st.p[3] = p;
   st.ny = c.ny; st.nz = c.nz;
@@ -504,23 +500,26 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   {
     c2<T>* ef[4] = {c.S("ef0"), c.S("ef1"), c.S("ef2"), c.S("ef3")};
     c2<T>* ex[4] = {c.S("ex0"), c.S("ex1"), c.S("ex2"), c.S("ex3")};
-    c2<T>* ey[4] = {c.S("ey0"), c.S("ey1"), c.S("ey2"), c.S("ey3")};
     LoadPtr<T> ld{};
     ld.p[0] = S.u; ld.p[1] = S.v; ld.p[2] = S.w; ld.p[3] = S.TH; ld.ny = c.ny; ld.nz = c.nz;
     y_r2c<T>(P, s, ld, c.nx, P->fy, P->tw_y, c.nz, d.cut0_y, 4, (void* const*)ef);
     XArgs xa{};
     for (int b = 0; b < 4; ++b) {
-      xa.in[b] = ef[b]; xa.dst[b][0] = ef[b]; xa.dst[b][1] = ex[b]; xa.dst[b][2] = ey[b];
+      xa.in[b] = ef[b]; xa.dst[b][0] = ef[b]; xa.dst[b][1] = ex[b];
     }
     xa.op[0] = xop(d.cut0_y, 1, d.cut0_x, XM_NONE, XMAP_ID, c.nx);
     xa.op[1] = xop(d.cut0_y, 1, d.cut0_x, XM_IKX, XMAP_ID, c.nx);
-    xa.op[2] = xop(d.cut0_y, 1, d.cut0_x, XM_IKY, XMAP_ID, c.nx);
-    xa.nops = 3; xa.ncy_in = d.cut0_y; xa.nz = c.nz; xa.ny_full = c.ny;
+    xa.nops = 2; xa.ncy_in = d.cut0_y; xa.nz = c.nz; xa.ny_full = c.ny;
     x_pass<T>(P, s, xa, 4, P->fx, P->fx, P->tw_x, P->tw_x);
+    // the y derivatives are the filtered spectra times i*ky, applied in the
+    // y C2R (i*ky commutes with the x transform)
     YC2RArgs ya{};
     T* outs[12] = {S.u, S.v, S.w, S.TH, dudx, dvdx, dwdx, dTx, dudy, dvdy, dwdy, dTy};
-    for (int b = 0; b < 4; ++b) { ya.in[b][0] = ef[b]; ya.in[4 + b][0] = ex[b]; ya.in[8 + b][0] = ey[b]; }
+    for (int b = 0; b < 4; ++b) {
+      ya.in[b][0] = ef[b]; ya.in[4 + b][0] = ex[b]; ya.in[8 + b][0] = ef[b]; ya.dy[8 + b] = 1;
+    }
     ya.nsum = 1; ya.ncy_in = d.cut0_y; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
+    ya.dyscale = 1.0;
     StorePtr<T> so{};
     for (int b = 0; b < 12; ++b) so.p[b] = outs[b];
     so.ny = c.ny; so.nz = c.nz;
