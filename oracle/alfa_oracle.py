@@ -764,7 +764,11 @@ def ScalarFluxesW_interior(g, dTHdz, S, CsPr):
 
 def LASDD_SM(g, u, v, w, S11, S22, S33, S12, S13, S23, S):
     """src/subgridscale/DynamicSGS_LASDD_SM.py:138-367 (optSgs 1 = LASDD,
-    3 = LAD with beta = 1)."""
+    3 = LAD with beta = 1).
+
+    Parity unpinned: the plane-mean sums a1..e2 and the quintic coefficient
+    algebra aa..ff (:314-319) have no independent check; their parts (the
+    filters, the Leonard terms, the root solve) are pinned separately."""
     nz = g.nz
     L, TFR = g.L, g.TFR
     u_ = u.copy(); v_ = v.copy()
@@ -832,7 +836,10 @@ def LASDD_SM(g, u, v, w, S11, S22, S33, S12, S13, S23, S):
 
 
 def ScalarLASDD_SM(g, mom, TH, dTHdx, dTHdy, dTHdz, S):
-    """src/subgridscale/DynamicSGS_ScalarLASDD_SM.py:139-316."""
+    """src/subgridscale/DynamicSGS_ScalarLASDD_SM.py:139-316.
+
+    Parity unpinned: the scalar sums and quintic coefficients (as in
+    LASDD_SM) have no independent check."""
     nz = g.nz
     L, TFR = g.L, g.TFR
     u_, v_, w_ = mom["u_"], mom["v_"], mom["w_"]
