@@ -61,6 +61,47 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned lin, unsigned total) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// 16-byte vector of W = 16 / sizeof(T) consecutive elements (4 float or 2
+// double): the unit of every vectorised global load and store.
+template <class T> struct Vec16 {
+  static constexpr int W = 16 / (int)sizeof(T);
+  T v[W];
+};
+template <class T> __device__ __forceinline__ Vec16<T> vld(const T* p) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  Vec16<T> r;
+  __builtin_memcpy(r.v, &q, 16);
+  return r;
+}
+template <class T> __device__ __forceinline__ void vst(T* p, const Vec16<T>& r) {
+  uint4 q;
+  __builtin_memcpy(&q, r.v, 16);
+  *reinterpret_cast<uint4*>(p) = q;
+}
+template <class T> __device__ __forceinline__ Vec16<T> vzero() {
+  Vec16<T> r;
+#pragma unroll
+  for (int j = 0; j < Vec16<T>::W; ++j) r.v[j] = T(0);
+  return r;
+}
+// two adjacent complex values (one 16-byte access for float, two for double)
+template <class T> __device__ __forceinline__ void cld2(const c2<T>* p, c2<T>& a, c2<T>& b) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    a = mkc<T>(q.x, q.y); b = mkc<T>(q.z, q.w);
+  } else {
+    a = p[0]; b = p[1];
+  }
+}
+template <class T> __device__ __forceinline__ void cst2(c2<T>* p, c2<T> a, c2<T> b) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(a.x, a.y, b.x, b.y);
+  } else {
+    p[0] = a; p[1] = b;
+  }
+}
+inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 // Maximum number of distinct fields one batched launch handles.
 constexpr int MAXB = 24;
 

@@ -234,49 +234,60 @@ struct TileGrid {
 };
 
 // ----------------------------------------------------------------------------
-// y-direction R2C: row x of field b (read through ld(b, x, y, z)) -> the
-// first ncy_out columns of its half spectrum, (rows, ncy_out, nz) complex.
+// y-direction R2C: row x of field b (read through the loader) -> the first
+// ncy_out columns of its half spectrum, (rows, ncy_out, nz) complex.  With
+// vec set (nz a multiple of the vector width, 16-byte aligned buffers) every
+// thread loads 16 bytes of z at a time through ld.vload and stores the two
+// complex values of one packed line together.
 // ----------------------------------------------------------------------------
 template <class T, class Loader, int MAXN>
 __global__ void __launch_bounds__(FFT_BS)
 k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
-        TileGrid tg) {
+        TileGrid tg, int vec) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
-  constexpr int PRE = (MAXN * ZT + FFT_BS - 1) / FFT_BS;
+  constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = fy.n, nEl = n * ZT;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
   T* bufr = reinterpret_cast<T*>(buf);
   for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
-  int zt, x, b, it = 0;
-  bool have = tg.tile(0, zt, x, b);
-  while (have) {
+  int zt, x, b;
+  for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
-    for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
-      const int y = i / ZT, z = zt * ZT + (i - y * ZT);
-      bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
+    if (vec) {
+      for (int e = threadIdx.x * W; e < nEl; e += FFT_BS * W) {
+        const int y = e / ZT, z = zt * ZT + (e - y * ZT);
+        vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
+      }
+    } else {
+      for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
+        const int y = i / ZT, z = zt * ZT + (i - y * ZT);
+        bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
+      }
     }
     __syncthreads();
-    int zn, xn, bn;
-    const bool hn = tg.tile(++it, zn, xn, bn);
     fft_lines<T, L, MAXN>(buf, stw, fy, -1);
+    // line p holds z0+2p (real part) and z0+2p+1 (imaginary part)
     c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
     const int z0 = zt * ZT;
-    for (int i = threadIdx.x; i < ncy_out * ZT; i += FFT_BS) {
-      const int k = i / ZT, zl = i - k * ZT;
-      const int z = z0 + zl;
+    for (int i = threadIdx.x; i < ncy_out * L; i += FFT_BS) {
+      const int k = i / L, p = i - k * L;
+      const int z = z0 + 2 * p;
       if (z >= nz) continue;
-      const int p = zl >> 1;
       const c2<T> Zk = buf[k * L + p];
       const c2<T> Zn = buf[((n - k) % n) * L + p];
-      c2<T> r;
-      if ((zl & 1) == 0) r = mkc<T>(T(0.5) * (Zk.x + Zn.x), T(0.5) * (Zk.y - Zn.y));
-      else r = mkc<T>(T(0.5) * (Zk.y + Zn.y), T(-0.5) * (Zk.x - Zn.x));
-      out[((size_t)x * ncy_out + k) * nz + z] = r;
+      const c2<T> r0 = mkc<T>(T(0.5) * (Zk.x + Zn.x), T(0.5) * (Zk.y - Zn.y));
+      const c2<T> r1 = mkc<T>(T(0.5) * (Zk.y + Zn.y), T(-0.5) * (Zk.x - Zn.x));
+      c2<T>* o = out + ((size_t)x * ncy_out + k) * nz + z;
+      if (vec) {
+        cst2<T>(o, r0, r1);
+      } else {
+        o[0] = r0;
+        if (z + 1 < nz) o[1] = r1;
+      }
     }
-    zt = zn; x = xn; b = bn; have = hn;
   }
 }
 
@@ -305,6 +316,7 @@ struct XArgs {
   int nz;
   int in_spectral;   // 1: input is already x-transformed
   int ny_full;       // regular-grid ny (ky Nyquist column = ny_full/2)
+  int vec;           // 16-byte accesses (set by the launcher: nz even, aligned)
 };
 
 // value of op `op` at input mode k of a forward spectrum
@@ -322,11 +334,22 @@ __device__ __forceinline__ c2<T> x_apply(const XOp& op, c2<T> v, int k, int nin,
   return cscale<T>(v, (T)op.scale);
 }
 
-// load tile (zt, ky) of field b into LDS, (k, line) at buf[k*L + line]
+// load tile (zt, ky) of field b into LDS, (k, line) at buf[k*L + line];
+// with vec, float lines are moved in pairs (16 bytes)
 template <class T, int L>
 __device__ __forceinline__ void x_load(c2<T>* buf, const XArgs& a, int nEl, int zt, int ky, int b) {
   const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b]);
   const int nz = a.nz;
+  if (sizeof(T) == 4 && a.vec) {
+    for (int e = 2 * threadIdx.x; e < nEl; e += 2 * FFT_BS) {
+      const int k = e / L, z = zt * L + (e - k * L);
+      c2<T> v0 = mkc<T>(0, 0), v1 = mkc<T>(0, 0);
+      if (z < nz) cld2<T>(in + ((size_t)k * a.ncy_in + ky) * nz + z, v0, v1);
+      buf[e] = v0;
+      buf[e + 1] = v1;
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
     const int k = i / L, z = zt * L + (i - k * L);
     buf[i] = (z < nz) ? in[((size_t)k * a.ncy_in + ky) * nz + z] : mkc<T>(0, 0);
@@ -335,9 +358,16 @@ __device__ __forceinline__ void x_load(c2<T>* buf, const XArgs& a, int nEl, int 
 
 template <class T, int L>
 __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, int nz, int nout,
-                                        int zt, int ky) {
+                                        int zt, int ky, int vec) {
   c2<T>* dst = reinterpret_cast<c2<T>*>(dstp);
   const int z0 = zt * L;
+  if (sizeof(T) == 4 && vec) {
+    for (int e = 2 * threadIdx.x; e < nout * L; e += 2 * FFT_BS) {
+      const int k = e / L, z = z0 + (e - k * L);
+      if (z < nz) cst2<T>(dst + ((size_t)k * ncy + ky) * nz + z, buf[e], buf[e + 1]);
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < nout * L; i += FFT_BS) {
     const int k = i / L, z = z0 + (i - k * L);
     if (z < nz) dst[((size_t)k * ncy + ky) * nz + z] = buf[i];
@@ -379,7 +409,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
       for (int i = threadIdx.x; i < nEl; i += FFT_BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
       __syncthreads();
       if (op.inverse) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
-      x_store<T, L>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky);
+      x_store<T, L>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky, a.vec);
     } else {
       c2<T> spec[MAXE];
 #pragma unroll
@@ -419,7 +449,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
           if (op.map == XMAP_ID) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
           else fft_lines<T, L, MAXN>(buf, stw_out, fout, +1);
         }
-        x_store<T, L>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky);
+        x_store<T, L>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky, a.vec);
       }
     }
   }
@@ -435,23 +465,37 @@ struct YC2RArgs {
   int nsum;
   int ncy_in;
   int nz;
+  int vec;          // 16-byte accesses (set by the launcher: nz % W == 0, aligned)
   double scale;
   double dyscale;
 };
 
-// sum of the nsum inputs of output b at (x, ky = k, z); the y wavenumber of
-// the Nyquist column (2k == n) is taken as 0, as in the x pass.
+// the sums of the nsum inputs of output b at (x, ky = k) for the z pair
+// (z, z+1) of one packed line; the y wavenumber of the Nyquist column
+// (2k == n) is taken as 0, as in the x pass.
 template <class T>
-__device__ __forceinline__ c2<T> yc2r_in(const YC2RArgs& a, int b, int x, int k, int z, int n) {
-  c2<T> X = mkc<T>(0, 0);
+__device__ __forceinline__ void yc2r_in2(const YC2RArgs& a, int b, int x, int k, int z, int n,
+                                         c2<T>& X1, c2<T>& X2) {
+  X1 = mkc<T>(0, 0);
+  X2 = mkc<T>(0, 0);
   const size_t base = ((size_t)x * a.ncy_in + k) * a.nz + z;
-  const int kyv = (2 * k == n) ? 0 : k;
+  const T kyv = (T)(a.dyscale * ((2 * k == n) ? 0 : k));
   for (int s = 0; s < a.nsum; ++s) {
-    c2<T> v = reinterpret_cast<const c2<T>*>(a.in[b][s])[base];
-    if ((a.dy[b] >> s) & 1) v = cscale<T>(cmul_i<T>(v, 1), (T)(a.dyscale * kyv));
-    X = cadd<T>(X, v);
+    const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b][s]) + base;
+    c2<T> v1, v2 = mkc<T>(0, 0);
+    if (a.vec) {
+      cld2<T>(in, v1, v2);
+    } else {
+      v1 = in[0];
+      if (z + 1 < a.nz) v2 = in[1];
+    }
+    if ((a.dy[b] >> s) & 1) {
+      v1 = cscale<T>(cmul_i<T>(v1, 1), kyv);
+      v2 = cscale<T>(cmul_i<T>(v2, 1), kyv);
+    }
+    X1 = cadd<T>(X1, v1);
+    X2 = cadd<T>(X2, v2);
   }
-  return X;
 }
 
 template <class T, class Storer, int MAXN>
@@ -459,7 +503,7 @@ __global__ void __launch_bounds__(FFT_BS)
 k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
-  constexpr int PRE = ((MAXN / 2 + 1) * L + FFT_BS - 1) / FFT_BS;
+  constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = fy.n, half = n / 2;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
@@ -468,9 +512,8 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
   const int kmax = imin(a.ncy_in, half + 1);
   const int nIn = kmax * L;
   for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
-  int zt, x, b, it = 0;
-  bool have = tg.tile(0, zt, x, b);
-  while (have) {
+  int zt, x, b;
+  for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
     // the modes kmax..n-kmax are zero
     for (int i = threadIdx.x; i < n * L; i += FFT_BS) {
@@ -481,25 +524,33 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
       const int k = i / L, p = i - k * L;
       const int z = zt * ZT + 2 * p;
       c2<T> X1 = mkc<T>(0, 0), X2 = mkc<T>(0, 0);
-      if (z < nz) X1 = yc2r_in<T>(a, b, x, k, z, n);
-      if (z + 1 < nz) X2 = yc2r_in<T>(a, b, x, k, z + 1, n);
+      if (z < nz) yc2r_in2<T>(a, b, x, k, z, n, X1, X2);
       if (k == 0 || k == half) { X1.y = T(0); X2.y = T(0); }
       buf[k * L + p] = mkc<T>(X1.x - X2.y, X1.y + X2.x);
       if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
     }
     __syncthreads();
-    int zn, xn, bn;
-    const bool hn = tg.tile(++it, zn, xn, bn);
     fft_lines<T, L, MAXN>(buf, stw, fy, +1);
     const T* bufr = reinterpret_cast<const T*>(buf);
     const T sc = (T)a.scale;
     const int z0 = zt * ZT;
-    for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
-      const int y = i / ZT, zl = i - y * ZT;
-      const int z = z0 + zl;
-      if (z < nz) st(b, x, y, z, bufr[i] * sc);
+    if (a.vec) {
+      for (int e = threadIdx.x * W; e < n * ZT; e += FFT_BS * W) {
+        const int y = e / ZT, zl = e - y * ZT;
+        const int z = z0 + zl;
+        if (z >= nz) continue;
+        Vec16<T> v = vld(bufr + e);
+#pragma unroll
+        for (int j = 0; j < W; ++j) v.v[j] *= sc;
+        st.vstore(b, x, y, z, v);
+      }
+    } else {
+      for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
+        const int y = i / ZT, zl = i - y * ZT;
+        const int z = z0 + zl;
+        if (z < nz) st(b, x, y, z, bufr[i] * sc);
+      }
     }
-    zt = zn; x = xn; b = bn; have = hn;
   }
 }
 
@@ -515,6 +566,14 @@ template <class T> struct LoadPtr {
   __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
     return p[b][((size_t)x * ny + y) * nz + z];
   }
+  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
+    return vld(p[b] + ((size_t)x * ny + y) * nz + z);
+  }
+  bool aligned(int nb) const {
+    for (int b = 0; b < nb; ++b)
+      if (!al16(p[b])) return false;
+    return true;
+  }
 };
 
 template <class T> struct StorePtr {
@@ -524,5 +583,13 @@ template <class T> struct StorePtr {
   int ny, nz;
   __device__ __forceinline__ void operator()(int b, int x, int y, int z, T v) const {
     p[b][((size_t)x * ny + y) * nz + z] = v;
+  }
+  __device__ __forceinline__ void vstore(int b, int x, int y, int z, const Vec16<T>& v) const {
+    vst(p[b] + ((size_t)x * ny + y) * nz + z, v);
+  }
+  bool aligned(int nb) const {
+    for (int b = 0; b < nb; ++b)
+      if (!al16(p[b])) return false;
+    return true;
   }
 };

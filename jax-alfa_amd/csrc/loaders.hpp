@@ -45,6 +45,68 @@ template <class T> struct LoadLasdd {
       default: return vv * ww;
     }
   }
+  // W consecutive z values from z (z % W == 0, z + W <= nz)
+  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
+    constexpr int W = Vec16<T>::W;
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    Vec16<T> r;
+    if (b >= 9) {
+      r = vld(S[(b - 9) % 6] + i);
+      if (b >= 15) {
+        const Vec16<T> m = vld(Sm + i);
+#pragma unroll
+        for (int j = 0; j < W; ++j) r.v[j] *= m.v[j];
+      }
+      return r;
+    }
+    Vec16<T> uu, vv, ww;
+    if (b != 2 && b != 5) {
+      uu = vld(u + i);
+      vv = vld(v + i);
+      if (shu) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) uu.v[j] = (T)((double)uu.v[j] - shu[z + j]);
+      }
+      if (shv) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) vv.v[j] = (T)((double)vv.v[j] - shv[z + j]);
+      }
+    }
+    if (b == 2 || b >= 5) {
+      // w averaged to uvp nodes: needs w[z .. z+W]
+      const Vec16<T> w0 = vld(w + i);
+      const T wn = (z + W < nz) ? w[i + W] : T(0);
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const int zz = z + j;
+        const T a = w0.v[j], c = (j + 1 < W) ? w0.v[j + 1] : wn;
+        if (zz == 0) ww.v[j] = T(0.5) * c;
+        else if (zz == nz - 1) ww.v[j] = (j > 0) ? w0.v[j - 1] : w[i - 1];
+        else ww.v[j] = T(0.5) * (a + c);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const T a = uu.v[j], c = vv.v[j], e = ww.v[j];
+      switch (b) {
+        case 0: r.v[j] = a; break;
+        case 1: r.v[j] = c; break;
+        case 2: r.v[j] = e; break;
+        case 3: r.v[j] = a * a; break;
+        case 4: r.v[j] = c * c; break;
+        case 5: r.v[j] = e * e; break;
+        case 6: r.v[j] = a * c; break;
+        case 7: r.v[j] = a * e; break;
+        default: r.v[j] = c * e; break;
+      }
+    }
+    return r;
+  }
+  bool aligned(int) const {
+    for (int n = 0; n < 6; ++n)
+      if (!al16(S[n])) return false;
+    return al16(u) && al16(v) && al16(w) && al16(Sm);
+  }
 };
 
 // The 10 scalar fields of DynamicSGS_ScalarLASDD_SM.py:182-221:
@@ -84,6 +146,65 @@ template <class T> struct LoadScalar {
     else ww = T(0.5) * (w[i] + w[i + 1]);
     return ww * th;
   }
+  // v averaged from the staggered field f (z .. z+W): the THz / w_ rule
+  __device__ __forceinline__ Vec16<T> stag(const T* f, size_t i, int z, bool wrule) const {
+    constexpr int W = Vec16<T>::W;
+    const Vec16<T> f0 = vld(f + i);
+    const T fn = (z + W < nz) ? f[i + W] : T(0);
+    Vec16<T> r;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int zz = z + j;
+      const T a = f0.v[j], c = (j + 1 < W) ? f0.v[j + 1] : fn;
+      if (wrule) {
+        if (zz == 0) r.v[j] = T(0.5) * c;
+        else if (zz == nz - 1) r.v[j] = (j > 0) ? f0.v[j - 1] : f[i - 1];
+        else r.v[j] = T(0.5) * (a + c);
+      } else {
+        r.v[j] = (zz == 0 || zz == nz - 1) ? a : T(0.5) * (a + c);
+      }
+    }
+    return r;
+  }
+  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
+    constexpr int W = Vec16<T>::W;
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    Vec16<T> r;
+    if (b >= 4) {
+      const int d = (b - 4) % 3;
+      r = d == 0 ? vld(dx + i) : d == 1 ? vld(dy + i) : stag(dz, i, z, false);
+      if (b >= 7) {
+        const Vec16<T> m = vld(Sm + i);
+#pragma unroll
+        for (int j = 0; j < W; ++j) r.v[j] *= m.v[j];
+      }
+      return r;
+    }
+    Vec16<T> th = vld(TH + i);
+    if (sht) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) th.v[j] = (T)((double)th.v[j] - sht[z + j]);
+    }
+    if (b == 0) return th;
+    Vec16<T> f;
+    if (b == 3) {
+      f = stag(w, i, z, true);
+    } else {
+      f = vld((b == 1 ? u : v) + i);
+      const double* sh = b == 1 ? shu : shv;
+      if (sh) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) f.v[j] = (T)((double)f.v[j] - sh[z + j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) r.v[j] = f.v[j] * th.v[j];
+    return r;
+  }
+  bool aligned(int) const {
+    return al16(u) && al16(v) && al16(w) && al16(TH) && al16(dx) && al16(dy) && al16(dz) &&
+           al16(Sm);
+  }
 };
 
 // Stress / flux divergence epilogue of the y-C2R: out = spectral part +
@@ -101,5 +222,33 @@ template <class T> struct StoreDiv {
     if (wnode[b]) dzv = (z < nz - 1) ? (f[i + 1] - f[i]) * idz : T(0);
     else dzv = (z > 0) ? (f[i] - f[i - 1]) * idz : T(0);
     out[b][i] = v + dzv;
+  }
+  __device__ __forceinline__ void vstore(int b, int x, int y, int z, const Vec16<T>& v) const {
+    constexpr int W = Vec16<T>::W;
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    const T* f = fz[b];
+    const Vec16<T> f0 = vld(f + i);
+    Vec16<T> r;
+    if (wnode[b]) {
+      const T fn = (z + W < nz) ? f[i + W] : T(0);
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const T c = (j + 1 < W) ? f0.v[j + 1] : fn;
+        r.v[j] = v.v[j] + ((z + j < nz - 1) ? (c - f0.v[j]) * idz : T(0));
+      }
+    } else {
+      const T fp = (z > 0) ? f[i - 1] : T(0);
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const T c = (j > 0) ? f0.v[j - 1] : fp;
+        r.v[j] = v.v[j] + ((z + j > 0) ? (f0.v[j] - c) * idz : T(0));
+      }
+    }
+    vst(out[b] + i, r);
+  }
+  bool aligned(int nb) const {
+    for (int b = 0; b < nb; ++b)
+      if (!al16(out[b]) || !al16(fz[b])) return false;
+    return true;
   }
 };

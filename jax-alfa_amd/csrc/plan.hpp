@@ -144,10 +144,16 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   ProfScope ps(P, s, std::string("k_y_r2c<") + Loader::NAME + ">",
                Loader::nin(nb) * cells * sizeof(T) +
                    (double)nb * rows * ncy * nz * sizeof(c2<T>));
+  // 16-byte loads need nz to be a multiple of the vector width and aligned
+  // buffers; the paired complex stores need 16-byte aligned outputs
+  bool vec = nz % Vec16<T>::W == 0 && ld.aligned(nb);
+  for (int b = 0; b < nb; ++b) vec = vec && al16(outs[b]);
   if (fyy.n <= 256)
-    k_y_r2c<T, Loader, 256><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg);
+    k_y_r2c<T, Loader, 256><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz,
+                                                                   ncy, pb, tg, vec ? 1 : 0);
   else
-    k_y_r2c<T, Loader, FFT_MAXN><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg);
+    k_y_r2c<T, Loader, FFT_MAXN><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(
+        ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
   launch_ok(P);
 }
 
@@ -162,9 +168,17 @@ void launch_kx(alfa_plan* P, hipStream_t s, unsigned grid, size_t lds, const XAr
 // x pass over nb spectra; the tiles cover ky in [0, rows) (rows = ncy_in by
 // default).
 template <class T>
-void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a, int nb, const Fft1D& fin,
+void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a_in, int nb, const Fft1D& fin,
             const Fft1D& fout, const void* twin, const void* twout, int rows = -1) {
   if (P->ws.dry) return;
+  XArgs a = a_in;
+  // float lines move in 16-byte pairs: nz even and aligned buffers
+  bool vec = a.nz % 2 == 0;
+  for (int b = 0; b < nb; ++b) {
+    vec = vec && al16(a.in[b]);
+    for (int o = 0; o < a.nops; ++o) vec = vec && al16(a.dst[b][o]);
+  }
+  a.vec = vec ? 1 : 0;
   constexpr int L = Lines<T>::L;
   if (rows < 0) rows = a.ncy_in;
   TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
@@ -187,9 +201,14 @@ void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a, int nb, const Fft1D& fi
 }
 
 template <class T, class Storer>
-void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a, int nb, int rows,
+void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, int nb, int rows,
            const Fft1D& fyy, const void* tw) {
   if (P->ws.dry) return;
+  YC2RArgs a = a_in;
+  bool vec = a.nz % Vec16<T>::W == 0 && st.aligned(nb);
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < a.nsum; ++k) vec = vec && al16(a.in[b][k]);
+  a.vec = vec ? 1 : 0;
   constexpr int ZT = 2 * Lines<T>::L;
   TileGrid tg{(a.nz + ZT - 1) / ZT, rows, nb, 0};
   const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);

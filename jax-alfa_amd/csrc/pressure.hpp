@@ -171,6 +171,19 @@ template <class T> struct LoadPressureR {
     const size_t i = ((size_t)x * ny + y) * nz + z;
     return R[b][i] - third * Rp[b][i] + c2dt * U[b][i];
   }
+  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    const Vec16<T> r = vld(R[b] + i), rp = vld(Rp[b] + i), u = vld(U[b] + i);
+    Vec16<T> o;
+#pragma unroll
+    for (int j = 0; j < Vec16<T>::W; ++j) o.v[j] = r.v[j] - third * rp.v[j] + c2dt * u.v[j];
+    return o;
+  }
+  bool aligned(int nb) const {
+    for (int b = 0; b < nb; ++b)
+      if (!al16(R[b]) || !al16(Rp[b]) || !al16(U[b])) return false;
+    return true;
+  }
 };
 
 // Two horizontal planes of one field (levels 0 and nz-1) as a 2-level field.
@@ -182,4 +195,12 @@ template <class T> struct LoadPlanes2 {
   __device__ __forceinline__ T operator()(int, int x, int y, int z) const {
     return F[((size_t)x * ny + y) * nz + (z == 0 ? 0 : nz - 1)];
   }
+  // the two levels are not adjacent in memory: never vectorised
+  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
+    Vec16<T> r;
+#pragma unroll
+    for (int j = 0; j < Vec16<T>::W; ++j) r.v[j] = (*this)(b, x, y, z + j);
+    return r;
+  }
+  bool aligned(int) const { return false; }
 };
