@@ -54,7 +54,8 @@ enum {
  * src/config/DerivedVars.py:39-119).  Lengths in m, times in s. */
 typedef struct alfa_config {
   int32_t nx, ny, nz;
-  int32_t optSgs;               /* 0 static SM, 1 LASDD-SM, 3 LAD-SM        */
+  int32_t optSgs;               /* 0 static SM, 1 LASDD-SM, 2 LASDD-WL,
+                                   3 LAD-SM, 4 LAD-WL (DynamicSGS_Main.py:25-27) */
   int32_t FGR;                  /* 1 (dealiased, TFR=2) or 2 (TFR=sqrt 2)   */
   int32_t dynamicSGS_call_time;
   int32_t optSurfFlux;          /* 0 homogeneous, 1 heterogeneous           */
@@ -72,7 +73,9 @@ typedef struct alfa_config {
   double f_coriolis;
   double inversion;
   double T_0;
-  double Cs2, Cs2PrRatio;
+  double Cs2, Cs2PrRatio;       /* initial coefficient fields; for the WL
+                                   models pass Cwl and CwlPrRatio here
+                                   (DerivedVars.py:101-106) */
   double z_damping, RelaxTime;
   double reserved_d[8];
 } alfa_config;

@@ -59,6 +59,10 @@ def config_struct(cfg: dict) -> AlfaConfig:
         setattr(s, n, int(cfg.get(n, 1 if n == "istep" else 0)))
     for n in _CFG_DBL:
         setattr(s, n, float(cfg.get(n, 0.0)))
+    if int(cfg.get("optSgs", 0)) in (2, 4):
+        # WL models start from Cwl / CwlPrRatio (src/config/DerivedVars.py:101-106)
+        s.Cs2 = float(cfg.get("Cwl", 0.1 ** 2))
+        s.Cs2PrRatio = float(cfg.get("CwlPrRatio", 0.1 ** 2))
     if "optBuoyancy" not in cfg:
         s.optBuoyancy = 1
     if "optDamping" not in cfg:

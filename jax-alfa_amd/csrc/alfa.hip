@@ -444,8 +444,10 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
     return set_error(ALFA_EINVAL, "grid %dx%dx%d: nx, ny >= 8 and nz >= 4, all even", nx, ny, nz);
   if (nz + 1 > TH_MAXN) return set_error(ALFA_EINVAL, "nz=%d exceeds %d", nz, TH_MAXN - 1);
   if (cfg->FGR != 1 && cfg->FGR != 2) return set_error(ALFA_EINVAL, "FGR must be 1 or 2");
-  if (!(cfg->optSgs == 0 || cfg->optSgs == 1 || cfg->optSgs == 3))
-    return set_error(ALFA_EINVAL, "optSgs=%d: supported are 0 (static SM), 1 (LASDD-SM), 3 (LAD-SM)",
+  if (cfg->optSgs < 0 || cfg->optSgs > 4)
+    return set_error(ALFA_EINVAL,
+                     "optSgs=%d: supported are 0 (static SM), 1 (LASDD-SM), 2 (LASDD-WL), "
+                     "3 (LAD-SM), 4 (LAD-WL)",
                      cfg->optSgs);
   if (cfg->optSurfBC < 0 || cfg->optSurfBC > 2) return set_error(ALFA_EINVAL, "optSurfBC must be 0..2");
   alfa_plan* P = new alfa_plan();

@@ -21,12 +21,14 @@ template <class T> struct LoadLasdd {
   const T* S[6]; const T* Sm;
   const double* shu; const double* shv;   // may be NULL
   int ny, nz;
+  int b0;   // field b of the launch is virtual field b + b0
   __device__ __forceinline__ T wbar(size_t i, int z) const {
     if (z == 0) return T(0.5) * w[i + 1];
     if (z == nz - 1) return w[i - 1];
     return T(0.5) * (w[i] + w[i + 1]);
   }
   __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
+    b += b0;
     const size_t i = ((size_t)x * ny + y) * nz + z;
     if (b >= 15) return Sm[i] * S[b - 15][i];
     if (b >= 9) return S[b - 9][i];
@@ -48,6 +50,7 @@ template <class T> struct LoadLasdd {
   // W consecutive z values from z (z % W == 0, z + W <= nz)
   __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
     constexpr int W = Vec16<T>::W;
+    b += b0;
     const size_t i = ((size_t)x * ny + y) * nz + z;
     Vec16<T> r;
     if (b >= 9) {
@@ -121,11 +124,13 @@ template <class T> struct LoadScalar {
   const T* dx; const T* dy; const T* dz; const T* Sm;
   const double* shu; const double* shv; const double* sht;
   int ny, nz;
+  int b0;   // field b of the launch is virtual field b + b0
   __device__ __forceinline__ T thz(size_t i, int z) const {
     if (z == 0 || z == nz - 1) return dz[i];
     return T(0.5) * (dz[i] + dz[i + 1]);
   }
   __device__ __forceinline__ T operator()(int b, int x, int y, int z) const {
+    b += b0;
     const size_t i = ((size_t)x * ny + y) * nz + z;
     switch (b) {
       case 4: return dx[i];
@@ -168,6 +173,7 @@ template <class T> struct LoadScalar {
   }
   __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
     constexpr int W = Vec16<T>::W;
+    b += b0;
     const size_t i = ((size_t)x * ny + y) * nz + z;
     Vec16<T> r;
     if (b >= 4) {

@@ -218,7 +218,8 @@ __global__ void k_smag(PwDims d, const T* a, const T* b, const T* c, const T* e,
 }
 
 // StressesUVPnodes (src/subgridscale/SGSStresses_SM.py:98-182) before any
-// Dealias2: tij = -2 L^2 Cs2 |S| Sij, top level 0.
+// Dealias2: tij = -2 L^2 Cs2 |S| Sij, top level 0.  With S == NULL the
+// Wong-Lilly form (SGSStresses_WL.py:70-141): m2L2 = -2 L^(4/3), no |S|.
 template <class T>
 __global__ void k_stress_uvp(PwDims d, const T* __restrict__ S11, const T* __restrict__ S22,
                              const T* __restrict__ S33, const T* __restrict__ S12,
@@ -228,13 +229,14 @@ __global__ void k_stress_uvp(PwDims d, const T* __restrict__ S11, const T* __res
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
     if (z == nz - 1) { txx[i] = tyy[i] = tzz[i] = txy[i] = T(0); continue; }
-    const T pre = m2L2 * Cs2[i] * S[i];
+    const T pre = m2L2 * Cs2[i] * (S ? S[i] : T(1));
     txx[i] = pre * S11[i]; tyy[i] = pre * S22[i]; tzz[i] = pre * S33[i]; txy[i] = pre * S12[i];
   }
 }
 
 // StressesWnodes (SGSStresses_SM.py:190-297) interior; level 0 gets the wall
 // stress of Wall() (:63-90) when `wall` is set, else 0 (padded branch).
+// Sw == NULL: Wong-Lilly (SGSStresses_WL.py:146-239).
 template <class T>
 __global__ void k_stress_w(PwDims d, const T* __restrict__ S13w, const T* __restrict__ S23w,
                            const T* __restrict__ Sw, const T* __restrict__ Cs2, T m2L2,
@@ -258,7 +260,7 @@ __global__ void k_stress_w(PwDims d, const T* __restrict__ S13w, const T* __rest
       }
       continue;
     }
-    const T pre = m2L2 * (T(0.5) * (Cs2[i - 1] + Cs2[i])) * Sw[i];
+    const T pre = m2L2 * (T(0.5) * (Cs2[i - 1] + Cs2[i])) * (Sw ? Sw[i] : T(1));
     txz[i] = pre * S13w[i];
     tyz[i] = pre * S23w[i];
   }
@@ -286,7 +288,8 @@ __global__ void k_level0(int ncol, int nz, T* txz, T* tyz, const T* __restrict__
 }
 
 // ScalarFluxes (src/subgridscale/ScalarSGSFluxes_SM.py:55-222) before any
-// Dealias2; qz level 0 = qsfc when qsfc != NULL, else 0.
+// Dealias2; qz level 0 = qsfc when qsfc != NULL, else 0.  S == Sw == NULL:
+// Wong-Lilly (ScalarSGSFluxes_WL.py:55-201) with m2L2 = -L^(4/3).
 template <class T>
 __global__ void k_scalar_flux(PwDims d, const T* __restrict__ dx, const T* __restrict__ dy,
                               const T* __restrict__ dz, const T* __restrict__ S,
@@ -296,11 +299,11 @@ __global__ void k_scalar_flux(PwDims d, const T* __restrict__ dx, const T* __res
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
     if (z == nz - 1) { qx[i] = qy[i] = qz[i] = T(0); continue; }
-    const T pre = m2L2 * CsPr[i] * S[i];
+    const T pre = m2L2 * CsPr[i] * (S ? S[i] : T(1));
     qx[i] = pre * dx[i];
     qy[i] = pre * dy[i];
     if (z == 0) qz[i] = qsfc ? qsfc[i / nz] : T(0);
-    else qz[i] = m2L2 * (T(0.5) * (CsPr[i - 1] + CsPr[i])) * Sw[i] * dz[i];
+    else qz[i] = m2L2 * (T(0.5) * (CsPr[i - 1] + CsPr[i])) * (Sw ? Sw[i] : T(1)) * dz[i];
   }
 }
 

@@ -261,6 +261,132 @@ __global__ void k_scalar_updn(PwDims d, ScaLevel<T> l1, const double* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wong-Lilly base model (DynamicSGS_LASDD_WL.py:128-313,
+// DynamicSGS_ScalarLASDD_WL.py:130-275).  Levels hold the filtered
+// velocities, the six products and S_ij; the level-2 set may hold only S_ij
+// (LAD-WL needs nothing else there).
+// ---------------------------------------------------------------------------
+template <class T> struct WlMom {
+  const T* uh[3];
+  const T* ph[6];
+  const T* Sh[6];
+};
+
+template <class T>
+__device__ __forceinline__ void wl_leonard(const WlMom<T>& m, size_t i, double* Lij) {
+  const double u0 = m.uh[0][i], u1 = m.uh[1][i], u2 = m.uh[2][i];
+  Lij[0] = (double)m.ph[0][i] - u0 * u0;
+  Lij[1] = (double)m.ph[1][i] - u1 * u1;
+  Lij[2] = (double)m.ph[2][i] - u2 * u2;
+  Lij[3] = (double)m.ph[3][i] - u0 * u1;
+  Lij[4] = (double)m.ph[4][i] - u0 * u2;
+  Lij[5] = (double)m.ph[5][i] - u1 * u2;
+}
+
+// a1 = <Q:S^>, a3 = <S^:S^>, a6 = <L:S^>, a8 = <S^^:S^^> (LASDD_WL.py:249-261)
+template <class T> struct FWlSums {
+  static constexpr const char* NAME = "FWlSums";
+  static constexpr int NFIELDS = 30;
+  WlMom<T> l1, l2;
+  int nz;
+  __device__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    double L1[6], Q[6], S1[6], S2[6];
+    wl_leonard<T>(l1, i, L1);
+    wl_leonard<T>(l2, i, Q);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) { S1[n] = l1.Sh[n][i]; S2[n] = l2.Sh[n][i]; }
+    acc[0] += con6<T>(Q, S1);
+    acc[1] += con6<T>(S1, S1);
+    acc[2] += con6<T>(L1, S1);
+    acc[3] += con6<T>(S2, S2);
+  }
+};
+
+// M_ij = 2 L^(4/3) (S^_ij - TFR^(4/3) beta S^^_ij), LM, MM (LASDD_WL.py:284-301)
+template <class T>
+__global__ void k_wl_lm(PwDims d, WlMom<T> l1, WlMom<T> l2, const double* __restrict__ beta,
+                        double L, double TFR, T* LM, T* MM) {
+  const int nz = d.nz;
+  const double T1 = 2.0 * pow(L, 4.0 / 3.0), T2 = 2.0 * pow(TFR * L, 4.0 / 3.0);
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    double Lij[6], M[6];
+    wl_leonard<T>(l1, i, Lij);
+    const double b = beta[z];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) M[n] = T1 * (double)l1.Sh[n][i] - T2 * b * (double)l2.Sh[n][i];
+    LM[i] = (T)con6<T>(Lij, M);
+    MM[i] = (T)con6<T>(M, M);
+  }
+}
+
+// Scalar level: TH^, (u,v,w)TH^, the gradient hats, and the velocity hats.
+template <class T> struct WlSca {
+  const T* th;
+  const T* uth[3];
+  const T* dh[3];
+  const T* uh[3];
+};
+
+template <class T>
+__device__ __forceinline__ void wl_sca_leonard(const WlSca<T>& s, size_t i, double* LT) {
+  const double th = s.th[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) LT[k] = (double)s.uth[k][i] - (double)s.uh[k][i] * th;
+}
+
+// a1 = <Q.d^>, a3 = <d^.d^>, a6 = <L.d^>, a8 = <d^^.d^^> (ScalarLASDD_WL.py:218-228)
+template <class T> struct FWlScalarSums {
+  static constexpr const char* NAME = "FWlScalarSums";
+  static constexpr int NFIELDS = 20;
+  WlSca<T> l1, l2;
+  int nz;
+  __device__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    double L1[3], Q[3];
+    wl_sca_leonard<T>(l1, i, L1);
+    wl_sca_leonard<T>(l2, i, Q);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double d1 = l1.dh[k][i], d2 = l2.dh[k][i];
+      acc[0] += Q[k] * d1;
+      acc[1] += d1 * d1;
+      acc[2] += L1[k] * d1;
+      acc[3] += d2 * d2;
+    }
+  }
+};
+
+// T_up, T_dn of ScalarLASDD_WL.py:252-268; l2 needs only the gradient hats
+template <class T>
+__global__ void k_wl_scalar_updn(PwDims d, WlSca<T> l1, WlSca<T> l2,
+                                 const double* __restrict__ beta, double L, double TFR, T* Tup,
+                                 T* Tdn) {
+  const int nz = d.nz;
+  const double L43 = pow(L, 4.0 / 3.0), L83 = pow(L, 8.0 / 3.0);
+  const double T43 = pow(TFR, 4.0 / 3.0), T83 = pow(TFR, 8.0 / 3.0);
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    double LT[3];
+    wl_sca_leonard<T>(l1, i, LT);
+    double a6 = 0, b6 = 0, a3 = 0, c36 = 0, a8 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double d1 = l1.dh[k][i], d2 = l2.dh[k][i];
+      a6 += LT[k] * d1;
+      b6 += LT[k] * d2;
+      a3 += d1 * d1;
+      c36 += d1 * d2;
+      a8 += d2 * d2;
+    }
+    const double b = beta[z];
+    Tup[i] = (T)(L43 * (a6 - T43 * b * b6));
+    Tdn[i] = (T)(L83 * (a3 - 2.0 * T43 * b * c36 + T83 * b * b * a8));
+  }
+}
+
 // Imfilter (Utilities.py:87-115) of both inputs at each level, the clipped
 // ratio (DynamicSGS_LASDD_SM.py:103-130, ScalarLASDD_SM.py:104-131), and the
 // plane sums of sqrt(C) and C for the profiles (LASDD_SM.py:359-360).
@@ -371,11 +497,25 @@ __global__ void k_beta(const double* __restrict__ coef, int n, double* beta) {
 
 // Quintic coefficients (LASDD_SM.py:314-319 momentum, ScalarLASDD_SM.py:
 // 296-302 scalar) from the plane means m[10][nz]; coef (6, nz).
-__global__ void k_quintic(const double* __restrict__ m, int nz, int scalar, double* coef) {
+// mode 0: SM momentum, 1: SM scalar, 2: Wong-Lilly (momentum or scalar,
+// LASDD_WL.py:255-274 from m[4][nz] = a1, a3, a6, a8).
+__global__ void k_quintic(const double* __restrict__ m, int nz, int scalar, double TFR,
+                          double* coef) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nz) return;
   double ff, ee, dd, cc, bb, aa;
-  if (!scalar) {
+  if (scalar == 2) {
+    const double a1 = m[0 * nz + k], a3 = m[1 * nz + k], a6 = m[2 * nz + k], a8 = m[3 * nz + k];
+    const double t43 = pow(TFR, 4.0 / 3.0), t83 = pow(TFR, 8.0 / 3.0), t163 = pow(TFR, 16.0 / 3.0);
+    const double a2 = -t83 * a1, a4 = -2.0 * t43 * a3, a5 = t83 * a3, a7 = -t43 * a6;
+    const double a9 = -2.0 * t83 * a8, a10 = t163 * a8;
+    aa = a1 * a3 - a6 * a8;
+    bb = a1 * a4 - a7 * a8;
+    cc = a2 * a3 + a1 * a5 - a6 * a9;
+    dd = a2 * a4 - a7 * a9;
+    ee = a2 * a5 - a6 * a10;
+    ff = -a7 * a10;
+  } else if (!scalar) {
     const double a1 = m[0 * nz + k], a2 = m[1 * nz + k], b1 = m[2 * nz + k], b2 = m[3 * nz + k];
     const double c1 = m[4 * nz + k], c2 = m[5 * nz + k], d1 = m[6 * nz + k], d2 = m[7 * nz + k];
     const double e1 = m[8 * nz + k], e2 = m[9 * nz + k];

@@ -113,43 +113,40 @@ void dealias2(Ctx<T>& c, int nb, const T* const* in, T* const* out) {
 // Test filters of nb virtual fields at one or two levels
 // (Filtering_Level1/2 of FFT(field), Filtering.py:97-165).
 template <class T, class Loader>
-void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level2, const std::string& tag,
-                 T* const* out1, T* const* out2) {
+void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level1, bool level2,
+                 const std::string& tag, T* const* out1, T* const* out2) {
   alfa_plan* P = c.P;
   const auto& d = P->d;
+  // the level-2 cut-off lies inside the level-1 one: the forward transform
+  // is pruned to the widest level asked for
+  const int ncy = level1 ? d.cut1_y : d.cut2_y;
   c2<T>* s1[MAXB];
   c2<T>* s2[MAXB];
   for (int b = 0; b < nb; ++b) {
-    s1[b] = c.S(tag + "1_" + std::to_string(b));
-    if (level2) s2[b] = c.S(tag + "2_" + std::to_string(b));
+    s1[b] = c.S(tag + (level1 ? "1_" : "2_") + std::to_string(b));
+    if (level1 && level2) s2[b] = c.S(tag + "2_" + std::to_string(b));
+    else s2[b] = s1[b];
   }
-  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, d.cut1_y, nb, (void* const*)s1);
+  y_r2c<T>(P, c.s, ld, c.nx, P->fy, P->tw_y, c.nz, ncy, nb, (void* const*)s1);
   XArgs xa{};
+  int nops = 0;
+  if (level1) xa.op[nops++] = xop(d.cut1_y, 1, d.cut1_x, XM_NONE, XMAP_ID, c.nx);
+  if (level2) xa.op[nops++] = xop(d.cut2_y, 1, d.cut2_x, XM_NONE, XMAP_ID, c.nx);
   for (int b = 0; b < nb; ++b) {
     xa.in[b] = s1[b];
     xa.dst[b][0] = s1[b];
-    if (level2) xa.dst[b][1] = s2[b];
+    if (level1 && level2) xa.dst[b][1] = s2[b];
   }
-  xa.op[0] = xop(d.cut1_y, 1, d.cut1_x, XM_NONE, XMAP_ID, c.nx);
-  xa.op[1] = xop(d.cut2_y, 1, d.cut2_x, XM_NONE, XMAP_ID, c.nx);
-  xa.nops = level2 ? 2 : 1; xa.ncy_in = d.cut1_y; xa.nz = c.nz; xa.ny_full = c.ny;
+  xa.nops = nops; xa.ncy_in = ncy; xa.nz = c.nz; xa.ny_full = c.ny;
   x_pass<T>(P, c.s, xa, nb, P->fx, P->fx, P->tw_x, P->tw_x);
   const double sc = 1.0 / ((double)c.nx * c.ny);
-  {
+  for (int lev = 1; lev <= 2; ++lev) {
+    if ((lev == 1 && !level1) || (lev == 2 && !level2)) continue;
     YC2RArgs ya{};
-    for (int b = 0; b < nb; ++b) ya.in[b][0] = s1[b];
-    ya.nsum = 1; ya.ncy_in = d.cut1_y; ya.nz = c.nz; ya.scale = sc;
+    for (int b = 0; b < nb; ++b) ya.in[b][0] = lev == 1 ? s1[b] : s2[b];
+    ya.nsum = 1; ya.ncy_in = lev == 1 ? d.cut1_y : d.cut2_y; ya.nz = c.nz; ya.scale = sc;
     StorePtr<T> st{};
-    for (int b = 0; b < nb; ++b) st.p[b] = out1[b];
-    st.ny = c.ny; st.nz = c.nz;
-    y_c2r<T>(P, c.s, st, ya, nb, c.nx, P->fy, P->tw_y);
-  }
-  if (level2) {
-    YC2RArgs ya{};
-    for (int b = 0; b < nb; ++b) ya.in[b][0] = s2[b];
-    ya.nsum = 1; ya.ncy_in = d.cut2_y; ya.nz = c.nz; ya.scale = sc;
-    StorePtr<T> st{};
-    for (int b = 0; b < nb; ++b) st.p[b] = out2[b];
+    for (int b = 0; b < nb; ++b) st.p[b] = lev == 1 ? out1[b] : out2[b];
     st.ny = c.ny; st.nz = c.nz;
     y_c2r<T>(P, c.s, st, ya, nb, c.nx, P->fy, P->tw_y);
   }
@@ -337,7 +334,7 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     f1[b] = c.W("lf1_" + std::to_string(b));
     f2[b] = lasdd ? c.W("lf2_" + std::to_string(b)) : nullptr;
   }
-  test_filter<T>(c, ld, 21, lasdd, "ls", f1, f2);
+  test_filter<T>(c, ld, 21, true, lasdd, "ls", f1, f2);
   MomLevel<T> l1{}, l2{};
   for (int k = 0; k < 3; ++k) { l1.uh[k] = f1[k]; l2.uh[k] = f2[k]; }
   for (int n = 0; n < 6; ++n) {
@@ -351,7 +348,7 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     FLasddSums<T> f{l1, l2, c.nz, d.L, d.TFR};
     plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasdd");
     if (!P->ws.dry) {
-      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 0, coef);
+      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 0, d.TFR, coef);
       k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
       launch_ok(P);
     }
@@ -395,7 +392,7 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* 
     f1[b] = c.W("sf1_" + std::to_string(b));
     f2[b] = lasdd ? c.W("sf2_" + std::to_string(b)) : nullptr;
   }
-  test_filter<T>(c, ld, 10, lasdd, "ss", f1, f2);
+  test_filter<T>(c, ld, 10, true, lasdd, "ss", f1, f2);
   ScaLevel<T> l1{}, l2{};
   l1.th = f1[0]; l2.th = f2[0];
   for (int k = 0; k < 3; ++k) {
@@ -413,7 +410,7 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* 
     FScalarSums<T> f{l1, l2, c.nz, d.L, d.TFR};
     plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasdd");
     if (!P->ws.dry) {
-      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 1, coef);
+      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 1, d.TFR, coef);
       k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
       launch_ok(P);
     }
@@ -426,6 +423,129 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* 
   PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
   double* s2 = c.D("ssum2", 2 * c.nz);
   FBoxClip<T> fb{Tup, Tdn, CsPr, c.nx, c.ny, c.nz};
+  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "sbox");
+  if (!P->ws.dry) {
+    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
+    launch_ok(P);
+    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+  }
+}
+
+// Dynamic procedure of the Wong-Lilly momentum model (DynamicSGS_LASDD_WL.py:
+// 128-313): 15 filtered fields at level 1; at level 2 all 15 for LASDD-WL
+// (optSgs 2, beta) or only S_ij for LAD-WL (optSgs 4).
+template <class T>
+void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const Sij[6],
+                    const double* shu, const double* shv, T* Cwl, double* avg1, double* avg2,
+                    double* beta_out) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  const bool lasdd = P->cfg.optSgs == 2;
+  LoadLasdd<T> ld{};
+  ld.u = u; ld.v = v; ld.w = w;
+  for (int n = 0; n < 6; ++n) ld.S[n] = Sij[n];
+  ld.Sm = nullptr; ld.shu = shu; ld.shv = shv; ld.ny = c.ny; ld.nz = c.nz;
+  T* f1[15];
+  T* f2[15];
+  for (int b = 0; b < 15; ++b) {
+    f1[b] = c.W("lf1_" + std::to_string(b));
+    f2[b] = (lasdd || b >= 9) ? c.W("lf2_" + std::to_string(b)) : nullptr;
+  }
+  if (lasdd) {
+    test_filter<T>(c, ld, 15, true, true, "ls", f1, f2);
+  } else {
+    test_filter<T>(c, ld, 15, true, false, "ls", f1, f2);
+    ld.b0 = 9;
+    test_filter<T>(c, ld, 6, false, true, "lw", f2 + 9, f2 + 9);
+  }
+  WlMom<T> l1{}, l2{};
+  for (int k = 0; k < 3; ++k) { l1.uh[k] = f1[k]; l2.uh[k] = f2[k]; }
+  for (int n = 0; n < 6; ++n) {
+    l1.ph[n] = f1[3 + n]; l1.Sh[n] = f1[9 + n];
+    l2.ph[n] = f2[3 + n]; l2.Sh[n] = f2[9 + n];
+  }
+  double* beta = c.D("beta1", c.nz);
+  if (lasdd) {
+    double* sums = c.D("lsums", 4 * c.nz);
+    double* coef = c.D("lcoef", 6 * c.nz);
+    FWlSums<T> f{l1, l2, c.nz};
+    plane_sums<4>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasddwl");
+    if (!P->ws.dry) {
+      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 2, d.TFR, coef);
+      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_ok(P);
+    }
+  } else if (!P->ws.dry) {
+    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_ok(P);
+  }
+  T* LM = c.W("LM");
+  T* MM = c.W("MM");
+  PW_LAUNCH(P, c.s, c.F, k_wl_lm<T>, c.dims(), l1, l2, beta, d.L, d.TFR, LM, MM);
+  double* s2 = c.D("bsums", 2 * c.nz);
+  FBoxClip<T> fb{LM, MM, Cwl, c.nx, c.ny, c.nz};
+  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "box");
+  if (!P->ws.dry) {
+    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
+    launch_ok(P);
+    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+  }
+}
+
+// Dynamic procedure of the Wong-Lilly scalar model
+// (DynamicSGS_ScalarLASDD_WL.py:130-275): TH, (u,v,w)TH and the gradients
+// at level 1; at level 2 all 7 (optSgs 2) or the gradients only (optSgs 4).
+template <class T>
+void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* TH, const T* dx,
+                     const T* dy, const T* dz, const double* shu, const double* shv,
+                     const double* sht, T* CwlPr, double* prof, double* beta_out) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
+  const bool lasdd = P->cfg.optSgs == 2;
+  LoadScalar<T> ld{};
+  ld.u = u; ld.v = v; ld.w = w; ld.TH = TH; ld.dx = dx; ld.dy = dy; ld.dz = dz; ld.Sm = nullptr;
+  ld.shu = shu; ld.shv = shv; ld.sht = sht; ld.ny = c.ny; ld.nz = c.nz;
+  T* f1[7];
+  T* f2[7];
+  for (int b = 0; b < 7; ++b) {
+    f1[b] = c.W("sf1_" + std::to_string(b));
+    f2[b] = (lasdd || b >= 4) ? c.W("sf2_" + std::to_string(b)) : nullptr;
+  }
+  if (lasdd) {
+    test_filter<T>(c, ld, 7, true, true, "ss", f1, f2);
+  } else {
+    test_filter<T>(c, ld, 7, true, false, "ss", f1, f2);
+    ld.b0 = 4;
+    test_filter<T>(c, ld, 3, false, true, "sw", f2 + 4, f2 + 4);
+  }
+  WlSca<T> l1{}, l2{};
+  l1.th = f1[0]; l2.th = f2[0];
+  for (int k = 0; k < 3; ++k) {
+    l1.uth[k] = f1[1 + k]; l1.dh[k] = f1[4 + k];
+    l2.uth[k] = f2[1 + k]; l2.dh[k] = f2[4 + k];
+    l1.uh[k] = c.W("lf1_" + std::to_string(k));
+    l2.uh[k] = lasdd ? c.W("lf2_" + std::to_string(k)) : nullptr;
+  }
+  double* beta = c.D("beta2", c.nz);
+  if (lasdd) {
+    double* sums = c.D("ssums", 4 * c.nz);
+    double* coef = c.D("scoef", 6 * c.nz);
+    FWlScalarSums<T> f{l1, l2, c.nz};
+    plane_sums<4>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasddwl");
+    if (!P->ws.dry) {
+      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 2, d.TFR, coef);
+      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_ok(P);
+    }
+  } else if (!P->ws.dry) {
+    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_ok(P);
+  }
+  T* Tup = c.W("Tup");
+  T* Tdn = c.W("Tdn");
+  PW_LAUNCH(P, c.s, c.F, k_wl_scalar_updn<T>, c.dims(), l1, l2, beta, d.L, d.TFR, Tup, Tdn);
+  double* s2 = c.D("ssum2", 2 * c.nz);
+  FBoxClip<T> fb{Tup, Tdn, CwlPr, c.nx, c.ny, c.nz};
   plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "sbox");
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
@@ -578,22 +698,28 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   const double* shu = mixed ? means : nullptr;
   const double* shv = mixed ? means + c.nz : nullptr;
   const double* sht = mixed ? means + 2 * c.nz : nullptr;
-  if (dynamic)
+  const bool wl = g.optSgs == 2 || g.optSgs == 4;
+  if (dynamic && wl)
+    stage_lasdd_wl<T>(c, S.u, S.v, S.w, Sij, shu, shv, S.Cs2, st->Cs2_1D_avg1, st->Cs2_1D_avg2,
+                      st->beta1_1D);
+  else if (dynamic)
     stage_lasdd<T>(c, S.u, S.v, S.w, Sij, Sm, shu, shv, S.Cs2, st->Cs2_1D_avg1, st->Cs2_1D_avg2,
                    st->beta1_1D);
   T *txx = c.W("txx"), *tyy = c.W("tyy"), *tzz = c.W("tzz"), *txy = c.W("txy");
   T *txz = c.W("txz"), *tyz = c.W("tyz");
   T *S13w = c.W("S13w"), *S23w = c.W("S23w"), *Sw = c.W("Sw");
   const double logm = std::log(0.5 * d.dz * d.z_scale / g.z0m);
-  const T m2L2 = (T)(-2.0 * d.L * d.L);
+  // stress prefactor: -2 L^2 (SM, times |S|) or -2 L^(4/3) (WL, no |S|)
+  const T m2L2 = wl ? (T)(-2.0 * std::pow(d.L, 4.0 / 3.0)) : (T)(-2.0 * d.L * d.L);
+  const T* Sst = wl ? nullptr : Sm;
   T *pS = nullptr, *pSw = nullptr;
   if (!dealias) {
     PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
               dwdz, S13w, S23w, Sw, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
-    PW_LAUNCH(P, s, c.F, k_stress_uvp<T>, c.dims(), dudx, dvdy, dwdz, S12, Sm, S.Cs2, m2L2, txx,
+    PW_LAUNCH(P, s, c.F, k_stress_uvp<T>, c.dims(), dudx, dvdy, dwdz, S12, Sst, S.Cs2, m2L2, txx,
               tyy, tzz, txy);
-    PW_LAUNCH(P, s, c.F, k_stress_w<T>, c.dims(), S13w, S23w, Sw, S.Cs2, m2L2, txz, tyz, 1, S.u,
-              S.v, sf.M, S.psim, S.psim0, logm, 0.4, g.Ugal);
+    PW_LAUNCH(P, s, c.F, k_stress_w<T>, c.dims(), S13w, S23w, wl ? nullptr : Sw, S.Cs2, m2L2, txz,
+              tyz, 1, S.u, S.v, sf.M, S.psim, S.psim0, logm, 0.4, g.Ugal);
   } else {
     const size_t PF = (size_t)c.mx * c.my * c.nz;
     T* ps[6];
@@ -605,8 +731,8 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     const T* cs[1] = {S.Cs2};
     dealias1<T>(c, 1, cs, &pc);
     T *ptxx = c.Pad("txx"), *ptyy = c.Pad("tyy"), *ptzz = c.Pad("tzz"), *ptxy = c.Pad("txy");
-    PW_LAUNCH(P, s, PF, k_stress_uvp<T>, c.pdims(), ps[0], ps[1], ps[2], ps[3], pS, pc, m2L2, ptxx,
-              ptyy, ptzz, ptxy);
+    PW_LAUNCH(P, s, PF, k_stress_uvp<T>, c.pdims(), ps[0], ps[1], ps[2], ps[3],
+              wl ? (const T*)nullptr : pS, pc, m2L2, ptxx, ptyy, ptzz, ptxy);
     const T* pt4[4] = {ptxx, ptyy, ptzz, ptxy};
     T* t4[4] = {txx, tyy, tzz, txy};
     dealias2<T>(c, 4, pt4, t4);
@@ -620,7 +746,8 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     pSw = c.Pad("Sw");
     PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pSw);
     T *ptxz = c.Pad("txz"), *ptyz = c.Pad("tyz");
-    PW_LAUNCH(P, s, PF, k_stress_w<T>, c.pdims(), pw[4], pw[5], pSw, pc, m2L2, ptxz, ptyz, 0,
+    PW_LAUNCH(P, s, PF, k_stress_w<T>, c.pdims(), pw[4], pw[5], wl ? (const T*)nullptr : pSw, pc,
+              m2L2, ptxz, ptyz, 0,
               (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
               (const T*)nullptr, logm, 0.4, g.Ugal);
     const T* pt2[2] = {ptxz, ptyz};
@@ -629,21 +756,27 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, txz, tyz, S.u, S.v, sf.M, S.psim,
               S.psim0, logm, 0.4, g.Ugal, (T*)nullptr, (const T*)nullptr);
   }
-  if (dynamic)
+  if (dynamic && wl)
+    stage_scalar_wl<T>(c, S.u, S.v, S.w, S.TH, dTx, dTy, dTz, shu, shv, sht, S.CsPr,
+                       st->Cs2PrRatio_1D, st->beta2_1D);
+  else if (dynamic)
     stage_scalar_lasdd<T>(c, S.u, S.v, S.w, S.TH, dTx, dTy, dTz, Sm, shu, shv, sht, S.CsPr,
                           st->Cs2PrRatio_1D, st->beta2_1D);
+  // flux prefactor: -2 L^2 (SM, times |S|) or -L^(4/3) (WL, no |S|)
+  const T mq = wl ? (T)(-std::pow(d.L, 4.0 / 3.0)) : m2L2;
   T *qx = c.W("qx"), *qy = c.W("qy"), *qzf = c.W("qz");
   if (!dealias) {
-    PW_LAUNCH(P, s, c.F, k_scalar_flux<T>, c.dims(), dTx, dTy, dTz, Sm, Sw, S.CsPr, m2L2, qx, qy,
-              qzf, (const T*)sf.qz);
+    PW_LAUNCH(P, s, c.F, k_scalar_flux<T>, c.dims(), dTx, dTy, dTz, Sst, wl ? nullptr : Sw,
+              S.CsPr, mq, qx, qy, qzf, (const T*)sf.qz);
   } else {
     const size_t PF = (size_t)c.mx * c.my * c.nz;
     const T* in4[4] = {dTx, dTy, dTz, S.CsPr};
     T* p4[4] = {c.Pad("q0"), c.Pad("q1"), c.Pad("q2"), c.Pad("q3")};
     dealias1<T>(c, 4, in4, p4);
     T *pqx = c.Pad("qx"), *pqy = c.Pad("qy"), *pqz = c.Pad("qz");
-    PW_LAUNCH(P, s, PF, k_scalar_flux<T>, c.pdims(), p4[0], p4[1], p4[2], pS, pSw, p4[3], m2L2, pqx,
-              pqy, pqz, (const T*)nullptr);
+    PW_LAUNCH(P, s, PF, k_scalar_flux<T>, c.pdims(), p4[0], p4[1], p4[2],
+              wl ? (const T*)nullptr : pS, wl ? (const T*)nullptr : pSw, p4[3], mq, pqx, pqy, pqz,
+              (const T*)nullptr);
     const T* pq3[3] = {pqx, pqy, pqz};
     T* q3[3] = {qx, qy, qzf};
     dealias2<T>(c, 3, pq3, q3);

@@ -762,6 +762,47 @@ def ScalarFluxesW_interior(g, dTHdz, S, CsPr):
     return qz
 
 
+def StressesUVP_WL(g, S11, S22, S33, S12, Cwl):
+    """SGSStresses_WL.py:111-141 (and :70-103 on padded inputs, without the
+    final Dealias2): tau_ij = -2 L^(4/3) C_WL S_ij."""
+    pre = -2 * g.L ** (4 / 3) * Cwl
+    out = [pre * S11, pre * S22, pre * S33, pre * S12]
+    for t in out:
+        t[:, :, -1] = 0
+    return out
+
+
+def StressesW_interior_WL(g, S13, S23, Cwl):
+    """SGSStresses_WL.py:196-239 (and :146-191 padded) without the wall row."""
+    nz = g.nz
+    txz = np.zeros_like(S13)
+    tyz = np.zeros_like(S13)
+    pre = -2 * g.L ** (4 / 3) * StagGridAvg(Cwl[:, :, :nz - 1])
+    txz[:, :, 1:nz - 1] = pre * S13[:, :, 1:nz - 1]
+    tyz[:, :, 1:nz - 1] = pre * S23[:, :, 1:nz - 1]
+    return txz, tyz
+
+
+def ScalarFluxesUVP_WL(g, dTHdx, dTHdy, CwlPr):
+    """ScalarSGSFluxes_WL.py:96-117 (and :55-91 padded): q = -L^(4/3) C grad."""
+    pre = -g.L ** (4 / 3) * CwlPr
+    qx = pre * dTHdx
+    qy = pre * dTHdy
+    qx[:, :, -1] = 0
+    qy[:, :, -1] = 0
+    return qx, qy
+
+
+def ScalarFluxesW_interior_WL(g, dTHdz, CwlPr):
+    """ScalarSGSFluxes_WL.py:169-201 (and :122-164 padded) without the
+    surface row."""
+    nz = g.nz
+    qz = np.zeros_like(dTHdz)
+    qz[:, :, 1:nz - 1] = (-g.L ** (4 / 3) * StagGridAvg(CwlPr[:, :, :nz - 1])
+                          * dTHdz[:, :, 1:nz - 1])
+    return qz
+
+
 def LASDD_SM(g, u, v, w, S11, S22, S33, S12, S13, S23, S):
     """src/subgridscale/DynamicSGS_LASDD_SM.py:138-367 (optSgs 1 = LASDD,
     3 = LAD with beta = 1).
@@ -901,9 +942,123 @@ def ScalarLASDD_SM(g, mom, TH, dTHdx, dTHdy, dTHdz, S):
                 T_up=T_up, T_dn=T_dn, beta_coeffs=coeffs)
 
 
+def _wl_quintic(g, a1, a3, a6, a8):
+    """The WL polynomial of ABL07 (DynamicSGS_LASDD_WL.py:255-274,
+    DynamicSGS_ScalarLASDD_WL.py:224-243): four independent plane means,
+    the other six scaled by powers of TFR.  Returns (ff, ee, dd, cc, bb, aa).
+
+    Parity unpinned: no independent check of this algebra."""
+    T = g.TFR
+    a2 = -(T ** (8 / 3)) * a1
+    a4 = -2 * T ** (4 / 3) * a3
+    a5 = T ** (8 / 3) * a3
+    a7 = -T ** (4 / 3) * a6
+    a9 = -2 * T ** (8 / 3) * a8
+    a10 = T ** (16 / 3) * a8
+    aa = a1 * a3 - a6 * a8
+    bb = a1 * a4 - a7 * a8
+    cc = a2 * a3 + a1 * a5 - a6 * a9
+    dd = a2 * a4 - a7 * a9
+    ee = a2 * a5 - a6 * a10
+    ff = -a7 * a10
+    return ff, ee, dd, cc, bb, aa
+
+
+def LASDD_WL(g, u, v, w, S11, S22, S33, S12, S13, S23):
+    """src/subgridscale/DynamicSGS_LASDD_WL.py:128-313 (optSgs 2 = LASDD-WL,
+    4 = LAD-WL with beta = 1): Wong-Lilly M_ij = 2 L^(4/3) (S^_ij -
+    TFR^(4/3) beta S^^_ij)."""
+    nz = g.nz
+    L, TFR = g.L, g.TFR
+    u_ = u.copy(); v_ = v.copy()
+    w_ = np.zeros_like(w)
+    w_[:, :, 1:nz - 1] = StagGridAvg(w[:, :, 1:nz])
+    w_[:, :, 0] = 0.5 * w[:, :, 1]
+    w_[:, :, nz - 1] = w[:, :, nz - 2]
+    F1 = lambda F: Filtering_Level1(g, FFT(F))
+    F2 = lambda F: Filtering_Level2(g, FFT(F))
+    prods = [u_ * u_, v_ * v_, w_ * w_, u_ * v_, u_ * w_, v_ * w_]
+    Sij = [S11, S22, S33, S12, S13, S23]
+    uh = [F1(x) for x in (u_, v_, w_)]
+    ud = [F2(x) for x in (u_, v_, w_)]
+    pp_h = [F1(x) for x in prods]
+    pp_d = [F2(x) for x in prods]
+    Sh = [F1(x) for x in Sij]
+    Sd = [F2(x) for x in Sij]
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    Lij = [pp_h[n] - uh[a] * uh[b] for n, (a, b) in enumerate(pairs)]
+    Qij = [pp_d[n] - ud[a] * ud[b] for n, (a, b) in enumerate(pairs)]
+
+    def contract(A, B):
+        return (A[0] * B[0] + A[1] * B[1] + A[2] * B[2]
+                + 2 * (A[3] * B[3] + A[4] * B[4] + A[5] * B[5]))
+
+    if g.optSgs == 2:
+        co = _wl_quintic(g, PlanarMean(contract(Qij, Sh)), PlanarMean(contract(Sh, Sh)),
+                         PlanarMean(contract(Lij, Sh)), PlanarMean(contract(Sd, Sd)))
+        beta1 = ComputeBeta(*co)
+        coeffs = np.stack(co)
+    else:
+        beta1 = np.ones(nz)
+        coeffs = None
+    b3 = beta1.reshape(1, 1, -1)
+    T1 = 2 * L ** (4 / 3)
+    T2 = 2 * (TFR * L) ** (4 / 3)
+    Mij = [T1 * Sh[n] - T2 * b3 * Sd[n] for n in range(6)]
+    LM = contract(Lij, Mij)
+    MM = contract(Mij, Mij)
+    Cwl = Cs2_3D_from_LM_MM(LM, MM)
+    return dict(u_=u_, v_=v_, w_=w_, u_hat=uh, u_hatd=ud, S_hat=_Smag(*Sh), S_hatd=_Smag(*Sd),
+                Cs2_3D=Cwl, Cs2_1D_avg1=PlanarMean(np.sqrt(Cwl)) ** 2,
+                Cs2_1D_avg2=PlanarMean(Cwl), beta1_1D=beta1, LM=LM, MM=MM, beta_coeffs=coeffs)
+
+
+def ScalarLASDD_WL(g, mom, TH, dTHdx, dTHdy, dTHdz):
+    """src/subgridscale/DynamicSGS_ScalarLASDD_WL.py:130-275."""
+    nz = g.nz
+    L, TFR = g.L, g.TFR
+    u_, v_, w_ = mom["u_"], mom["v_"], mom["w_"]
+    uh, ud = mom["u_hat"], mom["u_hatd"]
+    THz = np.zeros_like(TH)
+    THz[:, :, 1:nz - 1] = StagGridAvg(dTHdz[:, :, 1:nz])
+    THz[:, :, 0] = dTHdz[:, :, 0]
+    THz[:, :, nz - 1] = dTHdz[:, :, nz - 1]
+    F1 = lambda F: Filtering_Level1(g, FFT(F))
+    F2 = lambda F: Filtering_Level2(g, FFT(F))
+    TH_hat, TH_hatd = F1(TH), F2(TH)
+    uTH_hat = [F1(x * TH) for x in (u_, v_, w_)]
+    uTH_hatd = [F2(x * TH) for x in (u_, v_, w_)]
+    grads = (dTHdx, dTHdy, THz)
+    dh = [F1(x) for x in grads]
+    dd_ = [F2(x) for x in grads]
+    LT = [uTH_hat[i] - uh[i] * TH_hat for i in range(3)]
+    QT = [uTH_hatd[i] - ud[i] * TH_hatd for i in range(3)]
+    dot = lambda A, B: A[0] * B[0] + A[1] * B[1] + A[2] * B[2]
+    a3t = dot(dh, dh)
+    a6t = dot(LT, dh)
+    a8t = dot(dd_, dd_)
+    if g.optSgs == 2:
+        co = _wl_quintic(g, PlanarMean(dot(QT, dh)), PlanarMean(a3t), PlanarMean(a6t),
+                         PlanarMean(a8t))
+        beta2 = ComputeBeta(*co)
+        coeffs = np.stack(co)
+    else:
+        beta2 = np.ones(nz)
+        coeffs = None
+    b3 = beta2.reshape(1, 1, -1)
+    T_up = L ** (4 / 3) * (a6t - TFR ** (4 / 3) * b3 * dot(LT, dd_))
+    T_dn = L ** (8 / 3) * (a3t - 2 * TFR ** (4 / 3) * b3 * dot(dh, dd_)
+                           + TFR ** (8 / 3) * b3 ** 2 * a8t)
+    CwlPr = Cs2_3D_from_LM_MM(T_up, T_dn)
+    return dict(Cs2PrRatio_3D=CwlPr, Cs2PrRatio_1D=PlanarMean(CwlPr), beta2_1D=beta2,
+                T_up=T_up, T_dn=T_dn, beta_coeffs=coeffs)
+
+
 def SGS_momentum_stresses(g, grads, Cs2_3D, u, v, M_sfc_loc, psi2D_m, psi2D_m0):
     """Stresses on uvp and w nodes for either branch:
-    DynamicSGS_Main.py:119-282 / StaticSGS_Main.py:76-250 (SM base model)."""
+    DynamicSGS_Main.py:119-282 / StaticSGS_Main.py:76-250; the Wong-Lilly
+    base model (optSgs 2, 4) drops |S| and uses 2 L^(4/3) C_WL."""
+    wl = g.optSgs in (2, 4)
     uvp = StrainsUVPnodes(g, *grads)
     wn = StrainsWnodes(g, *grads)
     S11, S22, S33, S12, S13, S23, S_uvp = uvp
@@ -912,18 +1067,28 @@ def SGS_momentum_stresses(g, grads, Cs2_3D, u, v, M_sfc_loc, psi2D_m, psi2D_m0):
         P = [D(x) for x in (S11, S22, S33, S12, S13, S23)]
         S_uvp_pad = _Smag(*P)
         Cp = D(Cs2_3D)
-        tt = StressesUVP(g, P[0], P[1], P[2], P[3], S_uvp_pad, Cp)
+        if wl:
+            tt = StressesUVP_WL(g, P[0], P[1], P[2], P[3], Cp)
+        else:
+            tt = StressesUVP(g, P[0], P[1], P[2], P[3], S_uvp_pad, Cp)
         txx, tyy, tzz, txy = [Dealias2(g, FFT_pad(t)) for t in tt]
         Pw = [D(x) for x in wn[:6]]
         S_w_pad = _Smag(*Pw)
-        txz_p, tyz_p = StressesW_interior(g, Pw[4], Pw[5], S_w_pad, Cp)
+        if wl:
+            txz_p, tyz_p = StressesW_interior_WL(g, Pw[4], Pw[5], Cp)
+        else:
+            txz_p, tyz_p = StressesW_interior(g, Pw[4], Pw[5], S_w_pad, Cp)
         txz = Dealias2(g, FFT_pad(txz_p))
         tyz = Dealias2(g, FFT_pad(tyz_p))
     else:
         S_uvp_pad = S_uvp
         S_w_pad = wn[6]
-        txx, tyy, tzz, txy = StressesUVP(g, S11, S22, S33, S12, S_uvp, Cs2_3D)
-        txz, tyz = StressesW_interior(g, wn[4], wn[5], wn[6], Cs2_3D)
+        if wl:
+            txx, tyy, tzz, txy = StressesUVP_WL(g, S11, S22, S33, S12, Cs2_3D)
+            txz, tyz = StressesW_interior_WL(g, wn[4], wn[5], Cs2_3D)
+        else:
+            txx, tyy, tzz, txy = StressesUVP(g, S11, S22, S33, S12, S_uvp, Cs2_3D)
+            txz, tyz = StressesW_interior(g, wn[4], wn[5], wn[6], Cs2_3D)
     twx, twy = Wall(g, u, v, M_sfc_loc, psi2D_m, psi2D_m0)
     txz[:, :, 0] = twx
     tyz[:, :, 0] = twy
@@ -933,14 +1098,24 @@ def SGS_momentum_stresses(g, grads, Cs2_3D, u, v, M_sfc_loc, psi2D_m, psi2D_m0):
 
 
 def SGS_scalar_fluxes(g, st, CsPr, dTHdx, dTHdy, dTHdz, qz_sfc):
-    """DynamicSGS_Main.py:420-478 / StaticSGS_Main.py:258-360 (SM)."""
+    """DynamicSGS_Main.py:420-478 / StaticSGS_Main.py:258-360 (SM, or WL
+    for optSgs 2, 4)."""
+    wl = g.optSgs in (2, 4)
     if g.optDealias == 1:
         D = lambda F: Dealias1(g, FFT(F))
         dxp, dyp, dzp, cp = D(dTHdx), D(dTHdy), D(dTHdz), D(CsPr)
-        qxp, qyp = ScalarFluxesUVP(g, dxp, dyp, st["S_uvp_pad"], cp)
+        if wl:
+            qxp, qyp = ScalarFluxesUVP_WL(g, dxp, dyp, cp)
+            qzp = ScalarFluxesW_interior_WL(g, dzp, cp)
+        else:
+            qxp, qyp = ScalarFluxesUVP(g, dxp, dyp, st["S_uvp_pad"], cp)
+            qzp = ScalarFluxesW_interior(g, dzp, st["S_w_pad"], cp)
         qx = Dealias2(g, FFT_pad(qxp))
         qy = Dealias2(g, FFT_pad(qyp))
-        qz = Dealias2(g, FFT_pad(ScalarFluxesW_interior(g, dzp, st["S_w_pad"], cp)))
+        qz = Dealias2(g, FFT_pad(qzp))
+    elif wl:
+        qx, qy = ScalarFluxesUVP_WL(g, dTHdx, dTHdy, CsPr)
+        qz = ScalarFluxesW_interior_WL(g, dTHdz, CsPr)
     else:
         qx, qy = ScalarFluxesUVP(g, dTHdx, dTHdy, st["S_uvp"], CsPr)
         qz = ScalarFluxesW_interior(g, dTHdz, st["S_w"], CsPr)
@@ -1075,7 +1250,7 @@ def initial_state(g: Grid, u, v, w, TH) -> State:
 
 def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False) -> State:
     """One iteration of the time loop, src/Main.py:158-540 (dry,
-    optAdvection=0, optGeoWind=0, SM base models)."""
+    optAdvection=0, optGeoWind=0; SM or WL base models)."""
     if thomas is None:
         thomas = ThomasPressureInit(g)
     it = s.iteration
@@ -1102,10 +1277,16 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False) -> State:
     diag = {}
     if dynamic:
         uvp = StrainsUVPnodes(g, *grads)
-        mom = LASDD_SM(g, u, v, w, *uvp)
+        if g.optSgs in (2, 4):
+            mom = LASDD_WL(g, u, v, w, *uvp[:6])
+        else:
+            mom = LASDD_SM(g, u, v, w, *uvp)
         Cs2_3D = mom["Cs2_3D"]
         st = SGS_momentum_stresses(g, grads, Cs2_3D, u, v, M, MOST[0], MOST[1])
-        sc = ScalarLASDD_SM(g, mom, TH, dTHdx, dTHdy, dTHdz, uvp[6])
+        if g.optSgs in (2, 4):
+            sc = ScalarLASDD_WL(g, mom, TH, dTHdx, dTHdy, dTHdz)
+        else:
+            sc = ScalarLASDD_SM(g, mom, TH, dTHdx, dTHdy, dTHdz, uvp[6])
         CsPr = sc["Cs2PrRatio_3D"]
         diag.update(Cs2_1D_avg1=mom["Cs2_1D_avg1"], Cs2_1D_avg2=mom["Cs2_1D_avg2"],
                     beta1_1D=mom["beta1_1D"], Cs2PrRatio_1D=sc["Cs2PrRatio_1D"],

@@ -29,7 +29,7 @@ def test_struct_layout_matches_header():
 @pytest.mark.parametrize("over,frag", [
     (dict(nx=31), "even"),
     (dict(FGR=3), "FGR"),
-    (dict(optSgs=2), "optSgs"),
+    (dict(optSgs=5), "optSgs"),
     (dict(nx=14, ny=14), "factor"),
 ])
 def test_plan_create_rejects_bad_configs(over, frag):

@@ -24,6 +24,10 @@ CASES = {
     "convective32": ("convective", dict(nx=32, ny=32, nz=32)),         # FGR=2, LASDD, hetero
     "stable32": ("stable", dict(nx=32, ny=32, nz=32)),                 # FGR=2, LAD, prescribed T
     "stable_static": ("stable", dict(nx=32, ny=32, nz=32, dynamicSGS_call_time=3)),
+    # Wong-Lilly base model: LASDD-WL (beta), LAD-WL with dealiasing, static steps
+    "convective_wl": ("convective", dict(nx=32, ny=32, nz=32, optSgs=2)),
+    "neutral_wl": ("neutral", dict(optSgs=4)),
+    "stable_wl_static": ("stable", dict(nx=32, ny=32, nz=32, optSgs=4, dynamicSGS_call_time=2)),
 }
 
 
@@ -53,11 +57,15 @@ def test_step_fp64_short(key):
     # The coefficient fields are ratios of box-filtered contractions whose
     # terms cancel strongly while the flow is still nearly laminar (values
     # down to 1e-12), so they carry the FFT rounding differences amplified;
-    # the prognostic fields above are the parity bar.
+    # the prognostic fields above are the parity bar.  The Wong-Lilly
+    # M_ij = 2 L^(4/3) (S^ - TFR^(4/3) beta S^^) is a difference of two
+    # filtered strains that are equal to rounding wherever the flow is still
+    # smooth, which amplifies that rounding once more.
     ec = rel_err(les.host("Cs2_3D"), s.Cs2_3D)
     ep = rel_err(les.host("Cs2PrRatio_3D"), s.Cs2PrRatio_3D)
     print(key, "Cs2 rel err %.2e, Cs2PrRatio rel err %.2e" % (ec, ep))
-    assert ec < 1e-6 and ep < 1e-6
+    tol = 1e-5 if les.plan.cfg.get("optSgs") in (2, 4) else 1e-6
+    assert ec < tol and ep < tol
 
 
 def test_step_fp64_100_neutral():
@@ -66,16 +74,21 @@ def test_step_fp64_100_neutral():
     assert max(errs) < 1e-11
 
 
-@pytest.mark.parametrize("key", ["neutral32", "stable32"])
+@pytest.mark.parametrize("key", ["neutral32", "stable32", "convective32", "convective_wl"])
 def test_step_mixed_profiles(key):
     s, les, errs = run_both(key, 20, "mixed")
     f = les.fields()
-    for k in ("u", "v", "TH"):
-        pg = f[k].mean(axis=(0, 1))
-        po = getattr(s, k).mean(axis=(0, 1))
-        # profiles of the stored fields (u - Ugal, TH - T_0): compare against
-        # the absolute profile so the relative measure is the physical one
-        off = {"u": les.plan.cfg["Ugal"], "v": 0.0, "TH": les.plan.cfg["T_0"]}[k]
-        e = np.max(np.abs(pg - po)) / max(np.max(np.abs(po + off)), 1e-30)
-        print(key, k, "mixed profile rel err %.2e" % e)
+    cfg = les.plan.cfg
+    # mean wind profile (u + Ugal, v): error relative to the largest mean wind
+    # speed, the physical scale of both components (v alone can be near 0)
+    ug = {k: f[k].mean(axis=(0, 1)) for k in ("u", "v", "TH")}
+    uo = {k: getattr(s, k).mean(axis=(0, 1)) for k in ("u", "v", "TH")}
+    speed = np.max(np.hypot(uo["u"] + cfg["Ugal"], uo["v"]))
+    for k in ("u", "v"):
+        e = np.max(np.abs(ug[k] - uo[k])) / speed
+        print(key, k, "mixed profile err / max mean wind speed %.2e" % e)
         assert e < 1e-4
+    # potential temperature: relative to the absolute temperature profile
+    e = np.max(np.abs(ug["TH"] - uo["TH"])) / np.max(np.abs(uo["TH"] + cfg["T_0"]))
+    print(key, "TH mixed profile rel err %.2e" % e)
+    assert e < 1e-4

@@ -218,3 +218,26 @@ def test_rhs_momentum_sponge_and_buoyancy_terms():
     assert np.allclose(Ru, -rs * 0.3 * np.cos(Y), atol=1e-12)
     assert np.allclose(Rv, -rs * 0.2 * np.sin(X), atol=1e-12)
     assert np.allclose(Rw, buoy - rf * w, atol=1e-12)
+
+
+def test_wong_lilly_stresses_and_fluxes():
+    # Wong-Lilly: tau_ij = -2 C_WL Delta^(4/3) S_ij and q_i = -C Delta^(4/3) dtheta/dx_i,
+    # both free of |S|; C on w levels is the mean of the two uvp levels around it
+    g, X, Y, Zu, Zw = grid()
+    s, c = 0.37, 0.021
+    zero = np.zeros_like(X)
+    S12 = np.full_like(X, s)
+    C = np.full_like(X, c)
+    txx, tyy, tzz, txy = O.StressesUVP_WL(g, zero.copy(), zero.copy(), zero.copy(), S12, C)
+    d43 = g.L ** (4 / 3)
+    assert np.allclose(txy[:, :, :-1], -2 * c * d43 * s)
+    assert np.all(txy[:, :, -1] == 0)
+    Cz = c * (1 + Zu)                      # varies with height
+    txz, tyz = O.StressesW_interior_WL(g, S12, zero, Cz)
+    cw = c * (1 + Zw)                      # uvp mean at w levels = C at z_w (linear)
+    assert np.allclose(txz[:, :, 1:-1], -2 * d43 * cw[:, :, 1:-1] * s)
+    gx = np.cos(X)
+    qx, qy = O.ScalarFluxesUVP_WL(g, gx, zero.copy(), C)
+    assert np.allclose(qx[:, :, :-1], -c * d43 * gx[:, :, :-1])
+    qz = O.ScalarFluxesW_interior_WL(g, np.full_like(X, 2.0), Cz)
+    assert np.allclose(qz[:, :, 1:-1], -d43 * cw[:, :, 1:-1] * 2.0)
