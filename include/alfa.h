@@ -221,6 +221,35 @@ int alfa_strains_w(alfa_plan*, const void* const grads[9], void* out[3], alfa_st
 int alfa_lasdd_sm(alfa_plan*, const void* u, const void* v, const void* w,
                   const void* const strains[7], void* Cs2_3D, double* Cs2_1D_avg1,
                   double* Cs2_1D_avg2, double* beta1_1D, alfa_stream);
+/* LASDD (DynamicSGS_LASDD_WL.py:128) for optSgs 2 (LASDD-WL) or 4 (LAD-WL):
+ * strains = S11,S22,S33,S12,S13,S23; writes C_WL (Cwl_3D) and the profiles. */
+int alfa_lasdd_wl(alfa_plan*, const void* u, const void* v, const void* w,
+                  const void* const strains[6], void* Cwl_3D, double* Cwl_1D_avg1,
+                  double* Cwl_1D_avg2, double* beta1_1D, alfa_stream);
+/* ScalarLASDD (DynamicSGS_ScalarLASDD_SM.py:139 for optSgs 1/3,
+ * DynamicSGS_ScalarLASDD_WL.py:130 for optSgs 2/4), arguments as the
+ * reference: vel = u_, v_, w_ (w_ on uvp nodes), hat1 = u_hat, v_hat, w_hat,
+ * hat2 = u_hatd, v_hatd, w_hatd (may be NULL for optSgs 3, 4); S, S_hat,
+ * S_hatd are |S|, |S^|, |S^^| (SM only; S_hatd only for optSgs 1). */
+int alfa_scalar_lasdd(alfa_plan*, const void* const vel[3], const void* const hat1[3],
+                      const void* const hat2[3], const void* TH, const void* const dTH[3],
+                      const void* S, const void* S_hat, const void* S_hatd,
+                      void* Cs2PrRatio_3D, double* Cs2PrRatio_1D, double* beta2_1D,
+                      alfa_stream);
+/* SGS stresses from the velocity gradients (dudx,dvdx,dwdx,dudy,dvdy,dwdy,
+ * dudz,dvdz,dwdz): strain rates (StrainRates.py:56-368), then
+ * Stresses{UVP,W}nodes_{Dealias,NoDealias} of SGSStresses_SM.py:98-297 or
+ * SGSStresses_WL.py:70-239 per optSgs, with Wall (:63) at level 0, as in
+ * DynamicSGS_Main.py:136-282 / StaticSGS_Main.py:76-250.
+ * tau = txx, tyy, tzz, txy, txz, tyz. */
+int alfa_sgs_stresses(alfa_plan*, const void* const grads[9], const void* Cs2_3D,
+                      const void* u, const void* v, const void* M_sfc_loc,
+                      const void* psi_m, const void* psi_m0, void* tau[6], alfa_stream);
+/* SGS scalar fluxes q = qx, qy, qz (ScalarSGSFluxes_SM.py:55-222 or
+ * ScalarSGSFluxes_WL.py:55-201 per optSgs; qz = qz_sfc at level 0), as in
+ * DynamicSGS_Main.py:420-478 / StaticSGS_Main.py:258-360. */
+int alfa_scalar_fluxes(alfa_plan*, const void* const grads[9], const void* const dTH[3],
+                       const void* Cs2PrRatio_3D, const void* qz_sfc, void* q[3], alfa_stream);
 /* Wall (SGSStresses_SM.py:63): planes (nx,ny). */
 int alfa_wall(alfa_plan*, const void* u, const void* v, const void* M_sfc_loc,
               const void* psi_m, const void* psi_m0, void* txz, void* tyz,

@@ -109,6 +109,14 @@ _SIGS = {
     "alfa_strains_uvp": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _P],
     "alfa_strains_w": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _P],
     "alfa_lasdd_sm": [_P, _P, _P, _P, C.POINTER(C.c_void_p), _P, _P, _P, _P, _P],
+    "alfa_lasdd_wl": [_P, _P, _P, _P, C.POINTER(C.c_void_p), _P, _P, _P, _P, _P],
+    "alfa_scalar_lasdd": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                          C.POINTER(C.c_void_p), _P, C.POINTER(C.c_void_p), _P, _P, _P, _P, _P,
+                          _P, _P],
+    "alfa_sgs_stresses": [_P, C.POINTER(C.c_void_p), _P, _P, _P, _P, _P, _P,
+                          C.POINTER(C.c_void_p), _P],
+    "alfa_scalar_fluxes": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _P, _P,
+                           C.POINTER(C.c_void_p), _P],
     "alfa_wall": [_P] * 9,
     "alfa_surface_flux": [_P, _P, _P, _P, C.c_double] + [_P] * 12,
     "alfa_profile_begin": [_P, C.c_char_p, C.c_int],

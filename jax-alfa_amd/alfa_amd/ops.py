@@ -376,6 +376,62 @@ def LASDD(u, v, w, S11, S22, S33, S12, S13, S23, S, ZeRo3D=None):
     return cs, pr[0], pr[1], pr[2]
 
 
+def LASDD_WL(u, v, w, S11, S22, S33, S12, S13, S23, ZeRo3D=None):
+    """src/subgridscale/DynamicSGS_LASDD_WL.py:128 (optSgs 2 or 4 of the plan)
+    -> (Cwl_3D, Cwl_1D_avg1, Cwl_1D_avg2, beta1_1D)."""
+    torch = _torch()
+    P = _P()
+    cs = P.field()
+    pr = [P.zeros((P.nz,), dtype=torch.float64) for _ in range(3)]
+    P.call("alfa_lasdd_wl", P.ptr(u), P.ptr(v), P.ptr(w),
+           _ptr_array(P, [S11, S22, S33, S12, S13, S23]), P.ptr(cs),
+           *[P.ptr(t) for t in pr], current_stream())
+    return cs, pr[0], pr[1], pr[2]
+
+
+def ScalarLASDD(u_, v_, w_, u_hat, v_hat, w_hat, u_hatd, v_hatd, w_hatd, TH,
+                dTHdx, dTHdy, dTHdz, S=None, S_hat=None, S_hatd=None, ZeRo3D=None):
+    """DynamicSGS_ScalarLASDD_SM.py:139 (optSgs 1, 3) / DynamicSGS_ScalarLASDD_WL.py:130
+    (optSgs 2, 4) -> (Cs2PrRatio_3D, Cs2PrRatio_1D, beta2_1D).  u_hatd.. may be
+    None for the beta = 1 models; S, S_hat, S_hatd are used by SM only."""
+    torch = _torch()
+    P = _P()
+    cs = P.field()
+    pr = [P.zeros((P.nz,), dtype=torch.float64) for _ in range(2)]
+    h2 = None if u_hatd is None else _ptr_array(P, [u_hatd, v_hatd, w_hatd])
+    P.call("alfa_scalar_lasdd", _ptr_array(P, [u_, v_, w_]), _ptr_array(P, [u_hat, v_hat, w_hat]),
+           h2, P.ptr(TH), _ptr_array(P, [dTHdx, dTHdy, dTHdz]), P.ptr(S), P.ptr(S_hat),
+           P.ptr(S_hatd), P.ptr(cs), P.ptr(pr[0]), P.ptr(pr[1]), current_stream())
+    return cs, pr[0], pr[1]
+
+
+def SGSStresses(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, Cs2_3D,
+                u, v, M_sfc_loc, psi2D_m, psi2D_m0):
+    """Strain rates + Stresses{UVP,W}nodes (SM or WL per the plan's optSgs,
+    dealiased when FGR = 1) with the wall stress, as DynamicSGS_Main.py:136-282
+    and StaticSGS_Main.py:76-250 compute them -> (txx, tyy, tzz, txy, txz, tyz)."""
+    P = _P()
+    o = [P.field() for _ in range(6)]
+    P.call("alfa_sgs_stresses",
+           _ptr_array(P, [dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz]), P.ptr(Cs2_3D),
+           *[P.ptr(t) for t in (u, v, M_sfc_loc, psi2D_m, psi2D_m0)], _ptr_array(P, o),
+           current_stream())
+    return tuple(o)
+
+
+def ScalarFluxes(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, dTHdx, dTHdy, dTHdz,
+                 Cs2PrRatio_3D, qz_sfc):
+    """ScalarFluxes{UVP,W}nodes (SM or WL per the plan, dealiased when FGR = 1),
+    DynamicSGS_Main.py:420-478 / StaticSGS_Main.py:258-360 -> (qx, qy, qz)."""
+    P = _P()
+    o = [P.field() for _ in range(3)]
+    P.call("alfa_scalar_fluxes",
+           _ptr_array(P, [dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz]),
+           _ptr_array(P, [dTHdx, dTHdy, dTHdz]), P.ptr(Cs2PrRatio_3D), P.ptr(qz_sfc),
+           _ptr_array(P, o), current_stream())
+    return tuple(o)
+
+
 def Wall(u, v, M_sfc_loc, psi2D_m, psi2D_m0):
     """src/subgridscale/SGSStresses_SM.py:63 -> (txz, tyz) planes."""
     P = _P()

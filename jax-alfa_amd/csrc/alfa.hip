@@ -352,6 +352,70 @@ int lasdd(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void*
 }
 
 template <class T>
+int lasdd_wl(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* w,
+             const void* const str[6], void* Cwl, double* a1, double* a2, double* beta) {
+  Ctx<T> c(P, s);
+  const T* Sij[6];
+  for (int n = 0; n < 6; ++n) Sij[n] = (const T*)str[n];
+  stage_lasdd_wl<T>(c, (const T*)u, (const T*)v, (const T*)w, Sij, nullptr, nullptr, (T*)Cwl, a1,
+                    a2, beta);
+  return 0;
+}
+
+template <class T>
+int scalar_lasdd(alfa_plan* P, hipStream_t s, const void* const vel[3], const void* const h1[3],
+                 const void* const h2[3], const void* TH, const void* const dT[3], const void* S,
+                 const void* Shat, const void* Shatd, void* CsPr, double* prof, double* beta) {
+  Ctx<T> c(P, s);
+  MomHats<T> mh{};
+  for (int k = 0; k < 3; ++k) {
+    mh.uh1[k] = (const T*)h1[k];
+    mh.uh2[k] = h2 ? (const T*)h2[k] : nullptr;
+  }
+  mh.Shat = (const T*)Shat;
+  mh.Shatd = (const T*)Shatd;
+  if (wl_model(P))
+    stage_scalar_wl<T>(c, (const T*)vel[0], (const T*)vel[1], (const T*)vel[2], 1, (const T*)TH,
+                       (const T*)dT[0], (const T*)dT[1], (const T*)dT[2], mh, nullptr, nullptr,
+                       nullptr, (T*)CsPr, prof, beta);
+  else
+    stage_scalar_lasdd<T>(c, (const T*)vel[0], (const T*)vel[1], (const T*)vel[2], 1,
+                          (const T*)TH, (const T*)dT[0], (const T*)dT[1], (const T*)dT[2],
+                          (const T*)S, mh, nullptr, nullptr, nullptr, (T*)CsPr, prof, beta);
+  return 0;
+}
+
+template <class T>
+int sgs_stresses(alfa_plan* P, hipStream_t s, const void* const g[9], const void* Cs2,
+                 const void* u, const void* v, const void* M, const void* pm, const void* pm0,
+                 void* const tau[6]) {
+  Ctx<T> c(P, s);
+  const T* gg[9];
+  for (int n = 0; n < 9; ++n) gg[n] = (const T*)g[n];
+  SgsStrains<T> ss{};
+  stage_strains<T>(c, gg, ss);
+  T* t[6];
+  for (int n = 0; n < 6; ++n) t[n] = (T*)tau[n];
+  stage_stresses<T>(c, ss, (const T*)Cs2, (const T*)u, (const T*)v, (const T*)M, (const T*)pm,
+                    (const T*)pm0, t);
+  return 0;
+}
+
+template <class T>
+int scalar_fluxes(alfa_plan* P, hipStream_t s, const void* const g[9], const void* const dT[3],
+                  const void* CsPr, const void* qsfc, void* const q[3]) {
+  Ctx<T> c(P, s);
+  const T* gg[9];
+  for (int n = 0; n < 9; ++n) gg[n] = (const T*)g[n];
+  SgsStrains<T> ss{};
+  stage_strains<T>(c, gg, ss);
+  const T* d3[3] = {(const T*)dT[0], (const T*)dT[1], (const T*)dT[2]};
+  T* q3[3] = {(T*)q[0], (T*)q[1], (T*)q[2]};
+  stage_fluxes<T>(c, ss, d3, (const T*)CsPr, (const T*)qsfc, q3);
+  return 0;
+}
+
+template <class T>
 int surface(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* TH,
             double sfc_val, void* pm, void* pm0, void* ph, void* ph0, void* fm, void* fh,
             void* M, void* ustar, void* qz, void* invOB, double* qavg) {
@@ -399,7 +463,16 @@ template <class T> static void plan_dry_run(alfa_plan* P) {
   stage_div_stress<T>(c, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f, (T*)f);
   stage_div_flux<T>(c, (T*)f, (T*)f, (T*)f, (T*)f);
   const void* seven[7] = {f, f, f, f, f, f, f};
-  ops::lasdd<T>(P, s, f, f, f, seven, f, nullptr, nullptr, nullptr);
+  const void* nine[9] = {f, f, f, f, f, f, f, f, f};
+  void* six[6] = {f, f, f, f, f, f};
+  void* three_o[3] = {f, f, f};
+  const int o = P->cfg.optSgs;
+  if (o == 1 || o == 3) ops::lasdd<T>(P, s, f, f, f, seven, f, nullptr, nullptr, nullptr);
+  if (o == 2 || o == 4) ops::lasdd_wl<T>(P, s, f, f, f, seven, f, nullptr, nullptr, nullptr);
+  if (o >= 1 && o <= 4)
+    ops::scalar_lasdd<T>(P, s, three, three, three, f, three, f, f, f, f, nullptr, nullptr);
+  ops::sgs_stresses<T>(P, s, nine, f, f, f, f, f, f, six);
+  ops::scalar_fluxes<T>(P, s, nine, three, f, f, three_o);
 }
 
 template <class T> static int allocate_workspace(alfa_plan* P) {
@@ -818,6 +891,46 @@ extern "C" int alfa_lasdd_sm(alfa_plan* P, const void* u, const void* v, const v
   if (P && P->cfg.optSgs != 1 && P->cfg.optSgs != 3)
     return set_error(ALFA_EINVAL, "alfa_lasdd_sm: plan optSgs must be 1 or 3");
   DISPATCH(P, "alfa_lasdd_sm", ops::lasdd<T>(P, STREAM(s), u, v, w, str, Cs2, a1, a2, beta));
+}
+extern "C" int alfa_lasdd_wl(alfa_plan* P, const void* u, const void* v, const void* w,
+                             const void* const str[6], void* Cwl, double* a1, double* a2,
+                             double* beta, alfa_stream s) {
+  if (P && P->cfg.optSgs != 2 && P->cfg.optSgs != 4)
+    return set_error(ALFA_EINVAL, "alfa_lasdd_wl: plan optSgs must be 2 or 4");
+  DISPATCH(P, "alfa_lasdd_wl", ops::lasdd_wl<T>(P, STREAM(s), u, v, w, str, Cwl, a1, a2, beta));
+}
+extern "C" int alfa_scalar_lasdd(alfa_plan* P, const void* const vel[3], const void* const hat1[3],
+                                 const void* const hat2[3], const void* TH,
+                                 const void* const dTH[3], const void* S, const void* S_hat,
+                                 const void* S_hatd, void* Cs2PrRatio_3D, double* Cs2PrRatio_1D,
+                                 double* beta2, alfa_stream s) {
+  if (P) {
+    const int o = P->cfg.optSgs;
+    if (o < 1 || o > 4)
+      return set_error(ALFA_EINVAL, "alfa_scalar_lasdd: plan optSgs must be 1..4");
+    if ((o == 1 || o == 2) && !hat2)
+      return set_error(ALFA_EINVAL, "alfa_scalar_lasdd: level-2 velocity hats are required for beta");
+    if ((o == 1 || o == 3) && (!S || !S_hat))
+      return set_error(ALFA_EINVAL, "alfa_scalar_lasdd: S and S_hat are required (SM)");
+    if (o == 1 && !S_hatd)
+      return set_error(ALFA_EINVAL, "alfa_scalar_lasdd: S_hatd is required (LASDD-SM)");
+  }
+  DISPATCH(P, "alfa_scalar_lasdd",
+           ops::scalar_lasdd<T>(P, STREAM(s), vel, hat1, hat2, TH, dTH, S, S_hat, S_hatd,
+                                Cs2PrRatio_3D, Cs2PrRatio_1D, beta2));
+}
+extern "C" int alfa_sgs_stresses(alfa_plan* P, const void* const grads[9], const void* Cs2_3D,
+                                 const void* u, const void* v, const void* M_sfc_loc,
+                                 const void* psi_m, const void* psi_m0, void* tau[6],
+                                 alfa_stream s) {
+  DISPATCH(P, "alfa_sgs_stresses",
+           ops::sgs_stresses<T>(P, STREAM(s), grads, Cs2_3D, u, v, M_sfc_loc, psi_m, psi_m0, tau));
+}
+extern "C" int alfa_scalar_fluxes(alfa_plan* P, const void* const grads[9],
+                                  const void* const dTH[3], const void* Cs2PrRatio_3D,
+                                  const void* qz_sfc, void* q[3], alfa_stream s) {
+  DISPATCH(P, "alfa_scalar_fluxes",
+           ops::scalar_fluxes<T>(P, STREAM(s), grads, dTH, Cs2PrRatio_3D, qz_sfc, q));
 }
 extern "C" int alfa_wall(alfa_plan* P, const void* u, const void* v, const void* M,
                          const void* pm, const void* pm0, void* txz, void* tyz, alfa_stream s) {

@@ -124,7 +124,8 @@ template <class T> struct LoadScalar {
   const T* dx; const T* dy; const T* dz; const T* Sm;
   const double* shu; const double* shv; const double* sht;
   int ny, nz;
-  int b0;   // field b of the launch is virtual field b + b0
+  int b0;      // field b of the launch is virtual field b + b0
+  int w_uvp;   // w is given on uvp nodes already (no averaging)
   __device__ __forceinline__ T thz(size_t i, int z) const {
     if (z == 0 || z == nz - 1) return dz[i];
     return T(0.5) * (dz[i] + dz[i + 1]);
@@ -146,7 +147,8 @@ template <class T> struct LoadScalar {
     if (b == 1) return (shu ? (T)((double)u[i] - shu[z]) : u[i]) * th;
     if (b == 2) return (shv ? (T)((double)v[i] - shv[z]) : v[i]) * th;
     T ww;
-    if (z == 0) ww = T(0.5) * w[i + 1];
+    if (w_uvp) ww = w[i];
+    else if (z == 0) ww = T(0.5) * w[i + 1];
     else if (z == nz - 1) ww = w[i - 1];
     else ww = T(0.5) * (w[i] + w[i + 1]);
     return ww * th;
@@ -194,7 +196,7 @@ template <class T> struct LoadScalar {
     if (b == 0) return th;
     Vec16<T> f;
     if (b == 3) {
-      f = stag(w, i, z, true);
+      f = w_uvp ? vld(w + i) : stag(w, i, z, true);
     } else {
       f = vld((b == 1 ? u : v) + i);
       const double* sh = b == 1 ? shu : shv;

@@ -375,17 +375,39 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
 }
 
 // Dynamic procedure of the scalar model (DynamicSGS_ScalarLASDD_SM.py:139-316).
+// The momentum stage's filtered velocities and |S^| (its outputs u_hat..
+// S_hatd, DynamicSGS_Main.py:326-331) that the scalar models reuse.
+template <class T> struct MomHats {
+  const T* uh1[3];
+  const T* uh2[3];    // NULL where not computed (beta = 1 models)
+  const T* Shat;
+  const T* Shatd;
+};
+
+template <class T> MomHats<T> step_hats(Ctx<T>& c) {
+  MomHats<T> h{};
+  const int o = c.P->cfg.optSgs;
+  for (int k = 0; k < 3; ++k) {
+    h.uh1[k] = c.W("lf1_" + std::to_string(k));
+    h.uh2[k] = (o == 1 || o == 2) ? c.W("lf2_" + std::to_string(k)) : nullptr;
+  }
+  h.Shat = c.W("Shat");
+  h.Shatd = o == 1 ? c.W("Shatd") : nullptr;
+  return h;
+}
+
+// w_uvp: w is already averaged to uvp nodes (the reference's w_)
 template <class T>
-void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* TH,
-                        const T* dx, const T* dy, const T* dz, const T* Sm, const double* shu,
-                        const double* shv, const double* sht, T* CsPr, double* prof,
-                        double* beta_out) {
+void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, const T* TH,
+                        const T* dx, const T* dy, const T* dz, const T* Sm, const MomHats<T>& mh,
+                        const double* shu, const double* shv, const double* sht, T* CsPr,
+                        double* prof, double* beta_out) {
   alfa_plan* P = c.P;
   const auto& d = P->d;
   const bool lasdd = P->cfg.optSgs == 1;
   LoadScalar<T> ld{};
   ld.u = u; ld.v = v; ld.w = w; ld.TH = TH; ld.dx = dx; ld.dy = dy; ld.dz = dz; ld.Sm = Sm;
-  ld.shu = shu; ld.shv = shv; ld.sht = sht; ld.ny = c.ny; ld.nz = c.nz;
+  ld.shu = shu; ld.shv = shv; ld.sht = sht; ld.ny = c.ny; ld.nz = c.nz; ld.w_uvp = w_uvp;
   T* f1[10];
   T* f2[10];
   for (int b = 0; b < 10; ++b) {
@@ -398,11 +420,11 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* 
   for (int k = 0; k < 3; ++k) {
     l1.uth[k] = f1[1 + k]; l1.dh[k] = f1[4 + k]; l1.sdh[k] = f1[7 + k];
     l2.uth[k] = f2[1 + k]; l2.dh[k] = f2[4 + k]; l2.sdh[k] = f2[7 + k];
-    l1.uh[k] = c.W("lf1_" + std::to_string(k));
-    l2.uh[k] = lasdd ? c.W("lf2_" + std::to_string(k)) : nullptr;
+    l1.uh[k] = mh.uh1[k];
+    l2.uh[k] = mh.uh2[k];
   }
-  l1.Sm = c.W("Shat");
-  l2.Sm = lasdd ? c.W("Shatd") : nullptr;
+  l1.Sm = mh.Shat;
+  l2.Sm = mh.Shatd;
   double* beta = c.D("beta2", c.nz);
   if (lasdd) {
     double* sums = c.D("ssums", 10 * c.nz);
@@ -496,15 +518,16 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
 // (DynamicSGS_ScalarLASDD_WL.py:130-275): TH, (u,v,w)TH and the gradients
 // at level 1; at level 2 all 7 (optSgs 2) or the gradients only (optSgs 4).
 template <class T>
-void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* TH, const T* dx,
-                     const T* dy, const T* dz, const double* shu, const double* shv,
-                     const double* sht, T* CwlPr, double* prof, double* beta_out) {
+void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, const T* TH,
+                     const T* dx, const T* dy, const T* dz, const MomHats<T>& mh,
+                     const double* shu, const double* shv, const double* sht, T* CwlPr,
+                     double* prof, double* beta_out) {
   alfa_plan* P = c.P;
   const auto& d = P->d;
   const bool lasdd = P->cfg.optSgs == 2;
   LoadScalar<T> ld{};
   ld.u = u; ld.v = v; ld.w = w; ld.TH = TH; ld.dx = dx; ld.dy = dy; ld.dz = dz; ld.Sm = nullptr;
-  ld.shu = shu; ld.shv = shv; ld.sht = sht; ld.ny = c.ny; ld.nz = c.nz;
+  ld.shu = shu; ld.shv = shv; ld.sht = sht; ld.ny = c.ny; ld.nz = c.nz; ld.w_uvp = w_uvp;
   T* f1[7];
   T* f2[7];
   for (int b = 0; b < 7; ++b) {
@@ -523,8 +546,8 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* TH,
   for (int k = 0; k < 3; ++k) {
     l1.uth[k] = f1[1 + k]; l1.dh[k] = f1[4 + k];
     l2.uth[k] = f2[1 + k]; l2.dh[k] = f2[4 + k];
-    l1.uh[k] = c.W("lf1_" + std::to_string(k));
-    l2.uh[k] = lasdd ? c.W("lf2_" + std::to_string(k)) : nullptr;
+    l1.uh[k] = mh.uh1[k];
+    l2.uh[k] = mh.uh2[k];
   }
   double* beta = c.D("beta2", c.nz);
   if (lasdd) {
@@ -552,6 +575,131 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* TH,
     launch_ok(P);
     if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
   }
+}
+
+// Strain rates shared by the stress and flux stages (StrainRates.py:56-368):
+// S_ij and |S| on uvp nodes, S13, S23, |S| on w nodes, and with
+// dealiasing (FGR = 1) their padded copies.
+template <class T> struct SgsStrains {
+  const T* Sij[6];    // S11 (= du/dx), S22, S33, S12, S13, S23 on uvp nodes
+  T* Sm;              // |S| on uvp nodes
+  T *S13w, *S23w, *Sw;
+  T* ps[6];           // padded uvp components (dealias)
+  T* pw[6];           // padded w-node components (dealias)
+  T *pS, *pSw;        // padded |S| (dealias)
+};
+
+template <class T>
+void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o) {
+  alfa_plan* P = c.P;
+  hipStream_t s = c.s;
+  T *S12 = c.W("S12"), *S13 = c.W("S13"), *S23 = c.W("S23");
+  o.Sm = c.W("S");
+  PW_LAUNCH(P, s, c.F, k_strains_uvp<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7],
+            g[8], S12, S13, S23, o.Sm);
+  o.Sij[0] = g[0]; o.Sij[1] = g[4]; o.Sij[2] = g[8];
+  o.Sij[3] = S12; o.Sij[4] = S13; o.Sij[5] = S23;
+  o.S13w = c.W("S13w"); o.S23w = c.W("S23w"); o.Sw = c.W("Sw");
+  if (P->d.optDealias != 1) {
+    PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7],
+              g[8], o.S13w, o.S23w, o.Sw, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
+    o.pS = o.pSw = nullptr;
+    return;
+  }
+  const size_t PF = (size_t)c.mx * c.my * c.nz;
+  for (int n = 0; n < 6; ++n) o.ps[n] = c.Pad("su" + std::to_string(n));
+  dealias1<T>(c, 6, o.Sij, o.ps);
+  o.pS = c.Pad("S");
+  PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), o.ps[0], o.ps[1], o.ps[2], o.ps[3], o.ps[4], o.ps[5],
+            o.pS);
+  T *S11w = c.W("S11w"), *S22w = c.W("S22w"), *S33w = c.W("S33w"), *S12w = c.W("S12w");
+  PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7],
+            g[8], o.S13w, o.S23w, o.Sw, S11w, S22w, S33w, S12w);
+  const T* w6[6] = {S11w, S22w, S33w, S12w, o.S13w, o.S23w};
+  for (int n = 0; n < 6; ++n) o.pw[n] = c.Pad("sw" + std::to_string(n));
+  dealias1<T>(c, 6, w6, o.pw);
+  o.pSw = c.Pad("Sw");
+  PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), o.pw[0], o.pw[1], o.pw[2], o.pw[3], o.pw[4], o.pw[5],
+            o.pSw);
+}
+
+inline bool wl_model(const alfa_plan* P) { return P->cfg.optSgs == 2 || P->cfg.optSgs == 4; }
+
+// SGS stresses txx, tyy, tzz, txy, txz, tyz with the wall stress at level 0
+// (DynamicSGS_Main.py:136-282 / StaticSGS_Main.py:76-250; SGSStresses_SM.py,
+// SGSStresses_WL.py).
+template <class T>
+void stage_stresses(Ctx<T>& c, const SgsStrains<T>& ss, const T* Cs2, const T* u, const T* v,
+                    const T* M, const T* psim, const T* psim0, T* const tau[6]) {
+  alfa_plan* P = c.P;
+  hipStream_t s = c.s;
+  const auto& d = P->d;
+  const bool wl = wl_model(P);
+  const double logm = std::log(0.5 * d.dz * d.z_scale / P->cfg.z0m);
+  // prefactor: -2 L^2 (SM, times |S|) or -2 L^(4/3) (WL, no |S|)
+  const T pre = wl ? (T)(-2.0 * std::pow(d.L, 4.0 / 3.0)) : (T)(-2.0 * d.L * d.L);
+  const double Ugal = P->cfg.Ugal;
+  if (d.optDealias != 1) {
+    PW_LAUNCH(P, s, c.F, k_stress_uvp<T>, c.dims(), ss.Sij[0], ss.Sij[1], ss.Sij[2], ss.Sij[3],
+              wl ? (const T*)nullptr : ss.Sm, Cs2, pre, tau[0], tau[1], tau[2], tau[3]);
+    PW_LAUNCH(P, s, c.F, k_stress_w<T>, c.dims(), ss.S13w, ss.S23w,
+              wl ? (const T*)nullptr : ss.Sw, Cs2, pre, tau[4], tau[5], 1, u, v, M, psim, psim0,
+              logm, 0.4, Ugal);
+    return;
+  }
+  const size_t PF = (size_t)c.mx * c.my * c.nz;
+  T* pc = c.Pad("cs2");
+  const T* cs[1] = {Cs2};
+  dealias1<T>(c, 1, cs, &pc);
+  T *ptxx = c.Pad("txx"), *ptyy = c.Pad("tyy"), *ptzz = c.Pad("tzz"), *ptxy = c.Pad("txy");
+  PW_LAUNCH(P, s, PF, k_stress_uvp<T>, c.pdims(), ss.ps[0], ss.ps[1], ss.ps[2], ss.ps[3],
+            wl ? (const T*)nullptr : ss.pS, pc, pre, ptxx, ptyy, ptzz, ptxy);
+  const T* pt4[4] = {ptxx, ptyy, ptzz, ptxy};
+  T* t4[4] = {tau[0], tau[1], tau[2], tau[3]};
+  dealias2<T>(c, 4, pt4, t4);
+  T *ptxz = c.Pad("txz"), *ptyz = c.Pad("tyz");
+  PW_LAUNCH(P, s, PF, k_stress_w<T>, c.pdims(), ss.pw[4], ss.pw[5],
+            wl ? (const T*)nullptr : ss.pSw, pc, pre, ptxz, ptyz, 0, (const T*)nullptr,
+            (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, logm, 0.4,
+            Ugal);
+  const T* pt2[2] = {ptxz, ptyz};
+  T* t2[2] = {tau[4], tau[5]};
+  dealias2<T>(c, 2, pt2, t2);
+  PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, tau[4], tau[5], u, v, M, psim, psim0,
+            logm, 0.4, Ugal, (T*)nullptr, (const T*)nullptr);
+}
+
+// SGS scalar fluxes qx, qy, qz with qz = qsfc at level 0
+// (DynamicSGS_Main.py:420-478 / StaticSGS_Main.py:258-360; ScalarSGSFluxes_SM.py,
+// ScalarSGSFluxes_WL.py).
+template <class T>
+void stage_fluxes(Ctx<T>& c, const SgsStrains<T>& ss, const T* const dT[3], const T* CsPr,
+                  const T* qsfc, T* const q[3]) {
+  alfa_plan* P = c.P;
+  hipStream_t s = c.s;
+  const auto& d = P->d;
+  const bool wl = wl_model(P);
+  // prefactor: -2 L^2 (SM, times |S|) or -L^(4/3) (WL, no |S|)
+  const T pre = wl ? (T)(-std::pow(d.L, 4.0 / 3.0)) : (T)(-2.0 * d.L * d.L);
+  if (d.optDealias != 1) {
+    PW_LAUNCH(P, s, c.F, k_scalar_flux<T>, c.dims(), dT[0], dT[1], dT[2],
+              wl ? (const T*)nullptr : ss.Sm, wl ? (const T*)nullptr : ss.Sw, CsPr, pre, q[0], q[1],
+              q[2], qsfc);
+    return;
+  }
+  const size_t PF = (size_t)c.mx * c.my * c.nz;
+  const T* in4[4] = {dT[0], dT[1], dT[2], CsPr};
+  T* p4[4] = {c.Pad("q0"), c.Pad("q1"), c.Pad("q2"), c.Pad("q3")};
+  dealias1<T>(c, 4, in4, p4);
+  T *pqx = c.Pad("qx"), *pqy = c.Pad("qy"), *pqz = c.Pad("qz");
+  PW_LAUNCH(P, s, PF, k_scalar_flux<T>, c.pdims(), p4[0], p4[1], p4[2],
+            wl ? (const T*)nullptr : ss.pS, wl ? (const T*)nullptr : ss.pSw, p4[3], pre, pqx, pqy,
+            pqz, (const T*)nullptr);
+  const T* pq3[3] = {pqx, pqy, pqz};
+  dealias2<T>(c, 3, pq3, q);
+  PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, (T*)nullptr, (T*)nullptr,
+            (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+            (const T*)nullptr, 0.0, 0.4, 0.0, q[2], qsfc);
 }
 
 template <class T> struct FSum4 {
@@ -691,98 +839,33 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   // 6. SGS (Main.py:333-443)
   const bool dynamic = (g.optSgs >= 1 && g.optSgs <= 4) &&
                        (first || (g.dynamicSGS_call_time > 0 && it % g.dynamicSGS_call_time == 0));
-  T *S12 = c.W("S12"), *S13 = c.W("S13"), *S23 = c.W("S23"), *Sm = c.W("S");
-  PW_LAUNCH(P, s, c.F, k_strains_uvp<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
-            dwdz, S12, S13, S23, Sm);
-  const T* Sij[6] = {dudx, dvdy, dwdz, S12, S13, S23};
+  const T* grads[9] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz};
+  SgsStrains<T> ss{};
+  stage_strains<T>(c, grads, ss);
   const double* shu = mixed ? means : nullptr;
   const double* shv = mixed ? means + c.nz : nullptr;
   const double* sht = mixed ? means + 2 * c.nz : nullptr;
   const bool wl = g.optSgs == 2 || g.optSgs == 4;
   if (dynamic && wl)
-    stage_lasdd_wl<T>(c, S.u, S.v, S.w, Sij, shu, shv, S.Cs2, st->Cs2_1D_avg1, st->Cs2_1D_avg2,
-                      st->beta1_1D);
+    stage_lasdd_wl<T>(c, S.u, S.v, S.w, ss.Sij, shu, shv, S.Cs2, st->Cs2_1D_avg1,
+                      st->Cs2_1D_avg2, st->beta1_1D);
   else if (dynamic)
-    stage_lasdd<T>(c, S.u, S.v, S.w, Sij, Sm, shu, shv, S.Cs2, st->Cs2_1D_avg1, st->Cs2_1D_avg2,
-                   st->beta1_1D);
-  T *txx = c.W("txx"), *tyy = c.W("tyy"), *tzz = c.W("tzz"), *txy = c.W("txy");
-  T *txz = c.W("txz"), *tyz = c.W("tyz");
-  T *S13w = c.W("S13w"), *S23w = c.W("S23w"), *Sw = c.W("Sw");
-  const double logm = std::log(0.5 * d.dz * d.z_scale / g.z0m);
-  // stress prefactor: -2 L^2 (SM, times |S|) or -2 L^(4/3) (WL, no |S|)
-  const T m2L2 = wl ? (T)(-2.0 * std::pow(d.L, 4.0 / 3.0)) : (T)(-2.0 * d.L * d.L);
-  const T* Sst = wl ? nullptr : Sm;
-  T *pS = nullptr, *pSw = nullptr;
-  if (!dealias) {
-    PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
-              dwdz, S13w, S23w, Sw, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
-    PW_LAUNCH(P, s, c.F, k_stress_uvp<T>, c.dims(), dudx, dvdy, dwdz, S12, Sst, S.Cs2, m2L2, txx,
-              tyy, tzz, txy);
-    PW_LAUNCH(P, s, c.F, k_stress_w<T>, c.dims(), S13w, S23w, wl ? nullptr : Sw, S.Cs2, m2L2, txz,
-              tyz, 1, S.u, S.v, sf.M, S.psim, S.psim0, logm, 0.4, g.Ugal);
-  } else {
-    const size_t PF = (size_t)c.mx * c.my * c.nz;
-    T* ps[6];
-    for (int n = 0; n < 6; ++n) ps[n] = c.Pad("su" + std::to_string(n));
-    dealias1<T>(c, 6, Sij, ps);
-    pS = c.Pad("S");
-    PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), ps[0], ps[1], ps[2], ps[3], ps[4], ps[5], pS);
-    T* pc = c.Pad("cs2");
-    const T* cs[1] = {S.Cs2};
-    dealias1<T>(c, 1, cs, &pc);
-    T *ptxx = c.Pad("txx"), *ptyy = c.Pad("tyy"), *ptzz = c.Pad("tzz"), *ptxy = c.Pad("txy");
-    PW_LAUNCH(P, s, PF, k_stress_uvp<T>, c.pdims(), ps[0], ps[1], ps[2], ps[3],
-              wl ? (const T*)nullptr : pS, pc, m2L2, ptxx, ptyy, ptzz, ptxy);
-    const T* pt4[4] = {ptxx, ptyy, ptzz, ptxy};
-    T* t4[4] = {txx, tyy, tzz, txy};
-    dealias2<T>(c, 4, pt4, t4);
-    T *S11w = c.W("S11w"), *S22w = c.W("S22w"), *S33w = c.W("S33w"), *S12w = c.W("S12w");
-    PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz,
-              dwdz, S13w, S23w, Sw, S11w, S22w, S33w, S12w);
-    const T* w6[6] = {S11w, S22w, S33w, S12w, S13w, S23w};
-    T* pw[6];
-    for (int n = 0; n < 6; ++n) pw[n] = c.Pad("sw" + std::to_string(n));
-    dealias1<T>(c, 6, w6, pw);
-    pSw = c.Pad("Sw");
-    PW_LAUNCH(P, s, PF, k_smag<T>, c.pdims(), pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pSw);
-    T *ptxz = c.Pad("txz"), *ptyz = c.Pad("tyz");
-    PW_LAUNCH(P, s, PF, k_stress_w<T>, c.pdims(), pw[4], pw[5], wl ? (const T*)nullptr : pSw, pc,
-              m2L2, ptxz, ptyz, 0,
-              (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
-              (const T*)nullptr, logm, 0.4, g.Ugal);
-    const T* pt2[2] = {ptxz, ptyz};
-    T* t2[2] = {txz, tyz};
-    dealias2<T>(c, 2, pt2, t2);
-    PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, txz, tyz, S.u, S.v, sf.M, S.psim,
-              S.psim0, logm, 0.4, g.Ugal, (T*)nullptr, (const T*)nullptr);
-  }
+    stage_lasdd<T>(c, S.u, S.v, S.w, ss.Sij, ss.Sm, shu, shv, S.Cs2, st->Cs2_1D_avg1,
+                   st->Cs2_1D_avg2, st->beta1_1D);
+  T* tau[6] = {c.W("txx"), c.W("tyy"), c.W("tzz"), c.W("txy"), c.W("txz"), c.W("tyz")};
+  stage_stresses<T>(c, ss, S.Cs2, S.u, S.v, sf.M, S.psim, S.psim0, tau);
+  T *txx = tau[0], *tyy = tau[1], *tzz = tau[2], *txy = tau[3], *txz = tau[4], *tyz = tau[5];
   if (dynamic && wl)
-    stage_scalar_wl<T>(c, S.u, S.v, S.w, S.TH, dTx, dTy, dTz, shu, shv, sht, S.CsPr,
-                       st->Cs2PrRatio_1D, st->beta2_1D);
+    stage_scalar_wl<T>(c, S.u, S.v, S.w, 0, S.TH, dTx, dTy, dTz, step_hats(c), shu, shv, sht,
+                       S.CsPr, st->Cs2PrRatio_1D, st->beta2_1D);
   else if (dynamic)
-    stage_scalar_lasdd<T>(c, S.u, S.v, S.w, S.TH, dTx, dTy, dTz, Sm, shu, shv, sht, S.CsPr,
-                          st->Cs2PrRatio_1D, st->beta2_1D);
-  // flux prefactor: -2 L^2 (SM, times |S|) or -L^(4/3) (WL, no |S|)
-  const T mq = wl ? (T)(-std::pow(d.L, 4.0 / 3.0)) : m2L2;
+    stage_scalar_lasdd<T>(c, S.u, S.v, S.w, 0, S.TH, dTx, dTy, dTz, ss.Sm, step_hats(c), shu, shv,
+                          sht, S.CsPr, st->Cs2PrRatio_1D, st->beta2_1D);
   T *qx = c.W("qx"), *qy = c.W("qy"), *qzf = c.W("qz");
-  if (!dealias) {
-    PW_LAUNCH(P, s, c.F, k_scalar_flux<T>, c.dims(), dTx, dTy, dTz, Sst, wl ? nullptr : Sw,
-              S.CsPr, mq, qx, qy, qzf, (const T*)sf.qz);
-  } else {
-    const size_t PF = (size_t)c.mx * c.my * c.nz;
-    const T* in4[4] = {dTx, dTy, dTz, S.CsPr};
-    T* p4[4] = {c.Pad("q0"), c.Pad("q1"), c.Pad("q2"), c.Pad("q3")};
-    dealias1<T>(c, 4, in4, p4);
-    T *pqx = c.Pad("qx"), *pqy = c.Pad("qy"), *pqz = c.Pad("qz");
-    PW_LAUNCH(P, s, PF, k_scalar_flux<T>, c.pdims(), p4[0], p4[1], p4[2],
-              wl ? (const T*)nullptr : pS, wl ? (const T*)nullptr : pSw, p4[3], mq, pqx, pqy, pqz,
-              (const T*)nullptr);
-    const T* pq3[3] = {pqx, pqy, pqz};
-    T* q3[3] = {qx, qy, qzf};
-    dealias2<T>(c, 3, pq3, q3);
-    PW_LAUNCH(P, s, (size_t)c.ncol, k_level0<T>, c.ncol, c.nz, (T*)nullptr, (T*)nullptr,
-              (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
-              (const T*)nullptr, 0.0, 0.4, 0.0, qzf, (const T*)sf.qz);
+  {
+    const T* dT[3] = {dTx, dTy, dTz};
+    T* q[3] = {qx, qy, qzf};
+    stage_fluxes<T>(c, ss, dT, S.CsPr, sf.qz, q);
   }
   T *dtx = c.W("divtx"), *dty = c.W("divty"), *dtz = c.W("divtz"), *divq = c.W("divq");
   stage_div_stress<T>(c, txx, tyy, tzz, txy, txz, tyz, dtx, dty, dtz);
