@@ -169,6 +169,13 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=60,
+                    help="steps per member (the full ensemble configuration uses 500)")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=(100, 400), metavar=("MIN", "MAX"),
+                    help="run-length range of served requests (the full configuration: 200 2000)")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 3 times (a plain copy "
                          "kernel of known size, to scale HBM traffic counters)")
@@ -208,6 +215,24 @@ def main():
                     launches=dom["launches"],
                     avg_launch_us=round(dom["total_ms"] * 1e3 / dom["launches"], 2),
                     bytes_per_launch=dom["bytes"] / dom["launches"])
+    ens = srv = None
+    if not args.no_ensemble:
+        from alfa_amd import ensemble as E
+        specs = E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps, seed=0)
+        ens = E.run_ensemble(specs)
+        ens["workload"] = (f"{args.ensemble_members} convective members, grids from "
+                           f"48/64/80/96^3, seeded Ug and surface flux, mixed, one hipGraph "
+                           f"per member step, {ens['streams']} streams")
+        log(f"[bench] ensemble: {ens}")
+        reqs = E.serving_requests(args.serve_requests, rate=2.0, steps=tuple(args.serve_steps),
+                                  seed=0)
+        server = E.Server(hbm_budget_bytes=96 << 30, precision="mixed")
+        srv = server.serve(reqs)
+        srv["workload"] = (f"Poisson stream (2/s) of {args.serve_requests} requests, grids from "
+                           f"32/64/128/192^3, neutral/convective/stable with seeded forcing, "
+                           f"{args.serve_steps[0]}-{args.serve_steps[1]} steps each, mixed, "
+                           f"96 GiB HBM budget")
+        log(f"[bench] serving: {srv}")
     if args.copy_probe > 0:
         nel = args.copy_probe * 2 ** 20 // 4
         src = torch.ones(nel, dtype=torch.float32, device="cuda")
@@ -244,6 +269,8 @@ def main():
         # tests; at 256^3 it stands in for the oracle, which would take hours
         "error_mixed_vs_fp64": errors,
         "roofline": roof,
+        "ensemble": ens,
+        "serving": srv,
         "cpu_baseline": cpu,
         "reference_a100": A100_REF,
     }

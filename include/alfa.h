@@ -122,6 +122,14 @@ int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);
 int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_val,
               alfa_stream stream);
 
+/* As alfa_step, but the surface value is read from device memory (one
+ * double) when the step's kernels run, so that a step captured once into a
+ * hipGraph can be replayed with the next SurfaceBC_series value written
+ * there before each replay.  No counterpart in the reference (whose jitted
+ * step takes the value as an argument, Main.py:224,233). */
+int alfa_step_sfc_dev(alfa_plan* plan, const alfa_state* st, int iteration,
+                      const double* sfc_val, alfa_stream stream);
+
 /* ---- operations (src/operations) ---------------------------------------- */
 /* FFT (src/operations/FFT.py:44): F (nx,ny,nz) -> F_fft (nx,ny/2+1,nz). */
 int alfa_fft(alfa_plan*, const void* F, void* F_fft, alfa_stream);

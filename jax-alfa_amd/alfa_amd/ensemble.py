@@ -1,0 +1,277 @@
+"""Many independent LES runs on one GPU: batched ensembles and a request loop.
+
+The reference advances one run per process (src/Main.py).  Here every run
+("member") owns one ``alfa_plan`` (its grid, constants and workspace) and its
+state tensors; members of any grid size share the device.
+
+* Each member's time step is captured once into a hipGraph
+  (``torch.cuda.CUDAGraph`` around ``alfa_step_sfc_dev``) after its first,
+  eager step.  A replay launches the whole step (about a hundred kernels)
+  with one call, and the surface value of the step is written to device
+  memory before the replay (``alfa_step_sfc_dev`` reads it there).
+* Members are spread over a few streams (default priority), so the kernels
+  of small members overlap on the 256 CUs instead of running one grid at a
+  time.
+* ``run_ensemble`` steps a fixed set of members together (a synchronous
+  ensemble); ``Server`` admits requests as they arrive, within an HBM budget,
+  steps every admitted member in rounds (continuous batching: a member that
+  arrives joins the next round, a member that finishes leaves), and reports
+  completion-latency percentiles and aggregate throughput.
+
+Results do not depend on the mix: every kernel is deterministic (fixed
+reduction orders) and each member has its own workspace, so a member's
+fields are bitwise the same alone and in any mix.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib, cases, core
+
+# base grid of each case: the time step is scaled with the grid spacing
+_BASE_N = {"neutral": 32, "convective": 128, "stable": 256}
+
+
+@dataclass
+class MemberSpec:
+    """One run: case, grid, forcing, length and arrival time (serving)."""
+    case: str
+    n: int
+    steps: int
+    seed: int = 0
+    over: dict = field(default_factory=dict)
+    cooling_K_per_h: float = 0.25
+    arrival: float = 0.0
+
+    def build(self):
+        """-> (cfg, u, v, w, TH, surface series) in stored form."""
+        base = cases.CASES[self.case]
+        over = dict(self.over)
+        over.setdefault("dt", base["dt"] * _BASE_N[self.case] / self.n)
+        cfg, u, v, w, TH = cases.initial_fields(self.case, seed=self.seed, nx=self.n, ny=self.n,
+                                                nz=self.n, **over)
+        ser = None
+        if cfg["optSurfBC"] == 2:
+            t = np.arange(self.steps + 2) * cfg["dt"]
+            ser = -self.cooling_K_per_h * t / 3600.0
+        elif cfg["optSurfBC"] == 1:
+            ser = np.full(self.steps + 2, cfg["SensibleHeatFlux"])
+        return cfg, u, v, w, TH, ser
+
+
+def ensemble_specs(nmembers=64, sizes=(48, 64, 80, 96), steps=500, seed=0):
+    """Config 3: convective physics, per-member Ug in [2, 12] m/s and surface
+    heat flux in [0.05, 0.3] K m/s (homogeneous flux), grids drawn from sizes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(nmembers):
+        n = int(rng.choice(sizes))
+        out.append(MemberSpec("convective", n, steps, seed=seed * 1000 + m,
+                              over=dict(Ug2=float(rng.uniform(2, 12)),
+                                        SensibleHeatFlux=float(rng.uniform(0.05, 0.3)),
+                                        optSurfFlux=0)))
+    return out
+
+
+def serving_requests(nreq=40, rate=2.0, sizes=(32, 64, 128, 192), steps=(200, 2000), seed=0):
+    """Config 4: a Poisson stream (rate per second) of requests mixing the
+    neutral, convective and stable setups with seeded forcing."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for r in range(nreq):
+        t += float(rng.exponential(1.0 / rate))
+        case = str(rng.choice(["neutral", "convective", "stable"]))
+        n = int(rng.choice(sizes))
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        if case == "neutral":
+            over = dict(Ug2=float(rng.uniform(5, 15)))
+        elif case == "convective":
+            over = dict(SensibleHeatFlux=float(rng.uniform(0.05, 0.3)), Ug2=float(rng.uniform(1, 5)))
+        else:
+            over = dict(Ug2=float(rng.uniform(5, 10)))
+        spec = MemberSpec(case, n, nsteps, seed=seed * 1000 + r, over=over,
+                          cooling_K_per_h=float(rng.uniform(0.1, 0.5)), arrival=t)
+        out.append(spec)
+    return out
+
+
+def state_bytes(n, precision):
+    """Caller-owned state of one member: 10 fields + 6 planes + profiles."""
+    es = 8 if precision == "fp64" else 4
+    return 10 * n ** 3 * es + 6 * n * n * es + 6 * n * 8
+
+
+class Member:
+    """One run on the device: plan, state, stream and captured step graphs."""
+
+    def __init__(self, spec: MemberSpec, precision="mixed", stream=None, graphs=True):
+        torch = core._torch()
+        self.spec = spec
+        cfg, u, v, w, TH, ser = spec.build()
+        self.cfg = cfg
+        self.stream = stream or torch.cuda.current_stream()
+        with torch.cuda.stream(self.stream):
+            self.plan = core.Plan(cfg, precision)
+            self.les = core.LES(self.plan, u, v, w, TH, sfc_series=ser)
+            self.sfc = torch.zeros(1, dtype=torch.float64, device=self.plan.device)
+        self.bytes = int(self.plan.derived["workspace_bytes"]) + state_bytes(spec.n, precision)
+        self.done = 0
+        self.use_graphs = graphs
+        self.graphs = {}
+        self.t_admit = None
+        self.t_done = None
+
+    @property
+    def points(self):
+        return self.spec.n ** 3
+
+    def _dynamic(self, it):
+        c = self.cfg
+        return c["optSgs"] >= 1 and (it == c.get("istep", 1) or it % c["dynamicSGS_call_time"] == 0)
+
+    def _launch(self, it):
+        P = self.plan
+        st = core.current_stream()
+        _lib.check(P.lib.alfa_step_sfc_dev(P.handle, C.byref(self.les.state), int(it),
+                                           C.c_void_p(self.sfc.data_ptr()), st),
+                   "alfa_step_sfc_dev")
+
+    def step(self, n=1):
+        """Enqueue n steps on the member's stream (graph replays after the
+        first step of each kind)."""
+        torch = core._torch()
+        les = self.les
+        with torch.cuda.stream(self.stream):
+            for _ in range(n):
+                it = les.iteration
+                sv = 0.0 if les.sfc_series is None else float(les.sfc_series[it - 1])
+                self.sfc.fill_(sv)
+                first = it == self.cfg.get("istep", 1)
+                key = self._dynamic(it)
+                if not self.use_graphs or first:
+                    self._launch(it)
+                elif key in self.graphs:
+                    self.graphs[key].replay()
+                else:
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=self.stream):
+                        self._launch(it)
+                    self.graphs[key] = g
+                    g.replay()
+                les.iteration += 1
+                self.done += 1
+
+    def close(self):
+        self.graphs.clear()
+        self.plan.close()
+
+
+def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mixed",
+                 graphs=True, log=None):
+    """Step all members together for `steps` steps (default: each spec's
+    own length, all equal in config 3).  Returns throughput figures."""
+    torch = core._torch()
+    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+    t0 = time.perf_counter()
+    members = [Member(s, precision, streams[i % nstreams], graphs) for i, s in enumerate(specs)]
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    nsteps = steps or specs[0].steps
+    # untimed: the eager first step and the graph captures (2 steps)
+    for m in members:
+        m.step(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps - 2):
+        for m in members:
+            m.step(1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pts = sum(m.points for m in members) * (nsteps - 2)
+    out = dict(members=len(members), steps_timed=nsteps - 2, wall_s=wall, setup_s=t_setup,
+               mpoint_steps_per_s=pts / wall / 1e6, streams=nstreams, graphs=graphs,
+               grids={str(n): sum(1 for s in specs if s.n == n) for n in sorted({s.n for s in specs})},
+               hbm_bytes=sum(m.bytes for m in members))
+    for m in members:
+        m.close()
+    return out
+
+
+class Server:
+    """Continuous-batching request loop with HBM-budget admission."""
+
+    def __init__(self, hbm_budget_bytes, precision="mixed", nstreams=4, round_steps=10,
+                 graphs=True):
+        torch = core._torch()
+        self.budget = int(hbm_budget_bytes)
+        self.precision = precision
+        self.streams = [torch.cuda.Stream() for _ in range(nstreams)]
+        self.round_steps = round_steps
+        self.graphs = graphs
+        self.results = {}
+
+    def estimate_bytes(self, spec):
+        # workspace per grid point measured at plan creation (mixed ~560 B,
+        # FP64 ~1120 B at FGR=2; FGR=1 carries padded buffers too)
+        per = 560 if self.precision == "mixed" else 1120
+        if cases.CASES[spec.case]["FGR"] == 1:
+            per *= 2
+        return per * spec.n ** 3 + state_bytes(spec.n, self.precision)
+
+    def serve(self, requests: List[MemberSpec], keep=None, log=None):
+        """Run the request stream in real time.  keep: request indices whose
+        final fields are returned (for the alone-versus-mix check)."""
+        torch = core._torch()
+        keep = set(keep or [])
+        pending = deque(sorted(range(len(requests)), key=lambda i: requests[i].arrival))
+        active = {}
+        used = 0
+        lat = {}
+        points = 0
+        t0 = time.perf_counter()
+        k = 0
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admission: arrived requests in order, while the budget allows
+            while pending and requests[pending[0]].arrival <= now:
+                i = pending[0]
+                need = self.estimate_bytes(requests[i])
+                if active and used + need > self.budget:
+                    break
+                pending.popleft()
+                m = Member(requests[i], self.precision, self.streams[k % len(self.streams)],
+                           self.graphs)
+                k += 1
+                m.t_admit = time.perf_counter() - t0
+                used += m.bytes
+                active[i] = m
+            if not active:
+                time.sleep(max(0.0, min(0.05, requests[pending[0]].arrival - now)))
+                continue
+            # one round: every admitted member advances up to round_steps
+            for i, m in active.items():
+                m.step(min(self.round_steps, m.spec.steps - m.done))
+            torch.cuda.synchronize()
+            now = time.perf_counter() - t0
+            for i in [i for i, m in active.items() if m.done >= m.spec.steps]:
+                m = active.pop(i)
+                lat[i] = now - requests[i].arrival
+                points += m.points * m.spec.steps
+                if i in keep:
+                    self.results[i] = {f: m.les.host(f) for f in ("u", "v", "w", "TH")}
+                used -= m.bytes
+                m.close()
+        wall = time.perf_counter() - t0
+        v = np.array([lat[i] for i in sorted(lat)])
+        return dict(requests=len(requests), wall_s=wall, p50_s=float(np.percentile(v, 50)),
+                    p99_s=float(np.percentile(v, 99)), max_s=float(v.max()),
+                    mpoint_steps_per_s=points / wall / 1e6, budget_bytes=self.budget,
+                    streams=len(self.streams), round_steps=self.round_steps)

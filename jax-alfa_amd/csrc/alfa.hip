@@ -695,6 +695,12 @@ extern "C" int alfa_step(alfa_plan* P, const alfa_state* st, int iteration, doub
   if (!st) return set_error(ALFA_EINVAL, "alfa_step: null state");
   DISPATCH(P, "alfa_step", step_t<T>(P, st, iteration, sfc_val, STREAM(stream)));
 }
+extern "C" int alfa_step_sfc_dev(alfa_plan* P, const alfa_state* st, int iteration,
+                                 const double* sfc_val, alfa_stream stream) {
+  if (!P || !st || !sfc_val) return set_error(ALFA_EINVAL, "alfa_step_sfc_dev: null argument");
+  DISPATCH(P, "alfa_step_sfc_dev",
+           step_t<T>(P, st, iteration, 0.0, STREAM(stream), sfc_val));
+}
 
 extern "C" int alfa_fft(alfa_plan* P, const void* F, void* out, alfa_stream s) {
   DISPATCH(P, "alfa_fft", ops::fft<T>(P, STREAM(s), F, out));

@@ -550,6 +550,13 @@ __global__ void k_profiles(const double* __restrict__ s, int nz, double inv_n, d
   if (avg2) avg2[k] = s[nz + k] * inv_n;
 }
 
+// device copy of n doubles (a kernel rather than a memcpy node, so that
+// captured steps hold kernels only)
+__global__ void k_copy_d(int n, const double* __restrict__ src, double* __restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
 __global__ void k_fill_d(int n, double* a, double v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) a[k] = v;

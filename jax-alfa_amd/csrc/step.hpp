@@ -164,7 +164,8 @@ template <class T> struct Surf {
 };
 
 template <class T>
-void stage_surface(Ctx<T>& c, const StateT<T>& S, const Surf<T>& sf, double sfc_val) {
+void stage_surface(Ctx<T>& c, const StateT<T>& S, const Surf<T>& sf, double sfc_val,
+                   const double* sfc_dev = nullptr) {
   alfa_plan* P = c.P;
   const auto& g = P->cfg;
   const auto& d = P->d;
@@ -174,7 +175,7 @@ void stage_surface(Ctx<T>& c, const StateT<T>& S, const Surf<T>& sf, double sfc_
   a.M = sf.M; a.ustar = sf.ustar; a.qz = sf.qz; a.invOB = sf.invOB;
   a.scal = P->scal;
   a.ncol = c.ncol; a.nz = c.nz; a.bc = g.optSurfBC; a.hetero = g.optSurfFlux != 0;
-  a.sfc_val = sfc_val; a.SHF = g.SensibleHeatFlux; a.Ugal = g.Ugal; a.T0 = g.T_0;
+  a.sfc_val = sfc_val; a.sfc_dev = sfc_dev; a.SHF = g.SensibleHeatFlux; a.Ugal = g.Ugal; a.T0 = g.T_0;
   a.vonk = 0.4; a.g = d.g_nondim; a.dz_half = 0.5 * d.dz; a.z1 = 0.0;
   const double zref = g.zTemperature > 0.0 ? g.zTemperature : g.z0T;
   a.z0m_nd = g.z0m / d.z_scale; a.zT_nd = zref / d.z_scale;
@@ -370,7 +371,7 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
-    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
   }
 }
 
@@ -449,7 +450,7 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
-    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
   }
 }
 
@@ -510,7 +511,7 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
-    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
   }
 }
 
@@ -573,7 +574,7 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
-    if (beta_out) (void)hipMemcpyAsync(beta_out, beta, c.nz * sizeof(double), hipMemcpyDeviceToDevice, c.s);
+    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
   }
 }
 
@@ -741,7 +742,8 @@ constexpr int AB2_BLOCKS = 2048;
 
 // One time step.
 template <class T>
-int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream_t s) {
+int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream_t s,
+           const double* sfc_dev = nullptr) {
   Ctx<T> c(P, s);
   const auto& g = P->cfg;
   const auto& d = P->d;
@@ -794,7 +796,7 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     y_c2r<T>(P, s, so, ya, 12, c.nx, P->fy, P->tw_y);
   }
   // 2. surface fluxes (Main.py:213-243)
-  stage_surface<T>(c, S, sf, sfc_val);
+  stage_surface<T>(c, S, sf, sfc_val, sfc_dev);
   // 3. vertical gradients (Derivz_M, Derivz_TH)
   PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim, S.fih,
             sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz), (T)(0.4 * 0.5 * d.dz),

@@ -11,6 +11,8 @@ template <class T> struct SurfArgs {
   double* scal;   // [1] qz_sfc_avg, [2] sqrt(mean ustar^2), [3] mean M
   int ncol, nz, bc, hetero;
   double sfc_val, SHF, Ugal, T0, vonk, g, dz_half, z1, z0m_nd, zT_nd, logm, logh;
+  const double* sfc_dev;   // when set, the surface value is read here at run time
+  __device__ __forceinline__ double sval() const { return sfc_dev ? *sfc_dev : sfc_val; }
 };
 
 
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(SURF_T) k_surf_q(SurfArgs<T> a, const double* 
     const double us = fmax(a.vonk * M / dm, 1e-3);
     const double dh = a.logh + (double)a.psih[c] - (double)a.psih0[c];
     const double tha = a.hetero ? th : THavg;
-    sq += us * a.vonk * (a.sfc_val - tha) / dh;
+    sq += us * a.vonk * (a.sval() - tha) / dh;
   }
   const double q = block_reduce_256(sq, sm);
   if (threadIdx.x == 0) part2[blockIdx.x] = q;
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(SURF_T) k_surf_final(SurfArgs<T> a, const doub
   const double THavg = sum_parts(part + 1, gridDim.x, 2, sm) * invn;
   double qavg;
   if (a.bc == 0) qavg = a.SHF;
-  else if (a.bc == 1) qavg = a.sfc_val;
+  else if (a.bc == 1) qavg = a.sval();
   else qavg = sum_parts(part2, gridDim.x, 1, sm) * invn;
   const bool stable = qavg <= 0.0;
   double sus = 0;
@@ -124,10 +126,10 @@ __global__ void __launch_bounds__(SURF_T) k_surf_final(SurfArgs<T> a, const doub
     if (a.bc == 2) {
       const double dh = a.logh + (double)a.psih[c] - (double)a.psih0[c];
       const double tha = a.hetero ? th : THavg;
-      q = us * a.vonk * (a.sfc_val - tha) / dh;
+      q = us * a.vonk * (a.sval() - tha) / dh;
       thref = THavg + a.T0;
     } else {
-      q = (a.bc == 0) ? a.SHF : a.sfc_val;
+      q = (a.bc == 0) ? a.SHF : a.sval();
       thref = a.hetero ? th + a.T0 : THavg + a.T0;
     }
     const double iob = -(a.vonk * a.g * qavg) / ((us * us * us) * thref);
