@@ -197,6 +197,77 @@ __device__ void fft_lines(c2<T>* buf, const c2<T>* tw, const Fft1D& f, int sign)
   }
 }
 
+// Compile-time length.  For the grid sizes the plans meet most (see
+// ct_length in plan.hpp) the kernels are instantiated with the transform
+// length, its radix plan and every pass's stride as constants: index
+// arithmetic strength-reduces, loop bounds are exact and the register arrays
+// have their exact size.  Same passes, same radix order (8, 4, 2, 3, 5) and
+// same arithmetic as fft_lines.
+__host__ __device__ constexpr int radix_of(int m) {
+  return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : 5;
+}
+
+template <class T, int R, int LPB, int N, int NS>
+__device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sign) {
+  constexpr int NR = N / R;
+  constexpr int TOTAL = NR * LPB;
+  constexpr int MAXIT = (TOTAL + FFT_BS - 1) / FFT_BS;
+  constexpr int TSTEP = N / (NS * R);
+  c2<T> v[MAXIT][R];
+#pragma unroll
+  for (int q = 0; q < MAXIT; ++q) {
+    const int it = threadIdx.x + q * FFT_BS;
+    if (TOTAL % FFT_BS == 0 || it < TOTAL) {
+      const int j = it / LPB, p = it % LPB;
+      const int jm = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        c2<T> a = buf[(j + r * NR) * LPB + p];
+        if (NS > 1 && r > 0) {
+          c2<T> w = tw[jm * r * TSTEP];
+          if (sign > 0) w.y = -w.y;
+          a = cmul<T>(a, w);
+        }
+        v[q][r] = a;
+      }
+      Dft<T, R>::run(v[q], sign);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXIT; ++q) {
+    const int it = threadIdx.x + q * FFT_BS;
+    if (TOTAL % FFT_BS == 0 || it < TOTAL) {
+      const int j = it / LPB, p = it % LPB;
+      const int jm = j % NS;
+      const int base = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * NS) * LPB + p] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, int LPB, int N, int NS, int M>
+struct FftCT {
+  static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
+    constexpr int R = radix_of(M);
+    fft_pass_ct<T, R, LPB, N, NS>(buf, tw, sign);
+    FftCT<T, LPB, N, NS * R, M / R>::run(buf, tw, sign);
+  }
+};
+template <class T, int LPB, int N, int NS>
+struct FftCT<T, LPB, N, NS, 1> {
+  static __device__ __forceinline__ void run(c2<T>*, const c2<T>*, int) {}
+};
+
+// transform of LPB lines: compile-time length NCT, or f.n when NCT == 0
+template <class T, int LPB, int MAXN, int NCT>
+__device__ __forceinline__ void fft_any(c2<T>* buf, const c2<T>* tw, const Fft1D& f, int sign) {
+  if constexpr (NCT > 0) FftCT<T, LPB, NCT, 1, NCT>::run(buf, tw, sign);
+  else fft_lines<T, LPB, MAXN>(buf, tw, f, sign);
+}
+
 struct PtrBatch {
   void* p[MAXB];
 };
@@ -240,7 +311,7 @@ struct TileGrid {
 // thread loads 16 bytes of z at a time through ld.vload and stores the two
 // complex values of one packed line together.
 // ----------------------------------------------------------------------------
-template <class T, class Loader, int MAXN>
+template <class T, class Loader, int MAXN, int NCT>
 __global__ void __launch_bounds__(FFT_BS)
 k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
         TileGrid tg, int vec) {
@@ -248,7 +319,7 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = fy.n, nEl = n * ZT;
+  const int n = NCT > 0 ? NCT : fy.n, nEl = n * ZT;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
   T* bufr = reinterpret_cast<T*>(buf);
@@ -268,7 +339,7 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
       }
     }
     __syncthreads();
-    fft_lines<T, L, MAXN>(buf, stw, fy, -1);
+    fft_any<T, L, MAXN, NCT>(buf, stw, fy, -1);
     // line p holds z0+2p (real part) and z0+2p+1 (imaginary part)
     c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
     const int z0 = zt * ZT;
@@ -379,15 +450,15 @@ __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, i
 // LDS (the common case; small register footprint).  MULTI = true: the
 // forward spectrum is kept in registers and every op (mask / derivative /
 // dealias map, optional inverse) is built from it in turn.
-template <class T, int MAXN, bool MULTI>
+template <class T, int MAXN, bool MULTI, int NCT>
 __global__ void __launch_bounds__(FFT_BS)
 k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     const c2<T>* __restrict__ tw_out, TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int MAXE = MAXN * L / FFT_BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int nin = fin.n;
-  const int nbuf = imax(nin, fout.n);
+  const int nin = NCT > 0 ? NCT : fin.n;   // NCT is set only when fin.n == fout.n
+  const int nbuf = NCT > 0 ? NCT : imax(nin, fout.n);
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw_in = buf + nbuf * L;
   c2<T>* stw_out = stw_in + nin;
@@ -402,13 +473,13 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     __syncthreads();
     x_load<T, L>(buf, a, nEl, zt, ky, b);
     __syncthreads();
-    if (!a.in_spectral) fft_lines<T, L, MAXN>(buf, stw_in, fin, -1);
+    if (!a.in_spectral) fft_any<T, L, MAXN, NCT>(buf, stw_in, fin, -1);
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
     if (!MULTI) {
       const XOp op = a.op[0];
       for (int i = threadIdx.x; i < nEl; i += FFT_BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
       __syncthreads();
-      if (op.inverse) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
+      if (op.inverse) fft_any<T, L, MAXN, NCT>(buf, stw_in, fin, +1);
       x_store<T, L>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky, a.vec);
     } else {
       c2<T> spec[MAXE];
@@ -446,8 +517,8 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
         }
         __syncthreads();
         if (op.inverse) {
-          if (op.map == XMAP_ID) fft_lines<T, L, MAXN>(buf, stw_in, fin, +1);
-          else fft_lines<T, L, MAXN>(buf, stw_out, fout, +1);
+          if (op.map == XMAP_ID) fft_any<T, L, MAXN, NCT>(buf, stw_in, fin, +1);
+          else fft_any<T, L, MAXN, NCT>(buf, stw_out, fout, +1);
         }
         x_store<T, L>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky, a.vec);
       }
@@ -498,14 +569,14 @@ __device__ __forceinline__ void yc2r_in2(const YC2RArgs& a, int b, int x, int k,
   }
 }
 
-template <class T, class Storer, int MAXN>
+template <class T, class Storer, int MAXN, int NCT>
 __global__ void __launch_bounds__(FFT_BS)
 k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
   constexpr int L = Lines<T>::L;
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = fy.n, half = n / 2;
+  const int n = NCT > 0 ? NCT : fy.n, half = n / 2;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
   const int nz = a.nz;
@@ -530,7 +601,7 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
       if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
     }
     __syncthreads();
-    fft_lines<T, L, MAXN>(buf, stw, fy, +1);
+    fft_any<T, L, MAXN, NCT>(buf, stw, fy, +1);
     const T* bufr = reinterpret_cast<const T*>(buf);
     const T sc = (T)a.scale;
     const int z0 = zt * ZT;

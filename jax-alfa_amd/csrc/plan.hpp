@@ -1,6 +1,7 @@
 // Plan object and launch helpers shared by the step driver and the
 // per-operation entry points.
 #pragma once
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <string>
@@ -128,6 +129,24 @@ inline unsigned pw_blocks(size_t n) {
   return (unsigned)b;
 }
 
+// Transform lengths with compile-time FFT kernels (the headline 256^3 grid
+// and the common ensemble sizes); every other length runs the generic
+// kernels of its register tier (<= 256, <= 384).  f(nct, maxn) is called
+// with both as std::integral_constant.
+template <class F> inline void ct_length(int n, F&& f) {
+  using std::integral_constant;
+  switch (n) {
+    case 64: f(integral_constant<int, 64>{}, integral_constant<int, 64>{}); break;
+    case 96: f(integral_constant<int, 96>{}, integral_constant<int, 96>{}); break;
+    case 128: f(integral_constant<int, 128>{}, integral_constant<int, 128>{}); break;
+    case 192: f(integral_constant<int, 192>{}, integral_constant<int, 192>{}); break;
+    case 256: f(integral_constant<int, 256>{}, integral_constant<int, 256>{}); break;
+    default:
+      if (n <= 256) f(integral_constant<int, 0>{}, integral_constant<int, 256>{});
+      else f(integral_constant<int, 0>{}, integral_constant<int, FFT_MAXN>{});
+  }
+}
+
 template <class T> inline size_t fft_lds(int n) { return (size_t)n * Lines<T>::L * sizeof(c2<T>) + (size_t)n * sizeof(c2<T>); }
 
 // y R2C of nb fields, rows x [0, rows), transform length fyy.n, output pitch ncy.
@@ -148,21 +167,27 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   // buffers; the paired complex stores need 16-byte aligned outputs
   bool vec = nz % Vec16<T>::W == 0 && ld.aligned(nb);
   for (int b = 0; b < nb; ++b) vec = vec && al16(outs[b]);
-  if (fyy.n <= 256)
-    k_y_r2c<T, Loader, 256><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz,
-                                                                   ncy, pb, tg, vec ? 1 : 0);
-  else
-    k_y_r2c<T, Loader, FFT_MAXN><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(
-        ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
+  ct_length(fyy.n, [&](auto nct, auto maxn) {
+    k_y_r2c<T, Loader, decltype(maxn)::value, decltype(nct)::value>
+        <<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg,
+                                                 vec ? 1 : 0);
+  });
   launch_ok(P);
 }
 
-template <class T, int MAXN, bool MULTI>
+template <class T, bool MULTI>
 void launch_kx(alfa_plan* P, hipStream_t s, unsigned grid, size_t lds, const XArgs& a,
                const Fft1D& fin, const Fft1D& fout, const void* twin, const void* twout,
                const TileGrid& tg) {
-  k_x<T, MAXN, MULTI><<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin,
-                                                (const c2<T>*)twout, tg);
+  // compile-time length only when input and output lengths agree
+  const int n = fin.n == fout.n ? fin.n : -1;
+  const int tier = imax(fin.n, fout.n);
+  auto go = [&](auto nct, auto maxn) {
+    k_x<T, decltype(maxn)::value, MULTI, decltype(nct)::value>
+        <<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+  };
+  if (n > 0) ct_length(n, go);
+  else ct_length(tier <= 256 ? 1 : 1000, go);   // generic kernel of the tier
 }
 
 // x pass over nb spectra; the tiles cover ky in [0, rows) (rows = ncy_in by
@@ -190,13 +215,8 @@ void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a_in, int nb, const Fft1D&
   for (int o = 0; o < a.nops; ++o)
     bytes += (double)nb * a.op[o].nout * imin(a.op[o].ncy, rows) * a.nz * sizeof(c2<T>);
   ProfScope ps(P, s, multi ? "k_x<multi>" : "k_x<single>", bytes);
-  if (nbuf <= 256) {
-    if (multi) launch_kx<T, 256, true>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
-    else launch_kx<T, 256, false>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
-  } else {
-    if (multi) launch_kx<T, FFT_MAXN, true>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
-    else launch_kx<T, FFT_MAXN, false>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
-  }
+  if (multi) launch_kx<T, true>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
+  else launch_kx<T, false>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
   launch_ok(P);
 }
 
@@ -215,10 +235,10 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
   ProfScope ps(P, s, std::string("k_y_c2r<") + Storer::NAME + ">",
                (double)nb * a.nsum * rows * a.ncy_in * a.nz * sizeof(c2<T>) +
                    (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T));
-  if (fyy.n <= 256)
-    k_y_c2r<T, Storer, 256><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw, tg);
-  else
-    k_y_c2r<T, Storer, FFT_MAXN><<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+  ct_length(fyy.n, [&](auto nct, auto maxn) {
+    k_y_c2r<T, Storer, decltype(maxn)::value, decltype(nct)::value>
+        <<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+  });
   launch_ok(P);
 }
 
