@@ -260,6 +260,25 @@ void plane_sums(alfa_plan* P, hipStream_t s, int ncol, int nz, const F& f, doubl
   launch_ok(P);
 }
 
+// Imfilter of num and den per level, the clipped ratio into out and the
+// plane sums of sqrt(out) and out (k_box_clip + k_plane_final).
+template <class T>
+void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T* num,
+                   const T* den, T* out, double* sums, const char* tag) {
+  const int ncol = nx * ny;
+  RedGeom g = red_geom(ncol, nz);
+  double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * 2 * nz * sizeof(double));
+  if (P->ws.dry) return;
+  dim3 grid(g.nbz, g.nbc);
+  {
+    ProfScope ps(P, s, "k_box_clip", 3.0 * ncol * nz * esize(P));
+    k_box_clip<T><<<grid, RED_BS, 0, s>>>(g, num, den, out, nx, ny, part);
+  }
+  launch_ok(P);
+  k_plane_final<<<(2 * nz + 3) / 4, 256, 0, s>>>(g.nbc, 2, nz, part, sums, 1.0);
+  launch_ok(P);
+}
+
 #define PW_LAUNCH(P, s, n, kern, ...)                                     \
   do {                                                                    \
     if (!(P)->ws.dry) {                                                   \

@@ -34,16 +34,29 @@ inline RedGeom red_geom(int ncol, int nz) {
 }
 
 // f(c, z, acc) adds NT term values of point (c, z) into acc[0..NT).
+// Blocks are renumbered so that each XCD owns one contiguous range of
+// column blocks: neighbouring columns (the box filter's x+-1 rows) share an
+// L2.  The partials keep their (column block, term, level) slots, so the sums
+// do not depend on the numbering.
+__device__ __forceinline__ void red_block(int& bz, int& bc) {
+  const unsigned tot = gridDim.x * gridDim.y;
+  const unsigned lg = xcd_logical(blockIdx.x + blockIdx.y * gridDim.x, tot);
+  bz = (int)(lg % gridDim.x);
+  bc = (int)(lg / gridDim.x);
+}
+
 template <int NT, class F>
 __global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double* part) {
   __shared__ double sm[NT][RED_BS];
+  int bz, bc;
+  red_block(bz, bc);
   const int zl = threadIdx.x % g.zb, cl = threadIdx.x / g.zb;
-  const int z = blockIdx.x * g.zb + zl;
+  const int z = bz * g.zb + zl;
   double acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.0;
   if (z < g.nz) {
-    const int c0 = blockIdx.y * g.cpb;
+    const int c0 = bc * g.cpb;
     const int c1 = imin(c0 + g.cpb, g.ncol);
     for (int c = c0 + cl; c < c1; c += g.lanes) f(c, z, acc);
   }
@@ -55,7 +68,7 @@ __global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double
     for (int t = 0; t < NT; ++t) {
       double s = 0.0;
       for (int l = 0; l < g.lanes; ++l) s += sm[t][l * g.zb + zl];
-      part[((size_t)blockIdx.y * NT + t) * g.nz + z] = s;
+      part[((size_t)bc * NT + t) * g.nz + z] = s;
     }
   }
 }
@@ -388,35 +401,71 @@ __global__ void k_wl_scalar_updn(PwDims d, WlSca<T> l1, WlSca<T> l2,
 }
 
 // Imfilter (Utilities.py:87-115) of both inputs at each level, the clipped
-// ratio (DynamicSGS_LASDD_SM.py:103-130, ScalarLASDD_SM.py:104-131), and the
-// plane sums of sqrt(C) and C for the profiles (LASDD_SM.py:359-360).
-template <class T> struct FBoxClip {
-  static constexpr const char* NAME = "FBoxClip";
-  static constexpr int NFIELDS = 3;
-  const T* num; const T* den; T* out; int nx, ny, nz;
-  __device__ void operator()(int c, int z, double* acc) const {
-    const int x = c / ny, y = c - x * ny;
-    const int xm = (x + nx - 1) % nx, xp = (x + 1) % nx;
-    const int ym = (y + ny - 1) % ny, yp = (y + 1) % ny;
-    const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp};
+// ratio (DynamicSGS_LASDD_SM.py:103-130, ScalarLASDD_SM.py:104-131, and the
+// WL twins), and the plane sums of sqrt(C) and C for the profiles
+// (LASDD_SM.py:359-360).  Each thread walks a contiguous run of columns
+// along y and keeps the 3-row column sums of y-1, y and y+1, so every step
+// loads one new column of the 3x3 box (3 values per field) instead of nine.
+// Partials in the layout of k_plane_partial (NT = 2: sqrt(C), C).
+template <class T>
+__global__ void __launch_bounds__(RED_BS)
+k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* __restrict__ out,
+           int nx, int ny, double* part) {
+  __shared__ double sm[2][RED_BS];
+  int bz, bc;
+  red_block(bz, bc);
+  const int zl = threadIdx.x % g.zb, cl = threadIdx.x / g.zb;
+  const int z = bz * g.zb + zl;
+  const int nz = g.nz;
+  const int chunk = g.cpb / g.lanes;
+  double s1 = 0.0, s2 = 0.0;
+  if (z < nz) {
+    const int c0 = bc * g.cpb + cl * chunk;
+    const int c1 = imin(c0 + chunk, g.ncol);
     const T inv9 = T(1.0 / 9.0);
-    T a = T(0), b = T(0);
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const size_t j = ((size_t)xs[p] * ny + ys[q]) * nz + z;
-        a += num[j] * inv9;
-        b += den[j] * inv9;
+    T a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;   // column sums at y-1, y, y+1
+    int xm = 0, x = -1, xp = 0;
+    auto col = [&](int yy, T& sa, T& sb) {
+      const size_t jm = ((size_t)xm * ny + yy) * nz + z;
+      const size_t j0 = ((size_t)x * ny + yy) * nz + z;
+      const size_t jp = ((size_t)xp * ny + yy) * nz + z;
+      sa = num[jm] + num[j0] + num[jp];
+      sb = den[jm] + den[j0] + den[jp];
+    };
+    for (int c = c0; c < c1; ++c) {
+      const int cx = c / ny, y = c - cx * ny;
+      const int yp = (y + 1 == ny) ? 0 : y + 1;
+      if (cx != x || c == c0) {
+        x = cx;
+        xm = (x + nx - 1) % nx;
+        xp = (x + 1) % nx;
+        col((y + ny - 1) % ny, a0, b0);
+        col(y, a1, b1);
+      } else {
+        a0 = a1; b0 = b1;
+        a1 = a2; b1 = b2;
       }
-    T r = a / b;
-    const bool mask = (fabs((double)b) < 1e-10) || (r < T(0)) || (r > T(1));
-    if (mask) r = T(0);
-    out[(size_t)c * nz + z] = r;
-    acc[0] += sqrt((double)r);
-    acc[1] += (double)r;
+      col(yp, a2, b2);
+      const T a = (a0 + a1 + a2) * inv9, b = (b0 + b1 + b2) * inv9;
+      T r = a / b;
+      if ((fabs((double)b) < 1e-10) || (r < T(0)) || (r > T(1))) r = T(0);
+      out[(size_t)c * nz + z] = r;
+      s1 += sqrt((double)r);
+      s2 += (double)r;
+    }
   }
-};
+  sm[0][threadIdx.x] = s1;
+  sm[1][threadIdx.x] = s2;
+  __syncthreads();
+  if (cl == 0 && z < nz) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      double s = 0.0;
+      for (int l = 0; l < g.lanes; ++l) s += sm[t][l * g.zb + zl];
+      part[((size_t)bc * 2 + t) * nz + z] = s;
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // beta: ComputeBeta1/2 (LASDD_SM.py:53-95, ScalarLASDD_SM.py:54-96) with the

@@ -366,8 +366,7 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     PW_LAUNCH(P, c.s, c.F, k_smag_level<T>, c.dims(), l2, Shatd);
   }
   double* s2 = c.D("bsums", 2 * c.nz);
-  FBoxClip<T> fb{LM, MM, Cs2, c.nx, c.ny, c.nz};
-  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "box");
+  box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cs2, s2, "box");
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
@@ -445,8 +444,7 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
   T* Tdn = c.W("Tdn");
   PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
   double* s2 = c.D("ssum2", 2 * c.nz);
-  FBoxClip<T> fb{Tup, Tdn, CsPr, c.nx, c.ny, c.nz};
-  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "sbox");
+  box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CsPr, s2, "sbox");
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
@@ -506,8 +504,7 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
   T* MM = c.W("MM");
   PW_LAUNCH(P, c.s, c.F, k_wl_lm<T>, c.dims(), l1, l2, beta, d.L, d.TFR, LM, MM);
   double* s2 = c.D("bsums", 2 * c.nz);
-  FBoxClip<T> fb{LM, MM, Cwl, c.nx, c.ny, c.nz};
-  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "box");
+  box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cwl, s2, "box");
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
@@ -569,8 +566,7 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
   T* Tdn = c.W("Tdn");
   PW_LAUNCH(P, c.s, c.F, k_wl_scalar_updn<T>, c.dims(), l1, l2, beta, d.L, d.TFR, Tup, Tdn);
   double* s2 = c.D("ssum2", 2 * c.nz);
-  FBoxClip<T> fb{Tup, Tdn, CwlPr, c.nx, c.ny, c.nz};
-  plane_sums<2>(P, c.s, c.ncol, c.nz, fb, s2, 1.0, "sbox");
+  box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CwlPr, s2, "sbox");
   if (!P->ws.dry) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
