@@ -258,6 +258,25 @@ int alfa_sgs_stresses(alfa_plan*, const void* const grads[9], const void* Cs2_3D
  * DynamicSGS_Main.py:420-478 / StaticSGS_Main.py:258-360. */
 int alfa_scalar_fluxes(alfa_plan*, const void* const grads[9], const void* const dTH[3],
                        const void* Cs2PrRatio_3D, const void* qz_sfc, void* q[3], alfa_stream);
+/* The Main.py SGS call sites (Main.py:333-443): DivStressDynamicSGS
+ * (NSE_SGSTerms.py:113) when `dynamic`, else DivStressStaticSGS (:186) with the
+ * given Cs2_3D: strain rates, the dynamic coefficient (written to Cs2_3D and
+ * the profiles), stresses with the wall stress, and their divergence
+ * divt = divtx, divty, divtz. */
+int alfa_div_stress_sgs(alfa_plan*, const void* const grads[9], const void* u, const void* v,
+                        const void* w, const void* M_sfc_loc, const void* psi_m,
+                        const void* psi_m0, int dynamic, void* Cs2_3D, double* Cs2_1D_avg1,
+                        double* Cs2_1D_avg2, double* beta1_1D, void* divt[3], alfa_stream);
+/* DivFluxDynamicSGS (SCL_SGSTerms.py:97) when `dynamic`, else
+ * DivFluxStaticSGS (:169): scalar coefficient (dynamic; it reuses the
+ * momentum procedure's filtered fields, so it follows alfa_div_stress_sgs on
+ * the same plan, as the reference passes dynamicSGSmomentum), the fluxes
+ * (qz out, with qz_sfc at level 0) and their divergence divq. */
+int alfa_div_flux_sgs(alfa_plan*, const void* const grads[9], const void* u, const void* v,
+                      const void* w, const void* TH, const void* const dTH[3],
+                      const void* qz_sfc, int dynamic, void* Cs2PrRatio_3D,
+                      double* Cs2PrRatio_1D, double* beta2_1D, void* qz, void* divq,
+                      alfa_stream);
 /* Wall (SGSStresses_SM.py:63): planes (nx,ny). */
 int alfa_wall(alfa_plan*, const void* u, const void* v, const void* M_sfc_loc,
               const void* psi_m, const void* psi_m0, void* txz, void* tyz,

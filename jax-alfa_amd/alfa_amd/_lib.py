@@ -118,6 +118,10 @@ _SIGS = {
                           C.POINTER(C.c_void_p), _P],
     "alfa_scalar_fluxes": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _P, _P,
                            C.POINTER(C.c_void_p), _P],
+    "alfa_div_stress_sgs": [_P, C.POINTER(C.c_void_p), _P, _P, _P, _P, _P, _P, C.c_int, _P, _P,
+                            _P, _P, C.POINTER(C.c_void_p), _P],
+    "alfa_div_flux_sgs": [_P, C.POINTER(C.c_void_p), _P, _P, _P, _P, C.POINTER(C.c_void_p), _P,
+                          C.c_int, _P, _P, _P, _P, _P, _P],
     "alfa_wall": [_P] * 9,
     "alfa_surface_flux": [_P, _P, _P, _P, C.c_double] + [_P] * 12,
     "alfa_profile_begin": [_P, C.c_char_p, C.c_int],

@@ -432,6 +432,80 @@ def ScalarFluxes(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, dTHdx, dT
     return tuple(o)
 
 
+class SGSMomentum:
+    """What DivStress*SGS hands to DivFlux*SGS (the reference's
+    dynamicSGSmomentum / staticSGSmomentum tuples, NSE_SGSTerms.py:178,249).
+    Here the filtered fields stay in the plan's workspace; the handle keeps
+    the inputs the flux call needs.  Main.py passes a slice of the tuple
+    (Main.py:352, :438): slicing returns the handle itself."""
+
+    def __init__(self, grads, u, v, w):
+        self.grads, self.u, self.v, self.w = grads, u, v, w
+
+    def __getitem__(self, k):
+        return self
+
+
+def _div_stress_sgs(grads, u, v, w, M_sfc_loc, MOSTfunctions, dynamic, Cs2_3D):
+    torch = _torch()
+    P = _P()
+    cs = P.field() if Cs2_3D is None else Cs2_3D
+    pr = [P.zeros((P.nz,), dtype=torch.float64) for _ in range(3)]
+    o = [P.field() for _ in range(3)]
+    P.call("alfa_div_stress_sgs", _ptr_array(P, grads), P.ptr(u), P.ptr(v), P.ptr(w),
+           P.ptr(M_sfc_loc), P.ptr(MOSTfunctions[0]), P.ptr(MOSTfunctions[1]), int(dynamic),
+           P.ptr(cs), *[P.ptr(t) for t in pr], _ptr_array(P, o), current_stream())
+    return o, cs, pr
+
+
+def DivStressDynamicSGS(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, u, v, w, M_sfc_loc,
+                        MOSTfunctions, ZeRo3D=None, ZeRo3D_fft=None, ZeRo3D_pad_fft=None, kx2=None,
+                        ky2=None):
+    """src/pde/NSE_SGSTerms.py:113 -> (divtx, divty, divtz, Cs2_1D_avg1, Cs2_1D_avg2,
+    beta1_1D, Cs2_3D, dynamicSGSmomentum)."""
+    grads = [dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz]
+    o, cs, pr = _div_stress_sgs(grads, u, v, w, M_sfc_loc, MOSTfunctions, 1, None)
+    return o[0], o[1], o[2], pr[0], pr[1], pr[2], cs, SGSMomentum(grads, u, v, w)
+
+
+def DivStressStaticSGS(dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, Cs2_3D, u, v,
+                       M_sfc_loc, MOSTfunctions, ZeRo3D=None, ZeRo3D_fft=None,
+                       ZeRo3D_pad_fft=None, kx2=None, ky2=None):
+    """src/pde/NSE_SGSTerms.py:186 -> (divtx, divty, divtz, staticSGSmomentum)."""
+    grads = [dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz]
+    o, _, _ = _div_stress_sgs(grads, u, v, u, M_sfc_loc, MOSTfunctions, 0, Cs2_3D)
+    return o[0], o[1], o[2], SGSMomentum(grads, u, v, None)
+
+
+def _div_flux_sgs(mom, TH, dTH, qz_sfc, dynamic, CsPr):
+    torch = _torch()
+    P = _P()
+    cs = P.field() if CsPr is None else CsPr
+    pr = [P.zeros((P.nz,), dtype=torch.float64) for _ in range(2)]
+    qz, divq = P.field(), P.field()
+    w = mom.w if mom.w is not None else mom.u
+    P.call("alfa_div_flux_sgs", _ptr_array(P, mom.grads), P.ptr(mom.u), P.ptr(mom.v), P.ptr(w),
+           P.ptr(TH if TH is not None else mom.u), _ptr_array(P, dTH), P.ptr(qz_sfc), int(dynamic),
+           P.ptr(cs), *[P.ptr(t) for t in pr], P.ptr(qz), P.ptr(divq), current_stream())
+    return qz, divq, cs, pr
+
+
+def DivFluxDynamicSGS(dynamicSGSmomentum, TH, dTHdx, dTHdy, dTHdz, SHFX, ZeRo3D=None,
+                      ZeRo3D_fft=None, ZeRo3D_pad_fft=None, kx2=None, ky2=None):
+    """src/pde/SCL_SGSTerms.py:97 (after DivStressDynamicSGS on the same plan)
+    -> (qz, divq, Cs2PrRatio_3D, Cs2PrRatio_1D, beta2_1D)."""
+    qz, divq, cs, pr = _div_flux_sgs(dynamicSGSmomentum, TH, [dTHdx, dTHdy, dTHdz], SHFX, 1, None)
+    return qz, divq, cs, pr[0], pr[1]
+
+
+def DivFluxStaticSGS(staticSGSmomentum, Cs2PrRatio_3D, dTHdx, dTHdy, dTHdz, SHFX, ZeRo3D=None,
+                     ZeRo3D_fft=None, ZeRo3D_pad_fft=None, kx2=None, ky2=None):
+    """src/pde/SCL_SGSTerms.py:169 -> (qz, divq)."""
+    qz, divq, _, _ = _div_flux_sgs(staticSGSmomentum, None, [dTHdx, dTHdy, dTHdz], SHFX, 0,
+                                   Cs2PrRatio_3D)
+    return qz, divq
+
+
 def Wall(u, v, M_sfc_loc, psi2D_m, psi2D_m0):
     """src/subgridscale/SGSStresses_SM.py:63 -> (txz, tyz) planes."""
     P = _P()

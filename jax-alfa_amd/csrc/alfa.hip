@@ -415,6 +415,60 @@ int scalar_fluxes(alfa_plan* P, hipStream_t s, const void* const g[9], const voi
   return 0;
 }
 
+// DivStressDynamicSGS / DivStressStaticSGS (NSE_SGSTerms.py:113,186): the
+// dynamic coefficient (when `dynamic`), the stresses and their divergence.
+template <class T>
+int div_stress_sgs(alfa_plan* P, hipStream_t s, const void* const g[9], const void* u,
+                   const void* v, const void* w, const void* M, const void* pm, const void* pm0,
+                   int dynamic, void* Cs2, double* a1, double* a2, double* beta,
+                   void* const divt[3]) {
+  Ctx<T> c(P, s);
+  const T* gg[9];
+  for (int n = 0; n < 9; ++n) gg[n] = (const T*)g[n];
+  SgsStrains<T> ss{};
+  stage_strains<T>(c, gg, ss);
+  if (dynamic && wl_model(P))
+    stage_lasdd_wl<T>(c, (const T*)u, (const T*)v, (const T*)w, ss.Sij, nullptr, nullptr, (T*)Cs2,
+                      a1, a2, beta);
+  else if (dynamic)
+    stage_lasdd<T>(c, (const T*)u, (const T*)v, (const T*)w, ss.Sij, ss.Sm, nullptr, nullptr,
+                   (T*)Cs2, a1, a2, beta);
+  T* tau[6] = {c.W("txx"), c.W("tyy"), c.W("tzz"), c.W("txy"), c.W("txz"), c.W("tyz")};
+  stage_stresses<T>(c, ss, (const T*)Cs2, (const T*)u, (const T*)v, (const T*)M, (const T*)pm,
+                    (const T*)pm0, tau);
+  stage_div_stress<T>(c, tau[0], tau[1], tau[2], tau[3], tau[4], tau[5], (T*)divt[0],
+                      (T*)divt[1], (T*)divt[2]);
+  return 0;
+}
+
+// DivFluxDynamicSGS / DivFluxStaticSGS (SCL_SGSTerms.py:97,169).  The dynamic
+// scalar procedure reuses the momentum procedure's filtered velocities and
+// |S^| (DynamicSGS_Main.py:326-331), which stay in the plan's workspace from
+// the preceding div_stress_sgs call on the same plan.
+template <class T>
+int div_flux_sgs(alfa_plan* P, hipStream_t s, const void* const g[9], const void* u,
+                 const void* v, const void* w, const void* TH, const void* const dT[3],
+                 const void* qsfc, int dynamic, void* CsPr, double* prof, double* beta,
+                 void* qz, void* divq) {
+  Ctx<T> c(P, s);
+  const T* gg[9];
+  for (int n = 0; n < 9; ++n) gg[n] = (const T*)g[n];
+  SgsStrains<T> ss{};
+  stage_strains<T>(c, gg, ss);
+  const T* d3[3] = {(const T*)dT[0], (const T*)dT[1], (const T*)dT[2]};
+  if (dynamic && wl_model(P))
+    stage_scalar_wl<T>(c, (const T*)u, (const T*)v, (const T*)w, 0, (const T*)TH, d3[0], d3[1],
+                       d3[2], step_hats(c), nullptr, nullptr, nullptr, (T*)CsPr, prof, beta);
+  else if (dynamic)
+    stage_scalar_lasdd<T>(c, (const T*)u, (const T*)v, (const T*)w, 0, (const T*)TH, d3[0],
+                          d3[1], d3[2], ss.Sm, step_hats(c), nullptr, nullptr, nullptr,
+                          (T*)CsPr, prof, beta);
+  T* q[3] = {c.W("qx"), c.W("qy"), (T*)qz};
+  stage_fluxes<T>(c, ss, d3, (const T*)CsPr, (const T*)qsfc, q);
+  stage_div_flux<T>(c, q[0], q[1], q[2], (T*)divq);
+  return 0;
+}
+
 template <class T>
 int surface(alfa_plan* P, hipStream_t s, const void* u, const void* v, const void* TH,
             double sfc_val, void* pm, void* pm0, void* ph, void* ph0, void* fm, void* fh,
@@ -472,6 +526,10 @@ template <class T> static void plan_dry_run(alfa_plan* P) {
   if (o >= 1 && o <= 4)
     ops::scalar_lasdd<T>(P, s, three, three, three, f, three, f, f, f, f, nullptr, nullptr);
   ops::sgs_stresses<T>(P, s, nine, f, f, f, f, f, f, six);
+  ops::div_stress_sgs<T>(P, s, nine, f, f, f, f, f, f, o >= 1 && o <= 4, f, nullptr, nullptr,
+                         nullptr, three_o);
+  ops::div_flux_sgs<T>(P, s, nine, f, f, f, f, three, f, o >= 1 && o <= 4, f, nullptr, nullptr, f,
+                       f);
   ops::scalar_fluxes<T>(P, s, nine, three, f, f, three_o);
 }
 
@@ -937,6 +995,28 @@ extern "C" int alfa_scalar_fluxes(alfa_plan* P, const void* const grads[9],
                                   const void* qz_sfc, void* q[3], alfa_stream s) {
   DISPATCH(P, "alfa_scalar_fluxes",
            ops::scalar_fluxes<T>(P, STREAM(s), grads, dTH, Cs2PrRatio_3D, qz_sfc, q));
+}
+extern "C" int alfa_div_stress_sgs(alfa_plan* P, const void* const grads[9], const void* u,
+                                   const void* v, const void* w, const void* M_sfc_loc,
+                                   const void* psi_m, const void* psi_m0, int dynamic,
+                                   void* Cs2_3D, double* Cs2_1D_avg1, double* Cs2_1D_avg2,
+                                   double* beta1_1D, void* divt[3], alfa_stream s) {
+  if (P && dynamic && (P->cfg.optSgs < 1 || P->cfg.optSgs > 4))
+    return set_error(ALFA_EINVAL, "alfa_div_stress_sgs: dynamic needs optSgs 1..4");
+  DISPATCH(P, "alfa_div_stress_sgs",
+           ops::div_stress_sgs<T>(P, STREAM(s), grads, u, v, w, M_sfc_loc, psi_m, psi_m0, dynamic,
+                                  Cs2_3D, Cs2_1D_avg1, Cs2_1D_avg2, beta1_1D, divt));
+}
+extern "C" int alfa_div_flux_sgs(alfa_plan* P, const void* const grads[9], const void* u,
+                                 const void* v, const void* w, const void* TH,
+                                 const void* const dTH[3], const void* qz_sfc, int dynamic,
+                                 void* Cs2PrRatio_3D, double* Cs2PrRatio_1D, double* beta2_1D,
+                                 void* qz, void* divq, alfa_stream s) {
+  if (P && dynamic && (P->cfg.optSgs < 1 || P->cfg.optSgs > 4))
+    return set_error(ALFA_EINVAL, "alfa_div_flux_sgs: dynamic needs optSgs 1..4");
+  DISPATCH(P, "alfa_div_flux_sgs",
+           ops::div_flux_sgs<T>(P, STREAM(s), grads, u, v, w, TH, dTH, qz_sfc, dynamic,
+                                Cs2PrRatio_3D, Cs2PrRatio_1D, beta2_1D, qz, divq));
 }
 extern "C" int alfa_wall(alfa_plan* P, const void* u, const void* v, const void* M,
                          const void* pm, const void* pm0, void* txz, void* tyz, alfa_stream s) {

@@ -140,3 +140,50 @@ def test_scalar_lasdd(grid, prec):
         e = rel_err(host(prof), ref["Cs2PrRatio_1D"])
         print(f"{grid} mixed: Cs2/Pr profile rel err {e:.2e}")
         assert e < 1e-2
+
+
+@pytest.mark.parametrize("key", ["lasdd_sm_fgr2", "lasdd_wl_fgr2", "lad_sm_fgr1"])
+def test_sgs_call_sites_match_the_oracle_step(key, prec):
+    """DivStressDynamicSGS -> DivFluxDynamicSGS and the static pair on the
+    intermediate fields of an oracle step (Main.py:333-443)."""
+    P, g = get_plan(key, prec)
+    name, over = GRIDS[key]
+    cfg, u, v, w, TH = cases.initial_fields(name, **over)
+    th = O.ThomasPressureInit(g)
+    ser = cases.surface_series(cfg, 4)
+    s = O.step(g, O.initial_state(g, u, v, w, TH), th, None if ser is None else ser[0])
+    s2 = O.step(g, s, th, None if ser is None else ser[1], keep=True)
+    dg = s2.diag
+    D = P.asdev
+    grads = [D(a) for a in dg["grads"]]
+    uf, vf, wf, Tf = D(dg["u_f"]), D(dg["v_f"]), D(dg["w_f"]), D(dg["TH_f"])
+    most = [D(m) for m in s2.MOST]
+    dtx, dty, dtz, a1, a2, b1, cs, mom = ops.DivStressDynamicSGS(*grads, uf, vf, wf, D(dg["M_sfc_loc"]),
+                                                               most)
+    qz, divq, cp, prof, b2 = ops.DivFluxDynamicSGS(mom[10:], Tf, *[D(a) for a in dg["dTH"]],
+                                                    D(dg["qz_sfc"]))
+    if prec == "fp64":
+        for got, ref in ((dtx, dg["divt"][0]), (dty, dg["divt"][1]), (dtz, dg["divt"][2]),
+                         (divq, dg["divq"]), (qz, dg["qz"])):
+            e = rel_err(host(got), ref)
+            print(key, prec, "dynamic call sites rel err %.2e" % e)
+            assert e < 1e-9
+    else:
+        # Two steps after a smooth start the dynamic coefficients are ratios of
+        # rounding-level contractions in single precision (the step-level
+        # mixed tests hold the flow itself); here the call sites must equal
+        # the same pipeline composed from the finer entry points.
+        S = ops.StrainsUVPnodes_NoDealias(*grads)
+        if g.optSgs in (2, 4):
+            cs_ref = ops.LASDD_WL(uf, vf, wf, *S[:6])[0]
+        else:
+            cs_ref = ops.LASDD(uf, vf, wf, *S)[0]
+        tau = ops.SGSStresses(*grads, cs_ref, uf, vf, D(dg["M_sfc_loc"]), most[0], most[1])
+        for a, b in zip((dtx, dty, dtz), ops.DivStress(*tau)):
+            assert torch.equal(a, b)
+        assert torch.equal(cs, cs_ref)
+    # the static pair with the coefficient fields the dynamic call produced
+    sx, sy, sz, smom = ops.DivStressStaticSGS(*grads, cs, uf, vf, D(dg["M_sfc_loc"]), most)
+    sq, sdq = ops.DivFluxStaticSGS(smom[6:], cp, *[D(a) for a in dg["dTH"]], D(dg["qz_sfc"]))
+    for a, b in ((sx, dtx), (sy, dty), (sz, dtz), (sdq, divq), (sq, qz)):
+        assert torch.equal(a, b)
