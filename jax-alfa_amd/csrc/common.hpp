@@ -41,6 +41,18 @@ constexpr int FFT_MAXRAD = 8;
 #ifndef ALFA_RADIX_MAX
 #define ALFA_RADIX_MAX 8
 #endif
+//   ALFA_C2R_BS, ALFA_R2C_BS, ALFA_X_BS: threads of the compile-time-length
+//   y C2R, y R2C and single-op x kernels (256 or 512; the lines per tile
+//   scale with them, 512 threads give 128-byte tile rows)
+#ifndef ALFA_C2R_BS
+#define ALFA_C2R_BS 512
+#endif
+#ifndef ALFA_R2C_BS
+#define ALFA_R2C_BS 256
+#endif
+#ifndef ALFA_X_BS
+#define ALFA_X_BS 256
+#endif
 template <class T> struct Lines { static constexpr int L = ALFA_LPB * 8 / (int)sizeof(c2<T>); };
 
 struct Fft1D {

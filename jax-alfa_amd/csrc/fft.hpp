@@ -207,17 +207,17 @@ __host__ __device__ constexpr int radix_of(int m) {
   return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : 5;
 }
 
-template <class T, int R, int LPB, int N, int NS>
+template <class T, int R, int LPB, int N, int NS, int BS>
 __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sign) {
   constexpr int NR = N / R;
   constexpr int TOTAL = NR * LPB;
-  constexpr int MAXIT = (TOTAL + FFT_BS - 1) / FFT_BS;
+  constexpr int MAXIT = (TOTAL + BS - 1) / BS;
   constexpr int TSTEP = N / (NS * R);
   c2<T> v[MAXIT][R];
 #pragma unroll
   for (int q = 0; q < MAXIT; ++q) {
-    const int it = threadIdx.x + q * FFT_BS;
-    if (TOTAL % FFT_BS == 0 || it < TOTAL) {
+    const int it = threadIdx.x + q * BS;
+    if (TOTAL % BS == 0 || it < TOTAL) {
       const int j = it / LPB, p = it % LPB;
       const int jm = j % NS;
 #pragma unroll
@@ -236,8 +236,8 @@ __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sig
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXIT; ++q) {
-    const int it = threadIdx.x + q * FFT_BS;
-    if (TOTAL % FFT_BS == 0 || it < TOTAL) {
+    const int it = threadIdx.x + q * BS;
+    if (TOTAL % BS == 0 || it < TOTAL) {
       const int j = it / LPB, p = it % LPB;
       const int jm = j % NS;
       const int base = (j / NS) * NS * R + jm;
@@ -248,23 +248,24 @@ __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sig
   __syncthreads();
 }
 
-template <class T, int LPB, int N, int NS, int M>
+template <class T, int LPB, int N, int NS, int M, int BS>
 struct FftCT {
   static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
     constexpr int R = radix_of(M);
-    fft_pass_ct<T, R, LPB, N, NS>(buf, tw, sign);
-    FftCT<T, LPB, N, NS * R, M / R>::run(buf, tw, sign);
+    fft_pass_ct<T, R, LPB, N, NS, BS>(buf, tw, sign);
+    FftCT<T, LPB, N, NS * R, M / R, BS>::run(buf, tw, sign);
   }
 };
-template <class T, int LPB, int N, int NS>
-struct FftCT<T, LPB, N, NS, 1> {
+template <class T, int LPB, int N, int NS, int BS>
+struct FftCT<T, LPB, N, NS, 1, BS> {
   static __device__ __forceinline__ void run(c2<T>*, const c2<T>*, int) {}
 };
 
-// transform of LPB lines: compile-time length NCT, or f.n when NCT == 0
-template <class T, int LPB, int MAXN, int NCT>
+// transform of LPB lines by BS threads: compile-time length NCT, or f.n when
+// NCT == 0 (the generic passes run with FFT_BS threads)
+template <class T, int LPB, int MAXN, int NCT, int BS = FFT_BS>
 __device__ __forceinline__ void fft_any(c2<T>* buf, const c2<T>* tw, const Fft1D& f, int sign) {
-  if constexpr (NCT > 0) FftCT<T, LPB, NCT, 1, NCT>::run(buf, tw, sign);
+  if constexpr (NCT > 0) FftCT<T, LPB, NCT, 1, NCT, BS>::run(buf, tw, sign);
   else fft_lines<T, LPB, MAXN>(buf, tw, f, sign);
 }
 
@@ -311,11 +312,11 @@ struct TileGrid {
 // thread loads 16 bytes of z at a time through ld.vload and stores the two
 // complex values of one packed line together.
 // ----------------------------------------------------------------------------
-template <class T, class Loader, int MAXN, int NCT>
-__global__ void __launch_bounds__(FFT_BS)
+template <class T, class Loader, int MAXN, int NCT, int BS>
+__global__ void __launch_bounds__(BS)
 k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
         TileGrid tg, int vec) {
-  constexpr int L = Lines<T>::L;
+  constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -323,27 +324,27 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
   T* bufr = reinterpret_cast<T*>(buf);
-  for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
+  for (int i = threadIdx.x; i < n; i += BS) stw[i] = tw[i];
   int zt, x, b;
   for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
     if (vec) {
-      for (int e = threadIdx.x * W; e < nEl; e += FFT_BS * W) {
+      for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
         const int y = e / ZT, z = zt * ZT + (e - y * ZT);
         vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
       }
     } else {
-      for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
+      for (int i = threadIdx.x; i < nEl; i += BS) {
         const int y = i / ZT, z = zt * ZT + (i - y * ZT);
         bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
       }
     }
     __syncthreads();
-    fft_any<T, L, MAXN, NCT>(buf, stw, fy, -1);
+    fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, -1);
     // line p holds z0+2p (real part) and z0+2p+1 (imaginary part)
     c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
     const int z0 = zt * ZT;
-    for (int i = threadIdx.x; i < ncy_out * L; i += FFT_BS) {
+    for (int i = threadIdx.x; i < ncy_out * L; i += BS) {
       const int k = i / L, p = i - k * L;
       const int z = z0 + 2 * p;
       if (z >= nz) continue;
@@ -407,12 +408,12 @@ __device__ __forceinline__ c2<T> x_apply(const XOp& op, c2<T> v, int k, int nin,
 
 // load tile (zt, ky) of field b into LDS, (k, line) at buf[k*L + line];
 // with vec, float lines are moved in pairs (16 bytes)
-template <class T, int L>
+template <class T, int L, int BS>
 __device__ __forceinline__ void x_load(c2<T>* buf, const XArgs& a, int nEl, int zt, int ky, int b) {
   const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b]);
   const int nz = a.nz;
   if (sizeof(T) == 4 && a.vec) {
-    for (int e = 2 * threadIdx.x; e < nEl; e += 2 * FFT_BS) {
+    for (int e = 2 * threadIdx.x; e < nEl; e += 2 * BS) {
       const int k = e / L, z = zt * L + (e - k * L);
       c2<T> v0 = mkc<T>(0, 0), v1 = mkc<T>(0, 0);
       if (z < nz) cld2<T>(in + ((size_t)k * a.ncy_in + ky) * nz + z, v0, v1);
@@ -421,25 +422,25 @@ __device__ __forceinline__ void x_load(c2<T>* buf, const XArgs& a, int nEl, int 
     }
     return;
   }
-  for (int i = threadIdx.x; i < nEl; i += FFT_BS) {
+  for (int i = threadIdx.x; i < nEl; i += BS) {
     const int k = i / L, z = zt * L + (i - k * L);
     buf[i] = (z < nz) ? in[((size_t)k * a.ncy_in + ky) * nz + z] : mkc<T>(0, 0);
   }
 }
 
-template <class T, int L>
+template <class T, int L, int BS>
 __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, int nz, int nout,
                                         int zt, int ky, int vec) {
   c2<T>* dst = reinterpret_cast<c2<T>*>(dstp);
   const int z0 = zt * L;
   if (sizeof(T) == 4 && vec) {
-    for (int e = 2 * threadIdx.x; e < nout * L; e += 2 * FFT_BS) {
+    for (int e = 2 * threadIdx.x; e < nout * L; e += 2 * BS) {
       const int k = e / L, z = z0 + (e - k * L);
       if (z < nz) cst2<T>(dst + ((size_t)k * ncy + ky) * nz + z, buf[e], buf[e + 1]);
     }
     return;
   }
-  for (int i = threadIdx.x; i < nout * L; i += FFT_BS) {
+  for (int i = threadIdx.x; i < nout * L; i += BS) {
     const int k = i / L, z = z0 + (i - k * L);
     if (z < nz) dst[((size_t)k * ncy + ky) * nz + z] = buf[i];
   }
@@ -450,12 +451,12 @@ __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, i
 // LDS (the common case; small register footprint).  MULTI = true: the
 // forward spectrum is kept in registers and every op (mask / derivative /
 // dealias map, optional inverse) is built from it in turn.
-template <class T, int MAXN, bool MULTI, int NCT>
-__global__ void __launch_bounds__(FFT_BS)
+template <class T, int MAXN, bool MULTI, int NCT, int BS>
+__global__ void __launch_bounds__(BS)
 k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     const c2<T>* __restrict__ tw_out, TileGrid tg) {
-  constexpr int L = Lines<T>::L;
-  constexpr int MAXE = MAXN * L / FFT_BS;
+  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int MAXE = MAXN * L / BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nin = NCT > 0 ? NCT : fin.n;   // NCT is set only when fin.n == fout.n
   const int nbuf = NCT > 0 ? NCT : imax(nin, fout.n);
@@ -464,28 +465,28 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
   c2<T>* stw_out = stw_in + nin;
   const int nz = a.nz;
   const int nEl = nin * L;
-  for (int i = threadIdx.x; i < nin; i += FFT_BS) stw_in[i] = tw_in[i];
+  for (int i = threadIdx.x; i < nin; i += BS) stw_in[i] = tw_in[i];
   if (MULTI)
-    for (int i = threadIdx.x; i < fout.n; i += FFT_BS) stw_out[i] = tw_out[i];
+    for (int i = threadIdx.x; i < fout.n; i += BS) stw_out[i] = tw_out[i];
   int zt, ky, b;
   for (int it = 0; tg.tile(it, zt, ky, b); ++it) {
     if (!MULTI && ky >= a.op[0].ncy) continue;
     __syncthreads();
-    x_load<T, L>(buf, a, nEl, zt, ky, b);
+    x_load<T, L, BS>(buf, a, nEl, zt, ky, b);
     __syncthreads();
-    if (!a.in_spectral) fft_any<T, L, MAXN, NCT>(buf, stw_in, fin, -1);
+    if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, -1);
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
     if (!MULTI) {
       const XOp op = a.op[0];
-      for (int i = threadIdx.x; i < nEl; i += FFT_BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
+      for (int i = threadIdx.x; i < nEl; i += BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
       __syncthreads();
-      if (op.inverse) fft_any<T, L, MAXN, NCT>(buf, stw_in, fin, +1);
-      x_store<T, L>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky, a.vec);
+      if (op.inverse) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, +1);
+      x_store<T, L, BS>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky, a.vec);
     } else {
       c2<T> spec[MAXE];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q) {
-        const int i = threadIdx.x + q * FFT_BS;
+        const int i = threadIdx.x + q * BS;
         spec[q] = (i < nEl) ? buf[i] : mkc<T>(0, 0);
       }
       for (int o = 0; o < a.nops; ++o) {
@@ -494,12 +495,12 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
         const int nout = op.nout;
         __syncthreads();
         if (op.map != XMAP_ID) {
-          for (int i = threadIdx.x; i < nout * L; i += FFT_BS) buf[i] = mkc<T>(0, 0);
+          for (int i = threadIdx.x; i < nout * L; i += BS) buf[i] = mkc<T>(0, 0);
           __syncthreads();
         }
 #pragma unroll
         for (int q = 0; q < MAXE; ++q) {
-          const int i = threadIdx.x + q * FFT_BS;
+          const int i = threadIdx.x + q * BS;
           if (i < nEl) {
             const int k = i / L, p = i - k * L;
             const c2<T> v = x_apply<T>(op, spec[q], k, nin, kyv);
@@ -517,10 +518,10 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
         }
         __syncthreads();
         if (op.inverse) {
-          if (op.map == XMAP_ID) fft_any<T, L, MAXN, NCT>(buf, stw_in, fin, +1);
-          else fft_any<T, L, MAXN, NCT>(buf, stw_out, fout, +1);
+          if (op.map == XMAP_ID) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, +1);
+          else fft_any<T, L, MAXN, NCT, BS>(buf, stw_out, fout, +1);
         }
-        x_store<T, L>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky, a.vec);
+        x_store<T, L, BS>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky, a.vec);
       }
     }
   }
@@ -569,10 +570,12 @@ __device__ __forceinline__ void yc2r_in2(const YC2RArgs& a, int b, int x, int k,
   }
 }
 
-template <class T, class Storer, int MAXN, int NCT>
-__global__ void __launch_bounds__(FFT_BS)
+// BS = 512 (compile-time lengths only): twice the lines per tile, so each
+// tile row is 128 bytes of z.
+template <class T, class Storer, int MAXN, int NCT, int BS>
+__global__ void __launch_bounds__(BS)
 k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
-  constexpr int L = Lines<T>::L;
+  constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -582,16 +585,16 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
   const int nz = a.nz;
   const int kmax = imin(a.ncy_in, half + 1);
   const int nIn = kmax * L;
-  for (int i = threadIdx.x; i < n; i += FFT_BS) stw[i] = tw[i];
+  for (int i = threadIdx.x; i < n; i += BS) stw[i] = tw[i];
   int zt, x, b;
   for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
     // the modes kmax..n-kmax are zero
-    for (int i = threadIdx.x; i < n * L; i += FFT_BS) {
+    for (int i = threadIdx.x; i < n * L; i += BS) {
       const int k = i / L;
       if (k >= kmax && k <= n - kmax) buf[i] = mkc<T>(0, 0);
     }
-    for (int i = threadIdx.x; i < nIn; i += FFT_BS) {
+    for (int i = threadIdx.x; i < nIn; i += BS) {
       const int k = i / L, p = i - k * L;
       const int z = zt * ZT + 2 * p;
       c2<T> X1 = mkc<T>(0, 0), X2 = mkc<T>(0, 0);
@@ -601,12 +604,12 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
       if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
     }
     __syncthreads();
-    fft_any<T, L, MAXN, NCT>(buf, stw, fy, +1);
+    fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, +1);
     const T* bufr = reinterpret_cast<const T*>(buf);
     const T sc = (T)a.scale;
     const int z0 = zt * ZT;
     if (a.vec) {
-      for (int e = threadIdx.x * W; e < n * ZT; e += FFT_BS * W) {
+      for (int e = threadIdx.x * W; e < n * ZT; e += BS * W) {
         const int y = e / ZT, zl = e - y * ZT;
         const int z = z0 + zl;
         if (z >= nz) continue;
@@ -616,7 +619,7 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
         st.vstore(b, x, y, z, v);
       }
     } else {
-      for (int i = threadIdx.x; i < n * ZT; i += FFT_BS) {
+      for (int i = threadIdx.x; i < n * ZT; i += BS) {
         const int y = i / ZT, zl = i - y * ZT;
         const int z = z0 + zl;
         if (z < nz) st(b, x, y, z, bufr[i] * sc);

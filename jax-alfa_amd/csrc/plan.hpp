@@ -156,9 +156,6 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   if (P->ws.dry) return;
   PtrBatch pb{};
   for (int b = 0; b < nb; ++b) pb.p[b] = outs[b];
-  constexpr int ZT = 2 * Lines<T>::L;
-  TileGrid tg{(nz + ZT - 1) / ZT, rows, nb, Loader::BATCH_FAST};
-  const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
   const double cells = (double)rows * fyy.n * nz;
   ProfScope ps(P, s, std::string("k_y_r2c<") + Loader::NAME + ">",
                Loader::nin(nb) * cells * sizeof(T) +
@@ -168,23 +165,35 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   bool vec = nz % Vec16<T>::W == 0 && ld.aligned(nb);
   for (int b = 0; b < nb; ++b) vec = vec && al16(outs[b]);
   ct_length(fyy.n, [&](auto nct, auto maxn) {
-    k_y_r2c<T, Loader, decltype(maxn)::value, decltype(nct)::value>
-        <<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg,
-                                                 vec ? 1 : 0);
+    constexpr int NCT = decltype(nct)::value;
+    constexpr int BS = NCT > 0 ? ALFA_R2C_BS : FFT_BS;
+    constexpr int L = Lines<T>::L * BS / FFT_BS;
+    TileGrid tg{(nz + 2 * L - 1) / (2 * L), rows, nb, Loader::BATCH_FAST};
+    const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
+    const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+    k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>
+        <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
   });
   launch_ok(P);
 }
 
 template <class T, bool MULTI>
-void launch_kx(alfa_plan* P, hipStream_t s, unsigned grid, size_t lds, const XArgs& a,
-               const Fft1D& fin, const Fft1D& fout, const void* twin, const void* twout,
-               const TileGrid& tg) {
+void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, const Fft1D& fin,
+               const Fft1D& fout, const void* twin, const void* twout) {
   // compile-time length only when input and output lengths agree
   const int n = fin.n == fout.n ? fin.n : -1;
   const int tier = imax(fin.n, fout.n);
   auto go = [&](auto nct, auto maxn) {
-    k_x<T, decltype(maxn)::value, MULTI, decltype(nct)::value>
-        <<<grid, FFT_BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+    constexpr int NCT = decltype(nct)::value;
+    constexpr int BS = (NCT > 0 && !MULTI) ? ALFA_X_BS : FFT_BS;
+    constexpr int L = Lines<T>::L * BS / FFT_BS;
+    TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
+    const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
+    const int nbuf = imax(fin.n, fout.n);
+    const size_t lds =
+        (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
+    k_x<T, decltype(maxn)::value, MULTI, NCT, BS>
+        <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
   };
   if (n > 0) ct_length(n, go);
   else ct_length(tier <= 256 ? 1 : 1000, go);   // generic kernel of the tier
@@ -204,19 +213,14 @@ void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a_in, int nb, const Fft1D&
     for (int o = 0; o < a.nops; ++o) vec = vec && al16(a.dst[b][o]);
   }
   a.vec = vec ? 1 : 0;
-  constexpr int L = Lines<T>::L;
   if (rows < 0) rows = a.ncy_in;
-  TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
-  const int nbuf = imax(fin.n, fout.n);
-  const size_t lds = (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
-  const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
   const bool multi = a.nops > 1 || a.op[0].map != XMAP_ID;
   double bytes = (double)nb * fin.n * imin(rows, a.ncy_in) * a.nz * sizeof(c2<T>);
   for (int o = 0; o < a.nops; ++o)
     bytes += (double)nb * a.op[o].nout * imin(a.op[o].ncy, rows) * a.nz * sizeof(c2<T>);
   ProfScope ps(P, s, multi ? "k_x<multi>" : "k_x<single>", bytes);
-  if (multi) launch_kx<T, true>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
-  else launch_kx<T, false>(P, s, grid, lds, a, fin, fout, twin, twout, tg);
+  if (multi) launch_kx<T, true>(P, s, a, rows, nb, fin, fout, twin, twout);
+  else launch_kx<T, false>(P, s, a, rows, nb, fin, fout, twin, twout);
   launch_ok(P);
 }
 
@@ -229,15 +233,19 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
   for (int b = 0; b < nb; ++b)
     for (int k = 0; k < a.nsum; ++k) vec = vec && al16(a.in[b][k]);
   a.vec = vec ? 1 : 0;
-  constexpr int ZT = 2 * Lines<T>::L;
-  TileGrid tg{(a.nz + ZT - 1) / ZT, rows, nb, 0};
-  const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
   ProfScope ps(P, s, std::string("k_y_c2r<") + Storer::NAME + ">",
                (double)nb * a.nsum * rows * a.ncy_in * a.nz * sizeof(c2<T>) +
                    (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T));
   ct_length(fyy.n, [&](auto nct, auto maxn) {
-    k_y_c2r<T, Storer, decltype(maxn)::value, decltype(nct)::value>
-        <<<grid, FFT_BS, fft_lds<T>(fyy.n), s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+    constexpr int NCT = decltype(nct)::value;
+    // compile-time lengths: ALFA_C2R_BS threads and lines in proportion
+    constexpr int BS = NCT > 0 ? ALFA_C2R_BS : FFT_BS;
+    constexpr int L = Lines<T>::L * BS / FFT_BS;
+    TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, nb, 0};
+    const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
+    const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+    k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>
+        <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
   });
   launch_ok(P);
 }
