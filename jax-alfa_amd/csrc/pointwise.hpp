@@ -126,6 +126,50 @@ __global__ void k_scalar_adv(PwDims d, const T* __restrict__ u, const T* __restr
   }
 }
 
+// k_advection and k_scalar_adv in one pass (the unpadded FGR = 2 path):
+// u, v, w are read once for both.
+template <class T>
+__global__ void k_advection_all(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                                const T* __restrict__ w, const T* __restrict__ dudy,
+                                const T* __restrict__ dudz, const T* __restrict__ dvdx,
+                                const T* __restrict__ dvdz, const T* __restrict__ dwdx,
+                                const T* __restrict__ dwdy, const T* __restrict__ sx,
+                                const T* __restrict__ sy, const T* __restrict__ sz, T* Cx, T* Cy,
+                                T* Cz, T* sadv) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const T ui = u[i], vi = v[i], wi = w[i];
+    const T a1x = vi * (dudy[i] - dvdx[i]);
+    const T a1y = ui * (dvdx[i] - dudy[i]);
+    const T cc1 = ui * sx[i] + vi * sy[i];
+    T cx, cy, cz, r;
+    if (z == 0) {
+      const T w1 = w[i + 1];
+      cx = a1x + T(0.5) * (w1 * (dudz[i + 1] - dwdx[i + 1]));
+      cy = a1y + T(0.5) * (w1 * (dvdz[i + 1] - dwdy[i + 1]));
+      cz = T(0);
+      r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
+    } else if (z < nz - 1) {
+      const T w1 = w[i + 1];
+      const T a2x0 = wi * (dudz[i] - dwdx[i]), a2x1 = w1 * (dudz[i + 1] - dwdx[i + 1]);
+      const T a2y0 = wi * (dvdz[i] - dwdy[i]), a2y1 = w1 * (dvdz[i + 1] - dwdy[i + 1]);
+      cx = a1x + T(0.5) * (a2x0 + a2x1);
+      cy = a1y + T(0.5) * (a2y0 + a2y1);
+      const T uG = T(0.5) * (u[i - 1] + ui);
+      const T vG = T(0.5) * (v[i - 1] + vi);
+      cz = uG * (dwdx[i] - dudz[i]) + vG * (dwdy[i] - dvdz[i]);
+      r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
+    } else {
+      cx = a1x + wi * (dudz[i] - dwdx[i]);
+      cy = a1y + wi * (dvdz[i] - dwdy[i]);
+      cz = T(0);
+      r = cc1 + wi * sz[i];
+    }
+    Cx[i] = cx; Cy[i] = cy; Cz[i] = cz; sadv[i] = r;
+  }
+}
+
 // BuoyancyOpt1/2 (src/pde/NSE_BuoyancyTerms.py:49-150), dry; bar = plane
 // mean of TH + T_0 (double).
 template <class T>
@@ -307,6 +351,61 @@ __global__ void k_scalar_flux(PwDims d, const T* __restrict__ dx, const T* __res
   }
 }
 
+// Unpadded SGS stresses and scalar fluxes in one pass (FGR = 2): the w-node
+// strain rates of k_strains_w are formed in registers, then the stresses of
+// k_stress_uvp / k_stress_w (wall row included) and the fluxes of
+// k_scalar_flux, with the same arithmetic.  Sm == NULL: Wong-Lilly (no |S|).
+template <class T> struct SgsFusedArgs {
+  const T *dudx, *dvdx, *dwdx, *dudy, *dvdy, *dwdy, *dudz, *dvdz, *dwdz;
+  const T *S12, *Sm, *Cs2, *CsPr, *dTx, *dTy, *dTz;
+  T pre_s, pre_q;
+  const T *u, *v, *M, *psim, *psim0, *qsfc;
+  double logz1z0, vonk, Ugal;
+  T *txx, *tyy, *tzz, *txy, *txz, *tyz, *qx, *qy, *qz;
+};
+
+template <class T>
+__global__ void k_sgs_fused(PwDims d, SgsFusedArgs<T> a) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    if (z == nz - 1) {
+      a.txx[i] = a.tyy[i] = a.tzz[i] = a.txy[i] = T(0);
+      a.txz[i] = a.tyz[i] = T(0);
+      a.qx[i] = a.qy[i] = a.qz[i] = T(0);
+      continue;
+    }
+    const T sm = a.Sm ? a.Sm[i] : T(1);
+    const T ps = a.pre_s * a.Cs2[i] * sm;
+    a.txx[i] = ps * a.dudx[i]; a.tyy[i] = ps * a.dvdy[i]; a.tzz[i] = ps * a.dwdz[i];
+    a.txy[i] = ps * a.S12[i];
+    const T pq = a.pre_q * a.CsPr[i] * sm;
+    a.qx[i] = pq * a.dTx[i];
+    a.qy[i] = pq * a.dTy[i];
+    if (z == 0) {
+      const size_t c = i / nz;
+      const T denom = (T)a.logz1z0 + a.psim[c] - a.psim0[c];
+      const T us = (T)a.vonk * a.M[c] / denom;
+      a.txz[i] = -(us * us) * (a.u[i] + (T)a.Ugal) / a.M[c];
+      a.tyz[i] = -(us * us) * a.v[i] / a.M[c];
+      a.qz[i] = a.qsfc ? a.qsfc[c] : T(0);
+      continue;
+    }
+    // w-node strains (k_strains_w, interior levels)
+    const T ux = T(0.5) * (a.dudx[i - 1] + a.dudx[i]), uy = T(0.5) * (a.dudy[i - 1] + a.dudy[i]);
+    const T vx = T(0.5) * (a.dvdx[i - 1] + a.dvdx[i]), vy = T(0.5) * (a.dvdy[i - 1] + a.dvdy[i]);
+    const T wz = T(0.5) * (a.dwdz[i - 1] + a.dwdz[i]);
+    const T s12 = T(0.5) * (uy + vx);
+    const T s13 = T(0.5) * (a.dudz[i] + a.dwdx[i]);
+    const T s23 = T(0.5) * (a.dvdz[i] + a.dwdy[i]);
+    const T sw = a.Sm ? smag<T>(ux, vy, wz, s12, s13, s23) : T(1);
+    const T pw = a.pre_s * (T(0.5) * (a.Cs2[i - 1] + a.Cs2[i])) * sw;
+    a.txz[i] = pw * s13;
+    a.tyz[i] = pw * s23;
+    a.qz[i] = a.pre_q * (T(0.5) * (a.CsPr[i - 1] + a.CsPr[i])) * sw * a.dTz[i];
+  }
+}
+
 // RHS_Momentum (src/pde/NSE_AllTerms.py:48-104) + RHS_Scalar
 // (src/pde/SCL_AllTerms.py:48-82), optAdvection = 0.  Profiles are double.
 template <class T>
@@ -320,14 +419,29 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
                       const double* __restrict__ Vg, const double* __restrict__ rf,
                       const double* __restrict__ rs, const double* __restrict__ ubar,
                       const double* __restrict__ vbar, const double* __restrict__ thbar,
-                      double f, double Ugal, int damping, T* Ru, T* Rv, T* Rw, T* RT) {
+                      double f, double Ugal, int damping, T* Ru, T* Rv, T* Rw, T* RT,
+                      const double* __restrict__ bbar = nullptr, double T0 = 0.0,
+                      double g = 0.0, int bopt = 0) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
     if (Ru) {
       T ru = -Cx[i] - dtx[i] - (T)f * ((T)Vg[z] - v[i]);
       T rv = -Cy[i] - dty[i] + (T)f * ((T)Ug[z] - (T)Ugal - u[i]);
-      T rw = -Cz[i] - dtz[i] + buoy[i];
+      T b;
+      if (buoy) {
+        b = buoy[i];
+      } else if (z == 0) {
+        b = T(0);
+      } else {
+        // k_buoyancy inline: bbar = plane mean of TH + T_0
+        const double t1 = (double)TH[i] + T0, t0 = (double)TH[i - 1] + T0;
+        double n1, n0;
+        if (bopt == 0) { n1 = (t1 - bbar[z]) / T0; n0 = (t0 - bbar[z - 1]) / T0; }
+        else { n1 = (t1 - bbar[z]) / bbar[z]; n0 = (t0 - bbar[z - 1]) / bbar[z - 1]; }
+        b = (T)(0.5 * g * (n1 + n0));
+      }
+      T rw = -Cz[i] - dtz[i] + b;
       if (damping) {
         const double s = rs[z];
         ru = (T)((double)ru - s * ((double)u[i] - ubar[z]));

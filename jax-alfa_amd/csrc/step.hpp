@@ -586,8 +586,9 @@ template <class T> struct SgsStrains {
   T *pS, *pSw;        // padded |S| (dealias)
 };
 
+// w_nodes = false: the unpadded w-node strains are left to k_sgs_fused.
 template <class T>
-void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o) {
+void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o, bool w_nodes = true) {
   alfa_plan* P = c.P;
   hipStream_t s = c.s;
   T *S12 = c.W("S12"), *S13 = c.W("S13"), *S23 = c.W("S23");
@@ -598,8 +599,10 @@ void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o) {
   o.Sij[3] = S12; o.Sij[4] = S13; o.Sij[5] = S23;
   o.S13w = c.W("S13w"); o.S23w = c.W("S23w"); o.Sw = c.W("Sw");
   if (P->d.optDealias != 1) {
-    PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7],
-              g[8], o.S13w, o.S23w, o.Sw, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
+    if (w_nodes)
+      PW_LAUNCH(P, s, c.F, k_strains_w<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6],
+                g[7], g[8], o.S13w, o.S23w, o.Sw, (T*)nullptr, (T*)nullptr, (T*)nullptr,
+                (T*)nullptr);
     o.pS = o.pSw = nullptr;
     return;
   }
@@ -806,9 +809,8 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   // 4. advection (Main.py:301-311)
   T *Cx = c.W("Cx"), *Cy = c.W("Cy"), *Cz = c.W("Cz"), *THadv = c.W("THadv");
   if (!dealias) {
-    PW_LAUNCH(P, s, c.F, k_advection<T>, c.dims(), S.u, S.v, S.w, dudy, dudz, dvdx, dvdz, dwdx,
-              dwdy, Cx, Cy, Cz);
-    PW_LAUNCH(P, s, c.F, k_scalar_adv<T>, c.dims(), S.u, S.v, S.w, dTx, dTy, dTz, THadv);
+    PW_LAUNCH(P, s, c.F, k_advection_all<T>, c.dims(), S.u, S.v, S.w, dudy, dudz, dvdx, dvdz,
+              dwdx, dwdy, dTx, dTy, dTz, Cx, Cy, Cz, THadv);
   } else {
     const T* in9[9] = {S.u, S.v, S.w, dudy, dudz, dvdx, dvdz, dwdx, dwdy};
     T* p9[9];
@@ -830,16 +832,13 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     T* o1[1] = {THadv};
     dealias2<T>(c, 1, pc1, o1);
   }
-  // 5. buoyancy (Main.py:322-327)
-  T* buoy = c.W("buoy");
-  PW_LAUNCH(P, s, c.F, k_buoyancy<T>, c.dims(), S.TH, means + 3 * c.nz, buoy, g.T_0, d.g_nondim,
-            g.optBuoyancy);
+  // 5. buoyancy (Main.py:322-327) is formed inside the momentum RHS kernel
   // 6. SGS (Main.py:333-443)
   const bool dynamic = (g.optSgs >= 1 && g.optSgs <= 4) &&
                        (first || (g.dynamicSGS_call_time > 0 && it % g.dynamicSGS_call_time == 0));
   const T* grads[9] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz};
   SgsStrains<T> ss{};
-  stage_strains<T>(c, grads, ss);
+  stage_strains<T>(c, grads, ss, dealias);
   const double* shu = mixed ? means : nullptr;
   const double* shv = mixed ? means + c.nz : nullptr;
   const double* sht = mixed ? means + 2 * c.nz : nullptr;
@@ -851,7 +850,7 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     stage_lasdd<T>(c, S.u, S.v, S.w, ss.Sij, ss.Sm, shu, shv, S.Cs2, st->Cs2_1D_avg1,
                    st->Cs2_1D_avg2, st->beta1_1D);
   T* tau[6] = {c.W("txx"), c.W("tyy"), c.W("tzz"), c.W("txy"), c.W("txz"), c.W("tyz")};
-  stage_stresses<T>(c, ss, S.Cs2, S.u, S.v, sf.M, S.psim, S.psim0, tau);
+  if (dealias) stage_stresses<T>(c, ss, S.Cs2, S.u, S.v, sf.M, S.psim, S.psim0, tau);
   T *txx = tau[0], *tyy = tau[1], *tzz = tau[2], *txy = tau[3], *txz = tau[4], *tyz = tau[5];
   if (dynamic && wl)
     stage_scalar_wl<T>(c, S.u, S.v, S.w, 0, S.TH, dTx, dTy, dTz, step_hats(c), shu, shv, sht,
@@ -860,10 +859,25 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     stage_scalar_lasdd<T>(c, S.u, S.v, S.w, 0, S.TH, dTx, dTy, dTz, ss.Sm, step_hats(c), shu, shv,
                           sht, S.CsPr, st->Cs2PrRatio_1D, st->beta2_1D);
   T *qx = c.W("qx"), *qy = c.W("qy"), *qzf = c.W("qz");
-  {
+  if (dealias) {
     const T* dT[3] = {dTx, dTy, dTz};
     T* q[3] = {qx, qy, qzf};
     stage_fluxes<T>(c, ss, dT, S.CsPr, sf.qz, q);
+  } else {
+    // stresses and fluxes in one pass (same arithmetic as stage_stresses +
+    // stage_fluxes on the unpadded grid)
+    SgsFusedArgs<T> fa{};
+    fa.dudx = dudx; fa.dvdx = dvdx; fa.dwdx = dwdx; fa.dudy = dudy; fa.dvdy = dvdy;
+    fa.dwdy = dwdy; fa.dudz = dudz; fa.dvdz = dvdz; fa.dwdz = dwdz;
+    fa.S12 = ss.Sij[3]; fa.Sm = wl ? nullptr : ss.Sm; fa.Cs2 = S.Cs2; fa.CsPr = S.CsPr;
+    fa.dTx = dTx; fa.dTy = dTy; fa.dTz = dTz;
+    fa.pre_s = wl ? (T)(-2.0 * std::pow(d.L, 4.0 / 3.0)) : (T)(-2.0 * d.L * d.L);
+    fa.pre_q = wl ? (T)(-std::pow(d.L, 4.0 / 3.0)) : (T)(-2.0 * d.L * d.L);
+    fa.u = S.u; fa.v = S.v; fa.M = sf.M; fa.psim = S.psim; fa.psim0 = S.psim0; fa.qsfc = sf.qz;
+    fa.logz1z0 = std::log(0.5 * d.dz * d.z_scale / g.z0m); fa.vonk = 0.4; fa.Ugal = g.Ugal;
+    fa.txx = txx; fa.tyy = tyy; fa.tzz = tzz; fa.txy = txy; fa.txz = txz; fa.tyz = tyz;
+    fa.qx = qx; fa.qy = qy; fa.qz = qzf;
+    PW_LAUNCH(P, s, c.F, k_sgs_fused<T>, c.dims(), fa);
   }
   T *dtx = c.W("divtx"), *dty = c.W("divty"), *dtz = c.W("divtz"), *divq = c.W("divq");
   stage_div_stress<T>(c, txx, tyy, tzz, txy, txz, tyz, dtx, dty, dtz);
@@ -872,9 +886,10 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   T *Rnu = c.W("Rnu"), *Rnv = c.W("Rnv"), *Rnw = c.W("Rnw"), *RnT = c.W("RnT");
   {
     const double* pr = P->prof;
-    PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, buoy, dtx, dty, dtz,
-              THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means, means + c.nz,
-              means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu, Rnv, Rnw, RnT);
+    PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, (const T*)nullptr,
+              dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
+              means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu, Rnv,
+              Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy);
   }
   // 8. pressure (Main.py:481-500)
   T *dpdx = c.W("dpdx"), *dpdy = c.W("dpdy"), *dpdz = c.W("dpdz");
