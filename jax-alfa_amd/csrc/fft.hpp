@@ -261,6 +261,26 @@ struct FftCT<T, LPB, N, NS, 1, BS> {
   static __device__ __forceinline__ void run(c2<T>*, const c2<T>*, int) {}
 };
 
+// the passes of a compile-time plan after the first and before the last
+// (kernels that fuse the first pass into their loads and the last pass into
+// their stores run these in between)
+template <class T, int LPB, int N, int NS, int M, int BS>
+struct FftCTMid {
+  static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
+    constexpr int R = radix_of(M);
+    if constexpr (M != R) {
+      fft_pass_ct<T, R, LPB, N, NS, BS>(buf, tw, sign);
+      FftCTMid<T, LPB, N, NS * R, M / R, BS>::run(buf, tw, sign);
+    }
+  }
+};
+
+// radix of the last pass of the compile-time plan of length n
+__host__ __device__ constexpr int last_radix(int m) {
+  while (m != radix_of(m)) m /= radix_of(m);
+  return m;
+}
+
 // transform of LPB lines by BS threads: compile-time length NCT, or f.n when
 // NCT == 0 (the generic passes run with FFT_BS threads)
 template <class T, int LPB, int MAXN, int NCT, int BS = FFT_BS>
@@ -572,7 +592,42 @@ __device__ __forceinline__ void yc2r_in2(const YC2RArgs& a, int b, int x, int k,
 
 // BS = 512 (compile-time lengths only): twice the lines per tile, so each
 // tile row is 128 bytes of z.
-template <class T, class Storer, int MAXN, int NCT, int BS>
+// Compile-time length, vector path: the last radix pass writes its outputs
+// straight to the storer, two z levels per store, instead of through LDS.
+// The arithmetic and its order are those of the LDS path below.
+template <class T, class Storer, int N, int BS>
+__device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC2RArgs& a,
+                                                      const c2<T>* buf, const c2<T>* stw, int z0,
+                                                      int x, int b) {
+  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int R = last_radix(N), NS = N / R, TOT = NS * L, IT = (TOT + BS - 1) / BS;
+  const T sc = (T)a.scale;
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L, z = z0 + 2 * p;
+      c2<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        c2<T> e = buf[(j + r * NS) * L + p];
+        if (r > 0) {
+          c2<T> w = stw[j * r];
+          w.y = -w.y;
+          e = cmul<T>(e, w);
+        }
+        v[r] = e;
+      }
+      Dft<T, R>::run(v, +1);
+      if (z < a.nz) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st.store2(b, x, j + r * NS, z, v[r].x * sc, v[r].y * sc);
+      }
+    }
+  }
+}
+
+template <class T, class Storer, int MAXN, int NCT, int BS, bool FUSED = false>
 __global__ void __launch_bounds__(BS)
 k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
   constexpr int L = Lines<T>::L * BS / FFT_BS;
@@ -604,10 +659,15 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
       if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
     }
     __syncthreads();
+    const int z0 = zt * ZT;
+    if constexpr (FUSED) {  // NCT > 0 and a.vec
+      FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw, +1);
+      y_c2r_last_pass_store<T, Storer, NCT, BS>(st, a, buf, stw, z0, x, b);
+      continue;
+    }
     fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, +1);
     const T* bufr = reinterpret_cast<const T*>(buf);
     const T sc = (T)a.scale;
-    const int z0 = zt * ZT;
     if (a.vec) {
       for (int e = threadIdx.x * W; e < n * ZT; e += BS * W) {
         const int y = e / ZT, zl = e - y * ZT;
@@ -660,6 +720,10 @@ template <class T> struct StorePtr {
   }
   __device__ __forceinline__ void vstore(int b, int x, int y, int z, const Vec16<T>& v) const {
     vst(p[b] + ((size_t)x * ny + y) * nz + z, v);
+  }
+  // z and z + 1 (z even, nz even, aligned buffers)
+  __device__ __forceinline__ void store2(int b, int x, int y, int z, T v0, T v1) const {
+    *reinterpret_cast<c2<T>*>(p[b] + ((size_t)x * ny + y) * nz + z) = mkc<T>(v0, v1);
   }
   bool aligned(int nb) const {
     for (int b = 0; b < nb; ++b)

@@ -254,6 +254,21 @@ template <class T> struct StoreDiv {
     }
     vst(out[b] + i, r);
   }
+  // z and z + 1 (z even, nz even, aligned buffers)
+  __device__ __forceinline__ void store2(int b, int x, int y, int z, T v0, T v1) const {
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    const T* f = fz[b];
+    const c2<T> f01 = *reinterpret_cast<const c2<T>*>(f + i);
+    T d0, d1;
+    if (wnode[b]) {
+      d0 = (z < nz - 1) ? (f01.y - f01.x) * idz : T(0);
+      d1 = (z + 1 < nz - 1) ? (f[i + 2] - f01.y) * idz : T(0);
+    } else {
+      d0 = (z > 0) ? (f01.x - f[i - 1]) * idz : T(0);
+      d1 = (f01.y - f01.x) * idz;
+    }
+    *reinterpret_cast<c2<T>*>(out[b] + i) = mkc<T>(v0 + d0, v1 + d1);
+  }
   bool aligned(int nb) const {
     for (int b = 0; b < nb; ++b)
       if (!al16(out[b]) || !al16(fz[b])) return false;

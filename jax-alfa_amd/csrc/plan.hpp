@@ -244,6 +244,13 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
     TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, nb, 0};
     const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
     const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+    if constexpr (NCT > 0) {
+      if (a.vec) {  // first and last radix passes fused into the loads and stores
+        k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>
+            <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+        return;
+      }
+    }
     k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>
         <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
   });
