@@ -316,7 +316,7 @@ int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, 
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
   if (!P->ws.dry) {
-    k_thomas<T><<<c.nx * c.nyc, 64, 0, s>>>(ta);
+    k_thomas<T><<<c.nx * c.nyc, 64, thomas_lds(c.nz), s>>>(ta);
     launch_ok(P);
   }
   XArgs xa{};

@@ -514,7 +514,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
         if (ky >= op.ncy) continue;
         const int nout = op.nout;
         __syncthreads();
-        if (op.map != XMAP_ID) {
+        if (NCT == 0 && op.map != XMAP_ID) {
           for (int i = threadIdx.x; i < nout * L; i += BS) buf[i] = mkc<T>(0, 0);
           __syncthreads();
         }
@@ -524,7 +524,8 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
           if (i < nEl) {
             const int k = i / L, p = i - k * L;
             const c2<T> v = x_apply<T>(op, spec[q], k, nin, kyv);
-            if (op.map == XMAP_ID) {
+            // compile-time plans have fin.n == fout.n: identity map only
+            if (NCT > 0 || op.map == XMAP_ID) {
               buf[i] = v;
             } else if (op.map == XMAP_D1) {
               if (k <= nin / 2) buf[k * L + p] = v;
@@ -538,7 +539,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
         }
         __syncthreads();
         if (op.inverse) {
-          if (op.map == XMAP_ID) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, +1);
+          if (NCT > 0 || op.map == XMAP_ID) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, +1);
           else fft_any<T, L, MAXN, NCT, BS>(buf, stw_out, fout, +1);
         }
         x_store<T, L, BS>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky, a.vec);

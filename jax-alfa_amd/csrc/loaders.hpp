@@ -62,20 +62,23 @@ template <class T> struct LoadLasdd {
       }
       return r;
     }
-    Vec16<T> uu, vv, ww;
-    if (b != 2 && b != 5) {
+    // only the inputs field b needs (b is uniform over the workgroup)
+    Vec16<T> uu = vzero<T>(), vv = vzero<T>(), ww = vzero<T>();
+    if (b == 0 || b == 3 || b == 6 || b == 7) {
       uu = vld(u + i);
-      vv = vld(v + i);
       if (shu) {
 #pragma unroll
         for (int j = 0; j < W; ++j) uu.v[j] = (T)((double)uu.v[j] - shu[z + j]);
       }
+    }
+    if (b == 1 || b == 4 || b == 6 || b == 8) {
+      vv = vld(v + i);
       if (shv) {
 #pragma unroll
         for (int j = 0; j < W; ++j) vv.v[j] = (T)((double)vv.v[j] - shv[z + j]);
       }
     }
-    if (b == 2 || b >= 5) {
+    if (b == 2 || b == 5 || b == 7 || b == 8) {
       // w averaged to uvp nodes: needs w[z .. z+W]
       const Vec16<T> w0 = vld(w + i);
       const T wn = (z + W < nz) ? w[i + W] : T(0);

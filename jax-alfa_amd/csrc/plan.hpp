@@ -316,3 +316,15 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
       launch_ok(P);                                                       \
     }                                                                     \
   } while (0)
+
+// KLAUNCH with dynamic LDS bytes
+#define KLAUNCH_LDS(P, s, name, bytes, kern, grid, block, lds, ...)       \
+  do {                                                                    \
+    if (!(P)->ws.dry) {                                                   \
+      {                                                                   \
+        ProfScope ps_(P, s, name, bytes);                                 \
+        kern<<<grid, block, lds, s>>>(__VA_ARGS__);                       \
+      }                                                                   \
+      launch_ok(P);                                                       \
+    }                                                                     \
+  } while (0)

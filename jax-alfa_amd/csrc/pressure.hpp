@@ -27,16 +27,22 @@ template <class T> struct ThomasArgs {
 };
 
 constexpr int TH_MAXN = 520;   // nz + 1 <= 520
+// dynamic LDS of one column: 4 arrays of nz + 1 doubles
+inline size_t thomas_lds(int nz) { return 4 * (size_t)(nz + 1) * sizeof(double); }
 
 __device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
 __device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
 
 template <class T>
 __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
-  __shared__ double dr[TH_MAXN], di[TH_MAXN], sb[TH_MAXN], smm[TH_MAXN];
+  extern __shared__ double th_smem[];
   const int col = blockIdx.x;
   const int lane = threadIdx.x;
   const int nz = a.nz, N = nz + 1;
+  double* dr = th_smem;
+  double* di = dr + N;
+  double* sb = di + N;
+  double* smm = sb + N;
   const int i = col / a.nyc, j = col % a.nyc;
   const double kr = (i < a.nx / 2) ? (double)i : (double)(i - a.nx);
   const double kc = (double)j;

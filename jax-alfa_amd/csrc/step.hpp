@@ -273,8 +273,8 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   {
     // 3 spectra in, 2 spectra out, 2 double factor tables
     const double spb = (double)c.nx * c.nyc * c.nz * sizeof(c2<T>);
-    KLAUNCH(P, c.s, "k_thomas", 5.0 * spb + 2.0 * c.nx * c.nyc * (c.nz + 1) * 8.0, k_thomas<T>,
-            c.nx * c.nyc, 64, ta);
+    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 2.0 * c.nx * c.nyc * (c.nz + 1) * 8.0,
+                k_thomas<T>, c.nx * c.nyc, 64, thomas_lds(c.nz), ta);
   }
   XArgs xa{};
   xa.ncy_in = c.nyc; xa.nz = c.nz; xa.ny_full = c.ny; xa.in_spectral = 1;
