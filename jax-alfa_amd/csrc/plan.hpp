@@ -110,7 +110,7 @@ inline size_t esize(const alfa_plan* P) { return P->prec == ALFA_FP64 ? 8 : 4; }
 // full-field passes (reads + writes) of each pointwise kernel of the step
 inline int pw_fields(const char* name) {
   static const struct { const char* n; int f; } tab[] = {
-      {"k_derivz", 8}, {"k_derivz_generic", 2}, {"k_stag_avg", 2}, {"k_advection", 12}, {"k_advection_all", 16}, {"k_sgs_fused", 25},
+      {"k_derivz", 8}, {"k_derivz_generic", 2}, {"k_stag_avg", 2}, {"k_advection", 12}, {"k_advection_all", 16}, {"k_sgs_fused", 25}, {"k_sgs_fused_v", 25},
       {"k_scalar_adv", 7}, {"k_buoyancy", 2}, {"k_strains_uvp", 13}, {"k_strains_w", 12},
       {"k_smag", 7}, {"k_stress_uvp", 10}, {"k_stress_w", 6}, {"k_scalar_flux", 9},
       {"k_rhs", 17}, {"k_ab2_plain", 4}, {"k_imfilter", 2}, {"k_lasdd_lm", 24},
@@ -293,6 +293,18 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
   k_plane_final<<<(2 * nz + 3) / 4, 256, 0, s>>>(g.nbc, 2, nz, part, sums, 1.0);
   launch_ok(P);
 }
+
+// n points handled w per thread
+#define PW_LAUNCH_W(P, s, n, w, kern, ...)                                \
+  do {                                                                    \
+    if (!(P)->ws.dry) {                                                   \
+      {                                                                   \
+        ProfScope ps_(P, s, #kern, (double)pw_fields(#kern) * (n) * esize(P)); \
+        kern<<<pw_blocks((n) / (w)), 256, 0, s>>>(__VA_ARGS__);           \
+      }                                                                   \
+      launch_ok(P);                                                       \
+    }                                                                     \
+  } while (0)
 
 #define PW_LAUNCH(P, s, n, kern, ...)                                     \
   do {                                                                    \

@@ -406,6 +406,79 @@ __global__ void k_sgs_fused(PwDims d, SgsFusedArgs<T> a) {
   }
 }
 
+// k_sgs_fused with W = 16 / sizeof(T) consecutive levels per thread and
+// 16-byte accesses (nz % W == 0, all arrays 16-byte aligned); the lower
+// neighbours of the first level come from one scalar load per array.
+template <class T>
+__global__ void k_sgs_fused_v(PwDims d, SgsFusedArgs<T> a) {
+  constexpr int W = Vec16<T>::W;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    const Vec16<T> ux = vld(a.dudx + i0), vy = vld(a.dvdy + i0), wz = vld(a.dwdz + i0);
+    const Vec16<T> s12 = vld(a.S12 + i0), cs = vld(a.Cs2 + i0), cp = vld(a.CsPr + i0);
+    const Vec16<T> tx = vld(a.dTx + i0), ty = vld(a.dTy + i0), tz = vld(a.dTz + i0);
+    const Vec16<T> uy = vld(a.dudy + i0), vx = vld(a.dvdx + i0);
+    const Vec16<T> uz = vld(a.dudz + i0), vz = vld(a.dvdz + i0);
+    const Vec16<T> wx = vld(a.dwdx + i0), wy = vld(a.dwdy + i0);
+    const Vec16<T> sm = a.Sm ? vld(a.Sm + i0) : Vec16<T>{};
+    // level z0 - 1 (used only when z0 > 0)
+    T pux = 0, puy = 0, pvx = 0, pvy = 0, pwz = 0, pcs = 0, pcp = 0;
+    if (z0 > 0) {
+      pux = a.dudx[i0 - 1]; puy = a.dudy[i0 - 1]; pvx = a.dvdx[i0 - 1]; pvy = a.dvdy[i0 - 1];
+      pwz = a.dwdz[i0 - 1]; pcs = a.Cs2[i0 - 1]; pcp = a.CsPr[i0 - 1];
+    }
+    Vec16<T> txx, tyy, tzz, txy, txz, tyz, qx, qy, qz;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int z = z0 + j;
+      if (z == nz - 1) {
+        txx.v[j] = tyy.v[j] = tzz.v[j] = txy.v[j] = T(0);
+        txz.v[j] = tyz.v[j] = T(0);
+        qx.v[j] = qy.v[j] = qz.v[j] = T(0);
+        continue;
+      }
+      const T smv = a.Sm ? sm.v[j] : T(1);
+      const T ps = a.pre_s * cs.v[j] * smv;
+      txx.v[j] = ps * ux.v[j]; tyy.v[j] = ps * vy.v[j]; tzz.v[j] = ps * wz.v[j];
+      txy.v[j] = ps * s12.v[j];
+      const T pq = a.pre_q * cp.v[j] * smv;
+      qx.v[j] = pq * tx.v[j];
+      qy.v[j] = pq * ty.v[j];
+      if (z == 0) {
+        const size_t c = (i0 + j) / nz;
+        const T denom = (T)a.logz1z0 + a.psim[c] - a.psim0[c];
+        const T us = (T)a.vonk * a.M[c] / denom;
+        txz.v[j] = -(us * us) * (a.u[i0 + j] + (T)a.Ugal) / a.M[c];
+        tyz.v[j] = -(us * us) * a.v[i0 + j] / a.M[c];
+        qz.v[j] = a.qsfc ? a.qsfc[c] : T(0);
+        continue;
+      }
+      const T lux = j > 0 ? ux.v[j - 1] : pux, luy = j > 0 ? uy.v[j - 1] : puy;
+      const T lvx = j > 0 ? vx.v[j - 1] : pvx, lvy = j > 0 ? vy.v[j - 1] : pvy;
+      const T lwz = j > 0 ? wz.v[j - 1] : pwz;
+      const T lcs = j > 0 ? cs.v[j - 1] : pcs, lcp = j > 0 ? cp.v[j - 1] : pcp;
+      const T aux = T(0.5) * (lux + ux.v[j]), auy = T(0.5) * (luy + uy.v[j]);
+      const T avx = T(0.5) * (lvx + vx.v[j]), avy = T(0.5) * (lvy + vy.v[j]);
+      const T awz = T(0.5) * (lwz + wz.v[j]);
+      const T w12 = T(0.5) * (auy + avx);
+      const T w13 = T(0.5) * (uz.v[j] + wx.v[j]);
+      const T w23 = T(0.5) * (vz.v[j] + wy.v[j]);
+      const T sw = a.Sm ? smag<T>(aux, avy, awz, w12, w13, w23) : T(1);
+      const T pw = a.pre_s * (T(0.5) * (lcs + cs.v[j])) * sw;
+      txz.v[j] = pw * w13;
+      tyz.v[j] = pw * w23;
+      qz.v[j] = a.pre_q * (T(0.5) * (lcp + cp.v[j])) * sw * tz.v[j];
+    }
+    vst(a.txx + i0, txx); vst(a.tyy + i0, tyy); vst(a.tzz + i0, tzz); vst(a.txy + i0, txy);
+    vst(a.txz + i0, txz); vst(a.tyz + i0, tyz);
+    vst(a.qx + i0, qx); vst(a.qy + i0, qy); vst(a.qz + i0, qz);
+  }
+}
+
 // RHS_Momentum (src/pde/NSE_AllTerms.py:48-104) + RHS_Scalar
 // (src/pde/SCL_AllTerms.py:48-82), optAdvection = 0.  Profiles are double.
 template <class T>

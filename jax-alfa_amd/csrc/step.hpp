@@ -877,7 +877,14 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     fa.logz1z0 = std::log(0.5 * d.dz * d.z_scale / g.z0m); fa.vonk = 0.4; fa.Ugal = g.Ugal;
     fa.txx = txx; fa.tyy = tyy; fa.tzz = tzz; fa.txy = txy; fa.txz = txz; fa.tyz = tyz;
     fa.qx = qx; fa.qy = qy; fa.qz = qzf;
-    PW_LAUNCH(P, s, c.F, k_sgs_fused<T>, c.dims(), fa);
+    const void* vp[] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, fa.S12, fa.Sm, fa.Cs2,
+                        fa.CsPr, dTx, dTy, dTz, txx, tyy, tzz, txy, txz, tyz, qx, qy, qzf};
+    bool vec = c.nz % Vec16<T>::W == 0;
+    for (const void* q : vp) vec = vec && (q == nullptr || al16(q));
+    if (vec)
+      PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_sgs_fused_v<T>, c.dims(), fa);
+    else
+      PW_LAUNCH(P, s, c.F, k_sgs_fused<T>, c.dims(), fa);
   }
   T *dtx = c.W("divtx"), *dty = c.W("divty"), *dtz = c.W("divtz"), *divq = c.W("divq");
   stage_div_stress<T>(c, txx, tyy, tzz, txy, txz, tyz, dtx, dty, dtz);
