@@ -289,6 +289,42 @@ __device__ __forceinline__ void fft_any(c2<T>* buf, const c2<T>* tw, const Fft1D
   else fft_lines<T, LPB, MAXN>(buf, tw, f, sign);
 }
 
+// Twiddles of an n-point plan, loaded to registers at kernel start and
+// written to LDS after the first tile's loads: the two loads are in flight
+// together instead of one after the other.
+// DEFER = false: written to LDS at once (where the held registers would
+// cost occupancy).
+template <class T, int BS, int MAXN, bool DEFER = true>
+struct TwStage {
+  static constexpr int K = DEFER ? (MAXN + BS - 1) / BS : 1;
+  c2<T> r[K];
+  int n;
+  bool pending;
+  __device__ __forceinline__ void load(const c2<T>* tw, int n_, c2<T>* stw) {
+    n = n_;
+    pending = DEFER;
+    if (!DEFER) {
+      for (int i = threadIdx.x; i < n; i += BS) stw[i] = tw[i];
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = threadIdx.x + q * BS;
+      if (i < n) r[q] = tw[i];
+    }
+  }
+  // call between a tile's LDS stores and the barrier before its transform
+  __device__ __forceinline__ void store(c2<T>* stw) {
+    if (!DEFER || !pending) return;
+    pending = false;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = threadIdx.x + q * BS;
+      if (i < n) stw[i] = r[q];
+    }
+  }
+};
+
 struct PtrBatch {
   void* p[MAXB];
 };
@@ -344,7 +380,8 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
   c2<T>* stw = buf + n * L;
   T* bufr = reinterpret_cast<T*>(buf);
-  for (int i = threadIdx.x; i < n; i += BS) stw[i] = tw[i];
+  TwStage<T, BS, (NCT > 0 ? NCT : MAXN)> tws;
+  tws.load(tw, n, stw);
   int zt, x, b;
   for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
@@ -359,6 +396,7 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
         bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
       }
     }
+    tws.store(stw);
     __syncthreads();
     fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, -1);
     // line p holds z0+2p (real part) and z0+2p+1 (imaginary part)
@@ -485,7 +523,8 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
   c2<T>* stw_out = stw_in + nin;
   const int nz = a.nz;
   const int nEl = nin * L;
-  for (int i = threadIdx.x; i < nin; i += BS) stw_in[i] = tw_in[i];
+  TwStage<T, BS, (NCT > 0 ? NCT : MAXN)> tws;
+  tws.load(tw_in, nin, stw_in);
   if (MULTI)
     for (int i = threadIdx.x; i < fout.n; i += BS) stw_out[i] = tw_out[i];
   int zt, ky, b;
@@ -493,6 +532,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     if (!MULTI && ky >= a.op[0].ncy) continue;
     __syncthreads();
     x_load<T, L, BS>(buf, a, nEl, zt, ky, b);
+    tws.store(stw_in);
     __syncthreads();
     if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, -1);
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
@@ -641,7 +681,8 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
   const int nz = a.nz;
   const int kmax = imin(a.ncy_in, half + 1);
   const int nIn = kmax * L;
-  for (int i = threadIdx.x; i < n; i += BS) stw[i] = tw[i];
+  TwStage<T, BS, (NCT > 0 ? NCT : MAXN), sizeof(T) == 4> tws;
+  tws.load(tw, n, stw);
   int zt, x, b;
   for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
@@ -659,6 +700,7 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
       buf[k * L + p] = mkc<T>(X1.x - X2.y, X1.y + X2.x);
       if (k > 0 && k < half) buf[(n - k) * L + p] = mkc<T>(X1.x + X2.y, X2.x - X1.y);
     }
+    tws.store(stw);
     __syncthreads();
     const int z0 = zt * ZT;
     if constexpr (FUSED) {  // NCT > 0 and a.vec

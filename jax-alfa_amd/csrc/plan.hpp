@@ -68,12 +68,14 @@ struct alfa_plan {
   int ncu = 256;   // compute units of the plan's device
 };
 
-// Persistent grid of the tiled FFT kernels: as many workgroups as stay
-// resident (LDS-limited, at most 4 per CU), a multiple of the 8 XCDs, and
-// no more than the tiles rounded up to 8.
-// one workgroup per tile, rounded up to whole XCD rounds
+// Grid of the tiled FFT kernels: one workgroup per tile, rounded up to whole
+// XCD rounds, at most
+// TILE_GRID_MAX workgroups (each then walks several tiles, which spreads the
+// per-workgroup set-up over them)
+constexpr size_t TILE_GRID_MAX = 16384;
 inline unsigned tile_grid(size_t tiles) {
-  return (unsigned)((tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
+  const size_t g = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  return (unsigned)(g < TILE_GRID_MAX ? g : TILE_GRID_MAX);
 }
 
 int set_error(int code, const char* fmt, ...);
