@@ -1,5 +1,6 @@
 // Common device/host helpers for libalfa (MI355X / gfx950).
 #pragma once
+#include <initializer_list>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
@@ -113,6 +114,14 @@ template <class T> __device__ __forceinline__ void cst2(c2<T>* p, c2<T> a, c2<T>
   }
 }
 inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+// the 16-byte vector kernels apply: nz a multiple of the vector width and
+// every (non-null) array 16-byte aligned
+template <class T> inline bool vec_ok(int nz, std::initializer_list<const void*> ps) {
+  if (nz % Vec16<T>::W != 0) return false;
+  for (const void* p : ps)
+    if (p != nullptr && !al16(p)) return false;
+  return true;
+}
 
 // Maximum number of distinct fields one batched launch handles.
 constexpr int MAXB = 24;

@@ -50,6 +50,61 @@ __global__ void k_derivz(PwDims d, const T* __restrict__ u, const T* __restrict_
   }
 }
 
+// k_derivz with W = 16 / sizeof(T) consecutive levels per thread and
+// 16-byte accesses (nz % W == 0, arrays 16-byte aligned): the same
+// arithmetic, the neighbour levels across the vector edge read as scalars.
+template <class T>
+__global__ void k_derivz_v(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                           const T* __restrict__ w, const T* __restrict__ TH,
+                           const T* __restrict__ ustar, const T* __restrict__ M,
+                           const T* __restrict__ fim, const T* __restrict__ fih,
+                           const T* __restrict__ qz, T* dudz, T* dvdz, T* dwdz, T* dTHdz,
+                           T Ugal, T idz, T half_dz_vonk, T vonk_half_dz) {
+  constexpr int W = Vec16<T>::W;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    const size_t c = i0 / nz;
+    if (dudz) {
+      const Vec16<T> uu = vld(u + i0), vv = vld(v + i0), ww = vld(w + i0);
+      const T up = z0 > 0 ? u[i0 - 1] : T(0), vp = z0 > 0 ? v[i0 - 1] : T(0);
+      const T wn = z0 + W < nz ? w[i0 + W] : T(0);
+      Vec16<T> a, b, e;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const int z = z0 + j;
+        if (z == 0) {
+          const T den = M[c] * half_dz_vonk;
+          a.v[j] = fim[c] * ustar[c] * (uu.v[j] + Ugal) / den;
+          b.v[j] = fim[c] * ustar[c] * vv.v[j] / den;
+        } else if (z < nz - 1) {
+          a.v[j] = (uu.v[j] - (j > 0 ? uu.v[j - 1] : up)) * idz;
+          b.v[j] = (vv.v[j] - (j > 0 ? vv.v[j - 1] : vp)) * idz;
+        } else {
+          a.v[j] = T(0); b.v[j] = T(0);
+        }
+        e.v[j] = (z < nz - 1) ? ((j + 1 < W ? ww.v[j + 1] : wn) - ww.v[j]) * idz : T(0);
+      }
+      vst(dudz + i0, a); vst(dvdz + i0, b); vst(dwdz + i0, e);
+    }
+    if (dTHdz) {
+      const Vec16<T> th = vld(TH + i0);
+      const T tp = z0 > 0 ? TH[i0 - 1] : T(0);
+      Vec16<T> o;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const int z = z0 + j;
+        o.v[j] = (z == 0) ? fih[c] * (-qz[c] / ustar[c]) / vonk_half_dz
+                          : (th.v[j] - (j > 0 ? th.v[j - 1] : tp)) * idz;
+      }
+      vst(dTHdz + i0, o);
+    }
+  }
+}
+
 // Derivz_Generic_uvp (Derivatives.py:310) / Derivz_Generic_w (:340).
 template <class T>
 __global__ void k_derivz_generic(PwDims d, const T* __restrict__ F, T* out, T idz, int wnode) {
@@ -531,6 +586,66 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
   }
 }
 
+// k_rhs (both outputs, buoyancy formed inline) with W consecutive levels per
+// thread and 16-byte accesses (nz % W == 0, arrays 16-byte aligned).  Same
+// arithmetic, element by element.
+template <class T>
+__global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+                        const T* __restrict__ w, const T* __restrict__ TH,
+                        const T* __restrict__ Cx, const T* __restrict__ Cy,
+                        const T* __restrict__ Cz, const T* __restrict__ dtx,
+                        const T* __restrict__ dty, const T* __restrict__ dtz,
+                        const T* __restrict__ thadv, const T* __restrict__ divq,
+                        const double* __restrict__ Ug, const double* __restrict__ Vg,
+                        const double* __restrict__ rf, const double* __restrict__ rs,
+                        const double* __restrict__ ubar, const double* __restrict__ vbar,
+                        const double* __restrict__ thbar, double f, double Ugal, int damping,
+                        T* Ru, T* Rv, T* Rw, T* RT, const double* __restrict__ bbar, double T0,
+                        double g, int bopt) {
+  constexpr int W = Vec16<T>::W;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    const Vec16<T> uu = vld(u + i0), vv = vld(v + i0), ww = vld(w + i0), th = vld(TH + i0);
+    const Vec16<T> cx = vld(Cx + i0), cy = vld(Cy + i0), cz = vld(Cz + i0);
+    const Vec16<T> tx = vld(dtx + i0), ty = vld(dty + i0), tz = vld(dtz + i0);
+    const Vec16<T> ta = vld(thadv + i0), dq = vld(divq + i0);
+    const T thp = z0 > 0 ? TH[i0 - 1] : T(0);
+    Vec16<T> ru, rv, rw, rt;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int z = z0 + j;
+      T a = -cx.v[j] - tx.v[j] - (T)f * ((T)Vg[z] - vv.v[j]);
+      T b = -cy.v[j] - ty.v[j] + (T)f * ((T)Ug[z] - (T)Ugal - uu.v[j]);
+      T bu;
+      if (z == 0) {
+        bu = T(0);
+      } else {
+        const double t1 = (double)th.v[j] + T0;
+        const double t0 = (double)(j > 0 ? th.v[j - 1] : thp) + T0;
+        double n1, n0;
+        if (bopt == 0) { n1 = (t1 - bbar[z]) / T0; n0 = (t0 - bbar[z - 1]) / T0; }
+        else { n1 = (t1 - bbar[z]) / bbar[z]; n0 = (t0 - bbar[z - 1]) / bbar[z - 1]; }
+        bu = (T)(0.5 * g * (n1 + n0));
+      }
+      T c = -cz.v[j] - tz.v[j] + bu;
+      T e = -ta.v[j] - dq.v[j];
+      if (damping) {
+        const double sr = rs[z];
+        a = (T)((double)a - sr * ((double)uu.v[j] - ubar[z]));
+        b = (T)((double)b - sr * ((double)vv.v[j] - vbar[z]));
+        c = (T)((double)c - rf[z] * (double)ww.v[j]);
+        e = (T)((double)e - sr * ((double)th.v[j] - thbar[z]));
+      }
+      ru.v[j] = a; rv.v[j] = b; rw.v[j] = c; rt.v[j] = e;
+    }
+    vst(Ru + i0, ru); vst(Rv + i0, rv); vst(Rw + i0, rw); vst(RT + i0, rt);
+  }
+}
+
 // AB2_uvw (NSE_TimeAdvancement.py:45-76) / AB2_TH (SCL_TimeAdvancement.py:
 // 46-71) alone, in place; the thread of level nz-2 writes the top level.
 template <class T>
@@ -635,6 +750,75 @@ __global__ void k_ab2(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ R
       w[i] = wn;
       mw = fmaxf(mw, (float)fabs((double)wn));
     }
+  }
+  __shared__ float sm[3][256];
+  sm[0][threadIdx.x] = mu; sm[1][threadIdx.x] = mv; sm[2][threadIdx.x] = mw;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmaxf(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) cflpart[3 * blockIdx.x + k] = sm[k][0];
+}
+
+// k_ab2 with W consecutive levels per thread and 16-byte accesses (nz % W
+// == 0, arrays 16-byte aligned); the top level copies level nz-2 from the
+// same vector.  Same arithmetic and CFL maxima.
+template <class T>
+__global__ void k_ab2_v(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ Rnu,
+                        const T* __restrict__ Rnv, const T* __restrict__ Rnw,
+                        const T* __restrict__ RnT, T* Rpu, T* Rpv, T* Rpw, T* RpT,
+                        const T* __restrict__ dpdx, const T* __restrict__ dpdy,
+                        const T* __restrict__ dpdz, T dt, T th_top, int first,
+                        double* cflpart) {
+  constexpr int W = Vec16<T>::W;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  float mu = 0.f, mv = 0.f, mw = 0.f;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    const Vec16<T> nu = vld(Rnu + i0), nv_ = vld(Rnv + i0), nw = vld(Rnw + i0), nt = vld(RnT + i0);
+    const Vec16<T> px = vld(dpdx + i0), py = vld(dpdy + i0), pz = vld(dpdz + i0);
+    Vec16<T> ru, rv, rw, rpu, rpv, rpw, rpt;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      ru.v[j] = nu.v[j] - px.v[j];
+      rv.v[j] = nv_.v[j] - py.v[j];
+      rw.v[j] = nw.v[j] - pz.v[j];
+    }
+    if (first) {
+      rpu = ru; rpv = rv; rpw = rw; rpt = nt;
+    } else {
+      rpu = vld(Rpu + i0); rpv = vld(Rpv + i0); rpw = vld(Rpw + i0); rpt = vld(RpT + i0);
+    }
+    vst(Rpu + i0, ru); vst(Rpv + i0, rv); vst(Rpw + i0, rw); vst(RpT + i0, nt);
+    Vec16<T> uu = vld(u + i0), vv = vld(v + i0), tt = vld(TH + i0), ww = vld(w + i0);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int z = z0 + j;
+      if (z < nz - 1) {
+        uu.v[j] = uu.v[j] + dt * (T(1.5) * ru.v[j] - T(0.5) * rpu.v[j]);
+        vv.v[j] = vv.v[j] + dt * (T(1.5) * rv.v[j] - T(0.5) * rpv.v[j]);
+        tt.v[j] = tt.v[j] + dt * (T(1.5) * nt.v[j] - T(0.5) * rpt.v[j]);
+        mu = fmaxf(mu, (float)fabs((double)uu.v[j]));
+        mv = fmaxf(mv, (float)fabs((double)vv.v[j]));
+      } else if (j > 0) {   // z == nz - 1: copies of level nz - 2
+        uu.v[j] = uu.v[j - 1];
+        vv.v[j] = vv.v[j - 1];
+        tt.v[j] = tt.v[j - 1] + th_top;
+      }
+      if (z == 0 || z == nz - 1) {
+        ww.v[j] = T(0);
+      } else {
+        ww.v[j] = ww.v[j] + dt * (T(1.5) * rw.v[j] - T(0.5) * rpw.v[j]);
+        mw = fmaxf(mw, (float)fabs((double)ww.v[j]));
+      }
+    }
+    vst(u + i0, uu); vst(v + i0, vv); vst(TH + i0, tt); vst(w + i0, ww);
   }
   __shared__ float sm[3][256];
   sm[0][threadIdx.x] = mu; sm[1][threadIdx.x] = mv; sm[2][threadIdx.x] = mw;

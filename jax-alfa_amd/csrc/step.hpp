@@ -797,9 +797,14 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   // 2. surface fluxes (Main.py:213-243)
   stage_surface<T>(c, S, sf, sfc_val, sfc_dev);
   // 3. vertical gradients (Derivz_M, Derivz_TH)
-  PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim, S.fih,
-            sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz), (T)(0.4 * 0.5 * d.dz),
-            (T)(0.4 * 0.5 * d.dz));
+  if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, dudz, dvdz, dwdz, dTz}))
+    PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_derivz_v<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar,
+                sf.M, S.fim, S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz),
+                (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
+  else
+    PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim,
+              S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz),
+              (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
   // plane means of u, v, TH, TH + T_0
   double* means = c.D("means", 4 * c.nz);
   {
@@ -877,11 +882,9 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     fa.logz1z0 = std::log(0.5 * d.dz * d.z_scale / g.z0m); fa.vonk = 0.4; fa.Ugal = g.Ugal;
     fa.txx = txx; fa.tyy = tyy; fa.tzz = tzz; fa.txy = txy; fa.txz = txz; fa.tyz = tyz;
     fa.qx = qx; fa.qy = qy; fa.qz = qzf;
-    const void* vp[] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, fa.S12, fa.Sm, fa.Cs2,
-                        fa.CsPr, dTx, dTy, dTz, txx, tyy, tzz, txy, txz, tyz, qx, qy, qzf};
-    bool vec = c.nz % Vec16<T>::W == 0;
-    for (const void* q : vp) vec = vec && (q == nullptr || al16(q));
-    if (vec)
+    if (vec_ok<T>(c.nz, {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz, fa.S12, fa.Sm,
+                         fa.Cs2, fa.CsPr, dTx, dTy, dTz, txx, tyy, tzz, txy, txz, tyz, qx, qy,
+                         qzf}))
       PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_sgs_fused_v<T>, c.dims(), fa);
     else
       PW_LAUNCH(P, s, c.F, k_sgs_fused<T>, c.dims(), fa);
@@ -893,10 +896,17 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   T *Rnu = c.W("Rnu"), *Rnv = c.W("Rnv"), *Rnw = c.W("Rnw"), *RnT = c.W("RnT");
   {
     const double* pr = P->prof;
-    PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, (const T*)nullptr,
-              dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
-              means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu, Rnv,
-              Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy);
+    if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, Cx, Cy, Cz, dtx, dty, dtz, THadv, divq, Rnu, Rnv,
+                         Rnw, RnT}))
+      PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_rhs_v<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz,
+                  dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
+                  means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu,
+                  Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy);
+    else
+      PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, (const T*)nullptr,
+                dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
+                means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu,
+                Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy);
   }
   // 8. pressure (Main.py:481-500)
   T *dpdx = c.W("dpdx"), *dpdy = c.W("dpdy"), *dpdz = c.W("dpdz");
@@ -908,7 +918,13 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   }
   // 9. AB2 + CFL (Main.py:506-540)
   double* cflpart = c.D("cflpart", 3 * AB2_BLOCKS);
-  KLAUNCH(P, s, "k_ab2", 23.0 * c.FB, k_ab2<T>, AB2_BLOCKS, 256, c.dims(), S.u, S.v, S.w, S.TH,
+  if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, Rnu, Rnv, Rnw, RnT, S.Ru, S.Rv, S.Rw, S.RT, dpdx,
+                       dpdy, dpdz}))
+    KLAUNCH(P, s, "k_ab2", 23.0 * c.FB, k_ab2_v<T>, AB2_BLOCKS, 256, c.dims(), S.u, S.v, S.w, S.TH,
+          Rnu, Rnv, Rnw, RnT, S.Ru, S.Rv, S.Rw, S.RT, dpdx, dpdy, dpdz, (T)d.dt_nondim,
+          (T)(d.inversion_nondim * d.dz), first ? 1 : 0, cflpart);
+  else
+    KLAUNCH(P, s, "k_ab2", 23.0 * c.FB, k_ab2<T>, AB2_BLOCKS, 256, c.dims(), S.u, S.v, S.w, S.TH,
           Rnu, Rnv, Rnw, RnT, S.Ru, S.Rv, S.Rw, S.RT, dpdx, dpdy, dpdz, (T)d.dt_nondim,
           (T)(d.inversion_nondim * d.dz), first ? 1 : 0, cflpart);
   if (!P->ws.dry) {
