@@ -263,7 +263,7 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
 template <int NT, class F>
 void plane_sums(alfa_plan* P, hipStream_t s, int ncol, int nz, const F& f, double* out,
                 double scale, const char* tag) {
-  RedGeom g = red_geom(ncol, nz);
+  RedGeom g = red_geom(ncol, nz, esize(P));
   double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * NT * nz * sizeof(double));
   if (P->ws.dry) return;
   dim3 grid(g.nbz, g.nbc);
@@ -283,7 +283,7 @@ template <class T>
 void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T* num,
                    const T* den, T* out, double* sums, const char* tag) {
   const int ncol = nx * ny;
-  RedGeom g = red_geom(ncol, nz);
+  RedGeom g = red_geom(ncol, nz, esize(P));
   double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * 2 * nz * sizeof(double));
   if (P->ws.dry) return;
   dim3 grid(g.nbz, g.nbc);

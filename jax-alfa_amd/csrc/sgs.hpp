@@ -15,20 +15,23 @@
 #include "pointwise.hpp"
 
 constexpr int RED_BS = 256;
+// columns per thread of the plane reductions: short runs (more workgroups)
+// for float fields, long runs for double (measured at 256^3)
+inline int red_cpt(size_t esize) { return esize == 4 ? 16 : 64; }
 constexpr int RED_MAXT = 10;
 
 struct RedGeom {
   int ncol, nz, zb, lanes, cpb, nbz, nbc;
 };
 
-inline RedGeom red_geom(int ncol, int nz) {
+inline RedGeom red_geom(int ncol, int nz, size_t esize) {
   RedGeom g;
   g.ncol = ncol;
   g.nz = nz;
   g.zb = nz <= 32 ? 32 : 64;
   g.lanes = RED_BS / g.zb;
   g.nbz = (nz + g.zb - 1) / g.zb;
-  g.cpb = g.lanes * 64;
+  g.cpb = g.lanes * red_cpt(esize);
   g.nbc = (ncol + g.cpb - 1) / g.cpb;
   return g;
 }
@@ -450,7 +453,8 @@ k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* _
       T r = a / b;
       if ((fabs((double)b) < 1e-10) || (r < T(0)) || (r > T(1))) r = T(0);
       out[(size_t)c * nz + z] = r;
-      s1 += sqrt((double)r);
+      // the root in the precision of the field (the sums are double)
+      s1 += (double)sqrt(r);
       s2 += (double)r;
     }
   }
