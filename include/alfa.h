@@ -122,6 +122,24 @@ int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);
 int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_val,
               alfa_stream stream);
 
+/* ---- statistics (src/utilities/Statistics.py:47-330; Main.py:545-617) ----
+ * Adds one sample of ComputeStats (ResetFlag 0, dry) into acc: the planar
+ * statistics of the state after alfa_step and of that step's intermediates
+ * (dudz, dvdz, dTHdz, txy, txz, tyz, qz, M_sfc_loc, ustar, qz_sfc_avg),
+ * which the call reads from the plan workspace: call it after the
+ * alfa_step of the same plan on the same stream, before the next step.
+ * acc: ALFA_STATS_LEN(nz) doubles, zeroed by the caller to reset (the
+ * ResetStats branch), laid out as the ALFA_STATS_NPROF profiles of nz in
+ * the order U V W TH dUdz dVdz dTHdz u2 v2 w2 TH2 uv uw vw uTH vTH wTH txy
+ * txz tyz qz Cs2_1 Cs2_2 Cs2PrRatio Beta1 Beta2, then M_sfc, ustar, qz_sfc.
+ * The coefficient profiles are taken from the state's optional pointers
+ * (0 where NULL).  Averages are acc / samples (Main.py:589-595). */
+#define ALFA_STATS_NPROF 26
+#define ALFA_STATS_NSCAL 3
+#define ALFA_STATS_LEN(nz) (ALFA_STATS_NPROF * (nz) + ALFA_STATS_NSCAL)
+int alfa_stats_accumulate(alfa_plan* plan, const alfa_state* st, double* acc,
+                          alfa_stream stream);
+
 /* As alfa_step, but the surface value is read from device memory (one
  * double) when the step's kernels run, so that a step captured once into a
  * hipGraph can be replayed with the next SurfaceBC_series value written

@@ -1,0 +1,87 @@
+"""The reference's on-disk run inputs, read and written in its formats.
+
+* ``input/vel.npy``: an (N, 3) float64 column stack of u, v, w, each the
+  Fortran-order flattening of an (nx, ny, nz) field (written by the
+  examples' ``CreateInputs_*.py``, e.g. CreateInputs_NBL_A94.py:68-77; read
+  by src/initialization/Initialization.py:58-79, which removes Ugal).
+* ``input/TH.npy``: the flat Fortran-order potential temperature (K)
+  (Initialization.py:86-101, which removes T_0).
+* ``input/SurfaceBC.npz``: ``data_series`` and ``t_series`` of nsteps + 1
+  values, ``dt_surf`` and ``optSurfBC`` (CreateSurfaceBC_GABLS1.py:121-125),
+  validated against the run's dt, optSurfBC and length and made
+  nondimensional by Initialization.py:107-160.
+
+Fields come back in the stored form the step uses: u minus Ugal, TH minus
+T_0 (u_scale = TH_scale = 1, src/config/DerivedVars.py:49-50).
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+
+def nsteps_of(cfg) -> int:
+    """src/config/DerivedVars.py:90: ceil(SimTime / dt)."""
+    return int(math.ceil(cfg["SimTime"] / cfg["dt"]))
+
+
+def write_initial_fields(input_dir, u, v, w, TH):
+    """u, v, w (m/s, u with Ugal included) and TH (K) as the examples write
+    them: vel.npy (N, 3) and TH.npy, Fortran-order flattening."""
+    os.makedirs(input_dir, exist_ok=True)
+    vel = np.column_stack([np.asarray(a, dtype=np.float64).reshape(-1, order="F")
+                           for a in (u, v, w)])
+    np.save(os.path.join(input_dir, "vel.npy"), vel)
+    np.save(os.path.join(input_dir, "TH.npy"),
+            np.asarray(TH, dtype=np.float64).reshape(-1, order="F"))
+
+
+def read_initial_fields(input_dir, cfg):
+    """Initialize_uvw + Initialize_TH: -> (u - Ugal, v, w, TH - T_0), each
+    (nx, ny, nz) float64."""
+    nx, ny, nz = int(cfg["nx"]), int(cfg["ny"]), int(cfg["nz"])
+    vel = np.load(os.path.join(input_dir, "vel.npy"), allow_pickle=False)
+    if vel.shape != (nx * ny * nz, 3):
+        raise ValueError(f"vel.npy has shape {vel.shape}, expected ({nx * ny * nz}, 3)")
+    Ugal = float(cfg.get("Ugal", 0.0))
+    u = np.reshape(vel[:, 0] - Ugal, (nx, ny, nz), order="F")
+    v = np.reshape(vel[:, 1], (nx, ny, nz), order="F")
+    w = np.reshape(vel[:, 2], (nx, ny, nz), order="F")
+    TH = np.load(os.path.join(input_dir, "TH.npy"), allow_pickle=False)
+    if TH.size != nx * ny * nz:
+        raise ValueError(f"TH.npy has {TH.size} values, expected {nx * ny * nz}")
+    TH = np.reshape(TH, (nx, ny, nz), order="F") - float(cfg["T_0"])
+    return u, v, w, TH
+
+
+def write_surface_bc(path, data_series, dt, optSurfBC):
+    """SurfaceBC.npz as CreateSurfaceBC_GABLS1.py:121-125 writes it:
+    data_series (K or K m/s), t_series = arange(len) * dt, dt_surf,
+    optSurfBC (int32)."""
+    data = np.asarray(data_series, dtype=np.float64)
+    np.savez(path, data_series=data, t_series=np.arange(data.size) * float(dt),
+             dt_surf=np.float64(dt), optSurfBC=np.int32(optSurfBC))
+
+
+def read_surface_bc(path, cfg):
+    """Initialize_SurfaceBC (Initialization.py:107-160): checks dt_surf,
+    optSurfBC and the length nsteps + 1, then -> the nondimensional series
+    (heat flux as is for optSurfBC 1, temperature minus T_0 for 2)."""
+    with np.load(path, allow_pickle=False) as f:
+        dt_surf = float(f["dt_surf"])
+        opt = int(f["optSurfBC"])
+        series = np.asarray(f["data_series"], dtype=np.float64)
+    if abs(dt_surf - float(cfg["dt"])) > 1e-6:
+        raise ValueError(f"SurfaceBC file dt_surf={dt_surf:.6f} s does not match "
+                         f"dt={float(cfg['dt']):.6f} s")
+    if opt != int(cfg["optSurfBC"]):
+        raise ValueError(f"SurfaceBC file optSurfBC={opt} does not match "
+                         f"optSurfBC={int(cfg['optSurfBC'])}")
+    n = nsteps_of(cfg) + 1
+    if series.size != n:
+        raise ValueError(f"SurfaceBC series length {series.size} != nsteps+1={n}")
+    if opt == 1:
+        return series.copy()
+    return series - float(cfg["T_0"])

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include "step.hpp"
+#include "stats.hpp"
 
 using namespace alfa;
 
@@ -526,6 +527,7 @@ template <class T> static void plan_dry_run(alfa_plan* P) {
   if (o >= 1 && o <= 4)
     ops::scalar_lasdd<T>(P, s, three, three, three, f, three, f, f, f, f, nullptr, nullptr);
   ops::sgs_stresses<T>(P, s, nine, f, f, f, f, f, f, six);
+  stats_accumulate<T>(P, s, &st, (double*)f);
   ops::div_stress_sgs<T>(P, s, nine, f, f, f, f, f, f, o >= 1 && o <= 4, f, nullptr, nullptr,
                          nullptr, three_o);
   ops::div_flux_sgs<T>(P, s, nine, f, f, f, f, three, f, o >= 1 && o <= 4, f, nullptr, nullptr, f,
@@ -758,6 +760,14 @@ extern "C" int alfa_step_sfc_dev(alfa_plan* P, const alfa_state* st, int iterati
   if (!P || !st || !sfc_val) return set_error(ALFA_EINVAL, "alfa_step_sfc_dev: null argument");
   DISPATCH(P, "alfa_step_sfc_dev",
            step_t<T>(P, st, iteration, 0.0, STREAM(stream), sfc_val));
+}
+
+static_assert(ALFA_STATS_NPROF == STATS_NPROF && ALFA_STATS_NSCAL == STATS_NSCAL,
+              "statistics layout");
+extern "C" int alfa_stats_accumulate(alfa_plan* P, const alfa_state* st, double* acc,
+                                     alfa_stream stream) {
+  if (!P || !st || !acc) return set_error(ALFA_EINVAL, "alfa_stats_accumulate: null argument");
+  DISPATCH(P, "alfa_stats_accumulate", stats_accumulate<T>(P, STREAM(stream), st, acc));
 }
 
 extern "C" int alfa_fft(alfa_plan* P, const void* F, void* out, alfa_stream s) {

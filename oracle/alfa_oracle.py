@@ -1338,6 +1338,108 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False) -> State:
                  iteration=it + 1, CFLmax=max(s.CFLmax, CFL), diag=diag)
 
 
+# ---------------------------------------------------------------------------
+# Statistics (src/utilities/Statistics.py; Main.py:545-617)
+# ---------------------------------------------------------------------------
+
+# profile keys of the statistics dictionary in the order of Statistics.py:
+# 319-330 (dry runs: the moisture keys Q, dQdz, Q2, wQ, qHz are zero
+# profiles and qm_sfc is 0.0)
+STATS_PROFILES = ("U", "V", "W", "TH", "dUdz", "dVdz", "dTHdz", "u2", "v2", "w2", "TH2",
+                  "uv", "uw", "vw", "uTH", "vTH", "wTH", "txy", "txz", "tyz", "qz",
+                  "Cs2_1", "Cs2_2", "Cs2PrRatio", "Beta1", "Beta2")
+STATS_SCALARS = ("M_sfc", "ustar", "qz_sfc")
+STATS_MOIST_PROFILES = ("Q", "dQdz", "Q2", "wQ", "qHz")
+
+
+def InitializeStats(g: Grid) -> dict:
+    """Statistics.py:333-382 (InitializeStats) and the ResetStats branch of
+    ComputeStats (:134-149): zero accumulators plus the constants."""
+    z = np.zeros(g.nz)
+    d = {k: z.copy() for k in STATS_PROFILES + STATS_MOIST_PROFILES}
+    d.update({k: 0.0 for k in STATS_SCALARS + ("qm_sfc",)})
+    d["Ugal"] = g.Ugal
+    d["ZeRo1D"] = z.copy()
+    return d
+
+
+def ComputeStats(g: Grid, acc: dict, u, v, w, TH, dudz, dvdz, dTHdz, M_sfc_loc, ustar,
+                 qz_sfc_avg, txy, txz, tyz, qz, Cs2_1D_avg1, Cs2_1D_avg2, Cs2PrRatio_1D,
+                 beta1_1D, beta2_1D) -> dict:
+    """Statistics.py:47-330, ComputeStats with ResetFlag = 0 (dry: Q = 0),
+    u_scale = TH_scale = Q_scale = 1 (DerivedVars.py:49-69).  u, v, w, TH
+    are the state after the step's AB2; the other inputs are the step's
+    own intermediates (Main.py:550-557)."""
+    Ugal = g.Ugal
+    nz = g.nz
+    out = dict(acc)
+    # :158-183 mean profiles and mean gradients
+    mU = np.mean(u, axis=(0, 1)) + Ugal
+    mV = np.mean(v, axis=(0, 1))
+    mW = np.mean(w, axis=(0, 1))
+    mTH_anom = np.mean(TH, axis=(0, 1))
+    mTH = mTH_anom + g.T_0_nondim
+    out["U"] = acc["U"] + mU
+    out["V"] = acc["V"] + mV
+    out["W"] = acc["W"] + mW
+    out["TH"] = acc["TH"] + mTH
+    out["dUdz"] = acc["dUdz"] + np.mean(dudz, axis=(0, 1)) / g.z_scale
+    out["dVdz"] = acc["dVdz"] + np.mean(dvdz, axis=(0, 1)) / g.z_scale
+    out["dTHdz"] = acc["dTHdz"] + np.mean(dTHdz, axis=(0, 1)) / g.z_scale
+    # :188-221 level variances and horizontal fluxes (fluctuations about the
+    # level means)
+    uf = u + Ugal - mU[None, None, :]
+    vf = v - mV[None, None, :]
+    wf = w - mW[None, None, :]
+    tf = TH - mTH_anom[None, None, :]
+    for key, a, b in (("u2", uf, uf), ("v2", vf, vf), ("w2", wf, wf), ("TH2", tf, tf),
+                      ("uv", uf, vf), ("uTH", uf, tf), ("vTH", vf, tf)):
+        out[key] = acc[key] + np.mean(a * b, axis=(0, 1))
+    # :226-265 vertical fluxes on the staggered grid: u, v, TH averaged to
+    # w nodes for k >= 1, level 0 taken as is (u with Ugal added)
+    us = np.zeros_like(u)
+    vs = np.zeros_like(v)
+    ts = np.zeros_like(TH)
+    us[:, :, 1:nz] = StagGridAvg(u)
+    vs[:, :, 1:nz] = StagGridAvg(v)
+    ts[:, :, 1:nz] = StagGridAvg(TH)
+    us[:, :, 0] = u[:, :, 0] + Ugal
+    vs[:, :, 0] = v[:, :, 0]
+    ts[:, :, 0] = TH[:, :, 0]
+    mus = np.mean(us, axis=(0, 1)) + Ugal
+    mvs = np.mean(vs, axis=(0, 1))
+    mws = np.mean(w, axis=(0, 1))
+    mts = np.mean(ts, axis=(0, 1))
+    usf = us + Ugal - mus[None, None, :]
+    vsf = vs - mvs[None, None, :]
+    wsf = w - mws[None, None, :]
+    tsf = ts - mts[None, None, :]
+    out["uw"] = acc["uw"] + np.mean(usf * wsf, axis=(0, 1))
+    out["vw"] = acc["vw"] + np.mean(vsf * wsf, axis=(0, 1))
+    out["wTH"] = acc["wTH"] + np.mean(wsf * tsf, axis=(0, 1))
+    # :270-283 SGS stress and flux profiles
+    out["txy"] = acc["txy"] + np.mean(txy, axis=(0, 1))
+    out["txz"] = acc["txz"] + np.mean(txz, axis=(0, 1))
+    out["tyz"] = acc["tyz"] + np.mean(tyz, axis=(0, 1))
+    out["qz"] = acc["qz"] + np.mean(qz, axis=(0, 1))
+    # :288-296 surface variables
+    out["M_sfc"] = acc["M_sfc"] + float(np.mean(M_sfc_loc))
+    out["ustar"] = acc["ustar"] + float(np.sqrt(np.mean(ustar ** 2)))
+    out["qz_sfc"] = acc["qz_sfc"] + float(qz_sfc_avg)
+    # :301-305 SGS coefficient profiles
+    z = np.zeros(nz)
+    for key, prof in (("Cs2_1", Cs2_1D_avg1), ("Cs2_2", Cs2_1D_avg2),
+                      ("Cs2PrRatio", Cs2PrRatio_1D), ("Beta1", beta1_1D), ("Beta2", beta2_1D)):
+        out[key] = acc[key] + (z if prof is None else np.asarray(prof, dtype=np.float64))
+    return out
+
+
+def AverageStats(acc: dict, samples: int) -> dict:
+    """Main.py:589-595: every accumulated key divided by the sample count,
+    the constants Ugal and ZeRo1D kept as they are."""
+    return {k: (v if k in ("Ugal", "ZeRo1D") else v / samples) for k, v in acc.items()}
+
+
 def run(g: Grid, s: State, nsteps: int, sfc_series=None) -> State:
     """Advance ``nsteps`` iterations; ``sfc_series[i]`` is the surface BC
     value for iteration i+1 (Main.py:224, :233 index with iteration-1)."""

@@ -1,0 +1,65 @@
+"""The reference's run-input formats (alfa_amd.io), on the CPU.
+
+vel.npy / TH.npy are Fortran-order flattenings (first index fastest:
+element (i, j, k) at i + nx*(j + ny*k)), checked element by element from
+that definition; SurfaceBC.npz round-trips and is validated as
+Initialization.py:107-160 validates it.
+"""
+import numpy as np
+import pytest
+
+from alfa_amd import cases, io
+
+
+def test_initial_fields_layout_and_round_trip(tmp_path):
+    cfg, u, v, w, TH = cases.initial_fields("stable", nx=6, ny=5, nz=4, seed=2, Ugal=5.0)
+    Ugal, T0 = cfg["Ugal"], cfg["T_0"]
+    io.write_initial_fields(tmp_path, u + Ugal, v, w, TH + T0)
+    vel = np.load(tmp_path / "vel.npy")
+    th = np.load(tmp_path / "TH.npy")
+    nx, ny, nz = u.shape
+    assert vel.shape == (nx * ny * nz, 3) and th.shape == (nx * ny * nz,)
+    for (i, j, k) in [(0, 0, 0), (5, 0, 0), (0, 4, 0), (2, 3, 1), (5, 4, 3)]:
+        n = i + nx * (j + ny * k)
+        assert vel[n, 0] == u[i, j, k] + Ugal
+        assert vel[n, 1] == v[i, j, k] and vel[n, 2] == w[i, j, k]
+        assert th[n] == TH[i, j, k] + T0
+    ru, rv, rw, rth = io.read_initial_fields(tmp_path, cfg)
+    np.testing.assert_allclose(ru, u, rtol=0, atol=1e-14)
+    np.testing.assert_array_equal(rv, v)
+    np.testing.assert_array_equal(rw, w)
+    np.testing.assert_allclose(rth, TH, rtol=0, atol=1e-12)
+
+
+def test_initial_fields_shape_checks(tmp_path):
+    cfg, u, v, w, TH = cases.initial_fields("neutral", nx=4, ny=4, nz=4)
+    io.write_initial_fields(tmp_path, u, v, w, TH)
+    bad = dict(cfg, nz=5)
+    with pytest.raises(ValueError):
+        io.read_initial_fields(tmp_path, bad)
+
+
+def test_surface_bc_round_trip_and_validation(tmp_path):
+    cfg, *_ = cases.initial_fields("stable", nx=8, ny=8, nz=8)
+    cfg = dict(cfg, SimTime=1.0, dt=0.05)
+    n = io.nsteps_of(cfg)
+    assert n == 20                              # ceil(1.0 / 0.05)
+    t = np.arange(n + 1) * cfg["dt"]
+    T_s = cfg["T_0"] - 0.25 * t / 3600.0         # the GABLS1 linear cooling
+    p = tmp_path / "SurfaceBC.npz"
+    io.write_surface_bc(p, T_s, cfg["dt"], 2)
+    with np.load(p) as f:
+        assert set(f.keys()) == {"data_series", "t_series", "dt_surf", "optSurfBC"}
+        np.testing.assert_array_equal(f["t_series"], t)
+    got = io.read_surface_bc(p, cfg)
+    np.testing.assert_allclose(got, cases.surface_series(cfg, n), rtol=0, atol=1e-12)
+    with pytest.raises(ValueError):
+        io.read_surface_bc(p, dict(cfg, dt=0.1, SimTime=2.0))      # dt mismatch
+    with pytest.raises(ValueError):
+        io.read_surface_bc(p, dict(cfg, optSurfBC=1))              # mode mismatch
+    with pytest.raises(ValueError):
+        io.read_surface_bc(p, dict(cfg, SimTime=2.0))              # length mismatch
+    # heat flux series: returned as is (u_scale = TH_scale = 1)
+    q = np.full(n + 1, 0.24)
+    io.write_surface_bc(p, q, cfg["dt"], 1)
+    np.testing.assert_array_equal(io.read_surface_bc(p, dict(cfg, optSurfBC=1)), q)
