@@ -122,6 +122,20 @@ int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);
 int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_val,
               alfa_stream stream);
 
+/* ---- time-varying forcing (src/Main.py:167-188) --------------------------
+ * Sets the geostrophic wind profiles Ug, Vg (optGeoWind >= 1:
+ * GeoWind_U/V[iteration-1], Initialization.py:173-254) and the large-scale
+ * advection tendencies Uadv, Vadv, THadv (optAdvection >= 1:
+ * AdvForcing_*[iteration-1], Initialization.py:253-323, added to
+ * RHS_Momentum / RHS_Scalar, NSE_AllTerms.py:86-88, SCL_AllTerms.py:70-71)
+ * used by the following steps and RHS ops of this plan.  Each argument is a
+ * device pointer to nz nondimensional doubles, or NULL to keep the current
+ * profile (Ug2/Vg2 constants and zero advection at plan creation).  Once an
+ * advection profile is given the plan adds the three tendencies to every
+ * RHS.  A stream-ordered copy kernel: no allocation, capturable. */
+int alfa_set_forcing(alfa_plan* plan, const double* Ug, const double* Vg, const double* Uadv,
+                     const double* Vadv, const double* THadv, alfa_stream stream);
+
 /* ---- statistics (src/utilities/Statistics.py:47-330; Main.py:545-617) ----
  * Adds one sample of ComputeStats (ResetFlag 0, dry) into acc: the planar
  * statistics of the state after alfa_step and of that step's intermediates

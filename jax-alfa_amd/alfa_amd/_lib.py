@@ -81,6 +81,7 @@ _SIGS = {
     "alfa_step": [_P, C.POINTER(AlfaState), C.c_int, C.c_double, _P],
     "alfa_step_sfc_dev": [_P, C.POINTER(AlfaState), C.c_int, _P, _P],
     "alfa_stats_accumulate": [_P, C.POINTER(AlfaState), _P, _P],
+    "alfa_set_forcing": [_P, _P, _P, _P, _P, _P, _P],
     "alfa_fft": [_P, _P, _P, _P],
     "alfa_fft_pad": [_P, _P, _P, _P],
     "alfa_filtering_explicit": [_P, _P, _P, _P, _P],

@@ -151,6 +151,39 @@ class LES:
         self.iteration = int(P.cfg.get("istep", 1))
         self.sfc_series = None if sfc_series is None else np.asarray(sfc_series, dtype=float)
         self.CFLmax = 0.0
+        self.forcing = {}
+
+    def set_forcing_series(self, Ug=None, Vg=None, Uadv=None, Vadv=None, THadv=None):
+        """Time-varying forcing of Main.py:167-188: (nsteps+1, nz)
+        nondimensional series (GeoWind_U/V, AdvForcing_U/V/TH); row
+        iteration-1 is applied before each step.  Kept on the device."""
+        torch = _torch()
+        P = self.plan
+        for k, a in (("Ug", Ug), ("Vg", Vg), ("Uadv", Uadv), ("Vadv", Vadv), ("THadv", THadv)):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+            if a.ndim != 2 or a.shape[1] != P.nz:
+                raise ValueError(f"{k} series must be (nsteps+1, nz={P.nz}), got {a.shape}")
+            self.forcing[k] = torch.from_numpy(a).to(P.device)
+        if any(k in self.forcing for k in ("Uadv", "Vadv", "THadv")):
+            # optAdvection: the three tendencies together (missing ones zero)
+            n = next(iter(self.forcing.values())).shape[0]
+            for k in ("Uadv", "Vadv", "THadv"):
+                self.forcing.setdefault(k, torch.zeros((n, P.nz), dtype=torch.float64,
+                                                       device=P.device))
+
+    def _apply_forcing(self, st):
+        if not self.forcing:
+            return
+        row = self.iteration - 1
+        ptr = []
+        for k in ("Ug", "Vg", "Uadv", "Vadv", "THadv"):
+            t = self.forcing.get(k)
+            if t is not None and row >= t.shape[0]:
+                raise IndexError(f"{k} series has no row for iteration {self.iteration}")
+            ptr.append(None if t is None else C.c_void_p(t[row].data_ptr()))
+        self.plan.call("alfa_set_forcing", *ptr, st)
 
     def step(self, n: int = 1):
         P = self.plan
@@ -159,6 +192,7 @@ class LES:
             sv = 0.0
             if self.sfc_series is not None:
                 sv = float(self.sfc_series[self.iteration - 1])
+            self._apply_forcing(st)
             P.call("alfa_step", C.byref(self.state), int(self.iteration), C.c_double(sv), st)
             self.iteration += 1
 

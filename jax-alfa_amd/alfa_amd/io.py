@@ -10,6 +10,10 @@
   values, ``dt_surf`` and ``optSurfBC`` (CreateSurfaceBC_GABLS1.py:121-125),
   validated against the run's dt, optSurfBC and length and made
   nondimensional by Initialization.py:107-160.
+* the geostrophic-wind file (``Ug_series``/``Vg_series``, or compact
+  ``t_profile``/``Ug_profile``/``Vg_profile`` interpolated in time) and the
+  large-scale advection file (``Uadv_series``, ``Vadv_series``, optional
+  ``THadv_series``), Initialization.py:173-323.
 
 Fields come back in the stored form the step uses: u minus Ugal, TH minus
 T_0 (u_scale = TH_scale = 1, src/config/DerivedVars.py:49-50).
@@ -85,3 +89,64 @@ def read_surface_bc(path, cfg):
     if opt == 1:
         return series.copy()
     return series - float(cfg["T_0"])
+
+
+def read_geowind(path, cfg):
+    """Initialize_GeoWind_Varying (Initialization.py:173-250): -> (Ug, Vg),
+    each (nsteps+1, nz) nondimensional (u_scale = 1).  Either full series or
+    compact profiles linearly interpolated to t = i dt per level."""
+    nz = int(cfg["nz"])
+    n = nsteps_of(cfg) + 1
+    dt = float(cfg["dt"])
+    with np.load(path, allow_pickle=False) as f:
+        if abs(float(f["dt_geo"]) - dt) > 1e-6:
+            raise ValueError(f"GeoWind file dt_geo={float(f['dt_geo']):.6f} s does not match "
+                             f"dt={dt:.6f} s")
+        if int(f["optGeoWind"]) != int(cfg.get("optGeoWind", 0)):
+            raise ValueError(f"GeoWind file optGeoWind={int(f['optGeoWind'])} does not match "
+                             f"optGeoWind={int(cfg.get('optGeoWind', 0))}")
+        if "Ug_series" in f and "Vg_series" in f:
+            Ug = np.asarray(f["Ug_series"], dtype=np.float64)
+            Vg = np.asarray(f["Vg_series"], dtype=np.float64)
+            if Ug.shape[0] != n:
+                raise ValueError(f"GeoWind series length {Ug.shape[0]} != nsteps+1={n}")
+            if Ug.shape[1] != nz:
+                raise ValueError(f"GeoWind nz={Ug.shape[1]} != nz={nz}")
+        else:
+            t = np.asarray(f["t_profile"], dtype=np.float64)
+            Up = np.asarray(f["Ug_profile"], dtype=np.float64)
+            Vp = np.asarray(f["Vg_profile"], dtype=np.float64)
+            if Up.shape[1] != nz:
+                raise ValueError(f"GeoWind nz={Up.shape[1]} != nz={nz}")
+            tt = np.arange(n) * dt
+            if tt[0] < t[0] - 1e-6 or tt[-1] > t[-1] + 1e-6:
+                raise ValueError("GeoWind compact profiles do not cover the simulation interval")
+            Ug = np.stack([np.interp(tt, t, Up[:, k]) for k in range(nz)], axis=1)
+            Vg = np.stack([np.interp(tt, t, Vp[:, k]) for k in range(nz)], axis=1)
+    return Ug, Vg
+
+
+def read_adv_forcing(path, cfg):
+    """Initialize_AdvForcing (Initialization.py:253-323), dry: -> (Uadv,
+    Vadv, THadv), each (nsteps+1, nz), times z_scale = l_x / (2 pi)
+    (u_scale = TH_scale = 1); a missing THadv series is zero."""
+    nz = int(cfg["nz"])
+    n = nsteps_of(cfg) + 1
+    dt = float(cfg["dt"])
+    zs = float(cfg["l_x"]) / (2 * math.pi)
+    with np.load(path, allow_pickle=False) as f:
+        if abs(float(f["dt_adv"]) - dt) > 1e-6:
+            raise ValueError(f"AdvForcing file dt_adv={float(f['dt_adv']):.6f} s does not match "
+                             f"dt={dt:.6f} s")
+        if int(f["optAdvection"]) != int(cfg.get("optAdvection", 0)):
+            raise ValueError(f"AdvForcing file optAdvection={int(f['optAdvection'])} does not "
+                             f"match optAdvection={int(cfg.get('optAdvection', 0))}")
+        U = np.asarray(f["Uadv_series"], dtype=np.float64)
+        V = np.asarray(f["Vadv_series"], dtype=np.float64)
+        if U.shape[0] != n:
+            raise ValueError(f"AdvForcing series length {U.shape[0]} != nsteps+1={n}")
+        if U.shape[1] != nz:
+            raise ValueError(f"AdvForcing nz={U.shape[1]} != nz={nz}")
+        TH = (np.asarray(f["THadv_series"], dtype=np.float64) if "THadv_series" in f
+              else np.zeros((n, nz)))
+    return U * zs, V * zs, TH * zs

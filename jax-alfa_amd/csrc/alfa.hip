@@ -266,7 +266,8 @@ int rhs_momentum(alfa_plan* P, hipStream_t s, const void* u, const void* v, cons
             (const T*)Cx, (const T*)Cy, (const T*)Cz, (const T*)b, (const T*)dtx, (const T*)dty,
             (const T*)dtz, (const T*)nullptr, (const T*)nullptr, pr, pr + c.nz, pr + 2 * c.nz,
             pr + 3 * c.nz, m3, m3 + c.nz, m3 + 2 * c.nz, P->d.f_coriolis_nondim, P->cfg.Ugal,
-            P->cfg.optDamping, (T*)Ru, (T*)Rv, (T*)Rw, (T*)nullptr);
+            P->cfg.optDamping, (T*)Ru, (T*)Rv, (T*)Rw, (T*)nullptr, (const double*)nullptr, 0.0,
+            0.0, 0, P->fadv());
   return 0;
 }
 
@@ -283,7 +284,7 @@ int rhs_scalar(alfa_plan* P, hipStream_t s, const void* TH, const void* adv, con
             (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
             (const T*)nullptr, (const T*)adv, (const T*)divq, pr, pr + c.nz, pr + 2 * c.nz,
             pr + 3 * c.nz, m, m, m, 0.0, 0.0, P->cfg.optDamping, (T*)nullptr, (T*)nullptr,
-            (T*)nullptr, (T*)RT);
+            (T*)nullptr, (T*)RT, (const double*)nullptr, 0.0, 0.0, 0, P->fadv());
   return 0;
 }
 
@@ -678,7 +679,7 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
   P->th_b = upload(hb);
   // profiles: Ug, Vg (Initialization.py:163-170), Rayleigh full / staggered (:395-460)
   {
-    std::vector<double> pr(4 * (size_t)nz, 0.0);
+    std::vector<double> pr(7 * (size_t)nz, 0.0);
     const double inv = 1.0 / d.RelaxTime_nondim;
     const double ztop = cfg->l_z / d.z_scale;
     const double thick = ztop - d.z_damping_nondim;
@@ -760,6 +761,18 @@ extern "C" int alfa_step_sfc_dev(alfa_plan* P, const alfa_state* st, int iterati
   if (!P || !st || !sfc_val) return set_error(ALFA_EINVAL, "alfa_step_sfc_dev: null argument");
   DISPATCH(P, "alfa_step_sfc_dev",
            step_t<T>(P, st, iteration, 0.0, STREAM(stream), sfc_val));
+}
+
+extern "C" int alfa_set_forcing(alfa_plan* P, const double* Ug, const double* Vg,
+                                const double* Uadv, const double* Vadv, const double* THadv,
+                                alfa_stream stream) {
+  if (!P) return set_error(ALFA_EINVAL, "alfa_set_forcing: null plan");
+  if (!Ug && !Vg && !Uadv && !Vadv && !THadv) return ALFA_OK;
+  hipStream_t s = STREAM(stream);
+  k_set_forcing<<<(P->nz + 255) / 256, 256, 0, s>>>(P->nz, Ug, Vg, Uadv, Vadv, THadv, P->prof);
+  launch_ok(P);
+  if (Uadv || Vadv || THadv) P->adv_on = true;
+  return check_hip(P, "alfa_set_forcing");
 }
 
 static_assert(ALFA_STATS_NPROF == STATS_NPROF && ALFA_STATS_NSCAL == STATS_NSCAL,

@@ -59,7 +59,9 @@ struct alfa_plan {
   Fft1D fx, fy, fmx, fmy;
   void *tw_x = nullptr, *tw_y = nullptr, *tw_mx = nullptr, *tw_my = nullptr;
   double *th_m = nullptr, *th_b = nullptr;    // (nx*nyc, nz+1)
-  double* prof = nullptr;                     // Ug, Vg, rf, rs: 4*nz
+  double* prof = nullptr;                     // Ug, Vg, rf, rs, Uadv, Vadv, THadv: 7*nz
+  bool adv_on = false;                        // advection forcing set (alfa_set_forcing)
+  const double* fadv() const { return adv_on ? prof + 4 * (size_t)nz : nullptr; }
   double* scal = nullptr;                     // 8 scalars
   Workspace ws;
   Prof pf;

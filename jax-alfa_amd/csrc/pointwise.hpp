@@ -549,13 +549,17 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
                       const double* __restrict__ vbar, const double* __restrict__ thbar,
                       double f, double Ugal, int damping, T* Ru, T* Rv, T* Rw, T* RT,
                       const double* __restrict__ bbar = nullptr, double T0 = 0.0,
-                      double g = 0.0, int bopt = 0) {
+                      double g = 0.0, int bopt = 0, const double* __restrict__ fadv = nullptr) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
     if (Ru) {
       T ru = -Cx[i] - dtx[i] - (T)f * ((T)Vg[z] - v[i]);
       T rv = -Cy[i] - dty[i] + (T)f * ((T)Ug[z] - (T)Ugal - u[i]);
+      if (fadv) {   // large-scale advection forcing (optAdvection >= 1)
+        ru += (T)fadv[z];
+        rv += (T)fadv[nz + z];
+      }
       T b;
       if (buoy) {
         b = buoy[i];
@@ -580,6 +584,7 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
     }
     if (RT) {
       T rt = -thadv[i] - divq[i];
+      if (fadv) rt += (T)fadv[2 * nz + z];
       if (damping) rt = (T)((double)rt - rs[z] * ((double)TH[i] - thbar[z]));
       RT[i] = rt;
     }
@@ -601,7 +606,7 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
                         const double* __restrict__ ubar, const double* __restrict__ vbar,
                         const double* __restrict__ thbar, double f, double Ugal, int damping,
                         T* Ru, T* Rv, T* Rw, T* RT, const double* __restrict__ bbar, double T0,
-                        double g, int bopt) {
+                        double g, int bopt, const double* __restrict__ fadv) {
   constexpr int W = Vec16<T>::W;
   const int nz = d.nz;
   const size_t nv = d.n / W;
@@ -620,6 +625,10 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
       const int z = z0 + j;
       T a = -cx.v[j] - tx.v[j] - (T)f * ((T)Vg[z] - vv.v[j]);
       T b = -cy.v[j] - ty.v[j] + (T)f * ((T)Ug[z] - (T)Ugal - uu.v[j]);
+      if (fadv) {
+        a += (T)fadv[z];
+        b += (T)fadv[nz + z];
+      }
       T bu;
       if (z == 0) {
         bu = T(0);
@@ -633,6 +642,7 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
       }
       T c = -cz.v[j] - tz.v[j] + bu;
       T e = -ta.v[j] - dq.v[j];
+      if (fadv) e += (T)fadv[2 * nz + z];
       if (damping) {
         const double sr = rs[z];
         a = (T)((double)a - sr * ((double)uu.v[j] - ubar[z]));
@@ -643,6 +653,20 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
       ru.v[j] = a; rv.v[j] = b; rw.v[j] = c; rt.v[j] = e;
     }
     vst(Ru + i0, ru); vst(Rv + i0, rv); vst(Rw + i0, rw); vst(RT + i0, rt);
+  }
+}
+
+// Per-iteration forcing profiles into the plan's profile table: Ug, Vg
+// (slots 0, 1) and Uadv, Vadv, THadv (slots 4, 5, 6); NULL keeps a slot.
+__global__ void k_set_forcing(int nz, const double* __restrict__ Ug, const double* __restrict__ Vg,
+                              const double* __restrict__ Ua, const double* __restrict__ Va,
+                              const double* __restrict__ Ta, double* prof) {
+  for (int z = blockIdx.x * blockDim.x + threadIdx.x; z < nz; z += gridDim.x * blockDim.x) {
+    if (Ug) prof[z] = Ug[z];
+    if (Vg) prof[nz + z] = Vg[z];
+    if (Ua) prof[4 * nz + z] = Ua[z];
+    if (Va) prof[5 * nz + z] = Va[z];
+    if (Ta) prof[6 * nz + z] = Ta[z];
   }
 }
 

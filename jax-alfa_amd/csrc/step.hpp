@@ -901,12 +901,13 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
       PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_rhs_v<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz,
                   dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
                   means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu,
-                  Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy);
+                  Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy, P->fadv());
     else
       PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, (const T*)nullptr,
                 dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
                 means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu,
-                Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy);
+                Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy,
+                P->fadv());
   }
   // 8. pressure (Main.py:481-500)
   T *dpdx = c.W("dpdx"), *dpdy = c.W("dpdy"), *dpdz = c.W("dpdz");

@@ -466,13 +466,20 @@ def DivFlux(g, qx, qy, qz):
             + Derivz_Generic_w(g, qz))
 
 
-def RHS_Momentum(g, u, v, w, Cx, Cy, Cz, buoyancy, divtx, divty, divtz):
-    """src/pde/NSE_AllTerms.py:48-104 (optAdvection=0)."""
-    Ug = g.Ug.reshape(1, 1, -1)
-    Vg = g.Vg.reshape(1, 1, -1)
+def RHS_Momentum(g, u, v, w, Cx, Cy, Cz, buoyancy, divtx, divty, divtz, Ug=None, Vg=None,
+                 Uadv=None, Vadv=None):
+    """src/pde/NSE_AllTerms.py:48-104.  Ug, Vg: the iteration's geostrophic
+    profiles (optGeoWind >= 1, Main.py:170-174; default the constant
+    Ug2/Vg2); Uadv, Vadv: the large-scale advection tendencies
+    (optAdvection >= 1, :86-88), None when off."""
+    Ug = (g.Ug if Ug is None else np.asarray(Ug)).reshape(1, 1, -1)
+    Vg = (g.Vg if Vg is None else np.asarray(Vg)).reshape(1, 1, -1)
     RHS_u = -Cx - divtx - g.f_coriolis_nondim * (Vg - v)
     RHS_v = -Cy - divty + g.f_coriolis_nondim * (Ug - g.Ugal - u)
     RHS_w = -Cz - divtz + buoyancy
+    if Uadv is not None:
+        RHS_u = RHS_u + np.asarray(Uadv).reshape(1, 1, -1)
+        RHS_v = RHS_v + np.asarray(Vadv).reshape(1, 1, -1)
     if g.optDamping == 1:
         rs = g.RayleighDampCoeff_stag.reshape(1, 1, -1)
         rf = g.RayleighDampCoeff.reshape(1, 1, -1)
@@ -482,9 +489,12 @@ def RHS_Momentum(g, u, v, w, Cx, Cy, Cz, buoyancy, divtx, divty, divtz):
     return RHS_u, RHS_v, RHS_w
 
 
-def RHS_Scalar(g, TH, adv, divq):
-    """src/pde/SCL_AllTerms.py:48-82 (optAdvection=0)."""
+def RHS_Scalar(g, TH, adv, divq, THadv=None):
+    """src/pde/SCL_AllTerms.py:48-82; THadv: the large-scale advection
+    tendency (optAdvection >= 1, :70-71), None when off."""
     R = -adv - divq
+    if THadv is not None:
+        R = R + np.asarray(THadv).reshape(1, 1, -1)
     if g.optDamping == 1:
         rs = g.RayleighDampCoeff_stag.reshape(1, 1, -1)
         R = R - rs * (TH - PlanarMean(TH).reshape(1, 1, -1))
@@ -1248,9 +1258,14 @@ def initial_state(g: Grid, u, v, w, TH) -> State:
                  iteration=g.istep)
 
 
-def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False) -> State:
-    """One iteration of the time loop, src/Main.py:158-540 (dry,
-    optAdvection=0, optGeoWind=0; SM or WL base models)."""
+def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None) -> State:
+    """One iteration of the time loop, src/Main.py:158-540 (dry; SM or WL
+    base models).  forcing: the iteration's profiles of Main.py:167-188,
+    dict with Ug, Vg (optGeoWind >= 1) and/or Uadv, Vadv, THadv
+    (optAdvection >= 1; all three when any is given), nondimensional."""
+    fc = forcing or {}
+    adv_on = any(k in fc for k in ("Uadv", "Vadv", "THadv"))
+    zprof = np.zeros(g.nz)
     if thomas is None:
         thomas = ThomasPressureInit(g)
     it = s.iteration
@@ -1303,8 +1318,11 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False) -> State:
     qx, qy, qz = SGS_scalar_fluxes(g, st, CsPr, dTHdx, dTHdy, dTHdz, qz2)
     divq = DivFlux(g, qx, qy, qz)
     # Main.py:464-473
-    RHS_u, RHS_v, RHS_w = RHS_Momentum(g, u, v, w, Cx, Cy, Cz, buoy, divtx, divty, divtz)
-    RHS_TH = RHS_Scalar(g, TH, THadv, divq)
+    RHS_u, RHS_v, RHS_w = RHS_Momentum(
+        g, u, v, w, Cx, Cy, Cz, buoy, divtx, divty, divtz, Ug=fc.get("Ug"), Vg=fc.get("Vg"),
+        Uadv=fc.get("Uadv", zprof) if adv_on else None,
+        Vadv=fc.get("Vadv", zprof) if adv_on else None)
+    RHS_TH = RHS_Scalar(g, TH, THadv, divq, THadv=fc.get("THadv", zprof) if adv_on else None)
     # Main.py:481-500
     RC_r, RC_i, fRz = PressureRC(g, u, v, w, RHS_u, RHS_v, RHS_w, Ru_p, Rv_p, Rw_p, divtz)
     if g.optPressureSolver == 1:

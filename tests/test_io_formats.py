@@ -63,3 +63,50 @@ def test_surface_bc_round_trip_and_validation(tmp_path):
     q = np.full(n + 1, 0.24)
     io.write_surface_bc(p, q, cfg["dt"], 1)
     np.testing.assert_array_equal(io.read_surface_bc(p, dict(cfg, optSurfBC=1)), q)
+
+
+def test_geowind_series_and_compact_profiles(tmp_path):
+    cfg, *_ = cases.initial_fields("convective", nx=8, ny=8, nz=6)
+    cfg = dict(cfg, SimTime=2.0, dt=0.5, optGeoWind=1)
+    n = io.nsteps_of(cfg) + 1
+    nz = cfg["nz"]
+    t = np.arange(n) * cfg["dt"]
+    z = np.arange(nz)
+    Ug = 5.0 + 0.1 * t[:, None] + 0.01 * z[None, :]
+    Vg = -1.0 + 0.2 * t[:, None] * np.ones((1, nz))
+    p = tmp_path / "GeoWind.npz"
+    np.savez(p, Ug_series=Ug, Vg_series=Vg, dt_geo=cfg["dt"], optGeoWind=np.int32(1))
+    a, b = io.read_geowind(p, cfg)
+    np.testing.assert_array_equal(a, Ug)
+    np.testing.assert_array_equal(b, Vg)
+    # compact form: profiles at t = 0, 1.5, 3 s, linear in time, so the
+    # interpolation onto t = i dt reproduces the linear law exactly
+    tp = np.array([0.0, 1.5, 3.0])
+    Up = 5.0 + 0.1 * tp[:, None] + 0.01 * z[None, :]
+    Vp = -1.0 + 0.2 * tp[:, None] * np.ones((1, nz))
+    np.savez(p, t_profile=tp, Ug_profile=Up, Vg_profile=Vp, dt_geo=cfg["dt"],
+             optGeoWind=np.int32(1))
+    a, b = io.read_geowind(p, cfg)
+    np.testing.assert_allclose(a, Ug, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(b, Vg, rtol=0, atol=1e-13)
+    with pytest.raises(ValueError):   # profiles ending before SimTime
+        np.savez(p, t_profile=tp[:2], Ug_profile=Up[:2], Vg_profile=Vp[:2], dt_geo=cfg["dt"],
+                 optGeoWind=np.int32(1))
+        io.read_geowind(p, cfg)
+
+
+def test_adv_forcing_scaling_and_defaults(tmp_path):
+    cfg, *_ = cases.initial_fields("convective", nx=8, ny=8, nz=6)
+    cfg = dict(cfg, SimTime=1.0, dt=0.5, optAdvection=1)
+    n, nz = io.nsteps_of(cfg) + 1, cfg["nz"]
+    U = np.full((n, nz), 1e-4)
+    V = np.full((n, nz), -2e-4)
+    p = tmp_path / "AdvForcing.npz"
+    np.savez(p, Uadv_series=U, Vadv_series=V, dt_adv=cfg["dt"], optAdvection=np.int32(1))
+    a, b, c = io.read_adv_forcing(p, cfg)
+    zs = cfg["l_x"] / (2 * np.pi)            # tendencies [m/s^2] times z_scale
+    np.testing.assert_allclose(a, 1e-4 * zs, rtol=1e-15)
+    np.testing.assert_allclose(b, -2e-4 * zs, rtol=1e-15)
+    assert c.shape == (n, nz) and not np.any(c)
+    with pytest.raises(ValueError):
+        io.read_adv_forcing(p, dict(cfg, optAdvection=0))

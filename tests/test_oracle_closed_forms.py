@@ -241,3 +241,26 @@ def test_wong_lilly_stresses_and_fluxes():
     assert np.allclose(qx[:, :, :-1], -c * d43 * gx[:, :, :-1])
     qz = O.ScalarFluxesW_interior_WL(g, np.full_like(X, 2.0), Cz)
     assert np.allclose(qz[:, :, 1:-1], -d43 * cw[:, :, 1:-1] * 2.0)
+
+
+def test_rhs_forcing_profiles_geostrophic_balance():
+    """Main.py:167-188 forcing: with u = Ug(z) - Ugal and v = Vg(z) the
+    Coriolis terms vanish for any (time-varying) geostrophic profile, so with
+    no other terms RHS_u, RHS_v, RHS_TH are exactly the large-scale advection
+    tendencies; without advection they are zero."""
+    g, *_ = grid(optDamping=0, Ugal=3.0)
+    rng = np.random.default_rng(5)
+    Ug, Vg = 4.0 + rng.uniform(-1, 1, g.nz), rng.uniform(-1, 1, g.nz)
+    Ua, Va, Ta = (rng.uniform(-1, 1, g.nz) for _ in range(3))
+    shape = (g.nx, g.ny, g.nz)
+    u = np.broadcast_to(Ug - g.Ugal, shape).copy()
+    v = np.broadcast_to(Vg, shape).copy()
+    z = np.zeros(shape)
+    Ru, Rv, Rw = O.RHS_Momentum(g, u, v, z, z, z, z, z, z, z, z, Ug=Ug, Vg=Vg, Uadv=Ua, Vadv=Va)
+    np.testing.assert_allclose(Ru, np.broadcast_to(Ua, shape), atol=1e-15)
+    np.testing.assert_allclose(Rv, np.broadcast_to(Va, shape), atol=1e-15)
+    assert not np.any(Rw)
+    Ru0, Rv0, _ = O.RHS_Momentum(g, u, v, z, z, z, z, z, z, z, z, Ug=Ug, Vg=Vg)
+    assert np.max(np.abs(Ru0)) < 1e-15 and np.max(np.abs(Rv0)) < 1e-15
+    RT = O.RHS_Scalar(g, z, z, z, THadv=Ta)
+    np.testing.assert_allclose(RT, np.broadcast_to(Ta, shape), atol=1e-15)
