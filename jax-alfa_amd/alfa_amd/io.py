@@ -150,3 +150,47 @@ def read_adv_forcing(path, cfg):
         TH = (np.asarray(f["THadv_series"], dtype=np.float64) if "THadv_series" in f
               else np.zeros((n, nz)))
     return U * zs, V * zs, TH * zs
+
+
+def save_3d_fields(outdir, iteration, les):
+    """Main.py:620-635: ALFA_3DFields_Iteration_<n>.npz with u + Ugal, v, w
+    and TH + T_0 (dry: no Q)."""
+    os.makedirs(outdir, exist_ok=True)
+    f = les.fields()
+    cfg = les.plan.cfg
+    path = os.path.join(outdir, f"ALFA_3DFields_Iteration_{iteration}.npz")
+    np.savez(path, u=f["u"] + float(cfg.get("Ugal", 0.0)), v=f["v"], w=f["w"],
+             TH=f["TH"] + float(cfg["T_0"]))
+    return path
+
+
+# carried state of one run: the fields and planes of alfa_state, the
+# coefficient profiles and scalars, the iteration counter and CFLmax
+def save_checkpoint(path, les):
+    """Exact-restart file of an LES: every carried array in its stored
+    precision (bitwise), the iteration and CFLmax."""
+    arrays = {f"t_{k}": v.cpu().numpy() for k, v in les.t.items()}
+    arrays.update({f"p_{k}": v.cpu().numpy() for k, v in les.prof.items()})
+    arrays["scalars"] = les.scalars.cpu().numpy()
+    np.savez(path, iteration=np.int64(les.iteration), CFLmax=np.float64(les.CFLmax),
+             precision=np.array(les.plan.precision), **arrays)
+
+
+def load_checkpoint(path, les):
+    """Restore save_checkpoint's file into an LES of the same configuration
+    and precision; the next step continues the saved run bit for bit."""
+    torch = __import__("torch")
+    with np.load(path, allow_pickle=False) as f:
+        if str(f["precision"]) != les.plan.precision:
+            raise ValueError(f"checkpoint precision {f['precision']} != {les.plan.precision}")
+        for k, v in les.t.items():
+            a = f[f"t_{k}"]
+            if a.shape != tuple(v.shape):
+                raise ValueError(f"checkpoint field {k} has shape {a.shape}, expected "
+                                 f"{tuple(v.shape)}")
+            v.copy_(torch.from_numpy(a))
+        for k, v in les.prof.items():
+            v.copy_(torch.from_numpy(f[f"p_{k}"]))
+        les.scalars.copy_(torch.from_numpy(f["scalars"]))
+        les.iteration = int(f["iteration"])
+        les.CFLmax = float(f["CFLmax"])

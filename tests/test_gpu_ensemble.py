@@ -66,3 +66,31 @@ def test_small_ensemble_runs():
     out = run_ensemble(specs, steps=6, nstreams=2)
     print(out)
     assert out["members"] == 4 and out["mpoint_steps_per_s"] > 0
+
+
+def test_checkpoint_restart_is_exact(tmp_path):
+    """alfa_amd.io checkpoint: 3 steps, save, restore into a fresh run, 3
+    more steps equals 6 uninterrupted steps bit for bit (dynamic and static
+    steps alternate: dynamicSGS_call_time = 2)."""
+    from alfa_amd import cases, io
+    cfg, u, v, w, TH = cases.initial_fields("stable", nx=32, ny=32, nz=32,
+                                            dynamicSGS_call_time=2)
+    ser = cases.surface_series(cfg, 10)
+    for prec in ("fp64", "mixed"):
+        ref = core.LES(core.Plan(cfg, prec), u, v, w, TH, sfc_series=ser)
+        ref.step(6)
+        a = core.LES(core.Plan(cfg, prec), u, v, w, TH, sfc_series=ser)
+        a.step(3)
+        p = tmp_path / f"ck_{prec}.npz"
+        io.save_checkpoint(p, a)
+        b = core.LES(core.Plan(cfg, prec), u * 0, v * 0, w * 0, TH * 0, sfc_series=ser)
+        io.load_checkpoint(p, b)
+        assert b.iteration == a.iteration
+        b.step(3)
+        torch.cuda.synchronize()
+        for k in ("u", "v", "w", "TH", "Cs2_3D", "Cs2PrRatio_3D"):
+            assert np.array_equal(b.host(k), ref.host(k)), (prec, k)
+        snap = io.save_3d_fields(str(tmp_path), 6, b)
+        with np.load(snap) as f:
+            assert sorted(f.keys()) == ["TH", "u", "v", "w"]
+            np.testing.assert_array_equal(f["u"], ref.host("u") + cfg.get("Ugal", 0.0))
