@@ -42,6 +42,7 @@ DEFAULTS = dict(
     optSgs=3, dynamicSGS_call_time=1, FGR=1, Cs2=0.1 ** 2, Cs2PrRatio=0.1 ** 2,
     Cwl=0.1 ** 2, CwlPrRatio=0.1 ** 2,
     optDamping=1, z_damping=700.0, RelaxTime=300.0, optPressureSolver=1,
+    optMoistureSurfBC=0, MoistureFlux=0.0, zMoisture=0.0, q_inversion=0.0,
     optAdvection=0, optGeoWind=0, optMoisture=0, istep=1,
 )
 
@@ -63,6 +64,8 @@ class Grid:
         self.f_coriolis_nondim = c["f_coriolis"] * self.z_scale  # :57
         self.T_0_nondim = c["T_0"]                               # :60
         self.inversion_nondim = c["inversion"] * self.z_scale    # :63
+        self.qm_sfc = c["MoistureFlux"] * np.ones((nx, ny))      # :72 (Q_scale = 1)
+        self.q_inversion_nondim = c["q_inversion"] * self.z_scale  # :75
         self.qz_sfc = c["SensibleHeatFlux"] * np.ones((nx, ny))  # :66
         self.dx = 2 * pi / nx                                    # :78
         self.dy = 2 * pi / ny                                    # :79
@@ -431,10 +434,13 @@ def ScalarAdvection(g, u, v, w, dsdx, dsdy, dsdz):
     return cc
 
 
-def Buoyancy(g, TH):
+def Buoyancy(g, TH, H=None):
     """src/pde/NSE_BuoyancyTerms.py:49-96 (optBuoyancy=0) and :104-150
-    (optBuoyancy=1), dry (H = 0)."""
+    (optBuoyancy=1): THv = (TH + T_0)(1 + 0.61 H), H = Q with moisture
+    (Main.py:323), dry H = 0."""
     THv = TH + g.T_0_nondim
+    if H is not None:
+        THv = THv * (1 + 0.61 * H)
     bar = PlanarMean(THv).reshape(1, 1, -1)
     if g.optBuoyancy == 0:
         nrm = (THv - bar) / g.T_0_nondim
@@ -499,6 +505,43 @@ def RHS_Scalar(g, TH, adv, divq, THadv=None):
         rs = g.RayleighDampCoeff_stag.reshape(1, 1, -1)
         R = R - rs * (TH - PlanarMean(TH).reshape(1, 1, -1))
     return R
+
+
+def RHS_Moisture(g, Q, adv, divqm, Qadv=None):
+    """src/pde/SCL_AllTerms.py:86-110: -adv - div qm + Qadv, then the sponge
+    on the fluctuation about the plane mean."""
+    R = -adv - divqm
+    R = R + (np.zeros(g.nz) if Qadv is None else np.asarray(Qadv)).reshape(1, 1, -1)
+    if g.optDamping == 1:
+        rs = g.RayleighDampCoeff_stag.reshape(1, 1, -1)
+        R = R - rs * (Q - PlanarMean(Q).reshape(1, 1, -1))
+    return R
+
+
+def AB2_Q(g, Q, R, Rp):
+    """src/pde/SCL_TimeAdvancement.py:79-99: AB2, top level = level nz-2 +
+    q_inversion dz."""
+    Qn = Q + g.dt_nondim * (1.5 * R - 0.5 * Rp)
+    Qn[:, :, -1] = Qn[:, :, -2] + g.q_inversion_nondim * g.dz
+    return Qn
+
+
+def SurfaceMoistureFlux(g, Q, ustar, MOST, qsfc_val=None):
+    """Main.py:253-266 with SurfaceFlux.py:451-504: the surface moisture
+    flux plane for optMoistureSurfBC 0 (constant MoistureFlux), 1 (series
+    flux) and 2 (prescribed surface Q, diagnosed through MOST with the heat
+    stability functions; homogeneous uses the plane mean of Q at level 0)."""
+    nx, ny = g.nx, g.ny
+    if g.optMoistureSurfBC == 0:
+        return g.qm_sfc.copy()
+    if g.optMoistureSurfBC == 1:
+        return qsfc_val * np.ones((nx, ny))
+    psi_h, psi_h0 = MOST[2], MOST[3]
+    zref = g.zMoisture if g.zMoisture > 0.0 else g.z0T
+    denom = np.log(0.5 * g.dz * g.z_scale / zref) + psi_h - psi_h0
+    if g.optSurfFlux == 0:
+        return ustar * VONK * (qsfc_val - np.mean(Q[:, :, 0])) * np.ones((nx, ny)) / denom
+    return ustar * VONK * (qsfc_val - Q[:, :, 0]) / denom
 
 
 def PressureRC(g, u, v, w, RHS_u, RHS_v, RHS_w, Ru_p, Rv_p, Rw_p, divtz):
@@ -1235,12 +1278,14 @@ class State:
     Cs2_3D: Optional[np.ndarray] = None
     Cs2PrRatio_3D: Optional[np.ndarray] = None
     MOST: tuple = ()
+    Q: Optional[np.ndarray] = None        # optMoisture >= 1 (Main.py:71-73)
+    RHS_Q: Optional[np.ndarray] = None
     iteration: int = 1
     CFLmax: float = 0.0
     diag: dict = field(default_factory=dict)
 
 
-def initial_state(g: Grid, u, v, w, TH) -> State:
+def initial_state(g: Grid, u, v, w, TH, Q=None) -> State:
     """Main.py:66-111 (state allocation; u is stored minus Ugal and TH minus
     T_0 by Initialization.py:58-100, the caller passes stored values)."""
     nx, ny, nz = g.nx, g.ny, g.nz
@@ -1255,14 +1300,20 @@ def initial_state(g: Grid, u, v, w, TH) -> State:
                  RHS_w=np.zeros((nx, ny, nz)), RHS_TH=np.zeros((nx, ny, nz)),
                  Cs2_3D=c0 * np.ones((nx, ny, nz)), Cs2PrRatio_3D=p0 * np.ones((nx, ny, nz)),
                  MOST=(z2.copy(), z2.copy(), z2.copy(), z2.copy(), 1.0 + z2, 1.0 + z2),
+                 Q=None if Q is None else np.asarray(Q, dtype=float).copy(),
+                 RHS_Q=None if Q is None else np.zeros((nx, ny, nz)),
                  iteration=g.istep)
 
 
-def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None) -> State:
-    """One iteration of the time loop, src/Main.py:158-540 (dry; SM or WL
-    base models).  forcing: the iteration's profiles of Main.py:167-188,
-    dict with Ug, Vg (optGeoWind >= 1) and/or Uadv, Vadv, THadv
-    (optAdvection >= 1; all three when any is given), nondimensional."""
+def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None,
+         qsfc_val=None) -> State:
+    """One iteration of the time loop, src/Main.py:158-540 (SM or WL base
+    models).  forcing: the iteration's profiles of Main.py:167-188, dict
+    with Ug, Vg (optGeoWind >= 1) and/or Uadv, Vadv, THadv (optAdvection >=
+    1; all three when any is given), nondimensional.  With optMoisture >= 1
+    the state carries Q and RHS_Q; qsfc_val is the iteration's moisture
+    surface value (MoistureSurfaceBC_series[iteration-1]) for
+    optMoistureSurfBC 1 and 2, and forcing may hold Qadv."""
     fc = forcing or {}
     adv_on = any(k in fc for k in ("Uadv", "Vadv", "THadv"))
     zprof = np.zeros(g.nz)
@@ -1276,17 +1327,27 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None)
     v, v_fft = Filtering_Explicit(g, FFT(s.v))
     w, w_fft = Filtering_Explicit(g, FFT(s.w))
     TH, _ = Filtering_Explicit(g, FFT(s.TH))
+    moist = g.optMoisture >= 1
+    if moist:
+        Q, _ = Filtering_Explicit(g, FFT(s.Q))               # Main.py:198-199
     # Main.py:213-243
     M, ustar, qz2, qz_avg, invOB, MOST = SurfaceFlux(g, u, v, TH, s.MOST, sfc_val)
+    if moist:                                                  # Main.py:253-266
+        qm2 = SurfaceMoistureFlux(g, Q, ustar, MOST, qsfc_val)
+        qm_avg = float(np.mean(qm2))
     # Main.py:275-290
     grads = velocityGradients(g, u, v, w, u_fft, v_fft, w_fft, ustar, M, MOST)
     dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz = grads
     dTHdx, dTHdy, dTHdz = potentialTemperatureGradients(g, TH, ustar, qz2, MOST)
+    if moist:   # moistureGradients (Derivatives.py:281-306): the TH rule with qm
+        dQdx, dQdy, dQdz = potentialTemperatureGradients(g, Q, ustar, qm2, MOST)
     # Main.py:301-311
     Cx, Cy, Cz = Advection(g, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy)
     THadv = ScalarAdvection(g, u, v, w, dTHdx, dTHdy, dTHdz)
+    if moist:
+        QAdv = ScalarAdvection(g, u, v, w, dQdx, dQdy, dQdz)   # Main.py:313-318
     # Main.py:322-327
-    buoy = Buoyancy(g, TH)
+    buoy = Buoyancy(g, TH, Q if moist else None)
     # Main.py:333-443 (SGS)
     dynamic = (1 <= g.optSgs <= 4) and (first or it % g.dynamicSGS_call_time == 0)
     diag = {}
@@ -1317,12 +1378,17 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None)
                                     st["txz"], st["tyz"])
     qx, qy, qz = SGS_scalar_fluxes(g, st, CsPr, dTHdx, dTHdy, dTHdz, qz2)
     divq = DivFlux(g, qx, qy, qz)
+    if moist:   # Main.py:356-370: the scalar model with Cs2PrRatio, qm at k = 0
+        qHx, qHy, qHz = SGS_scalar_fluxes(g, st, CsPr, dQdx, dQdy, dQdz, qm2)
+        divqm = DivFlux(g, qHx, qHy, qHz)
     # Main.py:464-473
     RHS_u, RHS_v, RHS_w = RHS_Momentum(
         g, u, v, w, Cx, Cy, Cz, buoy, divtx, divty, divtz, Ug=fc.get("Ug"), Vg=fc.get("Vg"),
         Uadv=fc.get("Uadv", zprof) if adv_on else None,
         Vadv=fc.get("Vadv", zprof) if adv_on else None)
     RHS_TH = RHS_Scalar(g, TH, THadv, divq, THadv=fc.get("THadv", zprof) if adv_on else None)
+    if moist:                                                    # Main.py:474-475
+        RHS_Q = RHS_Moisture(g, Q, QAdv, divqm, fc.get("Qadv"))
     # Main.py:481-500
     RC_r, RC_i, fRz = PressureRC(g, u, v, w, RHS_u, RHS_v, RHS_w, Ru_p, Rv_p, Rw_p, divtz)
     if g.optPressureSolver == 1:
@@ -1333,11 +1399,15 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None)
     RHS_v = RHS_v - dpdy
     RHS_w = RHS_w - dpdz
     # Main.py:506-512
+    RQ_p = s.RHS_Q
     if first:
         Ru_p, Rv_p, Rw_p, RT_p = RHS_u, RHS_v, RHS_w, RHS_TH
+        if moist:
+            RQ_p = RHS_Q
     # Main.py:518-526
     un, vn, wn = AB2_uvw(g, u, v, w, RHS_u, Ru_p, RHS_v, Rv_p, RHS_w, Rw_p)
     THn = AB2_TH(g, TH, RHS_TH, RT_p)
+    Qn = AB2_Q(g, Q, RHS_Q, RQ_p) if moist else None              # Main.py:528-529
     # Main.py:534-540
     CFL = max(np.max(np.abs(un)) * g.dt_nondim / g.dx,
               np.max(np.abs(vn)) * g.dt_nondim / g.dy,
@@ -1345,6 +1415,11 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None)
     diag.update(M_sfc_loc=M, ustar=ustar, qz_sfc=qz2, qz_sfc_avg=qz_avg, invOB=invOB,
                 CFL=CFL, txy=st["txy"], txz=st["txz"], tyz=st["tyz"], qz=qz,
                 dudz=dudz, dvdz=dvdz, dTHdz=dTHdz)
+    if moist:
+        diag.update(qm_sfc=qm2, qm_sfc_avg=qm_avg, dQdz=dQdz, qHz=qHz)
+        if keep:
+            diag.update(Q_f=Q, dQ=(dQdx, dQdy, dQdz), QAdv=QAdv, qHx=qHx, qHy=qHy,
+                        divqm=divqm, RHS_Q=RHS_Q)
     if keep:
         diag.update(u_f=u, v_f=v, w_f=w, TH_f=TH, grads=grads, dTH=(dTHdx, dTHdy, dTHdz),
                     Cx=Cx, Cy=Cy, Cz=Cz, THadv=THadv, buoyancy=buoy, stresses=st,
@@ -1353,6 +1428,7 @@ def step(g: Grid, s: State, thomas=None, sfc_val=None, keep=False, forcing=None)
                     RHS=(RHS_u, RHS_v, RHS_w, RHS_TH))
     return State(u=un, v=vn, w=wn, TH=THn, RHS_u=RHS_u, RHS_v=RHS_v, RHS_w=RHS_w,
                  RHS_TH=RHS_TH, Cs2_3D=Cs2_3D, Cs2PrRatio_3D=CsPr, MOST=MOST,
+                 Q=Qn, RHS_Q=RHS_Q if moist else None,
                  iteration=it + 1, CFLmax=max(s.CFLmax, CFL), diag=diag)
 
 
@@ -1383,11 +1459,12 @@ def InitializeStats(g: Grid) -> dict:
 
 def ComputeStats(g: Grid, acc: dict, u, v, w, TH, dudz, dvdz, dTHdz, M_sfc_loc, ustar,
                  qz_sfc_avg, txy, txz, tyz, qz, Cs2_1D_avg1, Cs2_1D_avg2, Cs2PrRatio_1D,
-                 beta1_1D, beta2_1D) -> dict:
-    """Statistics.py:47-330, ComputeStats with ResetFlag = 0 (dry: Q = 0),
-    u_scale = TH_scale = Q_scale = 1 (DerivedVars.py:49-69).  u, v, w, TH
-    are the state after the step's AB2; the other inputs are the step's
-    own intermediates (Main.py:550-557)."""
+                 beta1_1D, beta2_1D, Q=None, dQdz=None, qHz=None, qm_sfc_avg=0.0) -> dict:
+    """Statistics.py:47-330, ComputeStats with ResetFlag = 0, u_scale =
+    TH_scale = Q_scale = 1 (DerivedVars.py:49-69).  u, v, w, TH (and Q) are
+    the state after the step's AB2; the other inputs are the step's own
+    intermediates (Main.py:550-557).  Q = None: dry (the moisture keys add
+    zeros)."""
     Ugal = g.Ugal
     nz = g.nz
     out = dict(acc)
@@ -1432,6 +1509,19 @@ def ComputeStats(g: Grid, acc: dict, u, v, w, TH, dudz, dvdz, dTHdz, M_sfc_loc, 
     vsf = vs - mvs[None, None, :]
     wsf = w - mws[None, None, :]
     tsf = ts - mts[None, None, :]
+    # moisture (:175-176, :206-207, :235-264, :275, :291)
+    if Q is not None:
+        mQ = np.mean(Q, axis=(0, 1))
+        out["Q"] = acc["Q"] + mQ
+        out["dQdz"] = acc["dQdz"] + np.mean(dQdz, axis=(0, 1)) / g.z_scale
+        out["Q2"] = acc["Q2"] + np.mean((Q - mQ[None, None, :]) ** 2, axis=(0, 1))
+        qs = np.zeros_like(Q)
+        qs[:, :, 1:nz] = StagGridAvg(Q)
+        qs[:, :, 0] = Q[:, :, 0]
+        mqs = np.mean(qs, axis=(0, 1))
+        out["wQ"] = acc["wQ"] + np.mean(wsf * (qs - mqs[None, None, :]), axis=(0, 1))
+        out["qHz"] = acc["qHz"] + np.mean(qHz, axis=(0, 1))
+        out["qm_sfc"] = acc["qm_sfc"] + float(qm_sfc_avg)
     out["uw"] = acc["uw"] + np.mean(usf * wsf, axis=(0, 1))
     out["vw"] = acc["vw"] + np.mean(vsf * wsf, axis=(0, 1))
     out["wTH"] = acc["wTH"] + np.mean(wsf * tsf, axis=(0, 1))

@@ -264,3 +264,51 @@ def test_rhs_forcing_profiles_geostrophic_balance():
     assert np.max(np.abs(Ru0)) < 1e-15 and np.max(np.abs(Rv0)) < 1e-15
     RT = O.RHS_Scalar(g, z, z, z, THadv=Ta)
     np.testing.assert_allclose(RT, np.broadcast_to(Ta, shape), atol=1e-15)
+
+
+def test_moisture_terms_closed_forms():
+    """AB2_Q with its top rule, the virtual-temperature buoyancy and the
+    MOST moisture flux in the neutral limit (psi = 0: the log law)."""
+    g, X, Y, Zu, Zw = grid(q_inversion=2e-6, optBuoyancy=0, zMoisture=0.0)
+    rng = np.random.default_rng(4)
+    shape = X.shape
+    Q = 0.01 + 1e-3 * rng.standard_normal(shape)
+    R, Rp = 0.3 * np.ones(shape), -0.1 * np.ones(shape)
+    Qn = O.AB2_Q(g, Q, R, Rp)
+    inc = g.dt_nondim * (1.5 * 0.3 + 0.5 * 0.1)
+    np.testing.assert_allclose(Qn[:, :, :-1], Q[:, :, :-1] + inc, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(Qn[:, :, -1], Qn[:, :, -2] + g.q_inversion_nondim * g.dz,
+                               rtol=0, atol=1e-15)
+    # buoyancy: TH uniform, H = a cos x on every level -> THv' / T_0 = 0.61 a cos x
+    a = 2e-3
+    b = O.Buoyancy(g, np.zeros(shape), a * np.cos(X))
+    # (THv = 300 (1 + 0.61 H) carries ~3e-14 of rounding, times g_nondim ~3e3)
+    np.testing.assert_allclose(b[:, :, 1:], g.g_nondim * 0.61 * a * np.cos(X[:, :, 1:]),
+                               rtol=1e-12, atol=1e-10)
+    assert not np.any(b[:, :, 0])
+    # surface moisture flux, neutral (psi_h = psi_h0 = 0), prescribed Q_s
+    ustar = 0.3 + 0.1 * np.cos(X[:, :, 0])
+    z2 = np.zeros((g.nx, g.ny))
+    MOST = (z2, z2, z2, z2, 1 + z2, 1 + z2)
+    for het in (0, 1):
+        g2 = O.Grid(dict(g.cfg, optMoistureSurfBC=2, optSurfFlux=het))
+        qm = O.SurfaceMoistureFlux(g2, Q, ustar, MOST, 0.015)
+        q0 = Q[:, :, 0] if het else np.mean(Q[:, :, 0])
+        lg = np.log(0.5 * g.dz * g.z_scale / g.z0T)
+        np.testing.assert_allclose(qm, ustar * 0.4 * (0.015 - q0) / lg, rtol=1e-13)
+    g3 = O.Grid(dict(g.cfg, optMoistureSurfBC=0, MoistureFlux=1.5e-4))
+    assert np.all(O.SurfaceMoistureFlux(g3, Q, ustar, MOST) == 1.5e-4)
+
+
+def test_moisture_step_keeps_uniform_q():
+    """A uniform Q with no surface flux and no lapse stays uniform: every
+    moisture term (gradients, advection, SGS flux, sponge) vanishes."""
+    cfg, u, v, w, TH = cases.initial_fields("convective", nx=16, ny=16, nz=12)
+    cfg.update(optMoisture=1, optMoistureSurfBC=0, MoistureFlux=0.0, q_inversion=0.0)
+    g = O.Grid(cfg)
+    s = O.initial_state(g, u, v, w, TH, np.full(u.shape, 0.008))
+    th = O.ThomasPressureInit(g)
+    ser = cases.surface_series(cfg, 4)
+    for _ in range(2):
+        s = O.step(g, s, th, None if ser is None else ser[s.iteration - 1])
+    np.testing.assert_allclose(s.Q, 0.008, rtol=0, atol=1e-17)

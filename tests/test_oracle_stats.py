@@ -136,3 +136,25 @@ def test_accumulate_then_average():
     assert abs(avg["qz_sfc"] - 0.1) < 1e-15
     assert avg["Ugal"] == g.Ugal and np.all(avg["ZeRo1D"] == 0)
     assert set(O.STATS_MOIST_PROFILES) <= set(avg) and avg["qm_sfc"] == 0.0
+
+
+def test_moisture_statistics():
+    g, amp, cx, sy, c2x = setup()
+    u, v, w, TH = fields(g, amp, cx, sy, c2x)
+    c, d = amp["c"], amp["d"]
+    Q = 0.01 + 1e-3 * d * cx          # same mode as w's cos part
+    z = np.zeros_like(u)
+    k = np.arange(g.nz, dtype=float)
+    acc = O.ComputeStats(g, O.InitializeStats(g), u, v, w, TH, z, z, z,
+                         np.ones((g.nx, g.ny)), np.ones((g.nx, g.ny)), 0.0, z, z, z, z,
+                         None, None, None, None, None, Q=Q, dQdz=k + 0 * u, qHz=2 * k + 0 * u,
+                         qm_sfc_avg=3e-5)
+    a = 1e-3 * d
+    wn = a.copy()
+    wn[1:] = 0.5 * (a[:-1] + a[1:])
+    np.testing.assert_allclose(acc["Q"], 0.01, rtol=1e-13)
+    np.testing.assert_allclose(acc["Q2"], a * a / 2, rtol=1e-10)
+    np.testing.assert_allclose(acc["wQ"], wn * c / 2, rtol=1e-10)
+    np.testing.assert_allclose(acc["dQdz"], k / g.z_scale, rtol=1e-13)
+    np.testing.assert_allclose(acc["qHz"], 2 * k, rtol=1e-13)
+    assert acc["qm_sfc"] == 3e-5
