@@ -63,7 +63,10 @@ typedef struct alfa_config {
   int32_t optBuoyancy;          /* 0: /T_0, 1: /<theta_v>                   */
   int32_t optDamping;           /* 1: Rayleigh sponge                       */
   int32_t istep;                /* first iteration number                   */
-  int32_t reserved_i[7];
+  int32_t optMoisture;          /* 1: specific humidity Q is prognostic      */
+  int32_t optMoistureSurfBC;    /* 0 const MoistureFlux, 1 series flux,
+                                   2 series surface Q (ConfigLoader.py:102-124) */
+  int32_t reserved_i[5];
   double l_x, l_y, l_z;
   double dt;
   double Ugal;
@@ -77,7 +80,10 @@ typedef struct alfa_config {
                                    models pass Cwl and CwlPrRatio here
                                    (DerivedVars.py:101-106) */
   double z_damping, RelaxTime;
-  double reserved_d[8];
+  double MoistureFlux;          /* kg/kg m/s (optMoistureSurfBC 0)           */
+  double zMoisture;             /* reference height of the Q flux, 0: z0T    */
+  double q_inversion;           /* Q lapse above the top (kg/kg/m)           */
+  double reserved_d[5];
 } alfa_config;
 
 /* Derived constants the plan computed (DerivedVars.py:49-119,
@@ -103,10 +109,20 @@ typedef struct alfa_state {
   /* optional diagnostics written by the step when non-NULL: */
   double *Cs2_1D_avg1, *Cs2_1D_avg2, *beta1_1D;  /* (nz,) */
   double *Cs2PrRatio_1D, *beta2_1D;              /* (nz,) */
-  double *scalars;   /* [0] CFL, [1] qz_sfc_avg, [2] mean ustar, [3] M_sfc mean */
+  double *scalars;   /* [0] CFL, [1] qz_sfc_avg, [2] mean ustar, [3] M_sfc mean,
+                        [4] qm_sfc_avg (moisture) */
+  /* optMoisture >= 1 (Main.py:71-73, 164-165, 528-529): Q, RHS_Q_previous */
+  void *Q, *RHS_Q;                               /* (nx,ny,nz)          */
+  /* device pointer to MoistureSurfaceBC_series[iteration-1] (one double)
+   * for optMoistureSurfBC 1 (flux) and 2 (surface Q); read by the step */
+  const double* Q_sfc_val;
 } alfa_state;
 
 /* ---- plan --------------------------------------------------------------- */
+/* sizeof(alfa_config), sizeof(alfa_state), sizeof(alfa_derived) as the
+ * library was built, for bindings to check their struct layouts.  No
+ * counterpart in the reference. */
+int alfa_struct_sizes(size_t out[3]);
 int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan** out);
 int alfa_plan_destroy(alfa_plan* plan);
 int alfa_plan_derived(const alfa_plan* plan, alfa_derived* out);
@@ -125,31 +141,35 @@ int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_v
 /* ---- time-varying forcing (src/Main.py:167-188) --------------------------
  * Sets the geostrophic wind profiles Ug, Vg (optGeoWind >= 1:
  * GeoWind_U/V[iteration-1], Initialization.py:173-254) and the large-scale
- * advection tendencies Uadv, Vadv, THadv (optAdvection >= 1:
+ * advection tendencies Uadv, Vadv, THadv, Qadv (optAdvection >= 1:
  * AdvForcing_*[iteration-1], Initialization.py:253-323, added to
- * RHS_Momentum / RHS_Scalar, NSE_AllTerms.py:86-88, SCL_AllTerms.py:70-71)
+ * RHS_Momentum / RHS_Scalar / RHS_Moisture, NSE_AllTerms.py:86-88,
+ * SCL_AllTerms.py:70-71, 103)
  * used by the following steps and RHS ops of this plan.  Each argument is a
  * device pointer to nz nondimensional doubles, or NULL to keep the current
  * profile (Ug2/Vg2 constants and zero advection at plan creation).  Once an
  * advection profile is given the plan adds the three tendencies to every
  * RHS.  A stream-ordered copy kernel: no allocation, capturable. */
 int alfa_set_forcing(alfa_plan* plan, const double* Ug, const double* Vg, const double* Uadv,
-                     const double* Vadv, const double* THadv, alfa_stream stream);
+                     const double* Vadv, const double* THadv, const double* Qadv,
+                     alfa_stream stream);
 
 /* ---- statistics (src/utilities/Statistics.py:47-330; Main.py:545-617) ----
  * Adds one sample of ComputeStats (ResetFlag 0, dry) into acc: the planar
  * statistics of the state after alfa_step and of that step's intermediates
- * (dudz, dvdz, dTHdz, txy, txz, tyz, qz, M_sfc_loc, ustar, qz_sfc_avg),
+ * (dudz, dvdz, dTHdz, txy, txz, tyz, qz, M_sfc_loc, ustar, qz_sfc_avg; with
+ * moisture also Q, dQdz, qHz_q, qm_sfc_avg),
  * which the call reads from the plan workspace: call it after the
  * alfa_step of the same plan on the same stream, before the next step.
  * acc: ALFA_STATS_LEN(nz) doubles, zeroed by the caller to reset (the
  * ResetStats branch), laid out as the ALFA_STATS_NPROF profiles of nz in
  * the order U V W TH dUdz dVdz dTHdz u2 v2 w2 TH2 uv uw vw uTH vTH wTH txy
- * txz tyz qz Cs2_1 Cs2_2 Cs2PrRatio Beta1 Beta2, then M_sfc, ustar, qz_sfc.
+ * txz tyz qz Cs2_1 Cs2_2 Cs2PrRatio Beta1 Beta2 Q dQdz Q2 wQ qHz, then
+ * M_sfc, ustar, qz_sfc, qm_sfc (the moisture entries stay 0 in dry runs).
  * The coefficient profiles are taken from the state's optional pointers
  * (0 where NULL).  Averages are acc / samples (Main.py:589-595). */
-#define ALFA_STATS_NPROF 26
-#define ALFA_STATS_NSCAL 3
+#define ALFA_STATS_NPROF 31
+#define ALFA_STATS_NSCAL 4
 #define ALFA_STATS_LEN(nz) (ALFA_STATS_NPROF * (nz) + ALFA_STATS_NSCAL)
 int alfa_stats_accumulate(alfa_plan* plan, const alfa_state* st, double* acc,
                           alfa_stream stream);

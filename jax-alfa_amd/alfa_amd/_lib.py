@@ -18,15 +18,16 @@ ALFA_FP64 = 0
 ALFA_MIXED = 1
 
 _CFG_INT = ["nx", "ny", "nz", "optSgs", "FGR", "dynamicSGS_call_time", "optSurfFlux",
-            "optSurfBC", "optBuoyancy", "optDamping", "istep"]
+            "optSurfBC", "optBuoyancy", "optDamping", "istep", "optMoisture",
+            "optMoistureSurfBC"]
 _CFG_DBL = ["l_x", "l_y", "l_z", "dt", "Ugal", "z0m", "z0T", "zTemperature",
             "SensibleHeatFlux", "Ug2", "Vg2", "f_coriolis", "inversion", "T_0", "Cs2",
-            "Cs2PrRatio", "z_damping", "RelaxTime"]
+            "Cs2PrRatio", "z_damping", "RelaxTime", "MoistureFlux", "zMoisture", "q_inversion"]
 
 
 class AlfaConfig(C.Structure):
-    _fields_ = ([(n, C.c_int32) for n in _CFG_INT] + [("reserved_i", C.c_int32 * 7)]
-                + [(n, C.c_double) for n in _CFG_DBL] + [("reserved_d", C.c_double * 8)])
+    _fields_ = ([(n, C.c_int32) for n in _CFG_INT] + [("reserved_i", C.c_int32 * 5)]
+                + [(n, C.c_double) for n in _CFG_DBL] + [("reserved_d", C.c_double * 5)])
 
 
 class AlfaDerived(C.Structure):
@@ -49,8 +50,12 @@ STATE_FIELDS = ["u", "v", "w", "TH", "RHS_u", "RHS_v", "RHS_w", "RHS_TH", "Cs2_3
 STATE_DIAG = ["Cs2_1D_avg1", "Cs2_1D_avg2", "beta1_1D", "Cs2PrRatio_1D", "beta2_1D", "scalars"]
 
 
+STATE_MOIST = ["Q", "RHS_Q", "Q_sfc_val"]
+
+
 class AlfaState(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in STATE_FIELDS] + [(n, C.c_void_p) for n in STATE_DIAG]
+    _fields_ = ([(n, C.c_void_p) for n in STATE_FIELDS] + [(n, C.c_void_p) for n in STATE_DIAG]
+                + [(n, C.c_void_p) for n in STATE_MOIST])
 
 
 def config_struct(cfg: dict) -> AlfaConfig:
@@ -81,7 +86,8 @@ _SIGS = {
     "alfa_step": [_P, C.POINTER(AlfaState), C.c_int, C.c_double, _P],
     "alfa_step_sfc_dev": [_P, C.POINTER(AlfaState), C.c_int, _P, _P],
     "alfa_stats_accumulate": [_P, C.POINTER(AlfaState), _P, _P],
-    "alfa_set_forcing": [_P, _P, _P, _P, _P, _P, _P],
+    "alfa_set_forcing": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "alfa_struct_sizes": [_P],
     "alfa_fft": [_P, _P, _P, _P],
     "alfa_fft_pad": [_P, _P, _P, _P],
     "alfa_filtering_explicit": [_P, _P, _P, _P, _P],

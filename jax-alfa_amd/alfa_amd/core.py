@@ -131,9 +131,13 @@ class Plan:
 class LES:
     """Device-resident state of one run and its time loop (src/Main.py:156-540)."""
 
-    def __init__(self, plan: Plan, u, v, w, TH, sfc_series=None):
+    def __init__(self, plan: Plan, u, v, w, TH, sfc_series=None, Q=None, qsfc_series=None):
+        """Q, qsfc_series: the initial specific humidity and the
+        MoistureSurfaceBC_series (nondimensional, nsteps+1 values) of a
+        moisture plan (optMoisture >= 1; Initialization.py:330-386)."""
         self.plan = plan
         P = plan
+        self.moist = int(P.cfg.get("optMoisture", 0)) >= 1
         self.t = {n: P.field() for n in _lib.STATE_FIELDS[:10]}
         for n in _lib.STATE_FIELDS[10:]:
             self.t[n] = P.plane()
@@ -141,9 +145,23 @@ class LES:
         self.scalars = P.zeros((8,), dtype=_torch().float64)
         for n, a in zip(("u", "v", "w", "TH"), (u, v, w, TH)):
             self.t[n].copy_(P.asdev(a))
+        self.qsfc = None
+        if self.moist:
+            if Q is None:
+                raise ValueError("a moisture plan (optMoisture >= 1) needs the initial Q")
+            self.t["Q"] = P.asdev(Q).clone()
+            self.t["RHS_Q"] = P.field()
+            if int(P.cfg.get("optMoistureSurfBC", 0)) >= 1:
+                if qsfc_series is None:
+                    raise ValueError("optMoistureSurfBC >= 1 needs qsfc_series")
+                self.qsfc = _torch().tensor(np.asarray(qsfc_series, dtype=np.float64),
+                                            device=P.device)
         self.state = _lib.AlfaState()
         for n in _lib.STATE_FIELDS:
             setattr(self.state, n, self.t[n].data_ptr())
+        if self.moist:
+            self.state.Q = self.t["Q"].data_ptr()
+            self.state.RHS_Q = self.t["RHS_Q"].data_ptr()
         for n in _lib.STATE_DIAG[:5]:
             setattr(self.state, n, self.prof[n].data_ptr())
         self.state.scalars = self.scalars.data_ptr()
@@ -153,23 +171,24 @@ class LES:
         self.CFLmax = 0.0
         self.forcing = {}
 
-    def set_forcing_series(self, Ug=None, Vg=None, Uadv=None, Vadv=None, THadv=None):
+    def set_forcing_series(self, Ug=None, Vg=None, Uadv=None, Vadv=None, THadv=None, Qadv=None):
         """Time-varying forcing of Main.py:167-188: (nsteps+1, nz)
-        nondimensional series (GeoWind_U/V, AdvForcing_U/V/TH); row
+        nondimensional series (GeoWind_U/V, AdvForcing_U/V/TH/Q); row
         iteration-1 is applied before each step.  Kept on the device."""
         torch = _torch()
         P = self.plan
-        for k, a in (("Ug", Ug), ("Vg", Vg), ("Uadv", Uadv), ("Vadv", Vadv), ("THadv", THadv)):
+        for k, a in (("Ug", Ug), ("Vg", Vg), ("Uadv", Uadv), ("Vadv", Vadv), ("THadv", THadv),
+                     ("Qadv", Qadv)):
             if a is None:
                 continue
             a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
             if a.ndim != 2 or a.shape[1] != P.nz:
                 raise ValueError(f"{k} series must be (nsteps+1, nz={P.nz}), got {a.shape}")
             self.forcing[k] = torch.from_numpy(a).to(P.device)
-        if any(k in self.forcing for k in ("Uadv", "Vadv", "THadv")):
-            # optAdvection: the three tendencies together (missing ones zero)
+        if any(k in self.forcing for k in ("Uadv", "Vadv", "THadv", "Qadv")):
+            # optAdvection: the tendencies together (missing ones zero)
             n = next(iter(self.forcing.values())).shape[0]
-            for k in ("Uadv", "Vadv", "THadv"):
+            for k in ("Uadv", "Vadv", "THadv", "Qadv"):
                 self.forcing.setdefault(k, torch.zeros((n, P.nz), dtype=torch.float64,
                                                        device=P.device))
 
@@ -178,7 +197,7 @@ class LES:
             return
         row = self.iteration - 1
         ptr = []
-        for k in ("Ug", "Vg", "Uadv", "Vadv", "THadv"):
+        for k in ("Ug", "Vg", "Uadv", "Vadv", "THadv", "Qadv"):
             t = self.forcing.get(k)
             if t is not None and row >= t.shape[0]:
                 raise IndexError(f"{k} series has no row for iteration {self.iteration}")
@@ -193,12 +212,15 @@ class LES:
             if self.sfc_series is not None:
                 sv = float(self.sfc_series[self.iteration - 1])
             self._apply_forcing(st)
+            if self.qsfc is not None:   # MoistureSurfaceBC_series[iteration-1]
+                self.state.Q_sfc_val = self.qsfc[self.iteration - 1].data_ptr()
             P.call("alfa_step", C.byref(self.state), int(self.iteration), C.c_double(sv), st)
             self.iteration += 1
 
     def fields(self):
-        """Host copies (float64 numpy) of u, v, w, TH in stored form."""
-        return {k: self.t[k].double().cpu().numpy() for k in ("u", "v", "w", "TH")}
+        """Host copies (float64 numpy) of u, v, w, TH (and Q) in stored form."""
+        keys = ("u", "v", "w", "TH", "Q") if self.moist else ("u", "v", "w", "TH")
+        return {k: self.t[k].double().cpu().numpy() for k in keys}
 
     def host(self, name):
         return self.t[name].double().cpu().numpy()

@@ -6,9 +6,9 @@ The accumulator is one device vector of doubles (``alfa_stats_accumulate``
 adds a sample of the state after a step and of that step's intermediates);
 ``averaged()`` divides by the sample count as Main.py:589-595 does and
 returns the reference's dictionary keys, and ``save()`` writes
-``ALFA_Statistics_Iteration_<n>.npz`` with them.  Runs are dry, so the
-moisture keys (Q, dQdz, Q2, wQ, qHz, qm_sfc) are zero, as in the reference
-with optMoisture = 0.
+``ALFA_Statistics_Iteration_<n>.npz`` with them.  In dry runs the moisture
+keys (Q, dQdz, Q2, wQ, qHz, qm_sfc) are zero, as in the reference with
+optMoisture = 0.
 """
 from __future__ import annotations
 
@@ -21,8 +21,8 @@ from .core import LES, _torch, current_stream
 
 PROFILES = ("U", "V", "W", "TH", "dUdz", "dVdz", "dTHdz", "u2", "v2", "w2", "TH2", "uv", "uw",
             "vw", "uTH", "vTH", "wTH", "txy", "txz", "tyz", "qz", "Cs2_1", "Cs2_2",
-            "Cs2PrRatio", "Beta1", "Beta2")
-SCALARS = ("M_sfc", "ustar", "qz_sfc")
+            "Cs2PrRatio", "Beta1", "Beta2", "Q", "dQdz", "Q2", "wQ", "qHz")
+SCALARS = ("M_sfc", "ustar", "qz_sfc", "qm_sfc")
 # key order of the reference's dictionary (Statistics.py:319-330)
 KEY_ORDER = ("U", "V", "W", "TH", "dUdz", "dVdz", "dTHdz", "u2", "v2", "w2", "TH2", "uv", "uw",
              "vw", "uTH", "vTH", "wTH", "txy", "txz", "tyz", "qz", "M_sfc", "ustar", "qz_sfc",

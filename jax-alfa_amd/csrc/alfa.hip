@@ -497,6 +497,10 @@ template <class T> static void plan_dry_run(alfa_plan* P) {
   st.RHS_u = st.RHS_v = st.RHS_w = st.RHS_TH = f;
   st.Cs2_3D = st.Cs2PrRatio_3D = f;
   st.psi_m = st.psi_m0 = st.psi_h = st.psi_h0 = st.fi_m = st.fi_h = f;
+  if (P->cfg.optMoisture >= 1) {
+    st.Q = st.RHS_Q = f;
+    st.Q_sfc_val = (const double*)f;
+  }
   step_t<T>(P, &st, P->cfg.istep, 0.0, s);
   if (P->cfg.dynamicSGS_call_time > 1) step_t<T>(P, &st, P->cfg.istep + 1, 0.0, s);
   ops::fft<T>(P, s, f, f);
@@ -584,6 +588,10 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
                      "3 (LAD-SM), 4 (LAD-WL)",
                      cfg->optSgs);
   if (cfg->optSurfBC < 0 || cfg->optSurfBC > 2) return set_error(ALFA_EINVAL, "optSurfBC must be 0..2");
+  if (cfg->optMoisture < 0 || cfg->optMoisture > 1)
+    return set_error(ALFA_EINVAL, "optMoisture must be 0 or 1");
+  if (cfg->optMoistureSurfBC < 0 || cfg->optMoistureSurfBC > 2)
+    return set_error(ALFA_EINVAL, "optMoistureSurfBC must be 0..2");
   alfa_plan* P = new alfa_plan();
   P->cfg = *cfg;
   if (P->cfg.istep == 0) P->cfg.istep = 1;
@@ -679,7 +687,7 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
   P->th_b = upload(hb);
   // profiles: Ug, Vg (Initialization.py:163-170), Rayleigh full / staggered (:395-460)
   {
-    std::vector<double> pr(7 * (size_t)nz, 0.0);
+    std::vector<double> pr(8 * (size_t)nz, 0.0);
     const double inv = 1.0 / d.RelaxTime_nondim;
     const double ztop = cfg->l_z / d.z_scale;
     const double thick = ztop - d.z_damping_nondim;
@@ -747,31 +755,55 @@ extern "C" int alfa_state_init(alfa_plan* P, const alfa_state* st, alfa_stream s
     for (void* z : zero2) k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)z, T(0));
     k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)st->fi_m, T(1));
     k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)st->fi_h, T(1));
+    if (P->cfg.optMoisture >= 1 && st->RHS_Q)   // RHS_Q_previous = 0 (Main.py:73)
+      k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)st->RHS_Q, T(0));
     launch_ok(P);
   });
+}
+
+// the moisture fields and surface value a moisture plan needs
+static int check_moist_state(const alfa_plan* P, const alfa_state* st, const char* where) {
+  if (!P || P->cfg.optMoisture < 1) return ALFA_OK;
+  if (!st->Q || !st->RHS_Q)
+    return set_error(ALFA_EINVAL, "%s: optMoisture needs state Q and RHS_Q", where);
+  if (P->cfg.optMoistureSurfBC >= 1 && !st->Q_sfc_val)
+    return set_error(ALFA_EINVAL, "%s: optMoistureSurfBC=%d needs state Q_sfc_val", where,
+                     P->cfg.optMoistureSurfBC);
+  return ALFA_OK;
 }
 
 extern "C" int alfa_step(alfa_plan* P, const alfa_state* st, int iteration, double sfc_val,
                          alfa_stream stream) {
   if (!st) return set_error(ALFA_EINVAL, "alfa_step: null state");
+  if (int rc = check_moist_state(P, st, "alfa_step")) return rc;
   DISPATCH(P, "alfa_step", step_t<T>(P, st, iteration, sfc_val, STREAM(stream)));
 }
 extern "C" int alfa_step_sfc_dev(alfa_plan* P, const alfa_state* st, int iteration,
                                  const double* sfc_val, alfa_stream stream) {
   if (!P || !st || !sfc_val) return set_error(ALFA_EINVAL, "alfa_step_sfc_dev: null argument");
+  if (int rc = check_moist_state(P, st, "alfa_step_sfc_dev")) return rc;
   DISPATCH(P, "alfa_step_sfc_dev",
            step_t<T>(P, st, iteration, 0.0, STREAM(stream), sfc_val));
 }
 
+extern "C" int alfa_struct_sizes(size_t out[3]) {
+  if (!out) return set_error(ALFA_EINVAL, "alfa_struct_sizes: null argument");
+  out[0] = sizeof(alfa_config);
+  out[1] = sizeof(alfa_state);
+  out[2] = sizeof(alfa_derived);
+  return ALFA_OK;
+}
+
 extern "C" int alfa_set_forcing(alfa_plan* P, const double* Ug, const double* Vg,
                                 const double* Uadv, const double* Vadv, const double* THadv,
-                                alfa_stream stream) {
+                                const double* Qadv, alfa_stream stream) {
   if (!P) return set_error(ALFA_EINVAL, "alfa_set_forcing: null plan");
-  if (!Ug && !Vg && !Uadv && !Vadv && !THadv) return ALFA_OK;
+  if (!Ug && !Vg && !Uadv && !Vadv && !THadv && !Qadv) return ALFA_OK;
   hipStream_t s = STREAM(stream);
-  k_set_forcing<<<(P->nz + 255) / 256, 256, 0, s>>>(P->nz, Ug, Vg, Uadv, Vadv, THadv, P->prof);
+  k_set_forcing<<<(P->nz + 255) / 256, 256, 0, s>>>(P->nz, Ug, Vg, Uadv, Vadv, THadv, Qadv,
+                                                     P->prof);
   launch_ok(P);
-  if (Uadv || Vadv || THadv) P->adv_on = true;
+  if (Uadv || Vadv || THadv || Qadv) P->adv_on = true;
   return check_hip(P, "alfa_set_forcing");
 }
 

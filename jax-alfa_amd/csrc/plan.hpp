@@ -59,9 +59,11 @@ struct alfa_plan {
   Fft1D fx, fy, fmx, fmy;
   void *tw_x = nullptr, *tw_y = nullptr, *tw_mx = nullptr, *tw_my = nullptr;
   double *th_m = nullptr, *th_b = nullptr;    // (nx*nyc, nz+1)
-  double* prof = nullptr;                     // Ug, Vg, rf, rs, Uadv, Vadv, THadv: 7*nz
+  double* prof = nullptr;                     // Ug, Vg, rf, rs, Uadv, Vadv, THadv, Qadv: 8*nz
   bool adv_on = false;                        // advection forcing set (alfa_set_forcing)
   const double* fadv() const { return adv_on ? prof + 4 * (size_t)nz : nullptr; }
+  // for the scalar path of the RHS kernel on Q: its slot 2 is Qadv
+  const double* fadvQ() const { return adv_on ? prof + 5 * (size_t)nz : nullptr; }
   double* scal = nullptr;                     // 8 scalars
   Workspace ws;
   Prof pf;
@@ -114,7 +116,7 @@ inline size_t esize(const alfa_plan* P) { return P->prec == ALFA_FP64 ? 8 : 4; }
 // full-field passes (reads + writes) of each pointwise kernel of the step
 inline int pw_fields(const char* name) {
   static const struct { const char* n; int f; } tab[] = {
-      {"k_derivz", 8}, {"k_derivz_generic", 2}, {"k_stag_avg", 2}, {"k_advection", 12}, {"k_advection_all", 16}, {"k_sgs_fused", 25}, {"k_sgs_fused_v", 25}, {"k_derivz_v", 8}, {"k_rhs_v", 17},
+      {"k_derivz", 8}, {"k_derivz_generic", 2}, {"k_stag_avg", 2}, {"k_advection", 12}, {"k_advection_all", 16}, {"k_sgs_fused", 25}, {"k_sgs_fused_v", 25}, {"k_ab2_q", 5}, {"k_derivz_v", 8}, {"k_rhs_v", 17},
       {"k_scalar_adv", 7}, {"k_buoyancy", 2}, {"k_strains_uvp", 13}, {"k_strains_w", 12},
       {"k_smag", 7}, {"k_stress_uvp", 10}, {"k_stress_w", 6}, {"k_scalar_flux", 9},
       {"k_rhs", 17}, {"k_ab2_plain", 4}, {"k_imfilter", 2}, {"k_lasdd_lm", 24},

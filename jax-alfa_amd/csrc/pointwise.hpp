@@ -549,7 +549,8 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
                       const double* __restrict__ vbar, const double* __restrict__ thbar,
                       double f, double Ugal, int damping, T* Ru, T* Rv, T* Rw, T* RT,
                       const double* __restrict__ bbar = nullptr, double T0 = 0.0,
-                      double g = 0.0, int bopt = 0, const double* __restrict__ fadv = nullptr) {
+                      double g = 0.0, int bopt = 0, const double* __restrict__ fadv = nullptr,
+                      const T* __restrict__ Hq = nullptr) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
@@ -566,8 +567,12 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
       } else if (z == 0) {
         b = T(0);
       } else {
-        // k_buoyancy inline: bbar = plane mean of TH + T_0
-        const double t1 = (double)TH[i] + T0, t0 = (double)TH[i - 1] + T0;
+        // k_buoyancy inline: bbar = plane mean of THv = (TH + T_0)(1 + 0.61 Q)
+        double t1 = (double)TH[i] + T0, t0 = (double)TH[i - 1] + T0;
+        if (Hq) {
+          t1 = t1 * (1.0 + 0.61 * (double)Hq[i]);
+          t0 = t0 * (1.0 + 0.61 * (double)Hq[i - 1]);
+        }
         double n1, n0;
         if (bopt == 0) { n1 = (t1 - bbar[z]) / T0; n0 = (t0 - bbar[z - 1]) / T0; }
         else { n1 = (t1 - bbar[z]) / bbar[z]; n0 = (t0 - bbar[z - 1]) / bbar[z - 1]; }
@@ -606,7 +611,8 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
                         const double* __restrict__ ubar, const double* __restrict__ vbar,
                         const double* __restrict__ thbar, double f, double Ugal, int damping,
                         T* Ru, T* Rv, T* Rw, T* RT, const double* __restrict__ bbar, double T0,
-                        double g, int bopt, const double* __restrict__ fadv) {
+                        double g, int bopt, const double* __restrict__ fadv,
+                        const T* __restrict__ Hq) {
   constexpr int W = Vec16<T>::W;
   const int nz = d.nz;
   const size_t nv = d.n / W;
@@ -619,6 +625,12 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
     const Vec16<T> tx = vld(dtx + i0), ty = vld(dty + i0), tz = vld(dtz + i0);
     const Vec16<T> ta = vld(thadv + i0), dq = vld(divq + i0);
     const T thp = z0 > 0 ? TH[i0 - 1] : T(0);
+    Vec16<T> hq;
+    T hqp = T(0);
+    if (Hq) {
+      hq = vld(Hq + i0);
+      hqp = z0 > 0 ? Hq[i0 - 1] : T(0);
+    }
     Vec16<T> ru, rv, rw, rt;
 #pragma unroll
     for (int j = 0; j < W; ++j) {
@@ -633,8 +645,12 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
       if (z == 0) {
         bu = T(0);
       } else {
-        const double t1 = (double)th.v[j] + T0;
-        const double t0 = (double)(j > 0 ? th.v[j - 1] : thp) + T0;
+        double t1 = (double)th.v[j] + T0;
+        double t0 = (double)(j > 0 ? th.v[j - 1] : thp) + T0;
+        if (Hq) {
+          t1 = t1 * (1.0 + 0.61 * (double)hq.v[j]);
+          t0 = t0 * (1.0 + 0.61 * (double)(j > 0 ? hq.v[j - 1] : hqp));
+        }
         double n1, n0;
         if (bopt == 0) { n1 = (t1 - bbar[z]) / T0; n0 = (t0 - bbar[z - 1]) / T0; }
         else { n1 = (t1 - bbar[z]) / bbar[z]; n0 = (t0 - bbar[z - 1]) / bbar[z - 1]; }
@@ -657,16 +673,18 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
 }
 
 // Per-iteration forcing profiles into the plan's profile table: Ug, Vg
-// (slots 0, 1) and Uadv, Vadv, THadv (slots 4, 5, 6); NULL keeps a slot.
+// (slots 0, 1) and Uadv, Vadv, THadv, Qadv (slots 4-7); NULL keeps a slot.
 __global__ void k_set_forcing(int nz, const double* __restrict__ Ug, const double* __restrict__ Vg,
                               const double* __restrict__ Ua, const double* __restrict__ Va,
-                              const double* __restrict__ Ta, double* prof) {
+                              const double* __restrict__ Ta, const double* __restrict__ Qa,
+                              double* prof) {
   for (int z = blockIdx.x * blockDim.x + threadIdx.x; z < nz; z += gridDim.x * blockDim.x) {
     if (Ug) prof[z] = Ug[z];
     if (Vg) prof[nz + z] = Vg[z];
     if (Ua) prof[4 * nz + z] = Ua[z];
     if (Va) prof[5 * nz + z] = Va[z];
     if (Ta) prof[6 * nz + z] = Ta[z];
+    if (Qa) prof[7 * nz + z] = Qa[z];
   }
 }
 
@@ -687,6 +705,26 @@ __global__ void k_ab2_plain(PwDims d, T* F, const T* __restrict__ R, const T* __
     const T fn = F[i] + dt * (T(1.5) * R[i] - T(0.5) * Rp[i]);
     F[i] = fn;
     if (z == nz - 2) F[i + 1] = (kind == 2) ? fn + th_top : fn;
+  }
+}
+
+// AB2_Q (SCL_TimeAdvancement.py:79-99) in place, with the first-step rule
+// (RHS_Q_previous = RHS_Q, Main.py:506-512) and the rotation of RHS_Q
+// into the previous-step buffer (Main.py:164-165); level nz-1 becomes
+// level nz-2 plus q_top = q_inversion dz.
+template <class T>
+__global__ void k_ab2_q(PwDims d, T* Q, const T* __restrict__ R, T* Rp, T dt, T q_top,
+                        int first) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const T r = R[i];
+    const T rp = first ? r : Rp[i];
+    Rp[i] = r;
+    if (z == nz - 1) continue;
+    const T qn = Q[i] + dt * (T(1.5) * r - T(0.5) * rp);
+    Q[i] = qn;
+    if (z == nz - 2) Q[i + 1] = qn + q_top;
   }
 }
 

@@ -1,6 +1,6 @@
 // Planar statistics of one sample: ComputeStats with ResetFlag = 0
-// (src/utilities/Statistics.py:47-330), dry, u_scale = TH_scale = 1
-// (src/config/DerivedVars.py:49-50).
+// (src/utilities/Statistics.py:47-330), u_scale = TH_scale = Q_scale = 1
+// (src/config/DerivedVars.py:49-69); with moisture also the Q statistics.
 //
 // Two deterministic plane-sum passes in double over the state after the
 // step and the step's own intermediates (still in the plan workspace):
@@ -16,10 +16,10 @@
 
 namespace alfa {
 
-// accumulator layout: STATS_NPROF profiles of nz (order of STATS_KEYS) and
-// then STATS_NSCAL scalars
-constexpr int STATS_NPROF = 26;
-constexpr int STATS_NSCAL = 3;   // M_sfc, ustar, qz_sfc
+// accumulator layout: STATS_NPROF profiles of nz and then STATS_NSCAL
+// scalars (the order of the C header)
+constexpr int STATS_NPROF = 31;   // 26 dry profiles, then Q, dQdz, Q2, wQ, qHz
+constexpr int STATS_NSCAL = 4;    // M_sfc, ustar, qz_sfc, qm_sfc
 
 template <class T> struct FStatsMeans {
   static constexpr const char* NAME = "FStatsMeans";
@@ -73,6 +73,39 @@ template <class T> struct FStatsMoments {
   }
 };
 
+// moisture (Statistics.py:175-176, 206-207, 235-264, 275): level means of Q,
+// dQdz, qHz and the w-node Q; then Q2 and wQ about them
+template <class T> struct FStatsMeansQ {
+  static constexpr const char* NAME = "FStatsMeansQ";
+  static constexpr int NFIELDS = 3;
+  const T *Q, *dQdz, *qHz;
+  int nz;
+  __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    acc[0] += (double)Q[i];
+    acc[1] += (double)dQdz[i];
+    acc[2] += (double)qHz[i];
+    acc[3] += z == 0 ? (double)Q[i] : (double)(T(0.5) * (Q[i - 1] + Q[i]));
+  }
+};
+
+template <class T> struct FStatsMomentsQ {
+  static constexpr const char* NAME = "FStatsMomentsQ";
+  static constexpr int NFIELDS = 2;
+  const T *Q, *w;
+  const double* m;    // dry pass-1 means (w at row 2)
+  const double* mq;   // moisture means (4, nz)
+  int nz;
+  __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    const double qf = (double)Q[i] - mq[z];
+    const double wf = (double)w[i] - m[2 * nz + z];
+    const double qs = z == 0 ? (double)Q[i] : (double)(T(0.5) * (Q[i - 1] + Q[i]));
+    acc[0] += qf * qf;
+    acc[1] += wf * (qs - mq[3 * nz + z]);
+  }
+};
+
 // surface planes: M and ustar^2 (nz = 1 reduction over the columns)
 template <class T> struct FStatsSurface {
   static constexpr const char* NAME = "FStatsSurface";
@@ -94,6 +127,7 @@ struct StatsProfiles {
 __global__ void k_stats_add(int nz, const double* __restrict__ m, const double* __restrict__ mo,
                             const double* __restrict__ sfc, const double* __restrict__ scal,
                             StatsProfiles pr, double Ugal, double T0, double z_scale,
+                            const double* __restrict__ mq, const double* __restrict__ moq,
                             double* acc) {
   for (int z = threadIdx.x; z < nz; z += blockDim.x) {
     double* a = acc;
@@ -123,12 +157,20 @@ __global__ void k_stats_add(int nz, const double* __restrict__ m, const double* 
     a[23 * nz + z] += pr.cspr ? pr.cspr[z] : 0.0; // Cs2PrRatio
     a[24 * nz + z] += pr.b1 ? pr.b1[z] : 0.0;     // Beta1
     a[25 * nz + z] += pr.b2 ? pr.b2[z] : 0.0;     // Beta2
+    if (mq) {   // moisture (dry runs add nothing: the profiles stay 0)
+      a[26 * nz + z] += mq[z];                    // Q
+      a[27 * nz + z] += mq[nz + z] / z_scale;     // dQdz
+      a[28 * nz + z] += moq[z];                   // Q2
+      a[29 * nz + z] += moq[nz + z];              // wQ
+      a[30 * nz + z] += mq[2 * nz + z];           // qHz
+    }
   }
   if (threadIdx.x == 0) {
     double* s = acc + STATS_NPROF * nz;
     s[0] += sfc[0];             // M_sfc
     s[1] += sqrt(sfc[1]);       // ustar: sqrt of the mean of ustar^2
     s[2] += scal[1];            // qz_sfc (qz_sfc_avg of the step)
+    if (mq) s[3] += scal[4];    // qm_sfc (qm_sfc_avg of the step)
   }
 }
 
@@ -158,10 +200,20 @@ void stats_accumulate(alfa_plan* P, hipStream_t s, const alfa_state* st, double*
   f3.M = (const T*)P->ws.get("pl_M", PB);
   f3.ustar = (const T*)P->ws.get("pl_ustar", PB);
   plane_sums<2>(P, s, c.ncol, 1, f3, sfc, 1.0 / c.ncol, "stats3");
+  double *mq = nullptr, *moq = nullptr;
+  if (g.optMoisture >= 1) {
+    mq = c.D("st_meansq", 4 * (size_t)c.nz);
+    moq = c.D("st_momentsq", 2 * (size_t)c.nz);
+    FStatsMeansQ<T> fq{(const T*)st->Q, c.W("dQdz"), c.W("qHz"), c.nz};
+    plane_sums<4>(P, s, c.ncol, c.nz, fq, mq, 1.0 / c.ncol, "stats4");
+    FStatsMomentsQ<T> fq2{(const T*)st->Q, f1.w, m, mq, c.nz};
+    plane_sums<2>(P, s, c.ncol, c.nz, fq2, moq, 1.0 / c.ncol, "stats5");
+  }
   if (P->ws.dry) return;
   StatsProfiles pr{st->Cs2_1D_avg1, st->Cs2_1D_avg2, st->Cs2PrRatio_1D, st->beta1_1D,
                    st->beta2_1D};
-  k_stats_add<<<1, 256, 0, s>>>(c.nz, m, mo, sfc, P->scal, pr, g.Ugal, g.T_0, d.z_scale, acc);
+  k_stats_add<<<1, 256, 0, s>>>(c.nz, m, mo, sfc, P->scal, pr, g.Ugal, g.T_0, d.z_scale, mq,
+                                moq, acc);
   launch_ok(P);
 }
 

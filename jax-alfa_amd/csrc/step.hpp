@@ -702,6 +702,32 @@ void stage_fluxes(Ctx<T>& c, const SgsStrains<T>& ss, const T* const dT[3], cons
             (const T*)nullptr, 0.0, 0.4, 0.0, q[2], qsfc);
 }
 
+// u, v, TH, THv = (TH + T_0)(1 + 0.61 Q) and Q (moisture runs)
+template <class T> struct FSum5 {
+  static constexpr const char* NAME = "FSum5";
+  static constexpr int NFIELDS = 4;
+  const T* u; const T* v; const T* TH; const T* Q; int nz; double T0;
+  __device__ __forceinline__ void operator()(int c, int z, double* acc) const {
+    const size_t i = (size_t)c * nz + z;
+    const double th = TH[i], q = Q[i];
+    acc[0] += (double)u[i];
+    acc[1] += (double)v[i];
+    acc[2] += th;
+    acc[3] += (th + T0) * (1.0 + 0.61 * q);
+    acc[4] += q;
+  }
+};
+
+// level 0 of a field, and a plane (plane sums with nz = 1)
+template <class T> struct FLevel0 {
+  static constexpr const char* NAME = "FLevel0";
+  static constexpr int NFIELDS = 0;
+  const T* F; int nz;
+  __device__ __forceinline__ void operator()(int c, int, double* acc) const {
+    acc[0] += (double)F[(size_t)c * nz];
+  }
+};
+
 template <class T> struct FSum4 {
   static constexpr const char* NAME = "FSum4";
   static constexpr int NFIELDS = 3;
@@ -733,7 +759,7 @@ __global__ void k_cfl_final(const double* __restrict__ part, int nb, double cx, 
     const double cfl = fmax(fmax(sm[0][0] * cx, sm[1][0] * cy), sm[2][0] * cz);
     scal[0] = cfl;
     if (out)
-      for (int k = 0; k < 4; ++k) out[k] = scal[k];
+      for (int k = 0; k < 5; ++k) out[k] = scal[k];
   }
 }
 
@@ -763,39 +789,74 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   T *dudy = c.W("dudy"), *dvdy = c.W("dvdy"), *dwdy = c.W("dwdy");
   T *dudz = c.W("dudz"), *dvdz = c.W("dvdz"), *dwdz = c.W("dwdz");
   T *dTx = c.W("dTHdx"), *dTy = c.W("dTHdy"), *dTz = c.W("dTHdz");
+  // moisture (optMoisture >= 1): Q and its RHS_Q_previous in the state
+  const bool moist = g.optMoisture >= 1;
+  T* SQ = moist ? (T*)st->Q : nullptr;
+  T* SRQ = moist ? (T*)st->RHS_Q : nullptr;
+  T *dQx = nullptr, *dQy = nullptr, *dQz = nullptr;
+  if (moist) { dQx = c.W("dQdx"); dQy = c.W("dQdy"); dQz = c.W("dQdz"); }
 
-  // 1. explicit filter of u, v, w, TH and the spectral gradients
-  //    (Main.py:193-197, Derivatives.py:47-153)
+  // 1. explicit filter of u, v, w, TH (and Q) and the spectral gradients
+  //    (Main.py:193-199, Derivatives.py:47-153, 281-306)
   {
-    c2<T>* ef[4] = {c.S("ef0"), c.S("ef1"), c.S("ef2"), c.S("ef3")};
-    c2<T>* ex[4] = {c.S("ex0"), c.S("ex1"), c.S("ex2"), c.S("ex3")};
+    const int nf = moist ? 5 : 4;
+    c2<T>* ef[5];
+    c2<T>* ex[5];
+    for (int b = 0; b < nf; ++b) {
+      ef[b] = c.S("ef" + std::to_string(b));
+      ex[b] = c.S("ex" + std::to_string(b));
+    }
     LoadPtr<T> ld{};
-    ld.p[0] = S.u; ld.p[1] = S.v; ld.p[2] = S.w; ld.p[3] = S.TH; ld.ny = c.ny; ld.nz = c.nz;
-    y_r2c<T>(P, s, ld, c.nx, P->fy, P->tw_y, c.nz, d.cut0_y, 4, (void* const*)ef);
+    ld.p[0] = S.u; ld.p[1] = S.v; ld.p[2] = S.w; ld.p[3] = S.TH; ld.p[4] = SQ;
+    ld.ny = c.ny; ld.nz = c.nz;
+    y_r2c<T>(P, s, ld, c.nx, P->fy, P->tw_y, c.nz, d.cut0_y, nf, (void* const*)ef);
     XArgs xa{};
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < nf; ++b) {
       xa.in[b] = ef[b]; xa.dst[b][0] = ef[b]; xa.dst[b][1] = ex[b];
     }
     xa.op[0] = xop(d.cut0_y, 1, d.cut0_x, XM_NONE, XMAP_ID, c.nx);
     xa.op[1] = xop(d.cut0_y, 1, d.cut0_x, XM_IKX, XMAP_ID, c.nx);
     xa.nops = 2; xa.ncy_in = d.cut0_y; xa.nz = c.nz; xa.ny_full = c.ny;
-    x_pass<T>(P, s, xa, 4, P->fx, P->fx, P->tw_x, P->tw_x);
+    x_pass<T>(P, s, xa, nf, P->fx, P->fx, P->tw_x, P->tw_x);
     // the y derivatives are the filtered spectra times i*ky, applied in the
     // y C2R (i*ky commutes with the x transform)
     YC2RArgs ya{};
-    T* outs[12] = {S.u, S.v, S.w, S.TH, dudx, dvdx, dwdx, dTx, dudy, dvdy, dwdy, dTy};
-    for (int b = 0; b < 4; ++b) {
-      ya.in[b][0] = ef[b]; ya.in[4 + b][0] = ex[b]; ya.in[8 + b][0] = ef[b]; ya.dy[8 + b] = 1;
-    }
+    T* outs[15] = {S.u, S.v, S.w, S.TH, SQ, dudx, dvdx, dwdx, dTx, dQx,
+                   dudy, dvdy, dwdy, dTy, dQy};
+    T* o[15];
+    int no = 0;
+    for (int part = 0; part < 3; ++part)
+      for (int b = 0; b < nf; ++b) {
+        o[no] = outs[part * 5 + b];
+        ya.in[no][0] = part == 1 ? ex[b] : ef[b];
+        ya.dy[no] = part == 2 ? 1 : 0;
+        ++no;
+      }
     ya.nsum = 1; ya.ncy_in = d.cut0_y; ya.nz = c.nz; ya.scale = 1.0 / ((double)c.nx * c.ny);
     ya.dyscale = 1.0;
     StorePtr<T> so{};
-    for (int b = 0; b < 12; ++b) so.p[b] = outs[b];
+    for (int b = 0; b < no; ++b) so.p[b] = o[b];
     so.ny = c.ny; so.nz = c.nz;
-    y_c2r<T>(P, s, so, ya, 12, c.nx, P->fy, P->tw_y);
+    y_c2r<T>(P, s, so, ya, no, c.nx, P->fy, P->tw_y);
   }
-  // 2. surface fluxes (Main.py:213-243)
+  // 2. surface fluxes (Main.py:213-243) and the moisture flux (:253-266)
   stage_surface<T>(c, S, sf, sfc_val, sfc_dev);
+  T* qm = nullptr;
+  if (moist) {
+    qm = (T*)P->ws.get("pl_qm", PB);
+    double* qa0 = c.D("qm_avg0", 1);
+    if (g.optMoistureSurfBC == 2 && g.optSurfFlux == 0) {
+      FLevel0<T> f0{SQ, c.nz};
+      plane_sums<1>(P, s, c.ncol, 1, f0, qa0, 1.0 / c.ncol, "qm0");
+    }
+    const double zref = g.zMoisture > 0.0 ? g.zMoisture : g.z0T;
+    KLAUNCH(P, s, "k_surf_moist", 5.0 * PB, k_surf_moist<T>, pw_blocks(c.ncol), 256, c.ncol,
+            c.nz, g.optMoistureSurfBC, g.optSurfFlux != 0 ? 1 : 0, (const T*)SQ,
+            (const T*)sf.ustar, (const T*)S.psih, (const T*)S.psih0, (const double*)qa0,
+            st->Q_sfc_val, g.MoistureFlux, 0.4, std::log(0.5 * d.dz * d.z_scale / zref), qm);
+    FLevel0<T> fq{qm, 1};
+    plane_sums<1>(P, s, c.ncol, 1, fq, P->scal + 4, 1.0 / c.ncol, "qmavg");
+  }
   // 3. vertical gradients (Derivz_M, Derivz_TH)
   if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, dudz, dvdz, dwdz, dTz}))
     PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_derivz_v<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar,
@@ -805,17 +866,33 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim,
               S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz),
               (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
-  // plane means of u, v, TH, TH + T_0
-  double* means = c.D("means", 4 * c.nz);
-  {
+  if (moist) {   // Derivz_TH of Q with the moisture flux (Derivatives.py:304)
+    if (vec_ok<T>(c.nz, {SQ, dQz}))
+      PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_derivz_v<T>, c.dims(), S.u, S.v, S.w, SQ, sf.ustar,
+                  sf.M, S.fim, S.fih, qm, (T*)nullptr, (T*)nullptr, (T*)nullptr, dQz,
+                  (T)g.Ugal, (T)(1.0 / d.dz), (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
+    else
+      PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, SQ, sf.ustar, sf.M, S.fim,
+                S.fih, qm, (T*)nullptr, (T*)nullptr, (T*)nullptr, dQz, (T)g.Ugal,
+                (T)(1.0 / d.dz), (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
+  }
+  // plane means of u, v, TH, THv = (TH + T_0)(1 + 0.61 Q) (and Q)
+  double* means = c.D("means", (moist ? 5 : 4) * (size_t)c.nz);
+  if (moist) {
+    FSum5<T> f{S.u, S.v, S.TH, SQ, c.nz, g.T_0};
+    plane_sums<5>(P, s, c.ncol, c.nz, f, means, 1.0 / c.ncol, "means");
+  } else {
     FSum4<T> f{S.u, S.v, S.TH, c.nz, g.T_0};
     plane_sums<4>(P, s, c.ncol, c.nz, f, means, 1.0 / c.ncol, "means");
   }
   // 4. advection (Main.py:301-311)
   T *Cx = c.W("Cx"), *Cy = c.W("Cy"), *Cz = c.W("Cz"), *THadv = c.W("THadv");
+  T* QAdv = moist ? c.W("QAdv") : nullptr;
   if (!dealias) {
     PW_LAUNCH(P, s, c.F, k_advection_all<T>, c.dims(), S.u, S.v, S.w, dudy, dudz, dvdx, dvdz,
               dwdx, dwdy, dTx, dTy, dTz, Cx, Cy, Cz, THadv);
+    if (moist)   // ScalarAdvection of Q (Main.py:313-318)
+      PW_LAUNCH(P, s, c.F, k_scalar_adv<T>, c.dims(), S.u, S.v, S.w, dQx, dQy, dQz, QAdv);
   } else {
     const T* in9[9] = {S.u, S.v, S.w, dudy, dudz, dvdx, dvdz, dwdx, dwdy};
     T* p9[9];
@@ -836,6 +913,17 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     const T* pc1[1] = {pcs};
     T* o1[1] = {THadv};
     dealias2<T>(c, 1, pc1, o1);
+    if (moist) {
+      const T* inq[3] = {dQx, dQy, dQz};
+      T* pq[3] = {c.Pad("qadv0"), c.Pad("qadv1"), c.Pad("qadv2")};
+      dealias1<T>(c, 3, inq, pq);
+      T* pcq = c.Pad("cq");
+      PW_LAUNCH(P, s, (size_t)c.mx * c.my * c.nz, k_scalar_adv<T>, c.pdims(), p9[0], p9[1],
+                p9[2], pq[0], pq[1], pq[2], pcq);
+      const T* pq1[1] = {pcq};
+      T* oq[1] = {QAdv};
+      dealias2<T>(c, 1, pq1, oq);
+    }
   }
   // 5. buoyancy (Main.py:322-327) is formed inside the momentum RHS kernel
   // 6. SGS (Main.py:333-443)
@@ -843,7 +931,8 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
                        (first || (g.dynamicSGS_call_time > 0 && it % g.dynamicSGS_call_time == 0));
   const T* grads[9] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz};
   SgsStrains<T> ss{};
-  stage_strains<T>(c, grads, ss, dealias);
+  // the moisture fluxes need the w-node |S| of the unpadded grid as well
+  stage_strains<T>(c, grads, ss, dealias || moist);
   const double* shu = mixed ? means : nullptr;
   const double* shv = mixed ? means + c.nz : nullptr;
   const double* sht = mixed ? means + 2 * c.nz : nullptr;
@@ -892,6 +981,17 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   T *dtx = c.W("divtx"), *dty = c.W("divty"), *dtz = c.W("divtz"), *divq = c.W("divq");
   stage_div_stress<T>(c, txx, tyy, tzz, txy, txz, tyz, dtx, dty, dtz);
   stage_div_flux<T>(c, qx, qy, qzf, divq);
+  // moisture SGS flux: the scalar model with Cs2PrRatio and qm at level 0
+  // (Main.py:356-370, 441-452), and its divergence
+  T* divqm = nullptr;
+  if (moist) {
+    T *qHx = c.W("qHx"), *qHy = c.W("qHy"), *qHz = c.W("qHz");
+    const T* dQ[3] = {dQx, dQy, dQz};
+    T* q[3] = {qHx, qHy, qHz};
+    stage_fluxes<T>(c, ss, dQ, S.CsPr, qm, q);
+    divqm = c.W("divqm");
+    stage_div_flux<T>(c, qHx, qHy, qHz, divqm);
+  }
   // 7. right-hand sides (Main.py:464-473)
   T *Rnu = c.W("Rnu"), *Rnv = c.W("Rnv"), *Rnw = c.W("Rnw"), *RnT = c.W("RnT");
   {
@@ -901,13 +1001,28 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
       PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_rhs_v<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz,
                   dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
                   means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu,
-                  Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy, P->fadv());
+                  Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy, P->fadv(),
+                  (const T*)SQ);
     else
       PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), S.u, S.v, S.w, S.TH, Cx, Cy, Cz, (const T*)nullptr,
                 dtx, dty, dtz, THadv, divq, pr, pr + c.nz, pr + 2 * c.nz, pr + 3 * c.nz, means,
                 means + c.nz, means + 2 * c.nz, d.f_coriolis_nondim, g.Ugal, g.optDamping, Rnu,
                 Rnv, Rnw, RnT, means + 3 * c.nz, g.T_0, d.g_nondim, g.optBuoyancy,
-                P->fadv());
+                P->fadv(), (const T*)SQ);
+  }
+  // RHS_Moisture (SCL_AllTerms.py:86-110): the RT path of k_rhs on Q, its
+  // advection, div qm, the Qadv forcing and the sponge about the Q mean
+  T* RnQ = nullptr;
+  if (moist) {
+    RnQ = c.W("RnQ");
+    const double* pr = P->prof;
+    PW_LAUNCH(P, s, c.F, k_rhs<T>, c.dims(), (const T*)nullptr, (const T*)nullptr,
+              (const T*)nullptr, (const T*)SQ, (const T*)nullptr, (const T*)nullptr,
+              (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+              (const T*)nullptr, (const T*)QAdv, (const T*)divqm, pr, pr + c.nz, pr + 2 * c.nz,
+              pr + 3 * c.nz, means, means + c.nz, means + 4 * c.nz, 0.0, 0.0, g.optDamping,
+              (T*)nullptr, (T*)nullptr, (T*)nullptr, RnQ, (const double*)nullptr, 0.0, 0.0, 0,
+              P->fadvQ());
   }
   // 8. pressure (Main.py:481-500)
   T *dpdx = c.W("dpdx"), *dpdy = c.W("dpdy"), *dpdz = c.W("dpdz");
@@ -928,6 +1043,9 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     KLAUNCH(P, s, "k_ab2", 23.0 * c.FB, k_ab2<T>, AB2_BLOCKS, 256, c.dims(), S.u, S.v, S.w, S.TH,
           Rnu, Rnv, Rnw, RnT, S.Ru, S.Rv, S.Rw, S.RT, dpdx, dpdy, dpdz, (T)d.dt_nondim,
           (T)(d.inversion_nondim * d.dz), first ? 1 : 0, cflpart);
+  if (moist)   // AB2_Q (SCL_TimeAdvancement.py:79-99), RHS_Q rotation
+    PW_LAUNCH(P, s, c.F, k_ab2_q<T>, c.dims(), SQ, (const T*)RnQ, SRQ, (T)d.dt_nondim,
+              (T)(g.q_inversion * d.z_scale * d.dz), first ? 1 : 0);
   if (!P->ws.dry) {
     k_cfl_final<<<1, 256, 0, s>>>(cflpart, AB2_BLOCKS, d.dt_nondim / d.dx, d.dt_nondim / d.dy,
                                   d.dt_nondim / d.dz, P->scal, st->scalars);

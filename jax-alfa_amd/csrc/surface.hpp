@@ -163,3 +163,28 @@ __global__ void __launch_bounds__(SURF_T) k_surf_scal(int nb, int ncol, const do
     scal[3] = part3[2];
   }
 }
+
+// Surface moisture flux plane (Main.py:253-266 with SurfaceFlux.py:451-504):
+// mode 0 the constant MoistureFlux, 1 the series flux, 2 the flux
+// diagnosed from the prescribed surface Q through MOST with the heat
+// stability functions (homogeneous: against the plane mean of Q at level 0).
+template <class T>
+__global__ void k_surf_moist(int ncol, int nz, int mode, int hetero, const T* __restrict__ Q,
+                             const T* __restrict__ ustar, const T* __restrict__ psih,
+                             const T* __restrict__ psih0, const double* __restrict__ qavg0,
+                             const double* __restrict__ sval, double flux, double vonk,
+                             double logq, T* qm) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncol; c += gridDim.x * blockDim.x) {
+    double q;
+    if (mode == 0) {
+      q = flux;
+    } else if (mode == 1) {
+      q = *sval;
+    } else {
+      const double qa = hetero ? (double)Q[(size_t)c * nz] : *qavg0;
+      const double denom = logq + (double)psih[c] - (double)psih0[c];
+      q = (double)ustar[c] * vonk * (*sval - qa) / denom;
+    }
+    qm[c] = (T)q;
+  }
+}

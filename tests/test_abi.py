@@ -21,9 +21,16 @@ def test_every_binding_is_declared():
 
 
 def test_struct_layout_matches_header():
-    # alfa_config: 11 int32 + 7 reserved, 18 doubles + 8 reserved
+    # alfa_config: 13 int32 + 5 reserved, 21 doubles + 5 reserved; alfa_state:
+    # 16 fields + 6 diagnostics + Q, RHS_Q, Q_sfc_val
     assert C.sizeof(_lib.AlfaConfig) == 18 * 4 + 26 * 8
-    assert C.sizeof(_lib.AlfaState) == 22 * 8
+    assert C.sizeof(_lib.AlfaState) == 25 * 8
+    # and the library's own sizeof of each struct
+    lib = _lib.load()
+    sz = (C.c_size_t * 3)()
+    assert lib.alfa_struct_sizes(C.cast(sz, C.c_void_p)) == 0
+    assert list(sz) == [C.sizeof(_lib.AlfaConfig), C.sizeof(_lib.AlfaState),
+                        C.sizeof(_lib.AlfaDerived)]
 
 
 @pytest.mark.parametrize("over,frag", [
