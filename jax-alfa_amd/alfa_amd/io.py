@@ -194,3 +194,40 @@ def load_checkpoint(path, les):
         les.scalars.copy_(torch.from_numpy(f["scalars"]))
         les.iteration = int(f["iteration"])
         les.CFLmax = float(f["CFLmax"])
+
+
+def write_moisture(input_dir, Q):
+    """input/Q.npy: the flat Fortran-order specific humidity (kg/kg)."""
+    os.makedirs(input_dir, exist_ok=True)
+    np.save(os.path.join(input_dir, "Q.npy"), np.asarray(Q, dtype=np.float64).reshape(-1, order="F"))
+
+
+def read_moisture(input_dir, cfg):
+    """Initialize_Q (Initialization.py:330-343): Q.npy as (nx, ny, nz),
+    absolute kg/kg (no base state removed)."""
+    nx, ny, nz = int(cfg["nx"]), int(cfg["ny"]), int(cfg["nz"])
+    Q = np.load(os.path.join(input_dir, "Q.npy"), allow_pickle=False)
+    if Q.size != nx * ny * nz:
+        raise ValueError(f"Q.npy has {Q.size} values, expected {nx * ny * nz}")
+    return np.reshape(Q, (nx, ny, nz), order="F")
+
+
+def read_moisture_surface_bc(path, cfg):
+    """Initialize_MoistureSurfaceBC (Initialization.py:346-386): checks
+    dt_moist, optMoistureSurfBC and the length nsteps + 1; the series is
+    returned as stored (a flux for mode 1 with u_scale = Q_scale = 1, the
+    surface Q in kg/kg for mode 2)."""
+    with np.load(path, allow_pickle=False) as f:
+        dt_m = float(f["dt_moist"])
+        opt = int(f["optMoistureSurfBC"])
+        series = np.asarray(f["data_series"], dtype=np.float64)
+    if abs(dt_m - float(cfg["dt"])) > 1e-6:
+        raise ValueError(f"MoistureSurfaceBC file dt_moist={dt_m:.6f} s does not match "
+                         f"dt={float(cfg['dt']):.6f} s")
+    if opt != int(cfg.get("optMoistureSurfBC", 0)):
+        raise ValueError(f"MoistureSurfaceBC file optMoistureSurfBC={opt} does not match "
+                         f"optMoistureSurfBC={int(cfg.get('optMoistureSurfBC', 0))}")
+    n = nsteps_of(cfg) + 1
+    if series.size != n:
+        raise ValueError(f"MoistureSurfaceBC series length {series.size} != nsteps+1={n}")
+    return series.copy()

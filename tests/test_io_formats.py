@@ -110,3 +110,22 @@ def test_adv_forcing_scaling_and_defaults(tmp_path):
     assert c.shape == (n, nz) and not np.any(c)
     with pytest.raises(ValueError):
         io.read_adv_forcing(p, dict(cfg, optAdvection=0))
+
+
+def test_moisture_inputs(tmp_path):
+    cfg, u, *_ = cases.initial_fields("convective", nx=6, ny=4, nz=5)
+    cfg = dict(cfg, SimTime=1.0, dt=0.5, optMoistureSurfBC=2)
+    Q = 0.008 + 1e-4 * np.arange(u.size).reshape(u.shape)
+    io.write_moisture(tmp_path, Q)
+    q = np.load(tmp_path / "Q.npy")
+    nx, ny, nz = u.shape
+    i, j, k = 5, 3, 4
+    assert q[i + nx * (j + ny * k)] == Q[i, j, k]
+    np.testing.assert_array_equal(io.read_moisture(tmp_path, cfg), Q)
+    n = io.nsteps_of(cfg) + 1
+    p = tmp_path / "MoistureSurfaceBC.npz"
+    ser = np.linspace(0.009, 0.0085, n)
+    np.savez(p, data_series=ser, dt_moist=cfg["dt"], optMoistureSurfBC=np.int32(2))
+    np.testing.assert_array_equal(io.read_moisture_surface_bc(p, cfg), ser)
+    with pytest.raises(ValueError):
+        io.read_moisture_surface_bc(p, dict(cfg, optMoistureSurfBC=1))
