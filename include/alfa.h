@@ -138,6 +138,15 @@ int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);
 int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_val,
               alfa_stream stream);
 
+/* One step of each member of an ensemble, in table order on one stream:
+ * member m is (plans[m], states[m]) at iteration iterations[m] with surface
+ * value sfc_vals[m] (sfc_vals may be NULL: 0 for all).  The same result as
+ * n alfa_step calls; every member keeps its own plan (grid, precision,
+ * workspace).  Replaces the per-run time loop of src/Main.py:156-540 for a
+ * set of independent runs (the reference runs one per process). */
+int alfa_batch_step(alfa_plan* const* plans, const alfa_state* states, int n,
+                    const int* iterations, const double* sfc_vals, alfa_stream stream);
+
 /* ---- time-varying forcing (src/Main.py:167-188) --------------------------
  * Sets the geostrophic wind profiles Ug, Vg (optGeoWind >= 1:
  * GeoWind_U/V[iteration-1], Initialization.py:173-254) and the large-scale

@@ -88,6 +88,7 @@ _SIGS = {
     "alfa_stats_accumulate": [_P, C.POINTER(AlfaState), _P, _P],
     "alfa_set_forcing": [_P, _P, _P, _P, _P, _P, _P, _P],
     "alfa_struct_sizes": [_P],
+    "alfa_batch_step": [_P, _P, C.c_int, _P, _P, _P],
     "alfa_fft": [_P, _P, _P, _P],
     "alfa_fft_pad": [_P, _P, _P, _P],
     "alfa_filtering_explicit": [_P, _P, _P, _P, _P],

@@ -815,6 +815,25 @@ extern "C" int alfa_stats_accumulate(alfa_plan* P, const alfa_state* st, double*
   DISPATCH(P, "alfa_stats_accumulate", stats_accumulate<T>(P, STREAM(stream), st, acc));
 }
 
+extern "C" int alfa_batch_step(alfa_plan* const* plans, const alfa_state* states, int n,
+                               const int* iterations, const double* sfc_vals,
+                               alfa_stream stream) {
+  if (n < 0 || (n > 0 && (!plans || !states || !iterations)))
+    return set_error(ALFA_EINVAL, "alfa_batch_step: null member table");
+  for (int m = 0; m < n; ++m) {
+    alfa_plan* P = plans[m];
+    if (!P) return set_error(ALFA_EINVAL, "alfa_batch_step: member %d has no plan", m);
+    if (int rc = check_moist_state(P, &states[m], "alfa_batch_step")) return rc;
+    const double sv = sfc_vals ? sfc_vals[m] : 0.0;
+    int rc = ALFA_OK;
+    if (P->prec == ALFA_FP64) step_t<double>(P, &states[m], iterations[m], sv, STREAM(stream));
+    else step_t<float>(P, &states[m], iterations[m], sv, STREAM(stream));
+    rc = check_hip(P, "alfa_batch_step");
+    if (rc) return rc;
+  }
+  return ALFA_OK;
+}
+
 extern "C" int alfa_fft(alfa_plan* P, const void* F, void* out, alfa_stream s) {
   DISPATCH(P, "alfa_fft", ops::fft<T>(P, STREAM(s), F, out));
 }

@@ -94,3 +94,39 @@ def test_checkpoint_restart_is_exact(tmp_path):
         with np.load(snap) as f:
             assert sorted(f.keys()) == ["TH", "u", "v", "w"]
             np.testing.assert_array_equal(f["u"], ref.host("u") + cfg.get("Ugal", 0.0))
+
+
+def test_batch_step_equals_member_steps():
+    """alfa_batch_step over a member table (different grids, precisions and
+    cases) gives bitwise the fields of stepping each member alone."""
+    import ctypes as C
+    from alfa_amd import _lib, cases
+    specs = [("stable", dict(nx=32, ny=32, nz=32), "mixed"),
+             ("convective", dict(nx=48, ny=48, nz=32), "fp64"),
+             ("neutral", dict(), "mixed")]
+    batch, alone = [], []
+    for name, over, prec in specs:
+        cfg, u, v, w, TH = cases.initial_fields(name, **over)
+        ser = cases.surface_series(cfg, 8)
+        batch.append(core.LES(core.Plan(cfg, prec), u, v, w, TH, sfc_series=ser))
+        alone.append(core.LES(core.Plan(cfg, prec), u, v, w, TH, sfc_series=ser))
+    lib = batch[0].plan.lib
+    n = len(batch)
+    plans = (C.c_void_p * n)(*[m.plan.handle.value for m in batch])
+    states = (_lib.AlfaState * n)(*[m.state for m in batch])
+    for _ in range(4):
+        its = (C.c_int * n)(*[m.iteration for m in batch])
+        sv = (C.c_double * n)(*[0.0 if m.sfc_series is None else float(m.sfc_series[m.iteration - 1])
+                                for m in batch])
+        rc = lib.alfa_batch_step(C.cast(plans, C.c_void_p), C.cast(states, C.c_void_p), n,
+                                 C.cast(its, C.c_void_p), C.cast(sv, C.c_void_p),
+                                 core.current_stream())
+        assert rc == 0, lib.alfa_last_error().decode()
+        for m in batch:
+            m.iteration += 1
+        for m in alone:
+            m.step(1)
+    torch.cuda.synchronize()
+    for a, b in zip(batch, alone):
+        for k in ("u", "v", "w", "TH"):
+            assert np.array_equal(a.host(k), b.host(k)), k
