@@ -31,6 +31,10 @@ CASES = {
     # transform lengths with compile-time plans (64, 96, 128, 192, 256; the
     # dealiased FGR=1 grid pads 64 to 96), whose y C2R reads its first radix
     # pass from the spectra and writes its last one to the fields directly
+    # optPressureSolver = 0 (the reference's dense LU solve per wavenumber,
+    # NSE_PressureTerms.py:220-335): the oracle solves it with LU; the GPU
+    # solves the same tridiagonal system with the Thomas kernel
+    "stable_lu": ("stable", dict(nx=32, ny=32, nz=32, optPressureSolver=0)),
     "convective64x96": ("convective", dict(nx=64, ny=96, nz=16)),
     "convective128x192": ("convective", dict(nx=128, ny=192, nz=16)),
     "neutral64": ("neutral", dict(nx=64, ny=64, nz=16)),
