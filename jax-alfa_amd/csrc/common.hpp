@@ -54,6 +54,10 @@ constexpr int FFT_MAXRAD = 8;
 #ifndef ALFA_X_BS
 #define ALFA_X_BS 256
 #endif
+//   ALFA_BAND: band-limited passes for filtered spectra (1) or full passes (0)
+#ifndef ALFA_BAND
+#define ALFA_BAND 1
+#endif
 template <class T> struct Lines { static constexpr int L = ALFA_LPB * 8 / (int)sizeof(c2<T>); };
 
 struct Fft1D {

@@ -281,6 +281,124 @@ __host__ __device__ constexpr int last_radix(int m) {
   return m;
 }
 
+// ----------------------------------------------------------------------------
+// Band-limited passes.  A test- or explicit-filtered spectrum is nonzero only
+// at |k| < c, and the filters keep only those modes of a forward transform.
+// Where c <= N/8 (N/8 = the stride of the radix-8 first pass of a
+// compile-time plan), the first inverse pass has two nonzero inputs per
+// butterfly (k = j and k = j - N/8), and where c <= N/RL the last forward
+// pass (radix RL, stride N/RL) is needed only at outputs r = 0 and r = RL-1
+// of each butterfly.  These helpers compute just those.
+// ----------------------------------------------------------------------------
+
+// Radix-8 butterfly whose only nonzero inputs are v[0] = a and v[7] = b:
+// y_m = a + b exp(SIGN 2 pi i 7m/8) = a + b exp(-SIGN 2 pi i m/8).
+template <class T, int SIGN>
+__device__ __forceinline__ void band_r8(c2<T> a, c2<T> b, c2<T>* y) {
+  const T h = T(0.70710678118654752440);   // sqrt(2)/2
+  const T s = T(SIGN);
+  const c2<T> p = mkc<T>(h * (b.x + s * b.y), h * (b.y - s * b.x));   // b exp(-SIGN i pi/4)
+  const c2<T> q = mkc<T>(h * (b.x - s * b.y), h * (b.y + s * b.x));   // b exp(+SIGN i pi/4)
+  const c2<T> r = mkc<T>(s * b.y, -s * b.x);                          // b exp(-SIGN i pi/2)
+  y[0] = cadd<T>(a, b);
+  y[1] = cadd<T>(a, p);
+  y[2] = cadd<T>(a, r);
+  y[3] = csub<T>(a, q);
+  y[4] = csub<T>(a, b);
+  y[5] = csub<T>(a, p);
+  y[6] = csub<T>(a, r);
+  y[7] = cadd<T>(a, q);
+}
+
+// Output ROUT of butterfly jj of the last pass of the compile-time forward
+// plan of length N (radix RL, stride N/RL, twiddle step 1), read from LDS:
+// the value fft_pass_ct writes to element jj + ROUT*N/RL, by the same
+// arithmetic.
+template <class T, int N, int L, int ROUT>
+__device__ __forceinline__ c2<T> fwd_last_out(const c2<T>* buf, const c2<T>* tw, int jj, int p) {
+  constexpr int RL = last_radix(N), NSL = N / RL;
+  c2<T> v[RL];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    c2<T> a = buf[(jj + r * NSL) * L + p];
+    if (r > 0) a = cmul<T>(a, tw[jj * r]);
+    v[r] = a;
+  }
+  Dft<T, RL>::run(v, -1);
+  return v[ROUT];
+}
+
+// Last pass of the compile-time forward plan of length N over L lines, with
+// only the outputs k < ncy and N - k (0 < k < ncy) written (ncy <= N/RL):
+// output 0 of butterflies j < ncy and output RL-1 of butterflies j > N/RL -
+// ncy.  Ends with a barrier, like a full pass.
+template <class T, int L, int N, int BS>
+__device__ __forceinline__ void fwd_last_pass_band(c2<T>* buf, const c2<T>* tw, int ncy) {
+  constexpr int RL = last_radix(N), NSL = N / RL, TOT = NSL * L, IT = (TOT + BS - 1) / BS;
+  c2<T> lo[IT], hi[IT];
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L;
+      c2<T> v[RL];
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        c2<T> a = buf[(j + r * NSL) * L + p];
+        if (r > 0) a = cmul<T>(a, tw[j * r]);
+        v[r] = a;
+      }
+      Dft<T, RL>::run(v, -1);
+      lo[q] = v[0];
+      hi[q] = v[RL - 1];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L;
+      if (j < ncy) buf[j * L + p] = lo[q];
+      if (j > NSL - ncy) buf[(j + (RL - 1) * NSL) * L + p] = hi[q];
+    }
+  }
+  __syncthreads();
+}
+
+// First pass (radix 8, stride 1) of the inverse transform of L lines whose
+// modes are zero outside k < kmax and k > N - kmax (kmax <= N/8); only
+// those elements of buf are read.  Ends with a barrier, like a full pass.
+template <class T, int L, int N, int BS>
+__device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
+  constexpr int NR1 = N / 8, TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
+  c2<T> lo[IT], hi[IT];
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    lo[q] = mkc<T>(0, 0);
+    hi[q] = mkc<T>(0, 0);
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L;
+      if (j < kmax) lo[q] = buf[j * L + p];
+      if (j >= NR1 - kmax + 1) hi[q] = buf[(j + 7 * NR1) * L + p];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L;
+      c2<T> y[8];
+      band_r8<T, +1>(lo[q], hi[q], y);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
 // transform of LPB lines by BS threads: compile-time length NCT, or f.n when
 // NCT == 0 (the generic passes run with FFT_BS threads)
 template <class T, int LPB, int MAXN, int NCT, int BS = FFT_BS>
@@ -366,9 +484,11 @@ struct TileGrid {
 // ncy_out columns of its half spectrum, (rows, ncy_out, nz) complex.  With
 // vec set (nz a multiple of the vector width, 16-byte aligned buffers) every
 // thread loads 16 bytes of z at a time through ld.vload and stores the two
-// complex values of one packed line together.
+// complex values of one packed line together.  BAND (compile-time lengths
+// with ncy_out <= N/RL, chosen by the launcher): the last radix pass is
+// evaluated only where the store reads it.
 // ----------------------------------------------------------------------------
-template <class T, class Loader, int MAXN, int NCT, int BS>
+template <class T, class Loader, int MAXN, int NCT, int BS, bool BAND = false>
 __global__ void __launch_bounds__(BS)
 k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
         TileGrid tg, int vec) {
@@ -398,7 +518,14 @@ k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, 
     }
     tws.store(stw);
     __syncthreads();
-    fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, -1);
+    // band (compile-time lengths, ncy_out <= N/RL): the last pass is
+    // evaluated only at the outputs k < ncy_out and n - k that the store reads
+    if constexpr (BAND) {
+      FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw, -1);
+      fwd_last_pass_band<T, L, NCT, BS>(buf, stw, ncy_out);
+    } else {
+      fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, -1);
+    }
     // line p holds z0+2p (real part) and z0+2p+1 (imaginary part)
     c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
     const int z0 = zt * ZT;
@@ -504,12 +631,52 @@ __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, i
   }
 }
 
+// Forward spectrum -> op -> first inverse pass, for a single op with mask
+// c = op.cx <= N/8 followed by an inverse transform: after the forward
+// passes but the last, each item (j, line p) evaluates the forward outputs
+// k = j and k = N - N/8 + j (the only ones the mask can keep among the
+// inputs of inverse butterfly j), applies the op and runs the radix-8
+// inverse butterfly with those two inputs.  Ends with a barrier.
+template <class T, int L, int N, int BS>
+__device__ __forceinline__ void x_band_mid(c2<T>* buf, const c2<T>* tw, const XOp& op, int kyv) {
+  constexpr int RL = last_radix(N), NSL = N / RL, NR1 = N / 8;
+  constexpr int TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
+  c2<T> lo[IT], hi[IT];
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    lo[q] = mkc<T>(0, 0);
+    hi[q] = mkc<T>(0, 0);
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L;
+      const int kn = N - NR1 + j;   // = (NSL - NR1 + j) + (RL - 1) * NSL
+      if (j < op.cx) lo[q] = x_apply<T>(op, fwd_last_out<T, N, L, 0>(buf, tw, j, p), j, N, kyv);
+      if (kn >= N - op.cx + 1)
+        hi[q] = x_apply<T>(op, fwd_last_out<T, N, L, RL - 1>(buf, tw, NSL - NR1 + j, p), kn, N,
+                           kyv);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = threadIdx.x + q * BS;
+    if (TOT % BS == 0 || it < TOT) {
+      const int j = it / L, p = it % L;
+      c2<T> y[8];
+      band_r8<T, +1>(lo[q], hi[q], y);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
 // One ky column tile (L z lines, full x length) per iteration of a
 // workgroup.  MULTI = false: one op with the identity map, applied in place in
 // LDS (the common case; small register footprint).  MULTI = true: the
 // forward spectrum is kept in registers and every op (mask / derivative /
 // dealias map, optional inverse) is built from it in turn.
-template <class T, int MAXN, bool MULTI, int NCT, int BS>
+template <class T, int MAXN, bool MULTI, int NCT, int BS, bool BAND = false>
 __global__ void __launch_bounds__(BS)
 k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     const c2<T>* __restrict__ tw_out, TileGrid tg) {
@@ -534,8 +701,17 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     x_load<T, L, BS>(buf, a, nEl, zt, ky, b);
     tws.store(stw_in);
     __syncthreads();
-    if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, -1);
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
+    if constexpr (BAND) {
+      // (chosen by the launcher) x-space input, one op with the identity
+      // map, mask c <= N/8, inverse transform
+      FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw_in, -1);
+      x_band_mid<T, L, NCT, BS>(buf, stw_in, a.op[0], kyv);
+      FftCT<T, L, NCT, 8, NCT / 8, BS>::run(buf, stw_in, +1);
+      x_store<T, L, BS>(buf, a.dst[b][0], a.op[0].ncy, nz, nin, zt, ky, a.vec);
+      continue;
+    }
+    if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, -1);
     if (!MULTI) {
       const XOp op = a.op[0];
       for (int i = threadIdx.x; i < nEl; i += BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
@@ -668,7 +844,9 @@ __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC
   }
 }
 
-template <class T, class Storer, int MAXN, int NCT, int BS, bool FUSED = false>
+// BAND (compile-time lengths with ncy_in <= N/8, chosen by the launcher):
+// the first inverse pass reads only the nonzero modes.
+template <class T, class Storer, int MAXN, int NCT, int BS, bool FUSED = false, bool BAND = false>
 __global__ void __launch_bounds__(BS)
 k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
   constexpr int L = Lines<T>::L * BS / FFT_BS;
@@ -686,10 +864,12 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
   int zt, x, b;
   for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
-    // the modes kmax..n-kmax are zero
-    for (int i = threadIdx.x; i < n * L; i += BS) {
-      const int k = i / L;
-      if (k >= kmax && k <= n - kmax) buf[i] = mkc<T>(0, 0);
+    // the modes kmax..n-kmax are zero (band: the first pass reads none of them)
+    if constexpr (!BAND) {
+      for (int i = threadIdx.x; i < n * L; i += BS) {
+        const int k = i / L;
+        if (k >= kmax && k <= n - kmax) buf[i] = mkc<T>(0, 0);
+      }
     }
     for (int i = threadIdx.x; i < nIn; i += BS) {
       const int k = i / L, p = i - k * L;
@@ -704,11 +884,21 @@ k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid 
     __syncthreads();
     const int z0 = zt * ZT;
     if constexpr (FUSED) {  // NCT > 0 and a.vec
-      FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw, +1);
+      if constexpr (BAND) {
+        band_first_pass_inv<T, L, NCT, BS>(buf, kmax);
+        FftCTMid<T, L, NCT, 8, NCT / 8, BS>::run(buf, stw, +1);
+      } else {
+        FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw, +1);
+      }
       y_c2r_last_pass_store<T, Storer, NCT, BS>(st, a, buf, stw, z0, x, b);
       continue;
     }
-    fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, +1);
+    if constexpr (BAND) {
+      band_first_pass_inv<T, L, NCT, BS>(buf, kmax);
+      FftCT<T, L, NCT, 8, NCT / 8, BS>::run(buf, stw, +1);
+    } else {
+      fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, +1);
+    }
     const T* bufr = reinterpret_cast<const T*>(buf);
     const T sc = (T)a.scale;
     if (a.vec) {

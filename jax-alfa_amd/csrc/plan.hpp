@@ -177,6 +177,15 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
     TileGrid tg{(nz + 2 * L - 1) / (2 * L), rows, nb, Loader::BATCH_FAST};
     const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
     const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+    // pruned output within the stride of the last radix pass: that pass is
+    // evaluated only at the outputs the store reads
+    if constexpr (NCT > 0) {
+      if (ALFA_BAND && ncy <= NCT / last_radix(NCT)) {
+        k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS, true>
+            <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
+        return;
+      }
+    }
     k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>
         <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
   });
@@ -198,6 +207,17 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
     const int nbuf = imax(fin.n, fout.n);
     const size_t lds =
         (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
+    // one masked op between the forward and inverse transforms, with the
+    // kept modes inside the stride of the radix-8 first pass: band passes
+    if constexpr (NCT > 0 && !MULTI) {
+      const XOp& o = a.op[0];
+      if (ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops == 1 && o.inverse &&
+          o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 8) {
+        k_x<T, decltype(maxn)::value, MULTI, NCT, BS, true>
+            <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+        return;
+      }
+    }
     k_x<T, decltype(maxn)::value, MULTI, NCT, BS>
         <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
   };
@@ -251,8 +271,19 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
     const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
     const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
     if constexpr (NCT > 0) {
-      if (a.vec) {  // first and last radix passes fused into the loads and stores
-        k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>
+      // input modes inside the stride of the radix-8 first pass: band first pass
+      const bool band = ALFA_BAND && radix_of(NCT) == 8 && a.ncy_in <= NCT / 8;
+      if (a.vec) {  // last radix pass fused into the stores
+        if (band)
+          k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true, true>
+              <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+        else
+          k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>
+              <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+        return;
+      }
+      if (band) {
+        k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, false, true>
             <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
         return;
       }

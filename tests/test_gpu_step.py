@@ -28,14 +28,17 @@ CASES = {
     "convective_wl": ("convective", dict(nx=32, ny=32, nz=32, optSgs=2)),
     "neutral_wl": ("neutral", dict(optSgs=4)),
     "stable_wl_static": ("stable", dict(nx=32, ny=32, nz=32, optSgs=4, dynamicSGS_call_time=2)),
-    # transform lengths with compile-time plans (64, 96, 128, 192, 256; the
-    # dealiased FGR=1 grid pads 64 to 96), whose y C2R reads its first radix
-    # pass from the spectra and writes its last one to the fields directly
     # optPressureSolver = 0 (the reference's dense LU solve per wavenumber,
     # NSE_PressureTerms.py:220-335): the oracle solves it with LU; the GPU
     # solves the same tridiagonal system with the Thomas kernel
     "stable_lu": ("stable", dict(nx=32, ny=32, nz=32, optPressureSolver=0)),
+    # transform lengths with compile-time plans (64, 96, 128, 192, 256; the
+    # dealiased FGR=1 grid pads 64 to 96): the y C2R writes its last radix
+    # pass to the fields directly, and at FGR=2 the filtered spectra take the
+    # band-limited passes (|k| < n/8)
     "convective64x96": ("convective", dict(nx=64, ny=96, nz=16)),
+    # one test-filter level (LAD): its x pass runs the band-limited kernel
+    "convective64x96_lad": ("convective", dict(nx=64, ny=96, nz=16, optSgs=3)),
     "convective128x192": ("convective", dict(nx=128, ny=192, nz=16)),
     "neutral64": ("neutral", dict(nx=64, ny=64, nz=16)),
 }
