@@ -54,6 +54,12 @@ constexpr int FFT_MAXRAD = 8;
 #ifndef ALFA_X_BS
 #define ALFA_X_BS 256
 #endif
+//   ALFA_BATCH_LOADS: the tile loads of the compile-time-length single-op
+//   float x kernel are all issued before the first LDS write (1), or one loop
+//   iteration at a time (0)
+#ifndef ALFA_BATCH_LOADS
+#define ALFA_BATCH_LOADS 1
+#endif
 //   ALFA_BAND: band-limited passes for filtered spectra (1) or full passes (0)
 #ifndef ALFA_BAND
 #define ALFA_BAND 1

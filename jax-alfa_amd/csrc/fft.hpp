@@ -592,11 +592,40 @@ __device__ __forceinline__ c2<T> x_apply(const XOp& op, c2<T> v, int k, int nin,
 }
 
 // load tile (zt, ky) of field b into LDS, (k, line) at buf[k*L + line];
-// with vec, float lines are moved in pairs (16 bytes)
-template <class T, int L, int BS>
+// with vec, float lines are moved in pairs (16 bytes).  NEL > 0 (float,
+// compile-time nEl): every load of the tile is issued before the first LDS
+// write, one HBM round trip per tile instead of one per loop iteration
+// (measured faster for the single-op float kernel only: elsewhere the
+// registers it holds cost more occupancy than it gains).
+template <class T, int L, int BS, int NEL = 0>
 __device__ __forceinline__ void x_load(c2<T>* buf, const XArgs& a, int nEl, int zt, int ky, int b) {
   const c2<T>* in = reinterpret_cast<const c2<T>*>(a.in[b]);
   const int nz = a.nz;
+  if constexpr (NEL > 0 && sizeof(T) == 4) {
+    constexpr int NV = (NEL + 2 * BS - 1) / (2 * BS);
+    if (a.vec) {
+      c2<T> v0[NV], v1[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const int e = 2 * (threadIdx.x + q * BS);
+        v0[q] = mkc<T>(0, 0);
+        v1[q] = mkc<T>(0, 0);
+        if (NEL % (2 * BS) == 0 || e < NEL) {
+          const int k = e / L, z = zt * L + (e - k * L);
+          if (z < nz) cld2<T>(in + ((size_t)k * a.ncy_in + ky) * nz + z, v0[q], v1[q]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const int e = 2 * (threadIdx.x + q * BS);
+        if (NEL % (2 * BS) == 0 || e < NEL) {
+          buf[e] = v0[q];
+          buf[e + 1] = v1[q];
+        }
+      }
+      return;
+    }
+  }
   if (sizeof(T) == 4 && a.vec) {
     for (int e = 2 * threadIdx.x; e < nEl; e += 2 * BS) {
       const int k = e / L, z = zt * L + (e - k * L);
@@ -698,7 +727,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
   for (int it = 0; tg.tile(it, zt, ky, b); ++it) {
     if (!MULTI && ky >= a.op[0].ncy) continue;
     __syncthreads();
-    x_load<T, L, BS>(buf, a, nEl, zt, ky, b);
+    x_load<T, L, BS, (!MULTI && ALFA_BATCH_LOADS) ? NCT * L : 0>(buf, a, nEl, zt, ky, b);
     tws.store(stw_in);
     __syncthreads();
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;

@@ -61,7 +61,14 @@ __global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double
   if (z < g.nz) {
     const int c0 = bc * g.cpb;
     const int c1 = imin(c0 + g.cpb, g.ncol);
-    for (int c = c0 + cl; c < c1; c += g.lanes) f(c, z, acc);
+    if constexpr (NT <= 5 && F::NFIELDS <= 5) {
+      // few fields and terms: unrolled, so that the loads of several columns are in
+      // flight at once (the additions keep their order)
+#pragma unroll 4
+      for (int c = c0 + cl; c < c1; c += g.lanes) f(c, z, acc);
+    } else {
+      for (int c = c0 + cl; c < c1; c += g.lanes) f(c, z, acc);
+    }
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) sm[t][threadIdx.x] = acc[t];
