@@ -284,31 +284,13 @@ __host__ __device__ constexpr int last_radix(int m) {
 // ----------------------------------------------------------------------------
 // Band-limited passes.  A test- or explicit-filtered spectrum is nonzero only
 // at |k| < c, and the filters keep only those modes of a forward transform.
-// Where c <= N/8 (N/8 = the stride of the radix-8 first pass of a
-// compile-time plan), the first inverse pass has two nonzero inputs per
-// butterfly (k = j and k = j - N/8), and where c <= N/RL the last forward
-// pass (radix RL, stride N/RL) is needed only at outputs r = 0 and r = RL-1
-// of each butterfly.  These helpers compute just those.
+// With c <= N/4 (twice the stride N/8 of the radix-8 first pass of a
+// compile-time plan), inputs r = 2..5 of every first inverse butterfly are
+// zero: only r = 0, 1 (k = j, j + N/8) and r = 6, 7 (k = j - N/4, j - N/8)
+// can be nonzero.  With c <= N/RL (the stride of the last forward pass,
+// radix RL), the kept forward modes are outputs r = 0 and r = RL-1 of the
+// last pass.  These helpers read and compute only those.
 // ----------------------------------------------------------------------------
-
-// Radix-8 butterfly whose only nonzero inputs are v[0] = a and v[7] = b:
-// y_m = a + b exp(SIGN 2 pi i 7m/8) = a + b exp(-SIGN 2 pi i m/8).
-template <class T, int SIGN>
-__device__ __forceinline__ void band_r8(c2<T> a, c2<T> b, c2<T>* y) {
-  const T h = T(0.70710678118654752440);   // sqrt(2)/2
-  const T s = T(SIGN);
-  const c2<T> p = mkc<T>(h * (b.x + s * b.y), h * (b.y - s * b.x));   // b exp(-SIGN i pi/4)
-  const c2<T> q = mkc<T>(h * (b.x - s * b.y), h * (b.y + s * b.x));   // b exp(+SIGN i pi/4)
-  const c2<T> r = mkc<T>(s * b.y, -s * b.x);                          // b exp(-SIGN i pi/2)
-  y[0] = cadd<T>(a, b);
-  y[1] = cadd<T>(a, p);
-  y[2] = cadd<T>(a, r);
-  y[3] = csub<T>(a, q);
-  y[4] = csub<T>(a, b);
-  y[5] = csub<T>(a, p);
-  y[6] = csub<T>(a, r);
-  y[7] = cadd<T>(a, q);
-}
 
 // Output ROUT of butterfly jj of the last pass of the compile-time forward
 // plan of length N (radix RL, stride N/RL, twiddle step 1), read from LDS:
@@ -366,22 +348,33 @@ __device__ __forceinline__ void fwd_last_pass_band(c2<T>* buf, const c2<T>* tw, 
   __syncthreads();
 }
 
+// the inputs r of a radix-8 first-pass butterfly that a band |k| < c with
+// c <= N/4 can make nonzero
+__host__ __device__ constexpr bool band_r(int r) { return r < 2 || r > 5; }
+
 // First pass (radix 8, stride 1) of the inverse transform of L lines whose
-// modes are zero outside k < kmax and k > N - kmax (kmax <= N/8); only
-// those elements of buf are read.  Ends with a barrier, like a full pass.
+// modes are zero outside k < kmax and k > N - kmax (kmax <= N/4); only the
+// nonzero elements of buf are read.  The butterfly is the full radix-8 one
+// with zeros in place, so the result is that of the full pass.  Ends with a
+// barrier, like a full pass.
 template <class T, int L, int N, int BS>
 __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
   constexpr int NR1 = N / 8, TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
-  c2<T> lo[IT], hi[IT];
+  c2<T> v[IT][8];
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int it = threadIdx.x + q * BS;
-    lo[q] = mkc<T>(0, 0);
-    hi[q] = mkc<T>(0, 0);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[q][r] = mkc<T>(0, 0);
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
-      if (j < kmax) lo[q] = buf[j * L + p];
-      if (j >= NR1 - kmax + 1) hi[q] = buf[(j + 7 * NR1) * L + p];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        if (!band_r(r)) continue;
+        const int k = j + r * NR1;
+        if (r < 2 ? k < kmax : k > N - kmax) v[q][r] = buf[k * L + p];
+      }
+      Dft<T, 8>::run(v[q], +1);
     }
   }
   __syncthreads();
@@ -390,10 +383,8 @@ __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
     const int it = threadIdx.x + q * BS;
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
-      c2<T> y[8];
-      band_r8<T, +1>(lo[q], hi[q], y);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = y[m];
+      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = v[q][m];
     }
   }
   __syncthreads();
@@ -661,28 +652,37 @@ __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, i
 }
 
 // Forward spectrum -> op -> first inverse pass, for a single op with mask
-// c = op.cx <= N/8 followed by an inverse transform: after the forward
-// passes but the last, each item (j, line p) evaluates the forward outputs
-// k = j and k = N - N/8 + j (the only ones the mask can keep among the
-// inputs of inverse butterfly j), applies the op and runs the radix-8
-// inverse butterfly with those two inputs.  Ends with a barrier.
+// c = op.cx <= min(N/4, N/RL) followed by an inverse transform: after the
+// forward passes but the last, each item (j, line p) evaluates the forward
+// outputs at the modes k = j + r N/8 (r = 0, 1, 6, 7) the mask keeps, applies
+// the op and runs the radix-8 inverse butterfly j on them (zeros elsewhere,
+// as the full path has after the mask).  Ends with a barrier.
 template <class T, int L, int N, int BS>
 __device__ __forceinline__ void x_band_mid(c2<T>* buf, const c2<T>* tw, const XOp& op, int kyv) {
   constexpr int RL = last_radix(N), NSL = N / RL, NR1 = N / 8;
   constexpr int TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
-  c2<T> lo[IT], hi[IT];
+  c2<T> v[IT][8];
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int it = threadIdx.x + q * BS;
-    lo[q] = mkc<T>(0, 0);
-    hi[q] = mkc<T>(0, 0);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[q][r] = mkc<T>(0, 0);
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
-      const int kn = N - NR1 + j;   // = (NSL - NR1 + j) + (RL - 1) * NSL
-      if (j < op.cx) lo[q] = x_apply<T>(op, fwd_last_out<T, N, L, 0>(buf, tw, j, p), j, N, kyv);
-      if (kn >= N - op.cx + 1)
-        hi[q] = x_apply<T>(op, fwd_last_out<T, N, L, RL - 1>(buf, tw, NSL - NR1 + j, p), kn, N,
-                           kyv);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        if (!band_r(r)) continue;
+        const int k = j + r * NR1;
+        if (r < 2) {   // k < c <= N/RL: output 0 of forward butterfly k
+          if (k < op.cx)
+            v[q][r] = x_apply<T>(op, fwd_last_out<T, N, L, 0>(buf, tw, k, p), k, N, kyv);
+        } else {       // k > N - c >= N - N/RL: output RL-1
+          if (k >= N - op.cx + 1)
+            v[q][r] = x_apply<T>(op, fwd_last_out<T, N, L, RL - 1>(buf, tw, k - (RL - 1) * NSL, p),
+                                 k, N, kyv);
+        }
+      }
+      Dft<T, 8>::run(v[q], +1);
     }
   }
   __syncthreads();
@@ -691,10 +691,8 @@ __device__ __forceinline__ void x_band_mid(c2<T>* buf, const c2<T>* tw, const XO
     const int it = threadIdx.x + q * BS;
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
-      c2<T> y[8];
-      band_r8<T, +1>(lo[q], hi[q], y);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = y[m];
+      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = v[q][m];
     }
   }
   __syncthreads();
@@ -733,7 +731,7 @@ k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
     if constexpr (BAND) {
       // (chosen by the launcher) x-space input, one op with the identity
-      // map, mask c <= N/8, inverse transform
+      // map, mask c <= min(N/4, N/RL), inverse transform
       FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw_in, -1);
       x_band_mid<T, L, NCT, BS>(buf, stw_in, a.op[0], kyv);
       FftCT<T, L, NCT, 8, NCT / 8, BS>::run(buf, stw_in, +1);
@@ -873,7 +871,7 @@ __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC
   }
 }
 
-// BAND (compile-time lengths with ncy_in <= N/8, chosen by the launcher):
+// BAND (compile-time lengths with ncy_in <= N/4, chosen by the launcher):
 // the first inverse pass reads only the nonzero modes.
 template <class T, class Storer, int MAXN, int NCT, int BS, bool FUSED = false, bool BAND = false>
 __global__ void __launch_bounds__(BS)

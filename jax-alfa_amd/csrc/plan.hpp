@@ -208,11 +208,14 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
     const size_t lds =
         (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
     // one masked op between the forward and inverse transforms, with the
-    // kept modes inside the stride of the radix-8 first pass: band passes
-    if constexpr (NCT > 0 && !MULTI) {
+    // kept modes within twice the stride of the radix-8 first pass and within
+    // the stride of the last pass: band passes (float only: the double
+    // kernel needs 14 more VGPRs for them and loses a wave per SIMD)
+    if constexpr (NCT > 0 && !MULTI && sizeof(T) == 4) {
       const XOp& o = a.op[0];
       if (ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops == 1 && o.inverse &&
-          o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 8) {
+          o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 4 &&
+          o.cx <= NCT / last_radix(NCT)) {
         k_x<T, decltype(maxn)::value, MULTI, NCT, BS, true>
             <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
         return;
@@ -271,8 +274,9 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
     const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
     const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
     if constexpr (NCT > 0) {
-      // input modes inside the stride of the radix-8 first pass: band first pass
-      const bool band = ALFA_BAND && radix_of(NCT) == 8 && a.ncy_in <= NCT / 8;
+      // input modes within twice the stride of the radix-8 first pass: band
+      // first pass
+      const bool band = ALFA_BAND && radix_of(NCT) == 8 && a.ncy_in <= NCT / 4;
       if (a.vec) {  // last radix pass fused into the stores
         if (band)
           k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true, true>

@@ -35,10 +35,10 @@ CASES = {
     # transform lengths with compile-time plans (64, 96, 128, 192, 256; the
     # dealiased FGR=1 grid pads 64 to 96): the y C2R writes its last radix
     # pass to the fields directly, and at FGR=2 the filtered spectra take the
-    # band-limited passes (|k| < n/8)
+    # band-limited passes (|k| <= n/4)
     "convective64x96": ("convective", dict(nx=64, ny=96, nz=16)),
     # one test-filter level (LAD): its x pass runs the band-limited kernel
-    "convective64x96_lad": ("convective", dict(nx=64, ny=96, nz=16, optSgs=3)),
+    "convective128x96_lad": ("convective", dict(nx=128, ny=96, nz=16, optSgs=3)),
     "convective128x192": ("convective", dict(nx=128, ny=192, nz=16)),
     "neutral64": ("neutral", dict(nx=64, ny=64, nz=16)),
 }
