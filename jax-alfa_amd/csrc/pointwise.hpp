@@ -181,48 +181,48 @@ __global__ void k_scalar_adv(PwDims d, const T* __restrict__ u, const T* __restr
   }
 }
 
-// k_advection and k_scalar_adv in one pass (the unpadded FGR = 2 path):
-// u, v, w are read once for both.
+// Advection (rotational form) and scalar advection at point i (level z):
+// k_advection + k_scalar_adv of the unpadded grid.
+template <class T> struct AdvOut { T cx, cy, cz, r; };
 template <class T>
-__global__ void k_advection_all(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
-                                const T* __restrict__ w, const T* __restrict__ dudy,
-                                const T* __restrict__ dudz, const T* __restrict__ dvdx,
-                                const T* __restrict__ dvdz, const T* __restrict__ dwdx,
-                                const T* __restrict__ dwdy, const T* __restrict__ sx,
-                                const T* __restrict__ sy, const T* __restrict__ sz, T* Cx, T* Cy,
-                                T* Cz, T* sadv) {
-  const int nz = d.nz;
-  PW_LOOP(d.n) {
-    const int z = (int)(i % nz);
-    const T ui = u[i], vi = v[i], wi = w[i];
-    const T a1x = vi * (dudy[i] - dvdx[i]);
-    const T a1y = ui * (dvdx[i] - dudy[i]);
-    const T cc1 = ui * sx[i] + vi * sy[i];
-    T cx, cy, cz, r;
-    if (z == 0) {
-      const T w1 = w[i + 1];
-      cx = a1x + T(0.5) * (w1 * (dudz[i + 1] - dwdx[i + 1]));
-      cy = a1y + T(0.5) * (w1 * (dvdz[i + 1] - dwdy[i + 1]));
-      cz = T(0);
-      r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
-    } else if (z < nz - 1) {
-      const T w1 = w[i + 1];
-      const T a2x0 = wi * (dudz[i] - dwdx[i]), a2x1 = w1 * (dudz[i + 1] - dwdx[i + 1]);
-      const T a2y0 = wi * (dvdz[i] - dwdy[i]), a2y1 = w1 * (dvdz[i + 1] - dwdy[i + 1]);
-      cx = a1x + T(0.5) * (a2x0 + a2x1);
-      cy = a1y + T(0.5) * (a2y0 + a2y1);
-      const T uG = T(0.5) * (u[i - 1] + ui);
-      const T vG = T(0.5) * (v[i - 1] + vi);
-      cz = uG * (dwdx[i] - dudz[i]) + vG * (dwdy[i] - dvdz[i]);
-      r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
-    } else {
-      cx = a1x + wi * (dudz[i] - dwdx[i]);
-      cy = a1y + wi * (dvdz[i] - dwdy[i]);
-      cz = T(0);
-      r = cc1 + wi * sz[i];
-    }
-    Cx[i] = cx; Cy[i] = cy; Cz[i] = cz; sadv[i] = r;
+__device__ __forceinline__ AdvOut<T> adv_point(size_t i, int z, int nz, const T* __restrict__ u,
+                                               const T* __restrict__ v, const T* __restrict__ w,
+                                               const T* __restrict__ dudy,
+                                               const T* __restrict__ dudz,
+                                               const T* __restrict__ dvdx,
+                                               const T* __restrict__ dvdz,
+                                               const T* __restrict__ dwdx,
+                                               const T* __restrict__ dwdy,
+                                               const T* __restrict__ sx, const T* __restrict__ sy,
+                                               const T* __restrict__ sz) {
+  const T ui = u[i], vi = v[i], wi = w[i];
+  const T a1x = vi * (dudy[i] - dvdx[i]);
+  const T a1y = ui * (dvdx[i] - dudy[i]);
+  const T cc1 = ui * sx[i] + vi * sy[i];
+  AdvOut<T> o;
+  if (z == 0) {
+    const T w1 = w[i + 1];
+    o.cx = a1x + T(0.5) * (w1 * (dudz[i + 1] - dwdx[i + 1]));
+    o.cy = a1y + T(0.5) * (w1 * (dvdz[i + 1] - dwdy[i + 1]));
+    o.cz = T(0);
+    o.r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
+  } else if (z < nz - 1) {
+    const T w1 = w[i + 1];
+    const T a2x0 = wi * (dudz[i] - dwdx[i]), a2x1 = w1 * (dudz[i + 1] - dwdx[i + 1]);
+    const T a2y0 = wi * (dvdz[i] - dwdy[i]), a2y1 = w1 * (dvdz[i + 1] - dwdy[i + 1]);
+    o.cx = a1x + T(0.5) * (a2x0 + a2x1);
+    o.cy = a1y + T(0.5) * (a2y0 + a2y1);
+    const T uG = T(0.5) * (u[i - 1] + ui);
+    const T vG = T(0.5) * (v[i - 1] + vi);
+    o.cz = uG * (dwdx[i] - dudz[i]) + vG * (dwdy[i] - dvdz[i]);
+    o.r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
+  } else {
+    o.cx = a1x + wi * (dudz[i] - dwdx[i]);
+    o.cy = a1y + wi * (dvdz[i] - dwdy[i]);
+    o.cz = T(0);
+    o.r = cc1 + wi * sz[i];
   }
+  return o;
 }
 
 // BuoyancyOpt1/2 (src/pde/NSE_BuoyancyTerms.py:49-150), dry; bar = plane
@@ -246,8 +246,38 @@ template <class T> __device__ __forceinline__ T smag(T a, T b, T c, T d, T e, T 
   return sqrt(T(2) * (a * a + b * b + c * c + T(2) * d * d + T(2) * e * e + T(2) * f * f));
 }
 
-// StrainsUVPnodes_NoDealias (src/subgridscale/StrainRates.py:149-211): writes
-// S12, S13, S23 and |S| (S11 = dudx, S22 = dvdy, S33 = dwdz are aliases).
+// StrainsUVPnodes_NoDealias (src/subgridscale/StrainRates.py:149-211) at
+// point i: S12, S13, S23 and |S| (S11 = dudx, S22 = dvdy, S33 = dwdz).
+template <class T> struct StrainOut { T s12, s13, s23, S; };
+template <class T>
+__device__ __forceinline__ StrainOut<T> strains_point(size_t i, int z, int nz,
+                                                      const T* __restrict__ dudx,
+                                                      const T* __restrict__ dvdx,
+                                                      const T* __restrict__ dwdx,
+                                                      const T* __restrict__ dudy,
+                                                      const T* __restrict__ dvdy,
+                                                      const T* __restrict__ dwdy,
+                                                      const T* __restrict__ dudz,
+                                                      const T* __restrict__ dvdz,
+                                                      const T* __restrict__ dwdz) {
+  T uz, vz, wx, wy;
+  if (z == 0) {
+    uz = dudz[i]; vz = dvdz[i]; wx = T(0.5) * dwdx[i + 1]; wy = T(0.5) * dwdy[i + 1];
+  } else if (z < nz - 1) {
+    uz = T(0.5) * (dudz[i] + dudz[i + 1]); vz = T(0.5) * (dvdz[i] + dvdz[i + 1]);
+    wx = T(0.5) * (dwdx[i] + dwdx[i + 1]); wy = T(0.5) * (dwdy[i] + dwdy[i + 1]);
+  } else {
+    uz = dudz[i]; vz = dvdz[i]; wx = dwdx[i]; wy = dwdy[i];
+  }
+  StrainOut<T> o;
+  o.s12 = T(0.5) * (dudy[i] + dvdx[i]);
+  o.s13 = T(0.5) * (uz + wx);
+  o.s23 = T(0.5) * (vz + wy);
+  o.S = smag<T>(dudx[i], dvdy[i], dwdz[i], o.s12, o.s13, o.s23);
+  return o;
+}
+
+// StrainsUVPnodes_NoDealias: writes S12, S13, S23 and |S|.
 template <class T>
 __global__ void k_strains_uvp(PwDims d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
                               const T* __restrict__ dwdx, const T* __restrict__ dudy,
@@ -257,20 +287,32 @@ __global__ void k_strains_uvp(PwDims d, const T* __restrict__ dudx, const T* __r
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
-    T uz, vz, wx, wy;
-    if (z == 0) {
-      uz = dudz[i]; vz = dvdz[i]; wx = T(0.5) * dwdx[i + 1]; wy = T(0.5) * dwdy[i + 1];
-    } else if (z < nz - 1) {
-      uz = T(0.5) * (dudz[i] + dudz[i + 1]); vz = T(0.5) * (dvdz[i] + dvdz[i + 1]);
-      wx = T(0.5) * (dwdx[i] + dwdx[i + 1]); wy = T(0.5) * (dwdy[i] + dwdy[i + 1]);
-    } else {
-      uz = dudz[i]; vz = dvdz[i]; wx = dwdx[i]; wy = dwdy[i];
-    }
-    const T s12 = T(0.5) * (dudy[i] + dvdx[i]);
-    const T s13 = T(0.5) * (uz + wx);
-    const T s23 = T(0.5) * (vz + wy);
-    S12[i] = s12; S13[i] = s13; S23[i] = s23;
-    S[i] = smag<T>(dudx[i], dvdy[i], dwdz[i], s12, s13, s23);
+    const StrainOut<T> o =
+        strains_point<T>(i, z, nz, dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz);
+    S12[i] = o.s12; S13[i] = o.s13; S23[i] = o.s23; S[i] = o.S;
+  }
+}
+
+// Advection + scalar advection (adv_point) and the uvp-node strain rates
+// (strains_point) in one pass on the unpadded grid: u, v, w are read once for
+// both advections, and the six velocity derivatives both parts read (dudy,
+// dvdx, dudz, dvdz, dwdx, dwdy) once for both.
+template <class T> struct AdvStrainArgs {
+  const T *u, *v, *w, *sx, *sy, *sz;
+  const T *dudx, *dvdx, *dwdx, *dudy, *dvdy, *dwdy, *dudz, *dvdz, *dwdz;
+  T *Cx, *Cy, *Cz, *sadv, *S12, *S13, *S23, *S;
+};
+template <class T>
+__global__ void k_adv_strains(PwDims d, AdvStrainArgs<T> a) {
+  const int nz = d.nz;
+  PW_LOOP(d.n) {
+    const int z = (int)(i % nz);
+    const AdvOut<T> c = adv_point<T>(i, z, nz, a.u, a.v, a.w, a.dudy, a.dudz, a.dvdx, a.dvdz,
+                                     a.dwdx, a.dwdy, a.sx, a.sy, a.sz);
+    const StrainOut<T> o = strains_point<T>(i, z, nz, a.dudx, a.dvdx, a.dwdx, a.dudy, a.dvdy,
+                                            a.dwdy, a.dudz, a.dvdz, a.dwdz);
+    a.Cx[i] = c.cx; a.Cy[i] = c.cy; a.Cz[i] = c.cz; a.sadv[i] = c.r;
+    a.S12[i] = o.s12; a.S13[i] = o.s13; a.S23[i] = o.s23; a.S[i] = o.S;
   }
 }
 

@@ -587,14 +587,18 @@ template <class T> struct SgsStrains {
 };
 
 // w_nodes = false: the unpadded w-node strains are left to k_sgs_fused.
+// uvp_done: S12, S13, S23 and |S| on uvp nodes are already in their
+// workspace buffers (k_adv_strains).
 template <class T>
-void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o, bool w_nodes = true) {
+void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o, bool w_nodes = true,
+                   bool uvp_done = false) {
   alfa_plan* P = c.P;
   hipStream_t s = c.s;
   T *S12 = c.W("S12"), *S13 = c.W("S13"), *S23 = c.W("S23");
   o.Sm = c.W("S");
-  PW_LAUNCH(P, s, c.F, k_strains_uvp<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7],
-            g[8], S12, S13, S23, o.Sm);
+  if (!uvp_done)
+    PW_LAUNCH(P, s, c.F, k_strains_uvp<T>, c.dims(), g[0], g[1], g[2], g[3], g[4], g[5], g[6],
+              g[7], g[8], S12, S13, S23, o.Sm);
   o.Sij[0] = g[0]; o.Sij[1] = g[4]; o.Sij[2] = g[8];
   o.Sij[3] = S12; o.Sij[4] = S13; o.Sij[5] = S23;
   o.S13w = c.W("S13w"); o.S23w = c.W("S23w"); o.Sw = c.W("Sw");
@@ -889,8 +893,14 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   T *Cx = c.W("Cx"), *Cy = c.W("Cy"), *Cz = c.W("Cz"), *THadv = c.W("THadv");
   T* QAdv = moist ? c.W("QAdv") : nullptr;
   if (!dealias) {
-    PW_LAUNCH(P, s, c.F, k_advection_all<T>, c.dims(), S.u, S.v, S.w, dudy, dudz, dvdx, dvdz,
-              dwdx, dwdy, dTx, dTy, dTz, Cx, Cy, Cz, THadv);
+    // with the uvp-node strain rates of step 6 in the same pass
+    AdvStrainArgs<T> aa{};
+    aa.u = S.u; aa.v = S.v; aa.w = S.w; aa.sx = dTx; aa.sy = dTy; aa.sz = dTz;
+    aa.dudx = dudx; aa.dvdx = dvdx; aa.dwdx = dwdx; aa.dudy = dudy; aa.dvdy = dvdy;
+    aa.dwdy = dwdy; aa.dudz = dudz; aa.dvdz = dvdz; aa.dwdz = dwdz;
+    aa.Cx = Cx; aa.Cy = Cy; aa.Cz = Cz; aa.sadv = THadv;
+    aa.S12 = c.W("S12"); aa.S13 = c.W("S13"); aa.S23 = c.W("S23"); aa.S = c.W("S");
+    PW_LAUNCH(P, s, c.F, k_adv_strains<T>, c.dims(), aa);
     if (moist)   // ScalarAdvection of Q (Main.py:313-318)
       PW_LAUNCH(P, s, c.F, k_scalar_adv<T>, c.dims(), S.u, S.v, S.w, dQx, dQy, dQz, QAdv);
   } else {
@@ -932,7 +942,7 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   const T* grads[9] = {dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz};
   SgsStrains<T> ss{};
   // the moisture fluxes need the w-node |S| of the unpadded grid as well
-  stage_strains<T>(c, grads, ss, dealias || moist);
+  stage_strains<T>(c, grads, ss, dealias || moist, !dealias);
   const double* shu = mixed ? means : nullptr;
   const double* shv = mixed ? means + c.nz : nullptr;
   const double* sht = mixed ? means + 2 * c.nz : nullptr;
