@@ -181,20 +181,64 @@ __global__ void k_scalar_adv(PwDims d, const T* __restrict__ u, const T* __restr
   }
 }
 
-// Advection (rotational form) and scalar advection at point i (level z):
-// k_advection + k_scalar_adv of the unpadded grid.
+// Accessors of a z-derivative field at point i (level z, column c): read
+// from memory (PtrAt), or formed on the fly with the arithmetic of k_derivz
+// (DzUV: u or v on w nodes with the MOST wall value at level 0 and 0 at the
+// top level; DzW: w on uvp nodes; DzTH: TH on w nodes with the surface flux
+// at level 0).
+template <class T> struct PtrAt {
+  const T* __restrict__ p;
+  __device__ __forceinline__ T operator()(size_t i, int, size_t) const { return p[i]; }
+};
+template <class T, bool ADD> struct DzUV {
+  const T* __restrict__ f;
+  const T* __restrict__ fim;
+  const T* __restrict__ ustar;
+  const T* __restrict__ M;
+  T add, idz, half_dz_vonk;
+  int nz;
+  __device__ __forceinline__ T operator()(size_t i, int z, size_t c) const {
+    if (z == 0) {
+      const T den = M[c] * half_dz_vonk;
+      if constexpr (ADD) return fim[c] * ustar[c] * (f[i] + add) / den;
+      else return fim[c] * ustar[c] * f[i] / den;
+    }
+    return z < nz - 1 ? (f[i] - f[i - 1]) * idz : T(0);
+  }
+};
+template <class T> struct DzW {
+  const T* __restrict__ w;
+  T idz;
+  int nz;
+  __device__ __forceinline__ T operator()(size_t i, int z, size_t) const {
+    return z < nz - 1 ? (w[i + 1] - w[i]) * idz : T(0);
+  }
+};
+template <class T> struct DzTH {
+  const T* __restrict__ th;
+  const T* __restrict__ fih;
+  const T* __restrict__ qz;
+  const T* __restrict__ ustar;
+  T idz, vonk_half_dz;
+  __device__ __forceinline__ T operator()(size_t i, int z, size_t c) const {
+    return z == 0 ? fih[c] * (-qz[c] / ustar[c]) / vonk_half_dz : (th[i] - th[i - 1]) * idz;
+  }
+};
+
+// Advection (rotational form) and scalar advection at point i (level z,
+// column c): k_advection + k_scalar_adv of the unpadded grid.  uz, vz, sz:
+// accessors of du/dz, dv/dz (w nodes) and d(scalar)/dz.
 template <class T> struct AdvOut { T cx, cy, cz, r; };
-template <class T>
-__device__ __forceinline__ AdvOut<T> adv_point(size_t i, int z, int nz, const T* __restrict__ u,
-                                               const T* __restrict__ v, const T* __restrict__ w,
-                                               const T* __restrict__ dudy,
-                                               const T* __restrict__ dudz,
-                                               const T* __restrict__ dvdx,
-                                               const T* __restrict__ dvdz,
+template <class T, class UZ, class VZ, class SZ>
+__device__ __forceinline__ AdvOut<T> adv_point(size_t i, int z, size_t c, int nz,
+                                               const T* __restrict__ u, const T* __restrict__ v,
+                                               const T* __restrict__ w,
+                                               const T* __restrict__ dudy, const UZ& dudz,
+                                               const T* __restrict__ dvdx, const VZ& dvdz,
                                                const T* __restrict__ dwdx,
                                                const T* __restrict__ dwdy,
                                                const T* __restrict__ sx, const T* __restrict__ sy,
-                                               const T* __restrict__ sz) {
+                                               const SZ& sz) {
   const T ui = u[i], vi = v[i], wi = w[i];
   const T a1x = vi * (dudy[i] - dvdx[i]);
   const T a1y = ui * (dvdx[i] - dudy[i]);
@@ -202,25 +246,26 @@ __device__ __forceinline__ AdvOut<T> adv_point(size_t i, int z, int nz, const T*
   AdvOut<T> o;
   if (z == 0) {
     const T w1 = w[i + 1];
-    o.cx = a1x + T(0.5) * (w1 * (dudz[i + 1] - dwdx[i + 1]));
-    o.cy = a1y + T(0.5) * (w1 * (dvdz[i + 1] - dwdy[i + 1]));
+    o.cx = a1x + T(0.5) * (w1 * (dudz(i + 1, z + 1, c) - dwdx[i + 1]));
+    o.cy = a1y + T(0.5) * (w1 * (dvdz(i + 1, z + 1, c) - dwdy[i + 1]));
     o.cz = T(0);
-    o.r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
+    o.r = cc1 + T(0.5) * (wi * sz(i, z, c) + w1 * sz(i + 1, z + 1, c));
   } else if (z < nz - 1) {
     const T w1 = w[i + 1];
-    const T a2x0 = wi * (dudz[i] - dwdx[i]), a2x1 = w1 * (dudz[i + 1] - dwdx[i + 1]);
-    const T a2y0 = wi * (dvdz[i] - dwdy[i]), a2y1 = w1 * (dvdz[i + 1] - dwdy[i + 1]);
+    const T uz0 = dudz(i, z, c), vz0 = dvdz(i, z, c);
+    const T a2x0 = wi * (uz0 - dwdx[i]), a2x1 = w1 * (dudz(i + 1, z + 1, c) - dwdx[i + 1]);
+    const T a2y0 = wi * (vz0 - dwdy[i]), a2y1 = w1 * (dvdz(i + 1, z + 1, c) - dwdy[i + 1]);
     o.cx = a1x + T(0.5) * (a2x0 + a2x1);
     o.cy = a1y + T(0.5) * (a2y0 + a2y1);
     const T uG = T(0.5) * (u[i - 1] + ui);
     const T vG = T(0.5) * (v[i - 1] + vi);
-    o.cz = uG * (dwdx[i] - dudz[i]) + vG * (dwdy[i] - dvdz[i]);
-    o.r = cc1 + T(0.5) * (wi * sz[i] + w1 * sz[i + 1]);
+    o.cz = uG * (dwdx[i] - uz0) + vG * (dwdy[i] - vz0);
+    o.r = cc1 + T(0.5) * (wi * sz(i, z, c) + w1 * sz(i + 1, z + 1, c));
   } else {
-    o.cx = a1x + wi * (dudz[i] - dwdx[i]);
-    o.cy = a1y + wi * (dvdz[i] - dwdy[i]);
+    o.cx = a1x + wi * (dudz(i, z, c) - dwdx[i]);
+    o.cy = a1y + wi * (dvdz(i, z, c) - dwdy[i]);
     o.cz = T(0);
-    o.r = cc1 + wi * sz[i];
+    o.r = cc1 + wi * sz(i, z, c);
   }
   return o;
 }
@@ -247,33 +292,36 @@ template <class T> __device__ __forceinline__ T smag(T a, T b, T c, T d, T e, T 
 }
 
 // StrainsUVPnodes_NoDealias (src/subgridscale/StrainRates.py:149-211) at
-// point i: S12, S13, S23 and |S| (S11 = dudx, S22 = dvdy, S33 = dwdz).
+// point i (level z, column c): S12, S13, S23 and |S| (S11 = dudx, S22 =
+// dvdy, S33 = dwdz).  dudz, dvdz, dwdz: accessors (PtrAt or the on-the-fly
+// derivatives).
 template <class T> struct StrainOut { T s12, s13, s23, S; };
-template <class T>
-__device__ __forceinline__ StrainOut<T> strains_point(size_t i, int z, int nz,
+template <class T, class UZ, class VZ, class WZ>
+__device__ __forceinline__ StrainOut<T> strains_point(size_t i, int z, size_t c, int nz,
                                                       const T* __restrict__ dudx,
                                                       const T* __restrict__ dvdx,
                                                       const T* __restrict__ dwdx,
                                                       const T* __restrict__ dudy,
                                                       const T* __restrict__ dvdy,
                                                       const T* __restrict__ dwdy,
-                                                      const T* __restrict__ dudz,
-                                                      const T* __restrict__ dvdz,
-                                                      const T* __restrict__ dwdz) {
+                                                      const UZ& dudz, const VZ& dvdz,
+                                                      const WZ& dwdz) {
   T uz, vz, wx, wy;
   if (z == 0) {
-    uz = dudz[i]; vz = dvdz[i]; wx = T(0.5) * dwdx[i + 1]; wy = T(0.5) * dwdy[i + 1];
+    uz = dudz(i, z, c); vz = dvdz(i, z, c);
+    wx = T(0.5) * dwdx[i + 1]; wy = T(0.5) * dwdy[i + 1];
   } else if (z < nz - 1) {
-    uz = T(0.5) * (dudz[i] + dudz[i + 1]); vz = T(0.5) * (dvdz[i] + dvdz[i + 1]);
+    uz = T(0.5) * (dudz(i, z, c) + dudz(i + 1, z + 1, c));
+    vz = T(0.5) * (dvdz(i, z, c) + dvdz(i + 1, z + 1, c));
     wx = T(0.5) * (dwdx[i] + dwdx[i + 1]); wy = T(0.5) * (dwdy[i] + dwdy[i + 1]);
   } else {
-    uz = dudz[i]; vz = dvdz[i]; wx = dwdx[i]; wy = dwdy[i];
+    uz = dudz(i, z, c); vz = dvdz(i, z, c); wx = dwdx[i]; wy = dwdy[i];
   }
   StrainOut<T> o;
   o.s12 = T(0.5) * (dudy[i] + dvdx[i]);
   o.s13 = T(0.5) * (uz + wx);
   o.s23 = T(0.5) * (vz + wy);
-  o.S = smag<T>(dudx[i], dvdy[i], dwdz[i], o.s12, o.s13, o.s23);
+  o.S = smag<T>(dudx[i], dvdy[i], dwdz(i, z, c), o.s12, o.s13, o.s23);
   return o;
 }
 
@@ -287,32 +335,100 @@ __global__ void k_strains_uvp(PwDims d, const T* __restrict__ dudx, const T* __r
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
-    const StrainOut<T> o =
-        strains_point<T>(i, z, nz, dudx, dvdx, dwdx, dudy, dvdy, dwdy, dudz, dvdz, dwdz);
+    const StrainOut<T> o = strains_point<T>(i, z, 0, nz, dudx, dvdx, dwdx, dudy, dvdy, dwdy,
+                                            PtrAt<T>{dudz}, PtrAt<T>{dvdz}, PtrAt<T>{dwdz});
     S12[i] = o.s12; S13[i] = o.s13; S23[i] = o.s23; S[i] = o.S;
   }
 }
 
-// Advection + scalar advection (adv_point) and the uvp-node strain rates
-// (strains_point) in one pass on the unpadded grid: u, v, w are read once for
-// both advections, and the six velocity derivatives both parts read (dudy,
-// dvdx, dudz, dvdz, dwdx, dwdy) once for both.
-template <class T> struct AdvStrainArgs {
-  const T *u, *v, *w, *sx, *sy, *sz;
-  const T *dudx, *dvdx, *dwdx, *dudy, *dvdy, *dwdy, *dudz, *dvdz, *dwdz;
+// The vertical gradients (k_derivz), advection + scalar advection
+// (adv_point) and the uvp-node strain rates (strains_point) in one pass on
+// the unpadded grid.  du/dz, dv/dz, dw/dz and dTH/dz are formed from u, v,
+// w, TH where they are needed (the values at level z + 1 by the same
+// arithmetic as at z + 1 itself) and written once for the later stages;
+// u, v, w and the six velocity derivatives that both parts need are read
+// once.
+template <class T> struct DzAdvStrainArgs {
+  const T *u, *v, *w, *TH, *sx, *sy;
+  const T *dudx, *dvdx, *dwdx, *dudy, *dvdy, *dwdy;
+  const T *ustar, *M, *fim, *fih, *qz;   // surface planes (MOST wall values)
+  T Ugal, idz, half_dz_vonk;
+  T *dudz, *dvdz, *dwdz, *dTdz;
   T *Cx, *Cy, *Cz, *sadv, *S12, *S13, *S23, *S;
 };
 template <class T>
-__global__ void k_adv_strains(PwDims d, AdvStrainArgs<T> a) {
+__global__ void k_dz_adv_strains(PwDims d, DzAdvStrainArgs<T> a) {
   const int nz = d.nz;
+  const DzUV<T, true> uz{a.u, a.fim, a.ustar, a.M, a.Ugal, a.idz, a.half_dz_vonk, nz};
+  const DzUV<T, false> vz{a.v, a.fim, a.ustar, a.M, T(0), a.idz, a.half_dz_vonk, nz};
+  const DzW<T> wz{a.w, a.idz, nz};
+  const DzTH<T> tz{a.TH, a.fih, a.qz, a.ustar, a.idz, a.half_dz_vonk};
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
-    const AdvOut<T> c = adv_point<T>(i, z, nz, a.u, a.v, a.w, a.dudy, a.dudz, a.dvdx, a.dvdz,
-                                     a.dwdx, a.dwdy, a.sx, a.sy, a.sz);
-    const StrainOut<T> o = strains_point<T>(i, z, nz, a.dudx, a.dvdx, a.dwdx, a.dudy, a.dvdy,
-                                            a.dwdy, a.dudz, a.dvdz, a.dwdz);
-    a.Cx[i] = c.cx; a.Cy[i] = c.cy; a.Cz[i] = c.cz; a.sadv[i] = c.r;
-    a.S12[i] = o.s12; a.S13[i] = o.s13; a.S23[i] = o.s23; a.S[i] = o.S;
+    const size_t c = z == 0 ? i / nz : 0;   // the column is read only at level 0
+    const AdvOut<T> o = adv_point<T>(i, z, c, nz, a.u, a.v, a.w, a.dudy, uz, a.dvdx, vz, a.dwdx,
+                                     a.dwdy, a.sx, a.sy, tz);
+    const StrainOut<T> s = strains_point<T>(i, z, c, nz, a.dudx, a.dvdx, a.dwdx, a.dudy, a.dvdy,
+                                            a.dwdy, uz, vz, wz);
+    a.dudz[i] = uz(i, z, c); a.dvdz[i] = vz(i, z, c); a.dwdz[i] = wz(i, z, c);
+    a.dTdz[i] = tz(i, z, c);
+    a.Cx[i] = o.cx; a.Cy[i] = o.cy; a.Cz[i] = o.cz; a.sadv[i] = o.r;
+    a.S12[i] = s.s12; a.S13[i] = s.s13; a.S23[i] = s.s23; a.S[i] = s.S;
+  }
+}
+
+// k_dz_adv_strains with 16-byte accesses (nz % W == 0, 16-byte aligned
+// arrays): each thread takes W consecutive levels of one column.  The fields
+// read at neighbouring levels are held as windows of W + 2 values (index 0:
+// level z0 - 1, index W + 1: level z0 + W, read only inside the column), and
+// adv_point / strains_point run on the windows: the same arithmetic as the
+// scalar kernel.
+template <class T>
+__global__ void k_dz_adv_strains_v(PwDims d, DzAdvStrainArgs<T> a) {
+  constexpr int W = Vec16<T>::W;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    const size_t c = i0 / nz;
+    const bool lo = z0 > 0, hi = z0 + W < nz;
+    T u_[W + 2], v_[W + 2], w_[W + 2], th_[W + 2], wx_[W + 2], wy_[W + 2];
+    auto window = [&](const T* __restrict__ f, T* o) {
+      const Vec16<T> m = vld(f + i0);
+#pragma unroll
+      for (int j = 0; j < W; ++j) o[1 + j] = m.v[j];
+      o[0] = lo ? f[i0 - 1] : T(0);
+      o[W + 1] = hi ? f[i0 + W] : T(0);
+    };
+    window(a.u, u_); window(a.v, v_); window(a.w, w_); window(a.TH, th_);
+    window(a.dwdx, wx_); window(a.dwdy, wy_);
+    const Vec16<T> ux = vld(a.dudx + i0), vx = vld(a.dvdx + i0), uy = vld(a.dudy + i0),
+                   vy = vld(a.dvdy + i0), sx = vld(a.sx + i0), sy = vld(a.sy + i0);
+    // accessors on the windows: local index 1 + j is level z0 + j
+    const DzUV<T, true> uz{u_ + 1, a.fim + c, a.ustar + c, a.M + c, a.Ugal, a.idz,
+                           a.half_dz_vonk, nz};
+    const DzUV<T, false> vz{v_ + 1, a.fim + c, a.ustar + c, a.M + c, T(0), a.idz,
+                            a.half_dz_vonk, nz};
+    const DzW<T> wz{w_ + 1, a.idz, nz};
+    const DzTH<T> tz{th_ + 1, a.fih + c, a.qz + c, a.ustar + c, a.idz, a.half_dz_vonk};
+    Vec16<T> oUz, oVz, oWz, oTz, oCx, oCy, oCz, oR, o12, o13, o23, oS;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int z = z0 + j;
+      const AdvOut<T> o = adv_point<T>((size_t)j, z, 0, nz, u_ + 1, v_ + 1, w_ + 1, uy.v, uz,
+                                       vx.v, vz, wx_ + 1, wy_ + 1, sx.v, sy.v, tz);
+      const StrainOut<T> s = strains_point<T>((size_t)j, z, 0, nz, ux.v, vx.v, wx_ + 1, uy.v,
+                                              vy.v, wy_ + 1, uz, vz, wz);
+      oUz.v[j] = uz((size_t)j, z, 0); oVz.v[j] = vz((size_t)j, z, 0);
+      oWz.v[j] = wz((size_t)j, z, 0); oTz.v[j] = tz((size_t)j, z, 0);
+      oCx.v[j] = o.cx; oCy.v[j] = o.cy; oCz.v[j] = o.cz; oR.v[j] = o.r;
+      o12.v[j] = s.s12; o13.v[j] = s.s13; o23.v[j] = s.s23; oS.v[j] = s.S;
+    }
+    vst(a.dudz + i0, oUz); vst(a.dvdz + i0, oVz); vst(a.dwdz + i0, oWz); vst(a.dTdz + i0, oTz);
+    vst(a.Cx + i0, oCx); vst(a.Cy + i0, oCy); vst(a.Cz + i0, oCz); vst(a.sadv + i0, oR);
+    vst(a.S12 + i0, o12); vst(a.S13 + i0, o13); vst(a.S23 + i0, o23); vst(a.S + i0, oS);
   }
 }
 

@@ -588,7 +588,7 @@ template <class T> struct SgsStrains {
 
 // w_nodes = false: the unpadded w-node strains are left to k_sgs_fused.
 // uvp_done: S12, S13, S23 and |S| on uvp nodes are already in their
-// workspace buffers (k_adv_strains).
+// workspace buffers (k_dz_adv_strains).
 template <class T>
 void stage_strains(Ctx<T>& c, const T* const g[9], SgsStrains<T>& o, bool w_nodes = true,
                    bool uvp_done = false) {
@@ -861,15 +861,18 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     FLevel0<T> fq{qm, 1};
     plane_sums<1>(P, s, c.ncol, 1, fq, P->scal + 4, 1.0 / c.ncol, "qmavg");
   }
-  // 3. vertical gradients (Derivz_M, Derivz_TH)
-  if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, dudz, dvdz, dwdz, dTz}))
-    PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_derivz_v<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar,
-                sf.M, S.fim, S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz),
+  // 3. vertical gradients (Derivz_M, Derivz_TH); on the unpadded grid they
+  // are formed in the advection pass of step 4 (k_dz_adv_strains)
+  if (dealias) {
+    if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, dudz, dvdz, dwdz, dTz}))
+      PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_derivz_v<T>, c.dims(), S.u, S.v, S.w, S.TH,
+                  sf.ustar, sf.M, S.fim, S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal,
+                  (T)(1.0 / d.dz), (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
+    else
+      PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim,
+                S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz),
                 (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
-  else
-    PW_LAUNCH(P, s, c.F, k_derivz<T>, c.dims(), S.u, S.v, S.w, S.TH, sf.ustar, sf.M, S.fim,
-              S.fih, sf.qz, dudz, dvdz, dwdz, dTz, (T)g.Ugal, (T)(1.0 / d.dz),
-              (T)(0.4 * 0.5 * d.dz), (T)(0.4 * 0.5 * d.dz));
+  }
   if (moist) {   // Derivz_TH of Q with the moisture flux (Derivatives.py:304)
     if (vec_ok<T>(c.nz, {SQ, dQz}))
       PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_derivz_v<T>, c.dims(), S.u, S.v, S.w, SQ, sf.ustar,
@@ -893,14 +896,22 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   T *Cx = c.W("Cx"), *Cy = c.W("Cy"), *Cz = c.W("Cz"), *THadv = c.W("THadv");
   T* QAdv = moist ? c.W("QAdv") : nullptr;
   if (!dealias) {
-    // with the uvp-node strain rates of step 6 in the same pass
-    AdvStrainArgs<T> aa{};
-    aa.u = S.u; aa.v = S.v; aa.w = S.w; aa.sx = dTx; aa.sy = dTy; aa.sz = dTz;
+    // with the vertical gradients of step 3 and the uvp-node strain rates of
+    // step 6 in the same pass
+    DzAdvStrainArgs<T> aa{};
+    aa.u = S.u; aa.v = S.v; aa.w = S.w; aa.TH = S.TH; aa.sx = dTx; aa.sy = dTy;
     aa.dudx = dudx; aa.dvdx = dvdx; aa.dwdx = dwdx; aa.dudy = dudy; aa.dvdy = dvdy;
-    aa.dwdy = dwdy; aa.dudz = dudz; aa.dvdz = dvdz; aa.dwdz = dwdz;
+    aa.dwdy = dwdy;
+    aa.ustar = sf.ustar; aa.M = sf.M; aa.fim = S.fim; aa.fih = S.fih; aa.qz = sf.qz;
+    aa.Ugal = (T)g.Ugal; aa.idz = (T)(1.0 / d.dz); aa.half_dz_vonk = (T)(0.4 * 0.5 * d.dz);
+    aa.dudz = dudz; aa.dvdz = dvdz; aa.dwdz = dwdz; aa.dTdz = dTz;
     aa.Cx = Cx; aa.Cy = Cy; aa.Cz = Cz; aa.sadv = THadv;
     aa.S12 = c.W("S12"); aa.S13 = c.W("S13"); aa.S23 = c.W("S23"); aa.S = c.W("S");
-    PW_LAUNCH(P, s, c.F, k_adv_strains<T>, c.dims(), aa);
+    if (vec_ok<T>(c.nz, {S.u, S.v, S.w, S.TH, dTx, dTy, dudx, dvdx, dwdx, dudy, dvdy, dwdy,
+                         dudz, dvdz, dwdz, dTz, Cx, Cy, Cz, THadv, aa.S12, aa.S13, aa.S23, aa.S}))
+      PW_LAUNCH_W(P, s, c.F, Vec16<T>::W, k_dz_adv_strains_v<T>, c.dims(), aa);
+    else
+      PW_LAUNCH(P, s, c.F, k_dz_adv_strains<T>, c.dims(), aa);
     if (moist)   // ScalarAdvection of Q (Main.py:313-318)
       PW_LAUNCH(P, s, c.F, k_scalar_adv<T>, c.dims(), S.u, S.v, S.w, dQx, dQy, dQz, QAdv);
   } else {
