@@ -119,8 +119,8 @@ inline int pw_fields(const char* name) {
       {"k_derivz", 8}, {"k_derivz_generic", 2}, {"k_stag_avg", 2}, {"k_advection", 12}, {"k_dz_adv_strains", 24}, {"k_dz_adv_strains_v", 24}, {"k_sgs_fused", 25}, {"k_sgs_fused_v", 25}, {"k_ab2_q", 5}, {"k_derivz_v", 8}, {"k_rhs_v", 17},
       {"k_scalar_adv", 7}, {"k_buoyancy", 2}, {"k_strains_uvp", 13}, {"k_strains_w", 12},
       {"k_smag", 7}, {"k_stress_uvp", 10}, {"k_stress_w", 6}, {"k_scalar_flux", 9},
-      {"k_rhs", 17}, {"k_ab2_plain", 4}, {"k_imfilter", 2}, {"k_lasdd_lm", 24},
-      {"k_smag_level", 7}, {"k_scalar_updn", 16}, {"k_real_part", 3}, {"k_fill", 1}};
+      {"k_rhs", 17}, {"k_ab2_plain", 4}, {"k_imfilter", 2}, {"k_lasdd_lm", 24}, {"k_lasdd_lm_v", 24},
+      {"k_smag_level", 7}, {"k_scalar_updn", 16}, {"k_scalar_updn_v", 16}, {"k_real_part", 3}, {"k_fill", 1}};
   for (const auto& e : tab) {
     const size_t l = strlen(e.n);
     if (strncmp(name, e.n, l) == 0 && (name[l] == '<' || name[l] == 0)) return e.f;

@@ -177,27 +177,80 @@ template <class T> struct FLasddSums {
   }
 };
 
-// M_ij, LM, MM (DynamicSGS_LASDD_SM.py:327-355) and |S_hat| (:235).
+// M_ij, LM, MM (DynamicSGS_LASDD_SM.py:327-355) and |S_hat| (:235) at point
+// i; b = beta of the level.
+template <class T>
+__device__ __forceinline__ void lasdd_lm_point(const MomLevel<T>& l1, size_t i, double b,
+                                               double L, double TFR, double& lm, double& mm,
+                                               double& Sm) {
+  double Lij[6], Sij[6], SSij[6];
+  mom_load<T>(l1, i, Lij, Sij, SSij, Sm);
+  const double T1 = 2.0 * L * L, T2 = 2.0 * (TFR * L) * (TFR * L);
+  double M[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) M[n] = T1 * SSij[n] - T2 * b * Sm * Sij[n];
+  lm = (Lij[0] * M[0] + Lij[1] * M[1] + Lij[2] * M[2]) +
+       2.0 * (Lij[3] * M[3] + Lij[4] * M[4] + Lij[5] * M[5]);
+  mm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + 2.0 * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
+}
+
 template <class T>
 __global__ void k_lasdd_lm(PwDims d, MomLevel<T> l1, const double* __restrict__ beta, double L,
                            double TFR, T* LM, T* MM, T* Shat) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
-    double Lij[6], Sij[6], SSij[6], Sm;
-    mom_load<T>(l1, i, Lij, Sij, SSij, Sm);
-    const double T1 = 2.0 * L * L, T2 = 2.0 * (TFR * L) * (TFR * L);
-    const double b = beta[z];
-    double M[6];
-#pragma unroll
-    for (int n = 0; n < 6; ++n) M[n] = T1 * SSij[n] - T2 * b * Sm * Sij[n];
-    const double lm = (Lij[0] * M[0] + Lij[1] * M[1] + Lij[2] * M[2]) +
-                      2.0 * (Lij[3] * M[3] + Lij[4] * M[4] + Lij[5] * M[5]);
-    const double mm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] +
-                      2.0 * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
+    double lm, mm, Sm;
+    lasdd_lm_point<T>(l1, i, beta[z], L, TFR, lm, mm, Sm);
     LM[i] = (T)lm;
     MM[i] = (T)mm;
     if (Shat) Shat[i] = (T)Sm;
+  }
+}
+
+// Vector form (nz % 2 == 0, 16-byte aligned arrays): two consecutive levels
+// per thread, the 21 fields read as 2-element vectors into registers and the
+// point function run on them (same arithmetic).  Two, not 16 bytes' worth:
+// four floats of 21 fields would not fit the registers.
+template <class T>
+__global__ void k_lasdd_lm_v(PwDims d, MomLevel<T> l1, const double* __restrict__ beta,
+                             double L, double TFR, T* LM, T* MM, T* Shat) {
+  constexpr int W = 2;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    c2<T> f[21];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f[k] = *reinterpret_cast<const c2<T>*>(l1.uh[k] + i0);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      f[3 + n] = *reinterpret_cast<const c2<T>*>(l1.ph[n] + i0);
+      f[9 + n] = *reinterpret_cast<const c2<T>*>(l1.Sh[n] + i0);
+      f[15 + n] = *reinterpret_cast<const c2<T>*>(l1.SSh[n] + i0);
+    }
+    T v[21][W];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) { v[k][0] = f[k].x; v[k][1] = f[k].y; }
+    MomLevel<T> r;   // the level, on the registers
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.uh[k] = v[k];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      r.ph[n] = v[3 + n]; r.Sh[n] = v[9 + n]; r.SSh[n] = v[15 + n];
+    }
+    T oLM[W], oMM[W], oS[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      double lm, mm, Sm;
+      lasdd_lm_point<T>(r, (size_t)j, beta[z0 + j], L, TFR, lm, mm, Sm);
+      oLM[j] = (T)lm; oMM[j] = (T)mm; oS[j] = (T)Sm;
+    }
+    *reinterpret_cast<c2<T>*>(LM + i0) = mkc<T>(oLM[0], oLM[1]);
+    *reinterpret_cast<c2<T>*>(MM + i0) = mkc<T>(oMM[0], oMM[1]);
+    if (Shat) *reinterpret_cast<c2<T>*>(Shat + i0) = mkc<T>(oS[0], oS[1]);
   }
 }
 
@@ -268,19 +321,67 @@ template <class T> struct FScalarSums {
   }
 };
 
-// T_up, T_dn (DynamicSGS_ScalarLASDD_SM.py:304-312).
+// T_up, T_dn (DynamicSGS_ScalarLASDD_SM.py:304-312) at point i.
+template <class T>
+__device__ __forceinline__ void scalar_updn_point(const ScaLevel<T>& l1, size_t i, double b,
+                                                  double L, double TFR, T& up, T& dn) {
+  double t[5];
+  sca_terms<T>(l1, i, t);
+  const double L2 = L * L, L4 = L2 * L2, T2 = TFR * TFR, T4 = T2 * T2;
+  up = (T)(L2 * t[0] + (-L2 * T2) * t[1] * b);
+  dn = (T)(L4 * t[2] + (-2.0 * L4 * T2) * t[3] * b + (L4 * T4) * t[4] * b * b);
+}
+
 template <class T>
 __global__ void k_scalar_updn(PwDims d, ScaLevel<T> l1, const double* __restrict__ beta,
                               double L, double TFR, T* Tup, T* Tdn) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
-    double t[5];
-    sca_terms<T>(l1, i, t);
-    const double b = beta[z];
-    const double L2 = L * L, L4 = L2 * L2, T2 = TFR * TFR, T4 = T2 * T2;
-    Tup[i] = (T)(L2 * t[0] + (-L2 * T2) * t[1] * b);
-    Tdn[i] = (T)(L4 * t[2] + (-2.0 * L4 * T2) * t[3] * b + (L4 * T4) * t[4] * b * b);
+    scalar_updn_point<T>(l1, i, beta[z], L, TFR, Tup[i], Tdn[i]);
+  }
+}
+
+// Vector form (nz % 2 == 0, 16-byte aligned arrays): two consecutive levels
+// per thread, the 14 fields as 2-element vectors in registers and the point
+// function run on them (same arithmetic).
+template <class T>
+__global__ void k_scalar_updn_v(PwDims d, ScaLevel<T> l1, const double* __restrict__ beta,
+                                double L, double TFR, T* Tup, T* Tdn) {
+  constexpr int W = 2;
+  const int nz = d.nz;
+  const size_t nv = d.n / W;
+  auto ld2 = [](const T* p) { return *reinterpret_cast<const c2<T>*>(p); };
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = t * W;
+    const int z0 = (int)(i0 % nz);
+    c2<T> f[14];
+    f[0] = ld2(l1.th + i0);
+    f[1] = ld2(l1.Sm + i0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      f[2 + k] = ld2(l1.uth[k] + i0);
+      f[5 + k] = ld2(l1.dh[k] + i0);
+      f[8 + k] = ld2(l1.sdh[k] + i0);
+      f[11 + k] = ld2(l1.uh[k] + i0);
+    }
+    T v[14][W];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) { v[k][0] = f[k].x; v[k][1] = f[k].y; }
+    ScaLevel<T> r;   // the level, on the registers
+    r.th = v[0];
+    r.Sm = v[1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      r.uth[k] = v[2 + k]; r.dh[k] = v[5 + k]; r.sdh[k] = v[8 + k]; r.uh[k] = v[11 + k];
+    }
+    T up[W], dn[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      scalar_updn_point<T>(r, (size_t)j, beta[z0 + j], L, TFR, up[j], dn[j]);
+    *reinterpret_cast<c2<T>*>(Tup + i0) = mkc<T>(up[0], up[1]);
+    *reinterpret_cast<c2<T>*>(Tdn + i0) = mkc<T>(dn[0], dn[1]);
   }
 }
 
@@ -442,6 +543,7 @@ k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* _
       sa = num[jm] + num[j0] + num[jp];
       sb = den[jm] + den[j0] + den[jp];
     };
+#pragma unroll 2
     for (int c = c0; c < c1; ++c) {
       const int cx = c / ny, y = c - cx * ny;
       const int yp = (y + 1 == ny) ? 0 : y + 1;

@@ -360,7 +360,13 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
   T* LM = c.W("LM");
   T* MM = c.W("MM");
   T* Shat = c.W("Shat");
-  PW_LAUNCH(P, c.s, c.F, k_lasdd_lm<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
+  if (vec_ok<T>(c.nz, {l1.uh[0], l1.uh[1], l1.uh[2], l1.ph[0], l1.ph[1], l1.ph[2], l1.ph[3],
+                       l1.ph[4], l1.ph[5], l1.Sh[0], l1.Sh[1], l1.Sh[2], l1.Sh[3], l1.Sh[4],
+                       l1.Sh[5], l1.SSh[0], l1.SSh[1], l1.SSh[2], l1.SSh[3], l1.SSh[4],
+                       l1.SSh[5], LM, MM, Shat}))
+    PW_LAUNCH_W(P, c.s, c.F, 2, k_lasdd_lm_v<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
+  else
+    PW_LAUNCH(P, c.s, c.F, k_lasdd_lm<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
   if (lasdd) {
     T* Shatd = c.W("Shatd");
     PW_LAUNCH(P, c.s, c.F, k_smag_level<T>, c.dims(), l2, Shatd);
@@ -442,7 +448,12 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
   }
   T* Tup = c.W("Tup");
   T* Tdn = c.W("Tdn");
-  PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
+  if (vec_ok<T>(c.nz, {l1.th, l1.Sm, l1.uth[0], l1.uth[1], l1.uth[2], l1.dh[0], l1.dh[1],
+                       l1.dh[2], l1.sdh[0], l1.sdh[1], l1.sdh[2], l1.uh[0], l1.uh[1], l1.uh[2],
+                       Tup, Tdn}))
+    PW_LAUNCH_W(P, c.s, c.F, 2, k_scalar_updn_v<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
+  else
+    PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CsPr, s2, "sbox");
   if (!P->ws.dry) {
