@@ -25,6 +25,9 @@ CASES = {
     "stable32": ("stable", dict(nx=32, ny=32, nz=32)),                 # FGR=2, LAD, prescribed T
     "stable_static": ("stable", dict(nx=32, ny=32, nz=32, dynamicSGS_call_time=3)),
     # Wong-Lilly base model: LASDD-WL (beta), LAD-WL with dealiasing, static steps
+    # nz = 18 (not a multiple of 4): the float kernels take their scalar
+    # (one level per thread) forms
+    "convective_nz18": ("convective", dict(nx=32, ny=32, nz=18)),
     "convective_wl": ("convective", dict(nx=32, ny=32, nz=32, optSgs=2)),
     "neutral_wl": ("neutral", dict(optSgs=4)),
     "stable_wl_static": ("stable", dict(nx=32, ny=32, nz=32, optSgs=4, dynamicSGS_call_time=2)),
@@ -108,8 +111,8 @@ def test_step_fp64_100_neutral():
     assert max(errs) < 1e-11
 
 
-@pytest.mark.parametrize("key", ["neutral32", "stable32", "convective32", "convective_wl",
-                                 "convective64x96"])
+@pytest.mark.parametrize("key", ["neutral32", "stable32", "convective32", "convective_nz18",
+                                 "convective_wl", "convective64x96"])
 def test_step_mixed_profiles(key):
     s, les, errs = run_both(key, 20, "mixed")
     f = les.fields()
