@@ -60,6 +60,11 @@ constexpr int FFT_MAXRAD = 8;
 #ifndef ALFA_BATCH_LOADS
 #define ALFA_BATCH_LOADS 1
 #endif
+//   ALFA_C2R_PAIRS: the float y C2R's last pass stores two lines (four z
+//   levels, 16 bytes) per item (1) or one line (8 bytes) (0)
+#ifndef ALFA_C2R_PAIRS
+#define ALFA_C2R_PAIRS 1
+#endif
 //   ALFA_BAND: band-limited passes for filtered spectra (1) or full passes (0)
 #ifndef ALFA_BAND
 #define ALFA_BAND 1

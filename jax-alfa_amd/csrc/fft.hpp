@@ -846,6 +846,46 @@ __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC
   constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int R = last_radix(N), NS = N / R, TOT = NS * L, IT = (TOT + BS - 1) / BS;
   const T sc = (T)a.scale;
+  if constexpr (sizeof(T) == 4 && L % 2 == 0 && Storer::NOUT == 1 && ALFA_C2R_PAIRS) {
+    // float, plain stores: two adjacent lines (four z levels) per item, so
+    // each output row is one 16-byte store (and one 16-byte LDS read per
+    // input).  (The z-differencing storer keeps one line per item: its
+    // 16-byte form needs more registers than it saves.)
+    constexpr int LP = L / 2, TP = NS * LP, ITP = (TP + BS - 1) / BS;
+#pragma unroll
+    for (int q = 0; q < ITP; ++q) {
+      const int it = threadIdx.x + q * BS;
+      if (TP % BS == 0 || it < TP) {
+        const int j = it / LP, pp = it % LP, z = z0 + 4 * pp;
+        c2<T> v0[R], v1[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          c2<T> e0, e1;
+          cld2<T>(buf + (j + r * NS) * L + 2 * pp, e0, e1);
+          if (r > 0) {
+            c2<T> w = stw[j * r];
+            w.y = -w.y;
+            e0 = cmul<T>(e0, w);
+            e1 = cmul<T>(e1, w);
+          }
+          v0[r] = e0;
+          v1[r] = e1;
+        }
+        Dft<T, R>::run(v0, +1);
+        Dft<T, R>::run(v1, +1);
+        if (z < a.nz) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            Vec16<T> o;
+            o.v[0] = v0[r].x * sc; o.v[1] = v0[r].y * sc;
+            o.v[2] = v1[r].x * sc; o.v[3] = v1[r].y * sc;
+            st.vstore(b, x, j + r * NS, z, o);
+          }
+        }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int it = threadIdx.x + q * BS;
