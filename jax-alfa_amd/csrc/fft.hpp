@@ -846,11 +846,9 @@ __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC
   constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int R = last_radix(N), NS = N / R, TOT = NS * L, IT = (TOT + BS - 1) / BS;
   const T sc = (T)a.scale;
-  if constexpr (sizeof(T) == 4 && L % 2 == 0 && Storer::NOUT == 1 && ALFA_C2R_PAIRS) {
-    // float, plain stores: two adjacent lines (four z levels) per item, so
-    // each output row is one 16-byte store (and one 16-byte LDS read per
-    // input).  (The z-differencing storer keeps one line per item: its
-    // 16-byte form needs more registers than it saves.)
+  if constexpr (sizeof(T) == 4 && L % 2 == 0 && ALFA_C2R_PAIRS) {
+    // float: two adjacent lines (four z levels) per item, so each output row
+    // is one 16-byte store (and one 16-byte LDS read per input)
     constexpr int LP = L / 2, TP = NS * LP, ITP = (TP + BS - 1) / BS;
 #pragma unroll
     for (int q = 0; q < ITP; ++q) {
