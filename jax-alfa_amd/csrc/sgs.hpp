@@ -580,6 +580,100 @@ k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* _
   }
 }
 
+// k_box_clip with 16-byte accesses (nz % W == 0, 16-byte aligned arrays):
+// each thread takes W consecutive levels; the geometry is red_geom_v's
+// (zb levels per block, zb / W threads per column, red_cpt columns per
+// thread).  Same arithmetic per level; the partial sums of each level add
+// the column lanes in a fixed order.
+inline RedGeom red_geom_v(int ncol, int nz, size_t esize) {
+  RedGeom g = red_geom(ncol, nz, esize);
+  const int W = 16 / (int)esize;
+  g.lanes = RED_BS / (g.zb / W);
+  g.cpb = g.lanes * red_cpt(esize);
+  g.nbc = (ncol + g.cpb - 1) / g.cpb;
+  return g;
+}
+template <class T>
+__global__ void __launch_bounds__(RED_BS)
+k_box_clip_v(RedGeom g, const T* __restrict__ num, const T* __restrict__ den,
+             T* __restrict__ out, int nx, int ny, double* part) {
+  constexpr int W = Vec16<T>::W;
+  __shared__ double sm[2][RED_BS * W];
+  int bz, bc;
+  red_block(bz, bc);
+  const int tz = g.zb / W;
+  const int zl = threadIdx.x % tz, cl = threadIdx.x / tz;
+  const int z = bz * g.zb + zl * W;
+  const int nz = g.nz;
+  const int chunk = g.cpb / g.lanes;
+  double s1[W], s2[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) { s1[j] = 0.0; s2[j] = 0.0; }
+  if (z < nz) {
+    const int c0 = bc * g.cpb + cl * chunk;
+    const int c1 = imin(c0 + chunk, g.ncol);
+    const T inv9 = T(1.0 / 9.0);
+    Vec16<T> a0, a1, a2, b0, b1, b2;   // column sums at y-1, y, y+1
+    int xm = 0, x = -1, xp = 0;
+    auto col = [&](int yy, Vec16<T>& sa, Vec16<T>& sb) {
+      const size_t jm = ((size_t)xm * ny + yy) * nz + z;
+      const size_t j0 = ((size_t)x * ny + yy) * nz + z;
+      const size_t jp = ((size_t)xp * ny + yy) * nz + z;
+      const Vec16<T> nm = vld(num + jm), n0 = vld(num + j0), np = vld(num + jp);
+      const Vec16<T> dm = vld(den + jm), d0 = vld(den + j0), dp = vld(den + jp);
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        sa.v[j] = nm.v[j] + n0.v[j] + np.v[j];
+        sb.v[j] = dm.v[j] + d0.v[j] + dp.v[j];
+      }
+    };
+    for (int c = c0; c < c1; ++c) {
+      const int cx = c / ny, y = c - cx * ny;
+      const int yp = (y + 1 == ny) ? 0 : y + 1;
+      if (cx != x || c == c0) {
+        x = cx;
+        xm = (x + nx - 1) % nx;
+        xp = (x + 1) % nx;
+        col((y + ny - 1) % ny, a0, b0);
+        col(y, a1, b1);
+      } else {
+        a0 = a1; b0 = b1;
+        a1 = a2; b1 = b2;
+      }
+      col(yp, a2, b2);
+      Vec16<T> o;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const T a = (a0.v[j] + a1.v[j] + a2.v[j]) * inv9, b = (b0.v[j] + b1.v[j] + b2.v[j]) * inv9;
+        T r = a / b;
+        if ((fabs((double)b) < 1e-10) || (r < T(0)) || (r > T(1))) r = T(0);
+        o.v[j] = r;
+        // the root in the precision of the field (the sums are double)
+        s1[j] += (double)sqrt(r);
+        s2[j] += (double)r;
+      }
+      vst(out + (size_t)c * nz + z, o);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    sm[0][threadIdx.x * W + j] = s1[j];
+    sm[1][threadIdx.x * W + j] = s2[j];
+  }
+  __syncthreads();
+  if (cl == 0 && z < nz) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        double s = 0.0;
+        for (int l = 0; l < g.lanes; ++l) s += sm[t][(l * tz + zl) * W + j];
+        part[((size_t)bc * 2 + t) * nz + z + j] = s;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // beta: ComputeBeta1/2 (LASDD_SM.py:53-95, ScalarLASDD_SM.py:54-96) with the
 // Laguerre iteration of Roots (Utilities.py:123-241), in double complex.
