@@ -322,19 +322,17 @@ template <class T>
 void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T* num,
                    const T* den, T* out, double* sums, const char* tag) {
   const int ncol = nx * ny;
-  // 16-byte form where the levels and buffers allow it (the partials are
-  // sized for either geometry: the buffers' alignment is known at run time)
-  const RedGeom gs = red_geom(ncol, nz, esize(P)), gv = red_geom_v(ncol, nz, esize(P));
-  double* part = (double*)P->ws.get(std::string("red_") + tag,
-                                    (size_t)imax(gs.nbc, gv.nbc) * 2 * nz * sizeof(double));
+  RedGeom g = red_geom(ncol, nz, esize(P));
+  double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * 2 * nz * sizeof(double));
   if (P->ws.dry) return;
-  const bool vec = vec_ok<T>(nz, {num, den, out});
-  const RedGeom g = vec ? gv : gs;
   dim3 grid(g.nbz, g.nbc);
   {
     ProfScope ps(P, s, "k_box_clip", 3.0 * ncol * nz * esize(P));
-    if (vec) k_box_clip_v<T><<<grid, RED_BS, 0, s>>>(g, num, den, out, nx, ny, part);
-    else k_box_clip<T><<<grid, RED_BS, 0, s>>>(g, num, den, out, nx, ny, part);
+    // 16-byte form where the levels and buffers allow it (same partials)
+    if (vec_ok<T>(nz, {num, den, out}))
+      k_box_clip_v<T><<<grid, RED_BS / Vec16<T>::W, 0, s>>>(g, num, den, out, nx, ny, part);
+    else
+      k_box_clip<T><<<grid, RED_BS, 0, s>>>(g, num, den, out, nx, ny, part);
   }
   launch_ok(P);
   k_plane_final<<<(2 * nz + 3) / 4, 256, 0, s>>>(g.nbc, 2, nz, part, sums, 1.0);

@@ -581,24 +581,15 @@ k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* _
 }
 
 // k_box_clip with 16-byte accesses (nz % W == 0, 16-byte aligned arrays):
-// each thread takes W consecutive levels; the geometry is red_geom_v's
-// (zb levels per block, zb / W threads per column, red_cpt columns per
-// thread).  Same arithmetic per level; the partial sums of each level add
-// the column lanes in a fixed order.
-inline RedGeom red_geom_v(int ncol, int nz, size_t esize) {
-  RedGeom g = red_geom(ncol, nz, esize);
-  const int W = 16 / (int)esize;
-  g.lanes = RED_BS / (g.zb / W);
-  g.cpb = g.lanes * red_cpt(esize);
-  g.nbc = (ncol + g.cpb - 1) / g.cpb;
-  return g;
-}
+// launched with RED_BS / W threads on the geometry of red_geom, each thread
+// takes W consecutive levels of the columns that W threads of k_box_clip
+// take, in the same order, so the partial sums are those of k_box_clip.
 template <class T>
 __global__ void __launch_bounds__(RED_BS)
 k_box_clip_v(RedGeom g, const T* __restrict__ num, const T* __restrict__ den,
              T* __restrict__ out, int nx, int ny, double* part) {
   constexpr int W = Vec16<T>::W;
-  __shared__ double sm[2][RED_BS * W];
+  __shared__ double sm[2][RED_BS];
   int bz, bc;
   red_block(bz, bc);
   const int tz = g.zb / W;
