@@ -9,7 +9,7 @@ plain FP64 is timed beside it.  One JSON line goes to stdout:
   roofline    = the dominant kernel's algorithmic bytes / its HIP-event time,
                 measured on its launches inside the timed region
   cpu_baseline= the FP64 oracle (NumPy, one thread) on a bounded sample
-Diagnostics (per-kernel table) go to stderr and gpurun_out/bench_kernels.json.
+Diagnostics (per-kernel table) go to stderr, and to --kernels-json PATH if given.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
@@ -176,13 +176,15 @@ def main():
     ap.add_argument("--serve-steps", type=int, nargs=2, default=(100, 400), metavar=("MIN", "MAX"),
                     help="run-length range of served requests (the full configuration: 200 2000)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--kernels-json", default=None, metavar="PATH",
+                    help="write the per-kernel tables (HIP-event times, bytes) to PATH")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 3 times (a plain copy "
                          "kernel of known size, to scale HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the per-member step does not shard across GPUs in this "
-              "build (ensembles shard by member; see DESIGN.md). Run with --gpus 1.",
+              "build (ensembles shard by member). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     import torch
@@ -274,13 +276,12 @@ def main():
         "cpu_baseline": cpu,
         "reference_a100": A100_REF,
     }
-    try:
-        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-        with open(os.path.join(ROOT, "gpurun_out", "bench_kernels.json"), "w") as f:
+    if args.kernels_json:
+        d = os.path.dirname(os.path.abspath(args.kernels_json))
+        os.makedirs(d, exist_ok=True)
+        with open(args.kernels_json, "w") as f:
             json.dump(dict(mixed=mixed["table"], fp64=None if fp64 is None else fp64["table"],
                            mixed_live=mixed["live"]), f, indent=1)
-    except OSError:
-        pass
     print(json.dumps(out), flush=True)
 
 
