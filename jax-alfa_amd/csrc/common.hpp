@@ -54,6 +54,11 @@ constexpr int FFT_MAXRAD = 8;
 #ifndef ALFA_X_BS
 #define ALFA_X_BS 256
 #endif
+//   ALFA_X_BS_F64: the same for the double kernel (measured: 512 is 7% faster
+//   for double, 256 is 8% faster for float)
+#ifndef ALFA_X_BS_F64
+#define ALFA_X_BS_F64 512
+#endif
 //   ALFA_BATCH_LOADS: the tile loads of the compile-time-length single-op
 //   float x kernel are all issued before the first LDS write (1), or one loop
 //   iteration at a time (0)

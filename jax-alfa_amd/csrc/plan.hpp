@@ -200,7 +200,8 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
   const int tier = imax(fin.n, fout.n);
   auto go = [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
-    constexpr int BS = (NCT > 0 && !MULTI) ? ALFA_X_BS : FFT_BS;
+    constexpr int BS =
+        (NCT > 0 && !MULTI) ? (sizeof(T) == 8 ? ALFA_X_BS_F64 : ALFA_X_BS) : FFT_BS;
     constexpr int L = Lines<T>::L * BS / FFT_BS;
     TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
     const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
