@@ -860,9 +860,10 @@ def LASDD_SM(g, u, v, w, S11, S22, S33, S12, S13, S23, S):
     """src/subgridscale/DynamicSGS_LASDD_SM.py:138-367 (optSgs 1 = LASDD,
     3 = LAD with beta = 1).
 
-    Parity unpinned: the plane-mean sums a1..e2 and the quintic coefficient
-    algebra aa..ff (:314-319) have no independent check; their parts (the
-    filters, the Leonard terms, the root solve) are pinned separately."""
+    The plane-mean sums a1..e2 and the quintic aa..ff (:314-319) are pinned
+    against the two-level condition they encode, formed from M_ij(beta)
+    directly (tests/test_oracle_beta_algebra.py); "parts" returns the
+    filtered tensors for that check."""
     nz = g.nz
     L, TFR = g.L, g.TFR
     u_ = u.copy(); v_ = v.copy()
@@ -926,14 +927,15 @@ def LASDD_SM(g, u, v, w, S11, S22, S33, S12, S13, S23, S):
     Cs2_1D_avg2 = PlanarMean(Cs2_3D)
     return dict(u_=u_, v_=v_, w_=w_, u_hat=uh, u_hatd=ud, S_hat=S_hat, S_hatd=S_hatd,
                 Cs2_3D=Cs2_3D, Cs2_1D_avg1=Cs2_1D_avg1, Cs2_1D_avg2=Cs2_1D_avg2,
-                beta1_1D=beta1, LM=LM, MM=MM, beta_coeffs=coeffs)
+                beta1_1D=beta1, LM=LM, MM=MM, beta_coeffs=coeffs,
+                parts=dict(Lij=Lij, Qij=Qij, Sh=Sh, Sd=Sd, SSh=SSh, SSd=SSd))
 
 
 def ScalarLASDD_SM(g, mom, TH, dTHdx, dTHdy, dTHdz, S):
     """src/subgridscale/DynamicSGS_ScalarLASDD_SM.py:139-316.
 
-    Parity unpinned: the scalar sums and quintic coefficients (as in
-    LASDD_SM) have no independent check."""
+    The scalar sums and quintic coefficients are pinned as in LASDD_SM
+    (tests/test_oracle_beta_algebra.py)."""
     nz = g.nz
     L, TFR = g.L, g.TFR
     u_, v_, w_ = mom["u_"], mom["v_"], mom["w_"]
@@ -991,8 +993,11 @@ def ScalarLASDD_SM(g, mom, TH, dTHdx, dTHdy, dTHdz, S):
     T_dn = (L ** 4 * c2t + (-2 * L ** 4 * TFR ** 2) * d2t * b3
             + (L ** 4 * TFR ** 4) * e2t * b3 ** 2)
     CsPr = Cs2_3D_from_LM_MM(T_up, T_dn)
+    parts = dict(LT=LT, dh=dh, SdTh=SdTh)
+    if g.optSgs in (1, 2):
+        parts.update(QT=QT, dd=dd_, SdTd=SdTd)
     return dict(Cs2PrRatio_3D=CsPr, Cs2PrRatio_1D=PlanarMean(CsPr), beta2_1D=beta2,
-                T_up=T_up, T_dn=T_dn, beta_coeffs=coeffs)
+                T_up=T_up, T_dn=T_dn, beta_coeffs=coeffs, parts=parts)
 
 
 def _wl_quintic(g, a1, a3, a6, a8):
@@ -1000,7 +1005,9 @@ def _wl_quintic(g, a1, a3, a6, a8):
     DynamicSGS_ScalarLASDD_WL.py:224-243): four independent plane means,
     the other six scaled by powers of TFR.  Returns (ff, ee, dd, cc, bb, aa).
 
-    Parity unpinned: no independent check of this algebra."""
+    The expansion is checked against its product form; parity unpinned for
+    the TFR-power scalings of the six dependent means, which follow the
+    reference with no independent check."""
     T = g.TFR
     a2 = -(T ** (8 / 3)) * a1
     a4 = -2 * T ** (4 / 3) * a3
