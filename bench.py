@@ -113,6 +113,9 @@ def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None):
     log(f"[bench] {prec} {n}^3 plan ready, workspace {P.derived['workspace_bytes'] / 2**30:.2f} GiB")
     les.step(warmup)
     torch.cuda.synchronize()
+    # FFT tile choices: timed on steps from a copy of the warmed-up state
+    tiles = P.autotune(les)
+    log(f"[bench] {prec} FFT tiles (autotuned on steps): {tiles}")
     table = None
     if find_dominant:
         P.profile_begin(None, 16384)
@@ -135,7 +138,8 @@ def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None):
     torch.cuda.empty_cache()
     ms = (t1 - t0) * 1e3 / steps
     return dict(ms=ms, mpts=(n ** 3) / (ms * 1e-3) / 1e6, table=table, dominant=dominant,
-                live=live, cfl=cfl, stats=stats, total_steps=warmup + (2 if find_dominant else 0) + steps)
+                live=live, cfl=cfl, stats=stats, tiles=tiles,
+                total_steps=warmup + (2 if find_dominant else 0) + steps)
 
 
 def cpu_baseline(budget_s=15.0, n=64):
@@ -267,6 +271,9 @@ def main():
             "value": round(fp64["mpts"], 2), "ms_per_step": round(fp64["ms"], 4),
             "hbm_frac_step": round(fp64["mpts"] * 1e6 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9), 4)},
         "mixed_over_fp64": None if fp64 is None else round(mixed["mpts"] / fp64["mpts"], 3),
+        # threads per workgroup of the y R2C, x and y C2R kernels, as the
+        # plan's autotuner chose them after the warmup
+        "fft_tiles": {"mixed": mixed["tiles"], "fp64": None if fp64 is None else fp64["tiles"]},
         # the FP64 GPU path is held to the FP64 oracle at <= 1e-11 by the parity
         # tests; at 256^3 it stands in for the oracle, which would take hours
         "error_mixed_vs_fp64": errors,

@@ -127,6 +127,27 @@ int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan** out);
 int alfa_plan_destroy(alfa_plan* plan);
 int alfa_plan_derived(const alfa_plan* plan, alfa_derived* out);
 const char* alfa_last_error(void);
+/* Tile choices of the compile-time-length FFT kernels (transform lengths
+ * 64, 96, 128, 192, 256): threads per workgroup of the y R2C, x and y C2R
+ * kernels, in that order, each 256 or 512.  The choice changes timing only:
+ * every line is transformed by the same operations, so results are bitwise
+ * equal.  No counterpart in the reference (jnp.fft.rfft2/irfft2 leave such
+ * choices to XLA's cuFFT plans); they exist for callers that reuse a plan
+ * for many steps (ensembles, serving).
+ *   alfa_plan_autotune: times each kernel's two choices, one kernel at a
+ *   time, on `stream`, keeps the faster and writes the kept choices to
+ *   `chosen` (nullable).  With a state `st`, it times whole steps
+ *   (iterations `iteration`, `iteration`+1, ..., surface value `sfc_val`)
+ *   on a scratch copy of it: the caller's state is not modified.  With st
+ *   NULL it times a forward transform and three inverses (an x derivative,
+ *   the two test-filter levels) of a zero field.  Allocates and frees its
+ *   scratch (a copy of the state, or three fields); synchronizes `stream`.
+ *   alfa_plan_set_tiles: sets them (0 keeps a choice; else 256 or 512).
+ *   alfa_plan_get_tiles: reads them. */
+int alfa_plan_autotune(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_val,
+                       alfa_stream stream, int chosen[3]);
+int alfa_plan_set_tiles(alfa_plan* plan, const int tiles[3]);
+int alfa_plan_get_tiles(const alfa_plan* plan, int tiles[3]);
 /* Initial state values of Main.py:87-111 (RHS_prev = 0, Cs2 fields =
  * Cs2/Cs2PrRatio, psi = 0, fi = 1) written into caller buffers. */
 int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);

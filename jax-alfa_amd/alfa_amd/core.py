@@ -56,6 +56,44 @@ class Plan:
     def handle(self):
         return self._p
 
+    # -- tile choices of the FFT kernels -----------------------------------
+    TILE_KERNELS = ("y_r2c", "x", "y_c2r")
+
+    def tiles(self):
+        """Threads per workgroup of the compile-time-length y R2C, x and y
+        C2R kernels (256 or 512 each)."""
+        t = (C.c_int * 3)()
+        _lib.check(self.lib.alfa_plan_get_tiles(self._p, t), "alfa_plan_get_tiles")
+        return dict(zip(self.TILE_KERNELS, list(t)))
+
+    def set_tiles(self, y_r2c=0, x=0, y_c2r=0):
+        """Set tile choices (0 keeps one); results do not depend on them."""
+        t = (C.c_int * 3)(int(y_r2c), int(x), int(y_c2r))
+        _lib.check(self.lib.alfa_plan_set_tiles(self._p, t), "alfa_plan_set_tiles")
+        return self.tiles()
+
+    def autotune(self, les=None, stream=None):
+        """Time each FFT kernel at both tile choices on this device and keep
+        the faster (alfa_plan_autotune); returns the kept choices.  With an
+        LES of this plan, whole steps from its current state are timed on a
+        scratch copy (the run itself does not advance); without one, a
+        forward transform and three inverses of a zero field."""
+        torch = _torch()
+        with torch.cuda.device(self.device):
+            s = stream if stream is not None else torch.cuda.current_stream(self.device)
+            t = (C.c_int * 3)()
+            st, it, sv = None, 1, 0.0
+            if les is not None:
+                if les.qsfc is not None:   # as LES.step sets it for this iteration
+                    les.state.Q_sfc_val = les.qsfc[les.iteration - 1].data_ptr()
+                st, it = C.byref(les.state), int(les.iteration)
+                if les.sfc_series is not None:
+                    sv = float(les.sfc_series[les.iteration - 1])
+            _lib.check(self.lib.alfa_plan_autotune(self._p, st, it, C.c_double(sv),
+                                                   C.c_void_p(s.cuda_stream), t),
+                       "alfa_plan_autotune")
+        return dict(zip(self.TILE_KERNELS, list(t)))
+
     def close(self):
         if getattr(self, "_p", None) is not None and self._p.value:
             self.lib.alfa_plan_destroy(self._p)

@@ -103,6 +103,20 @@ def serving_requests(nreq=40, rate=2.0, sizes=(32, 64, 128, 192), steps=(200, 20
     return out
 
 
+# FFT tile choices per (device, precision, grid, filter ratio, SGS model),
+# autotuned on steps of the first member of that kind and reused for the rest
+_TILES = {}
+
+
+def _tuned_tiles(m):
+    P, cfg = m.plan, m.cfg
+    key = (str(P.device), P.precision, m.spec.n, cfg.get("FGR"), cfg.get("optSgs"))
+    if key in _TILES:
+        return P.set_tiles(**_TILES[key])
+    _TILES[key] = P.autotune(m.les, stream=m.stream)
+    return _TILES[key]
+
+
 def state_bytes(n, precision):
     """Caller-owned state of one member: 10 fields + 6 planes + profiles."""
     es = 8 if precision == "fp64" else 4
@@ -112,7 +126,7 @@ def state_bytes(n, precision):
 class Member:
     """One run on the device: plan, state, stream and captured step graphs."""
 
-    def __init__(self, spec: MemberSpec, precision="mixed", stream=None, graphs=True):
+    def __init__(self, spec: MemberSpec, precision="mixed", stream=None, graphs=True, tune=True):
         torch = core._torch()
         self.spec = spec
         cfg, u, v, w, TH, ser = spec.build()
@@ -122,6 +136,8 @@ class Member:
             self.plan = core.Plan(cfg, precision)
             self.les = core.LES(self.plan, u, v, w, TH, sfc_series=ser)
             self.sfc = torch.zeros(1, dtype=torch.float64, device=self.plan.device)
+            if tune:
+                self.tiles = _tuned_tiles(self)
         self.bytes = int(self.plan.derived["workspace_bytes"]) + state_bytes(spec.n, precision)
         self.done = 0
         self.use_graphs = graphs
@@ -247,8 +263,10 @@ class Server:
                 if active and used + need > self.budget:
                     break
                 pending.popleft()
+                # no tile autotuning at admission: its timed steps hold the
+                # request loop (and every queued request) for their duration
                 m = Member(requests[i], self.precision, self.streams[k % len(self.streams)],
-                           self.graphs)
+                           self.graphs, tune=False)
                 k += 1
                 m.t_admit = time.perf_counter() - t0
                 used += m.bytes

@@ -596,6 +596,9 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
   P->cfg = *cfg;
   if (P->cfg.istep == 0) P->cfg.istep = 1;
   P->prec = precision;
+  P->tile_x = precision == ALFA_FP64 ? ALFA_X_BS_F64 : ALFA_X_BS;
+  P->tile_c2r = ALFA_C2R_BS;
+  P->tile_r2c = ALFA_R2C_BS;
   P->nx = nx; P->ny = ny; P->nz = nz; P->nyc = ny / 2 + 1;
   P->mx = (int)(1.5 * nx); P->my = (int)(1.5 * ny); P->myc = P->my / 2 + 1;
   alfa_derived& d = P->d;
@@ -784,6 +787,164 @@ extern "C" int alfa_step_sfc_dev(alfa_plan* P, const alfa_state* st, int iterati
   if (int rc = check_moist_state(P, st, "alfa_step_sfc_dev")) return rc;
   DISPATCH(P, "alfa_step_sfc_dev",
            step_t<T>(P, st, iteration, 0.0, STREAM(stream), sfc_val));
+}
+
+// Tile choices of the compile-time-length FFT kernels: threads per workgroup
+// of the y R2C, x and y C2R kernels (256 or 512; lines per tile scale with
+// them).  Each line is transformed by the same sequence of operations at
+// either size, so the choice changes timing only, never the results.
+static bool tile_ok(int t) { return t == 256 || t == 512; }
+
+extern "C" int alfa_plan_set_tiles(alfa_plan* P, const int tiles[3]) {
+  if (!P || !tiles) return set_error(ALFA_EINVAL, "alfa_plan_set_tiles: null argument");
+  for (int i = 0; i < 3; ++i)
+    if (tiles[i] != 0 && !tile_ok(tiles[i]))
+      return set_error(ALFA_EINVAL, "alfa_plan_set_tiles: tile %d is %d (0, 256 or 512)", i,
+                       tiles[i]);
+  if (tiles[0]) P->tile_r2c = tiles[0];
+  if (tiles[1]) P->tile_x = tiles[1];
+  if (tiles[2]) P->tile_c2r = tiles[2];
+  return ALFA_OK;
+}
+
+extern "C" int alfa_plan_get_tiles(const alfa_plan* P, int tiles[3]) {
+  if (!P || !tiles) return set_error(ALFA_EINVAL, "alfa_plan_get_tiles: null argument");
+  tiles[0] = P->tile_r2c;
+  tiles[1] = P->tile_x;
+  tiles[2] = P->tile_c2r;
+  return ALFA_OK;
+}
+
+namespace {
+// What the autotuner times: whole steps on a scratch copy of a state when
+// the caller gives one, else one forward transform and three inverses (x
+// derivative over all modes, the two test-filter levels with their band
+// cut-offs) of a zero field.
+template <class T>
+struct TuneWork {
+  alfa_plan* P;
+  hipStream_t s;
+  const alfa_state* st = nullptr;   // scratch copy, or null
+  int iteration = 1, reps = 3;
+  double sfc_val = 0.0;
+  void *F = nullptr, *Ffft = nullptr, *out = nullptr;
+
+  int once(int r) {
+    if (st) return step_t<T>(P, st, iteration + r, sfc_val, s);
+    ops::fft<T>(P, s, F, Ffft);
+    ops::derivxy<T>(P, s, Ffft, 0, out);
+    ops::filtering_level<T>(P, s, Ffft, out, 1);
+    ops::filtering_level<T>(P, s, Ffft, out, 2);
+    return 0;
+  }
+  // reps after one untimed call (the first launch of a tile choice), ms
+  int time(hipEvent_t e0, hipEvent_t e1, float& ms) {
+    if (int rc = once(0)) return rc;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r)
+      if (int rc = once(r)) return rc;
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    return hipEventElapsedTime(&ms, e0, e1) == hipSuccess ? 0 : ALFA_EHIP;
+  }
+};
+
+template <class T>
+int autotune_tiles(alfa_plan* P, hipStream_t s, const alfa_state* st, int iteration,
+                   double sfc_val) {
+  bool ct_x = false, ct_y = false;
+  ct_length(P->nx, [&](auto nct, auto) { ct_x = decltype(nct)::value > 0; });
+  ct_length(P->ny, [&](auto nct, auto) { ct_y = decltype(nct)::value > 0; });
+  if (!ct_x && !ct_y) return ALFA_OK;   // generic kernels only: nothing to choose
+  const size_t nf = (size_t)P->nx * P->ny * P->nz * sizeof(T);
+  const size_t npl = (size_t)P->nx * P->ny * sizeof(T);
+  std::vector<void*> bufs;
+  hipError_t e = hipSuccess;
+  auto alloc = [&](size_t n) -> void* {
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&p, n);
+    if (e == hipSuccess) bufs.push_back(p);
+    return e == hipSuccess ? p : nullptr;
+  };
+  TuneWork<T> w{P, s};
+  alfa_state cp{};
+  if (st) {
+    // the step updates its state in place: time it on a copy
+    cp = *st;
+    cp.Cs2_1D_avg1 = cp.Cs2_1D_avg2 = cp.beta1_1D = cp.Cs2PrRatio_1D = cp.beta2_1D = nullptr;
+    cp.scalars = nullptr;
+    void** vol[] = {&cp.u, &cp.v, &cp.w, &cp.TH, &cp.RHS_u, &cp.RHS_v, &cp.RHS_w, &cp.RHS_TH,
+                    &cp.Cs2_3D, &cp.Cs2PrRatio_3D, &cp.Q, &cp.RHS_Q};
+    void** pl[] = {&cp.psi_m, &cp.psi_m0, &cp.psi_h, &cp.psi_h0, &cp.fi_m, &cp.fi_h};
+    const int nvol = P->cfg.optMoisture >= 1 ? 12 : 10;
+    for (int i = 0; i < nvol; ++i) {
+      void* src = *vol[i];
+      *vol[i] = alloc(nf);
+      if (e == hipSuccess) e = hipMemcpyAsync(*vol[i], src, nf, hipMemcpyDeviceToDevice, s);
+    }
+    for (void** f : pl) {
+      void* src = *f;
+      *f = alloc(npl);
+      if (e == hipSuccess) e = hipMemcpyAsync(*f, src, npl, hipMemcpyDeviceToDevice, s);
+    }
+    w.st = &cp;
+    w.iteration = iteration;
+    w.sfc_val = sfc_val;
+    // cover a whole cycle of dynamic and static coefficient steps
+    w.reps = std::min(8, std::max(3, P->cfg.dynamicSGS_call_time));
+  } else {
+    const size_t ns = (size_t)P->nx * P->nyc * P->nz * sizeof(c2<T>);
+    w.F = alloc(nf);
+    w.Ffft = alloc(ns);
+    w.out = alloc(nf);
+    if (e == hipSuccess) e = hipMemsetAsync(w.F, 0, nf, s);
+    w.reps = 5;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int rc = ALFA_OK;
+  if (e == hipSuccess) {
+    // one knob at a time (each moves one kernel), in pipeline order; each
+    // choice timed twice, interleaved, and the sums compared
+    int* knob[3] = {ct_y ? &P->tile_r2c : nullptr, ct_x ? &P->tile_x : nullptr,
+                    ct_y ? &P->tile_c2r : nullptr};
+    for (int k = 0; k < 3 && rc == ALFA_OK; ++k) {
+      if (!knob[k]) continue;
+      const int keep = *knob[k];
+      float t[2] = {0.f, 0.f};
+      for (int trial = 0; trial < 2 && rc == ALFA_OK; ++trial)
+        for (int c = 0; c < 2 && rc == ALFA_OK; ++c) {
+          *knob[k] = c ? 512 : 256;
+          float ms = 0.f;
+          rc = w.time(e0, e1, ms);
+          t[c] += ms;
+        }
+      *knob[k] = rc == ALFA_OK ? (t[1] < t[0] ? 512 : 256) : keep;
+    }
+    e = hipStreamSynchronize(s);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* p : bufs) (void)hipFree(p);
+  if (rc != ALFA_OK) return rc;
+  if (e != hipSuccess)
+    return set_error(ALFA_EHIP, "alfa_plan_autotune: %s", hipGetErrorString(e));
+  return ALFA_OK;
+}
+}  // namespace
+
+extern "C" int alfa_plan_autotune(alfa_plan* P, const alfa_state* st, int iteration,
+                                  double sfc_val, alfa_stream stream, int chosen[3]) {
+  if (!P) return set_error(ALFA_EINVAL, "alfa_plan_autotune: null plan");
+  if (st)
+    if (int rc = check_moist_state(P, st, "alfa_plan_autotune")) return rc;
+  hipStream_t s = STREAM(stream);
+  const int rc = P->prec == ALFA_FP64 ? autotune_tiles<double>(P, s, st, iteration, sfc_val)
+                                      : autotune_tiles<float>(P, s, st, iteration, sfc_val);
+  if (rc) return rc;
+  if (chosen) alfa_plan_get_tiles(P, chosen);
+  return check_hip(P, "alfa_plan_autotune");
 }
 
 extern "C" int alfa_struct_sizes(size_t out[3]) {

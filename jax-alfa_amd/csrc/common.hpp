@@ -42,9 +42,11 @@ constexpr int FFT_MAXRAD = 8;
 #ifndef ALFA_RADIX_MAX
 #define ALFA_RADIX_MAX 8
 #endif
-//   ALFA_C2R_BS, ALFA_R2C_BS, ALFA_X_BS: threads of the compile-time-length
-//   y C2R, y R2C and single-op x kernels (256 or 512; the lines per tile
-//   scale with them, 512 threads give 128-byte tile rows)
+//   ALFA_C2R_BS, ALFA_R2C_BS, ALFA_X_BS: a new plan's threads per workgroup
+//   of the compile-time-length y C2R, y R2C and single-op x kernels (256 or
+//   512; the lines per tile scale with them, 512 threads give 128-byte tile
+//   rows).  Both sizes are built; alfa_plan_autotune / alfa_plan_set_tiles
+//   change a plan's choice.
 #ifndef ALFA_C2R_BS
 #define ALFA_C2R_BS 512
 #endif

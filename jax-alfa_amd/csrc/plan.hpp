@@ -70,6 +70,10 @@ struct alfa_plan {
   std::vector<void*> allocs;
   hipError_t herr = hipSuccess;
   int ncu = 256;   // compute units of the plan's device
+  // threads per workgroup of the compile-time-length FFT kernels (256 or
+  // 512; the lines per tile scale with them): the build's defaults, or the
+  // choices of alfa_plan_autotune / alfa_plan_set_tiles
+  int tile_x = 256, tile_c2r = 512, tile_r2c = 256;
 };
 
 // Grid of the tiled FFT kernels: one workgroup per tile, rounded up to whole
@@ -172,24 +176,62 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   for (int b = 0; b < nb; ++b) vec = vec && al16(outs[b]);
   ct_length(fyy.n, [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
-    constexpr int BS = NCT > 0 ? ALFA_R2C_BS : FFT_BS;
-    constexpr int L = Lines<T>::L * BS / FFT_BS;
-    TileGrid tg{(nz + 2 * L - 1) / (2 * L), rows, nb, Loader::BATCH_FAST};
-    const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
-    const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
-    // pruned output within the stride of the last radix pass: that pass is
-    // evaluated only at the outputs the store reads
-    if constexpr (NCT > 0) {
-      if (ALFA_BAND && ncy <= NCT / last_radix(NCT)) {
-        k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS, true>
-            <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
-        return;
+    auto launch = [&](auto bs) {
+      constexpr int BS = decltype(bs)::value;
+      constexpr int L = Lines<T>::L * BS / FFT_BS;
+      TileGrid tg{(nz + 2 * L - 1) / (2 * L), rows, nb, Loader::BATCH_FAST};
+      const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
+      const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+      // pruned output within the stride of the last radix pass: that pass is
+      // evaluated only at the outputs the store reads
+      if constexpr (NCT > 0) {
+        if (ALFA_BAND && ncy <= NCT / last_radix(NCT)) {
+          k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS, true>
+              <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
+          return;
+        }
       }
+      k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>
+          <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
+    };
+    if constexpr (NCT > 0) {
+      if (P->tile_r2c == 512) launch(std::integral_constant<int, 512>{});
+      else launch(std::integral_constant<int, 256>{});
+    } else {
+      launch(std::integral_constant<int, FFT_BS>{});
     }
-    k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>
-        <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
   });
   launch_ok(P);
+}
+
+// one x launch at BS threads (NCT, MAXN: the plan's length constants)
+template <class T, bool MULTI, int BS, class NCTT, class MAXNT>
+void launch_kx_bs(hipStream_t s, const XArgs& a, int rows, int nb, const Fft1D& fin,
+                  const Fft1D& fout, const void* twin, const void* twout, NCTT, MAXNT) {
+  constexpr int NCT = NCTT::value;
+  using maxn_t = MAXNT;
+  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
+  const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
+  const int nbuf = imax(fin.n, fout.n);
+  const size_t lds =
+      (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
+  // one masked op between the forward and inverse transforms, with the
+  // kept modes within twice the stride of the radix-8 first pass and within
+  // the stride of the last pass: band passes (float only: the double
+  // kernel needs 14 more VGPRs for them and loses a wave per SIMD)
+  if constexpr (NCT > 0 && !MULTI && sizeof(T) == 4) {
+    const XOp& o = a.op[0];
+    if (ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops == 1 && o.inverse &&
+        o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 4 &&
+        o.cx <= NCT / last_radix(NCT)) {
+      k_x<T, maxn_t::value, MULTI, NCT, BS, true>
+          <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+      return;
+    }
+  }
+  k_x<T, maxn_t::value, MULTI, NCT, BS>
+      <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
 }
 
 template <class T, bool MULTI>
@@ -200,30 +242,14 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
   const int tier = imax(fin.n, fout.n);
   auto go = [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
-    constexpr int BS =
-        (NCT > 0 && !MULTI) ? (sizeof(T) == 8 ? ALFA_X_BS_F64 : ALFA_X_BS) : FFT_BS;
-    constexpr int L = Lines<T>::L * BS / FFT_BS;
-    TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
-    const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
-    const int nbuf = imax(fin.n, fout.n);
-    const size_t lds =
-        (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
-    // one masked op between the forward and inverse transforms, with the
-    // kept modes within twice the stride of the radix-8 first pass and within
-    // the stride of the last pass: band passes (float only: the double
-    // kernel needs 14 more VGPRs for them and loses a wave per SIMD)
-    if constexpr (NCT > 0 && !MULTI && sizeof(T) == 4) {
-      const XOp& o = a.op[0];
-      if (ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops == 1 && o.inverse &&
-          o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 4 &&
-          o.cx <= NCT / last_radix(NCT)) {
-        k_x<T, decltype(maxn)::value, MULTI, NCT, BS, true>
-            <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
-        return;
-      }
+    if constexpr (NCT > 0 && !MULTI) {
+      if (P->tile_x == 512)
+        launch_kx_bs<T, MULTI, 512>(s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
+      else
+        launch_kx_bs<T, MULTI, 256>(s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
+    } else {
+      launch_kx_bs<T, MULTI, FFT_BS>(s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
     }
-    k_x<T, decltype(maxn)::value, MULTI, NCT, BS>
-        <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
   };
   if (n > 0) ct_length(n, go);
   else ct_length(tier <= 256 ? 1 : 1000, go);   // generic kernel of the tier
@@ -268,33 +294,41 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
                    (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T));
   ct_length(fyy.n, [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
-    // compile-time lengths: ALFA_C2R_BS threads and lines in proportion
-    constexpr int BS = NCT > 0 ? ALFA_C2R_BS : FFT_BS;
-    constexpr int L = Lines<T>::L * BS / FFT_BS;
-    TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, nb, 0};
-    const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
-    const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+    auto launch = [&](auto bs) {
+      // compile-time lengths: tile_c2r threads and lines in proportion
+      constexpr int BS = decltype(bs)::value;
+      constexpr int L = Lines<T>::L * BS / FFT_BS;
+      TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, nb, 0};
+      const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
+      const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+      if constexpr (NCT > 0) {
+        // input modes within twice the stride of the radix-8 first pass: band
+        // first pass
+        const bool band = ALFA_BAND && radix_of(NCT) == 8 && a.ncy_in <= NCT / 4;
+        if (a.vec) {  // last radix pass fused into the stores
+          if (band)
+            k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true, true>
+                <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+          else
+            k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>
+                <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+          return;
+        }
+        if (band) {
+          k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, false, true>
+              <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+          return;
+        }
+      }
+      k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>
+          <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+    };
     if constexpr (NCT > 0) {
-      // input modes within twice the stride of the radix-8 first pass: band
-      // first pass
-      const bool band = ALFA_BAND && radix_of(NCT) == 8 && a.ncy_in <= NCT / 4;
-      if (a.vec) {  // last radix pass fused into the stores
-        if (band)
-          k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true, true>
-              <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
-        else
-          k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>
-              <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
-        return;
-      }
-      if (band) {
-        k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, false, true>
-            <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
-        return;
-      }
+      if (P->tile_c2r == 256) launch(std::integral_constant<int, 256>{});
+      else launch(std::integral_constant<int, 512>{});
+    } else {
+      launch(std::integral_constant<int, FFT_BS>{});
     }
-    k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>
-        <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
   });
   launch_ok(P);
 }
