@@ -313,7 +313,7 @@ int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, 
   c2<T>* dpz = c.S("dpzh");
   c2<T>* dpx = c.S("dpxh");
   ThomasArgs<T> ta{};
-  ta.m = P->th_m; ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
+  ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
   ta.RCin_r = (const T*)RCr; ta.RCin_i = (const T*)RCi; ta.fRzr_in = (const T*)fRzr;
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
@@ -655,7 +655,7 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
   // Thomas factors (ThomasPressureInit, NSE_PressureTerms_Thomas.py:151-214)
   const int N = nz + 1, nyc = P->nyc;
   const size_t ncolc = (size_t)nx * nyc;
-  std::vector<double> hm(ncolc * N, 0.0), hb(ncolc * N, 0.0);
+  std::vector<double> hb(ncolc * N, 0.0);   // pivots; the kernel forms the multipliers
   {
     const double idz2 = 1.0 / (d.dz * d.dz);
     std::vector<double> a(N, idz2), c(N, idz2), b(N);
@@ -668,12 +668,11 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
         if (i == 0 && j == 0)
           for (int k = 1; k < nz; ++k) b[k] = -(1.0 + 2.0 / (d.dz * d.dz));
         b[0] = -1.0; b[nz] = 1.0;
-        double* pm = &hm[((size_t)i * nyc + j) * N];
         double* pb = &hb[((size_t)i * nyc + j) * N];
         pb[0] = b[0];
         for (int k = 1; k < N; ++k) {
-          pm[k] = a[k] / pb[k - 1];
-          pb[k] = b[k] - pm[k] * c[k - 1];
+          const double m = a[k] / pb[k - 1];
+          pb[k] = b[k] - m * c[k - 1];
         }
       }
   }
@@ -686,7 +685,6 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
     e = hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
     return (double*)p;
   };
-  P->th_m = upload(hm);
   P->th_b = upload(hb);
   // profiles: Ug, Vg (Initialization.py:163-170), Rayleigh full / staggered (:395-460)
   {

@@ -58,7 +58,7 @@ struct alfa_plan {
   int nx, ny, nz, nyc, mx, my, myc;
   Fft1D fx, fy, fmx, fmy;
   void *tw_x = nullptr, *tw_y = nullptr, *tw_mx = nullptr, *tw_my = nullptr;
-  double *th_m = nullptr, *th_b = nullptr;    // (nx*nyc, nz+1)
+  double* th_b = nullptr;   // Thomas pivots (nx*nyc, nz+1)
   double* prof = nullptr;                     // Ug, Vg, rf, rs, Uadv, Vadv, THadv, Qadv: 8*nz
   bool adv_on = false;                        // advection forcing set (alfa_set_forcing)
   const double* fadv() const { return adv_on ? prof + 4 * (size_t)nz : nullptr; }

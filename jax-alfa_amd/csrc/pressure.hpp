@@ -17,7 +17,7 @@
 template <class T> struct ThomasArgs {
   const c2<T>* fRx; const c2<T>* fRy; const c2<T>* fRz;   // (nx, nyc, nz)
   const c2<T>* fbt;                                        // (nx, nyc, 2)
-  const double* m; const double* b;                        // (ncol, nz+1)
+  const double* b;   // (ncol, nz+1) pivots of ThomasPressureInit's elimination
   c2<T>* phat; c2<T>* dpz;                                 // (nx, nyc, nz)
   T* RCr; T* RCi;             // optional out (ncol, nz+1): RC before masking
   const T* RCin_r; const T* RCin_i;   // optional in (ncol, nz+1): RC given
@@ -71,7 +71,6 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
     if (a.RCr) { a.RCr[(size_t)col * N + k] = (T)rr; a.RCi[(size_t)col * N + k] = (T)ri; }
     dr[k] = special ? 0.0 : rr;
     di[k] = special ? 0.0 : ri;
-    smm[k] = a.m[(size_t)col * N + k];
     sb[k] = a.b[(size_t)col * N + k];
   }
   __syncthreads();
@@ -104,6 +103,11 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
   }
   const int C = (N + 63) / 64;
   const int k0 = lane * C, k1 = imin(N, k0 + C);
+  // elimination multipliers m_k = a_k / p_{k-1} from the pivots p (a_k =
+  // 1/dz^2 inside, -1 at the top row), as ThomasPressureInit forms them:
+  // the same division, so the same values.  Each lane forms those of the
+  // levels it sweeps.
+  for (int k = imax(k0, 1); k < k1; ++k) smm[k] = (k == nz ? -1.0 : a.c_in) / sb[k - 1];
   // forward sweep: y_k = d_k - m_k y_{k-1}   (_thomas_solve :125-131)
   double A = 1.0, Br = 0.0, Bi = 0.0;
   for (int k = k0; k < k1; ++k) {

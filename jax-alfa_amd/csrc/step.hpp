@@ -266,14 +266,14 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   c2<T>* dpx = c.S("dpxh");
   ThomasArgs<T> ta{};
   ta.fRx = fR[0]; ta.fRy = fR[1]; ta.fRz = fR[2]; ta.fbt = fbt;
-  ta.m = P->th_m; ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
+  ta.b = P->th_b; ta.phat = phat; ta.dpz = dpz;
   ta.RCr = RCr; ta.RCi = RCi;
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
   {
-    // 3 spectra in, 2 spectra out, 2 double factor tables
+    // 3 spectra in, 2 spectra out, the double pivot table
     const double spb = (double)c.nx * c.nyc * c.nz * sizeof(c2<T>);
-    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 2.0 * c.nx * c.nyc * (c.nz + 1) * 8.0,
+    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 1.0 * c.nx * c.nyc * (c.nz + 1) * 8.0,
                 k_thomas<T>, c.nx * c.nyc, 64, thomas_lds(c.nz), ta);
   }
   XArgs xa{};
