@@ -9,6 +9,9 @@ plain FP64 is timed beside it.  One JSON line goes to stdout:
   roofline    = the dominant kernel's algorithmic bytes / its HIP-event time,
                 measured on its launches inside the timed region
   cpu_baseline= the FP64 oracle (NumPy, one thread) on a bounded sample
+  fft_tiles   = the FFT kernels' workgroup sizes, chosen after the warmup by
+                the plan's autotuner (timed on steps from a copy of the
+                state, outside the timed region), or given by --tiles
 Diagnostics (per-kernel table) go to stderr, and to --kernels-json PATH if given.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5]
@@ -102,7 +105,7 @@ def error_report(mx, fp):
     return r
 
 
-def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None):
+def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None, tiles=None):
     import torch
     from alfa_amd import cases, core
     cfg, u, v, w, TH = cases.initial_fields("stable", nx=n, ny=n, nz=n)
@@ -113,9 +116,12 @@ def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None):
     log(f"[bench] {prec} {n}^3 plan ready, workspace {P.derived['workspace_bytes'] / 2**30:.2f} GiB")
     les.step(warmup)
     torch.cuda.synchronize()
-    # FFT tile choices: timed on steps from a copy of the warmed-up state
-    tiles = P.autotune(les)
-    log(f"[bench] {prec} FFT tiles (autotuned on steps): {tiles}")
+    if tiles:   # fixed FFT tile choices (y R2C, x, y C2R)
+        tiles = P.set_tiles(*tiles)
+        log(f"[bench] {prec} FFT tiles (given): {tiles}")
+    else:       # timed on steps from a copy of the warmed-up state
+        tiles = P.autotune(les)
+        log(f"[bench] {prec} FFT tiles (autotuned on steps): {tiles}")
     table = None
     if find_dominant:
         P.profile_begin(None, 16384)
@@ -180,6 +186,10 @@ def main():
     ap.add_argument("--serve-steps", type=int, nargs=2, default=(100, 400), metavar=("MIN", "MAX"),
                     help="run-length range of served requests (the full configuration: 200 2000)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--tiles", default=None, metavar="R2C,X,C2R",
+                    help="threads per workgroup of the y R2C, x and y C2R FFT kernels (256 or "
+                         "512 each) instead of the plan's autotuner, e.g. under a profiler, "
+                         "whose trace would also hold the autotuner's trial steps")
     ap.add_argument("--kernels-json", default=None, metavar="PATH",
                     help="write the per-kernel tables (HIP-event times, bytes) to PATH")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -194,7 +204,10 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = args.grid
-    mixed = run_gpu("mixed", n, args.steps, args.warmup)
+    tiles = [int(t) for t in args.tiles.split(",")] if args.tiles else None
+    if tiles and (len(tiles) != 3 or any(t not in (256, 512) for t in tiles)):
+        ap.error("--tiles takes three of 256 or 512, e.g. 256,256,512")
+    mixed = run_gpu("mixed", n, args.steps, args.warmup, tiles=tiles)
     log(f"[bench] mixed: {mixed['ms']:.3f} ms/step, {mixed['mpts']:.1f} Mpoint-steps/s, "
         f"CFL {mixed['cfl']:.3f}")
     for r in (mixed["table"] or [])[:25]:
@@ -205,7 +218,7 @@ def main():
     errors = None
     if not args.no_fp64:
         # same step count as the mixed run, so the fields can be compared
-        fp64 = run_gpu("fp64", n, args.steps, args.warmup, find_dominant=True)
+        fp64 = run_gpu("fp64", n, args.steps, args.warmup, find_dominant=True, tiles=tiles)
         log(f"[bench] fp64: {fp64['ms']:.3f} ms/step, {fp64['mpts']:.1f} Mpoint-steps/s")
         errors = error_report(mixed["stats"], fp64["stats"])
         errors["after_steps"] = mixed["total_steps"]
