@@ -28,7 +28,7 @@ template <class T> struct ThomasArgs {
 
 constexpr int TH_MAXN = 520;   // nz + 1 <= 520
 // dynamic LDS of one column: 4 arrays of nz + 1 doubles
-inline size_t thomas_lds(int nz) { return 4 * (size_t)(nz + 1) * sizeof(double); }
+inline size_t thomas_lds(int nz) { return 3 * (size_t)(nz + 1) * sizeof(double); }
 
 __device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
 __device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
@@ -42,7 +42,6 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
   double* dr = th_smem;
   double* di = dr + N;
   double* sb = di + N;
-  double* smm = sb + N;
   const int i = col / a.nyc, j = col % a.nyc;
   const double kr = (i < a.nx / 2) ? (double)i : (double)(i - a.nx);
   const double kc = (double)j;
@@ -78,7 +77,7 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
     if (col == 0) {
       // zero mode by vertical integration (NSE_PressureTerms_Thomas.py:271-275)
       for (int k = lane; k < nz; k += 64)
-        smm[k] = a.RCin_r ? (double)a.fRzr_in[cb + k] : (double)a.fRz[cb + k].x;
+        dr[k] = a.RCin_r ? (double)a.fRzr_in[cb + k] : (double)a.fRz[cb + k].x;
       __syncthreads();
       if (lane == 0) {
         const double zf = a.RCin_r ? (double)a.RCin_r[0] : -a.dz * (double)a.fbt[0].x;
@@ -86,7 +85,7 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
         a.phat[cb] = mkc<T>((T)zf, T(0));
         a.dpz[cb] = mkc<T>(T(0), T(0));
         for (int k = 1; k < nz; ++k) {
-          S += smm[k] * a.dz;
+          S += dr[k] * a.dz;
           const double p = zf + S;
           a.phat[cb + k] = mkc<T>((T)p, T(0));
           a.dpz[cb + k] = mkc<T>((T)((p - prev) * a.idz), T(0));
@@ -106,12 +105,21 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
   // elimination multipliers m_k = a_k / p_{k-1} from the pivots p (a_k =
   // 1/dz^2 inside, -1 at the top row), as ThomasPressureInit forms them:
   // the same division, so the same values.  Each lane forms those of the
-  // levels it sweeps.
-  for (int k = imax(k0, 1); k < k1; ++k) smm[k] = (k == nz ? -1.0 : a.c_in) / sb[k - 1];
+  // levels it sweeps, into registers (C <= CMAX).
+  constexpr int CMAX = (TH_MAXN + 63) / 64;
+  double ml[CMAX];
+#pragma unroll
+  for (int t = 0; t < CMAX; ++t) {
+    const int k = k0 + t;
+    ml[t] = (k >= 1 && k < k1) ? (k == nz ? -1.0 : a.c_in) / sb[k - 1] : 0.0;
+  }
   // forward sweep: y_k = d_k - m_k y_{k-1}   (_thomas_solve :125-131)
   double A = 1.0, Br = 0.0, Bi = 0.0;
-  for (int k = k0; k < k1; ++k) {
-    const double al = (k == 0) ? 0.0 : -smm[k];
+#pragma unroll
+  for (int t = 0; t < CMAX; ++t) {
+    const int k = k0 + t;
+    if (k >= k1) break;
+    const double al = (k == 0) ? 0.0 : -ml[t];
     A *= al;
     Br = al * Br + dr[k];
     Bi = al * Bi + di[k];
@@ -126,8 +134,11 @@ __global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
   }
   double yr = shfl_up_d(Br, 1), yi = shfl_up_d(Bi, 1);
   if (lane == 0) { yr = 0.0; yi = 0.0; }
-  for (int k = k0; k < k1; ++k) {
-    const double al = (k == 0) ? 0.0 : -smm[k];
+#pragma unroll
+  for (int t = 0; t < CMAX; ++t) {
+    const int k = k0 + t;
+    if (k >= k1) break;
+    const double al = (k == 0) ? 0.0 : -ml[t];
     yr = al * yr + dr[k];
     yi = al * yi + di[k];
     dr[k] = yr;
