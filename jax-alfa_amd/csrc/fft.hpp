@@ -708,7 +708,9 @@ __global__ void __launch_bounds__(BS)
 k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
     const c2<T>* __restrict__ tw_out, TileGrid tg) {
   constexpr int L = Lines<T>::L * BS / FFT_BS;
-  constexpr int MAXE = MAXN * L / BS;
+  // spectrum elements per thread in the multi-op path: rounded up, since
+  // MAXN * L need not be a multiple of BS (96 double lines at 256 threads)
+  constexpr int MAXE = (MAXN * L + BS - 1) / BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nin = NCT > 0 ? NCT : fin.n;   // NCT is set only when fin.n == fout.n
   const int nbuf = NCT > 0 ? NCT : imax(nin, fout.n);

@@ -146,7 +146,9 @@ inline unsigned pw_blocks(size_t n) {
 template <class F> inline void ct_length(int n, F&& f) {
   using std::integral_constant;
   switch (n) {
+    case 48: f(integral_constant<int, 48>{}, integral_constant<int, 48>{}); break;
     case 64: f(integral_constant<int, 64>{}, integral_constant<int, 64>{}); break;
+    case 80: f(integral_constant<int, 80>{}, integral_constant<int, 80>{}); break;
     case 96: f(integral_constant<int, 96>{}, integral_constant<int, 96>{}); break;
     case 128: f(integral_constant<int, 128>{}, integral_constant<int, 128>{}); break;
     case 192: f(integral_constant<int, 192>{}, integral_constant<int, 192>{}); break;

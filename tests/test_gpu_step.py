@@ -35,7 +35,7 @@ CASES = {
     # NSE_PressureTerms.py:220-335): the oracle solves it with LU; the GPU
     # solves the same tridiagonal system with the Thomas kernel
     "stable_lu": ("stable", dict(nx=32, ny=32, nz=32, optPressureSolver=0)),
-    # transform lengths with compile-time plans (64, 96, 128, 192, 256; the
+    # transform lengths with compile-time plans (48, 64, 80, 96, 128, 192, 256; the
     # dealiased FGR=1 grid pads 64 to 96): the y C2R writes its last radix
     # pass to the fields directly, and at FGR=2 the filtered spectra take the
     # band-limited passes (|k| <= n/4)
@@ -44,6 +44,11 @@ CASES = {
     "convective128x96_lad": ("convective", dict(nx=128, ny=96, nz=16, optSgs=3)),
     "convective128x192": ("convective", dict(nx=128, ny=192, nz=16)),
     "neutral64": ("neutral", dict(nx=64, ny=64, nz=16)),
+    # x lengths 96 and 48, y length 80: the two-op x pass of the explicit
+    # filter at compile-time lengths whose tile (n x lines) is not a multiple
+    # of the workgroup size, and the radix-5 last pass
+    "convective96x64": ("convective", dict(nx=96, ny=64, nz=16)),
+    "convective48x80": ("convective", dict(nx=48, ny=80, nz=16)),
 }
 
 

@@ -87,3 +87,41 @@ def test_set_tiles_rejects_other_sizes():
     assert lib.alfa_plan_set_tiles(P.handle, None) != 0
     assert lib.alfa_plan_get_tiles(P.handle, (C.c_int * 3)()) == 0
     P.close()
+
+
+def _moist_run(autotune, nsteps=3):
+    cfg, u, v, w, TH = cases.initial_fields("stable", nx=64, ny=64, nz=16, optMoistureSurfBC=2,
+                                            q_inversion=0.0)
+    cfg["optMoisture"] = 1
+    rng = np.random.default_rng(5)
+    Q = 0.008 + 1e-4 * rng.standard_normal(u.shape)
+    qser = 0.0095 - 1e-4 * np.arange(nsteps + 2) * cfg["dt"] / 60.0
+    ser = cases.surface_series(cfg, nsteps + 2)
+    P = core.Plan(cfg, "fp64")
+    les = core.LES(P, u, v, w, TH, sfc_series=ser, Q=Q, qsfc_series=qser)
+    les.step(1)
+    chosen = None
+    if autotune:
+        before = les.fields()
+        chosen = P.autotune(les)
+        torch.cuda.synchronize()
+        after = les.fields()
+        assert set(before) == {"u", "v", "w", "TH", "Q"}
+        for k in before:
+            np.testing.assert_array_equal(before[k], after[k], err_msg=f"state {k} changed")
+    les.step(nsteps - 1)
+    torch.cuda.synchronize()
+    f = les.fields()
+    P.close()
+    return chosen, f
+
+
+def test_autotune_on_a_moist_state():
+    """With optMoisture the autotuner's scratch state holds Q and RHS_Q too:
+    the caller's state, Q included, is left as it was and the run continues
+    bitwise as at the default tiles."""
+    chosen, a = _moist_run(True)
+    print("moist autotuned tiles", chosen)
+    _, b = _moist_run(False)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
