@@ -842,8 +842,11 @@ struct TuneWork {
     for (int r = 0; r < reps; ++r)
       if (int rc = once(r)) return rc;
     (void)hipEventRecord(e1, s);
-    (void)hipEventSynchronize(e1);
-    return hipEventElapsedTime(&ms, e0, e1) == hipSuccess ? 0 : ALFA_EHIP;
+    hipError_t e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+      return set_error(ALFA_EHIP, "alfa_plan_autotune: %s", hipGetErrorString(e));
+    return ALFA_OK;
   }
 };
 
