@@ -191,7 +191,7 @@ class Member:
 
 
 def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mixed",
-                 graphs=True, log=None):
+                 graphs=True, log=None, warmup=2):
     """Step all members together for `steps` steps (default: each spec's
     own length, all equal in config 3).  Returns throughput figures."""
     torch = core._torch()
@@ -201,18 +201,20 @@ def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mix
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     nsteps = steps or specs[0].steps
-    # untimed: the eager first step and the graph captures (2 steps)
+    # untimed: the eager first step, the graph captures (2 steps) and any
+    # further warmup steps
+    warmup = max(2, min(warmup, nsteps - 1))
     for m in members:
-        m.step(2)
+        m.step(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(nsteps - 2):
+    for _ in range(nsteps - warmup):
         for m in members:
             m.step(1)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    pts = sum(m.points for m in members) * (nsteps - 2)
-    out = dict(members=len(members), steps_timed=nsteps - 2, wall_s=wall, setup_s=t_setup,
+    pts = sum(m.points for m in members) * (nsteps - warmup)
+    out = dict(members=len(members), steps_timed=nsteps - warmup, wall_s=wall, setup_s=t_setup,
                mpoint_steps_per_s=pts / wall / 1e6, streams=nstreams, graphs=graphs,
                grids={str(n): sum(1 for s in specs if s.n == n) for n in sorted({s.n for s in specs})},
                hbm_bytes=sum(m.bytes for m in members))
