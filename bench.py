@@ -239,9 +239,9 @@ def main():
         from alfa_amd import ensemble as E
         specs = E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps, seed=0)
         ens = E.run_ensemble(specs)
-        # the first ensemble of a process runs about 25% slower than later,
-        # identical ones (the mapping of member streams onto hardware queues
-        # explains part of it); a second run is reported beside the first
+        # a second, identical run beside the first: the first streams of a
+        # process ran ensembles about 25% slower, which the runner avoids by
+        # setting them aside; the two figures should now agree
         rep = E.run_ensemble(E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps,
                                               seed=0))
         ens["repeat_mpoint_steps_per_s"] = rep["mpoint_steps_per_s"]

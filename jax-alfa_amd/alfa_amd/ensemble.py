@@ -117,6 +117,26 @@ def _tuned_tiles(m):
     return _TILES[key]
 
 
+# The first streams a process creates were measured to run an ensemble about
+# 25% slower than streams created after them (1319 against 1744 Mpoint-steps/s
+# for 64 members at 48-96^3 on 4 streams): they share hardware queues with
+# work the process issued before.  A first set of streams is therefore
+# created, used once and kept aside, and members get streams made after it.
+_SPARE_STREAMS = []
+
+
+def member_streams(n):
+    torch = core._torch()
+    if not _SPARE_STREAMS:
+        for _ in range(4):
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                torch.ones(1, device="cuda").add_(1)
+            _SPARE_STREAMS.append(st)
+        torch.cuda.synchronize()
+    return [torch.cuda.Stream() for _ in range(n)]
+
+
 def state_bytes(n, precision):
     """Caller-owned state of one member: 10 fields + 6 planes + profiles."""
     es = 8 if precision == "fp64" else 4
@@ -195,7 +215,7 @@ def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mix
     """Step all members together for `steps` steps (default: each spec's
     own length, all equal in config 3).  Returns throughput figures."""
     torch = core._torch()
-    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+    streams = member_streams(nstreams)
     t0 = time.perf_counter()
     members = [Member(s, precision, streams[i % nstreams], graphs) for i, s in enumerate(specs)]
     torch.cuda.synchronize()
@@ -231,7 +251,7 @@ class Server:
         torch = core._torch()
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
-        self.streams = [torch.cuda.Stream() for _ in range(nstreams)]
+        self.streams = member_streams(nstreams)
         self.round_steps = round_steps
         self.graphs = graphs
         self.results = {}
