@@ -1005,9 +1005,10 @@ def _wl_quintic(g, a1, a3, a6, a8):
     DynamicSGS_ScalarLASDD_WL.py:224-243): four independent plane means,
     the other six scaled by powers of TFR.  Returns (ff, ee, dd, cc, bb, aa).
 
-    The expansion is checked against its product form; parity unpinned for
-    the TFR-power scalings of the six dependent means, which follow the
-    reference with no independent check."""
+    Pinned: the polynomial equals the cross-multiplied two-level condition
+    of the scale-dependent procedure formed from the WL model tensors at
+    several beta, and vanishes where either level's model term does
+    (tests/test_oracle_surface_advection_pins.py)."""
     T = g.TFR
     a2 = -(T ** (8 / 3)) * a1
     a4 = -2 * T ** (4 / 3) * a3
