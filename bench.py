@@ -6,9 +6,12 @@ Smagorinsky, FGR=2, Thomas pressure solve, prescribed cooling surface,
 dt = 0.05 s), mixed FP32/FP64, fields resident in HBM.  The same workload in
 plain FP64 is timed beside it.  One JSON line goes to stdout:
   value       = whole-job point-steps/s of the mixed run, in Mpoint-steps/s
-  roofline    = the dominant kernel's algorithmic bytes / its HIP-event time,
-                measured on its launches inside the timed region
-  cpu_baseline= the FP64 oracle (NumPy, one thread) on a bounded sample
+  roofline    = the dominant kernel's algorithmic bytes (the reference data
+                flow: one full read + one full write per transformed field)
+                / its HIP-event time, measured on its launches inside the
+                timed region; the kernel's own distinct bytes beside it
+  cpu_baseline= the FP64 oracle (NumPy, scipy.fft on the host cores) on a
+                bounded sample
   fft_tiles   = the FFT kernels' workgroup sizes, chosen after the warmup by
                 the plan's autotuner (timed on steps from a copy of the
                 state, outside the timed region), or given by --tiles
@@ -148,27 +151,48 @@ def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None, tiles=Non
                 total_steps=warmup + (2 if find_dominant else 0) + steps)
 
 
-def cpu_baseline(budget_s=15.0, n=64):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import alfa_oracle as O
+def _oracle_rate(O, name, over, nsteps, budget_s):
+    """Point-steps/s of the FP64 oracle stepping case `name` from its initial
+    state: nsteps steps, or fewer if budget_s runs out (at least one)."""
     from alfa_amd import cases
-    cfg, u, v, w, TH = cases.initial_fields("stable", nx=n, ny=n, nz=n)
+    cfg, u, v, w, TH = cases.initial_fields(name, **over)
     g = O.Grid(cfg)
     s = O.initial_state(g, u, v, w, TH)
-    ser = cases.surface_series(cfg, 400)
+    ser = cases.surface_series(cfg, nsteps + 2)
     th = O.ThomasPressureInit(g)
-    s = O.step(g, s, th, ser[0])          # first step (startup costs) untimed
     k = 0
     t0 = time.perf_counter()
-    while True:
-        s = O.step(g, s, th, ser[s.iteration - 1])
+    while k < nsteps:
+        s = O.step(g, s, th, None if ser is None else ser[s.iteration - 1])
         k += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or k >= 200:
+        if time.perf_counter() - t0 > budget_s:
             break
-    return dict(value=n ** 3 * k / el / 1e6, unit="Mpoint-steps/s", cores=1, kind="port",
-                sample=f"FP64 NumPy oracle, stable case at {n}^3 (LAD-SM, FGR=2), {k} steps "
-                       f"in {el:.1f} s after one untimed step")
+    el = time.perf_counter() - t0
+    return g.nx * g.ny * g.nz * k / el / 1e6, k, el
+
+
+def cpu_baseline():
+    """The FP64 NumPy/SciPy oracle on the host cores: FFTs through scipy.fft
+    with one worker per core of the process's CPU share (at most 16, the
+    share of one GPU), the rest of the NumPy work on one thread.  Samples:
+    the stable layer of the GPU workload (same physics: LAD-SM, FGR=2,
+    prescribed cooling surface) at 128^3 for 2 steps, and the neutral 32^3
+    configuration for its 100 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import alfa_oracle as O
+    workers = min(len(os.sched_getaffinity(0)), 16)
+    O.set_fft_workers(workers)
+    try:
+        v128, k128, t128 = _oracle_rate(O, "stable", dict(nx=128, ny=128, nz=128), 2, 30.0)
+        v32, k32, t32 = _oracle_rate(O, "neutral", {}, 100, 30.0)
+    finally:
+        O.set_fft_workers(None)
+    return dict(value=round(v128, 4), unit="Mpoint-steps/s", cores=workers, kind="port",
+                sample=f"FP64 NumPy/SciPy oracle, stable layer 128^3 (LAD-SM, FGR=2), {k128} "
+                       f"steps in {t128:.1f} s from the initial state; scipy.fft workers="
+                       f"{workers}, other NumPy work single-threaded",
+                config0=dict(value=round(v32, 4), steps=k32, seconds=round(t32, 1),
+                             workload="neutral 32^3, LAD-SM, FGR=1 (dealiased), dt=0.5 s"))
 
 
 def main():
@@ -180,11 +204,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=64)
-    ap.add_argument("--ensemble-steps", type=int, default=60,
-                    help="steps per member (the full ensemble configuration uses 500)")
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="steps per member (the ensemble configuration's 500)")
     ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-steps", type=int, nargs=2, default=(100, 400), metavar=("MIN", "MAX"),
-                    help="run-length range of served requests (the full configuration: 200 2000)")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=(200, 2000), metavar=("MIN", "MAX"),
+                    help="run-length range of served requests (the serving configuration's "
+                         "200-2000)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--tiles", default=None, metavar="R2C,X,C2R",
                     help="threads per workgroup of the y R2C, x and y C2R FFT kernels (256 or "
@@ -227,13 +252,21 @@ def main():
     dom = live.get(mixed["dominant"])
     roof = None
     if dom and dom["total_ms"] > 0:
-        ach = dom["bytes"] / (dom["total_ms"] * 1e-3) / 1e9
+        # achieved: the reference data flow's bytes of the launches (each
+        # transformed field one full read + one full write, half spectra at
+        # (ny+2)/ny of a field) over their HIP-event time; achieved_distinct:
+        # the bytes the kernel itself must move (pruned spectra, fused loads)
+        sec = dom["total_ms"] * 1e-3
+        ach = dom["ref_bytes"] / sec / 1e9
+        ach_d = dom["bytes"] / sec / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=measured_traffic(mixed["dominant"]),
                     traffic_source="profiles/hbm_traffic.json", kernel=mixed["dominant"],
                     launches=dom["launches"],
                     avg_launch_us=round(dom["total_ms"] * 1e3 / dom["launches"], 2),
-                    bytes_per_launch=dom["bytes"] / dom["launches"])
+                    bytes_per_launch=dom["ref_bytes"] / dom["launches"],
+                    achieved_distinct=round(ach_d, 1), frac_distinct=round(ach_d / HBM_PEAK_GBS, 4),
+                    distinct_bytes_per_launch=dom["bytes"] / dom["launches"])
     ens = srv = None
     if not args.no_ensemble:
         from alfa_amd import ensemble as E

@@ -386,7 +386,12 @@ typedef struct alfa_kernel_stat {
   char name[96];
   int64_t launches;
   double total_ms;
-  double bytes;
+  double bytes;       /* summed distinct inputs + outputs of the launches */
+  double ref_bytes;   /* the same launches in the reference's data flow: each
+                         transformed field of an FFT kernel counts one full
+                         read of its input and one full write of its output
+                         (a half spectrum is (ny+2)/ny of a field); equal to
+                         bytes for the other kernels */
 } alfa_kernel_stat;
 int alfa_profile_begin(alfa_plan* plan, const char* name_filter, int max_launches);
 int alfa_profile_end(alfa_plan* plan, alfa_kernel_stat* out, int max_out, int* n_out);

@@ -42,7 +42,7 @@ class AlfaDerived(C.Structure):
 
 class AlfaKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 96), ("launches", C.c_int64), ("total_ms", C.c_double),
-                ("bytes", C.c_double)]
+                ("bytes", C.c_double), ("ref_bytes", C.c_double)]
 
 
 STATE_FIELDS = ["u", "v", "w", "TH", "RHS_u", "RHS_v", "RHS_w", "RHS_TH", "Cs2_3D",

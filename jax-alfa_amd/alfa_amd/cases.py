@@ -95,11 +95,14 @@ def surface_series(cfg, nsteps: int):
     """Non-dimensional surface BC series of length nsteps+1 (index i is time
     i*dt), as src/initialization/Initialization.py:107-160 returns it.
     optSurfBC=2: temperature anomaly T_s(t) - T_0 with T_s = T_0 - 0.25 K/h * t
-    (CreateSurfaceBC_GABLS1.py linear cooling); optSurfBC=1: the constant
-    SensibleHeatFlux; optSurfBC=0: None."""
+    (CreateSurfaceBC_GABLS1.py linear cooling); optSurfBC=1: a heat-flux
+    series that oscillates by +-30% about SensibleHeatFlux with a period of
+    40 time steps (the per-step change, about 5%, exposes a series read at
+    the wrong step); optSurfBC=0: None."""
     t = np.arange(nsteps + 1) * cfg["dt"]
     if cfg["optSurfBC"] == 2:
         return (cfg["T_0"] - 0.25 * t / 3600.0) - cfg["T_0"]
     if cfg["optSurfBC"] == 1:
-        return np.full(nsteps + 1, cfg["SensibleHeatFlux"])
+        period = 40.0 * cfg["dt"]
+        return cfg["SensibleHeatFlux"] * (1.0 + 0.3 * np.sin(2.0 * np.pi * t / period))
     return None

@@ -153,12 +153,15 @@ class Plan:
         _lib.check(self.lib.alfa_profile_begin(self._p, f, int(max_launches)), "alfa_profile_begin")
 
     def profile_end(self) -> list:
-        """-> [dict(name, launches, total_ms, bytes)] per kernel name."""
+        """-> [dict(name, launches, total_ms, bytes, ref_bytes)] per kernel
+        name: bytes are the launches' distinct inputs + outputs, ref_bytes
+        the same work in the reference's data flow (include/alfa.h)."""
         arr = (_lib.AlfaKernelStat * 256)()
         n = C.c_int(0)
         _lib.check(self.lib.alfa_profile_end(self._p, arr, 256, C.byref(n)), "alfa_profile_end")
         return [dict(name=arr[i].name.decode(), launches=int(arr[i].launches),
-                     total_ms=float(arr[i].total_ms), bytes=float(arr[i].bytes))
+                     total_ms=float(arr[i].total_ms), bytes=float(arr[i].bytes),
+                     ref_bytes=float(arr[i].ref_bytes))
                 for i in range(min(n.value, 256))]
 
     def call(self, name, *args):

@@ -211,9 +211,11 @@ class Member:
 
 
 def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mixed",
-                 graphs=True, log=None, warmup=2):
+                 graphs=True, log=None, warmup=2, keep_fields=False):
     """Step all members together for `steps` steps (default: each spec's
-    own length, all equal in config 3).  Returns throughput figures."""
+    own length, all equal in config 3).  Returns throughput figures, and
+    with keep_fields each member's final u, v, w, TH and CFL (host copies)
+    under "fields"."""
     torch = core._torch()
     streams = member_streams(nstreams)
     t0 = time.perf_counter()
@@ -238,6 +240,9 @@ def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mix
                mpoint_steps_per_s=pts / wall / 1e6, streams=nstreams, graphs=graphs,
                grids={str(n): sum(1 for s in specs if s.n == n) for n in sorted({s.n for s in specs})},
                hbm_bytes=sum(m.bytes for m in members))
+    if keep_fields:
+        out["fields"] = [dict({f: m.les.host(f) for f in ("u", "v", "w", "TH")},
+                              CFL=m.les.diag()["CFL"]) for m in members]
     for m in members:
         m.close()
     return out

@@ -154,13 +154,17 @@ def read_adv_forcing(path, cfg):
 
 def save_3d_fields(outdir, iteration, les):
     """Main.py:620-635: ALFA_3DFields_Iteration_<n>.npz with u + Ugal, v, w
-    and TH + T_0 (dry: no Q)."""
+    and TH + T_0, and Q when the run carries moisture (optMoisture >= 1,
+    Main.py:630-631)."""
     os.makedirs(outdir, exist_ok=True)
     f = les.fields()
     cfg = les.plan.cfg
     path = os.path.join(outdir, f"ALFA_3DFields_Iteration_{iteration}.npz")
-    np.savez(path, u=f["u"] + float(cfg.get("Ugal", 0.0)), v=f["v"], w=f["w"],
-             TH=f["TH"] + float(cfg["T_0"]))
+    out = dict(u=f["u"] + float(cfg.get("Ugal", 0.0)), v=f["v"], w=f["w"],
+               TH=f["TH"] + float(cfg["T_0"]))
+    if int(cfg.get("optMoisture", 0)) >= 1:
+        out["Q"] = f["Q"]
+    np.savez(path, **out)
     return path
 
 

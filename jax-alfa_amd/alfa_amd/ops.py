@@ -14,6 +14,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _lib
 from .core import Plan, current_stream
 
@@ -45,14 +47,49 @@ def Wavenumber():
     return WaveTable(0), WaveTable(1)
 
 
+def _wavenumbers(n, half):
+    """Preprocess.py:89-94: integer wavenumbers, the Nyquist entry set to 0"""
+    k = np.fft.rfftfreq(n, 1.0 / n) if half else np.fft.fftfreq(n, 1.0 / n)
+    k[n // 2] = 0.0
+    return k
+
+
 def _axis(k):
+    """Axis of a Derivxy wavenumber argument: a WaveTable, 0/1/"x"/"y", or
+    the reference's own broadcast table kx2 / ky2 (Preprocess.py:96-100),
+    shape (nx, ny//2+1[, nz]), numpy or torch.  A table is recognised by the
+    index along which it varies, and its values must be the plan's
+    wavenumbers (the kernels use the plan's own table)."""
     if isinstance(k, WaveTable):
         return k.axis
-    if k in (0, "x"):
+    if isinstance(k, (int, str)):
+        if k in (0, "x"):
+            return 0
+        if k in (1, "y"):
+            return 1
+        raise TypeError("kxy2 must be a WaveTable from Wavenumber(), an axis or a kx2/ky2 table")
+    shape = tuple(getattr(k, "shape", ()))
+    if len(shape) not in (2, 3):
+        raise TypeError("kxy2 must be a WaveTable from Wavenumber(), an axis or a kx2/ky2 table")
+    P = _P()
+    if shape[0] != P.nx or shape[1] != P.ny // 2 + 1 or (len(shape) == 3 and shape[2] != P.nz):
+        raise ValueError(f"wavenumber table of shape {shape} does not match the plan's "
+                         f"({P.nx}, {P.ny // 2 + 1}, {P.nz}) spectra")
+    idx = (slice(None), 0, 0) if len(shape) == 3 else (slice(None), 0)
+    idy = (0, slice(None), 0) if len(shape) == 3 else (0, slice(None))
+
+    def host(a):
+        a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+        return a.astype(np.float64)
+
+    along_x, along_y = host(k[idx]), host(k[idy])
+    kx, ky = _wavenumbers(P.nx, False), _wavenumbers(P.ny, True)
+    if np.array_equal(along_x, kx) and np.all(along_y == along_y[0]):
         return 0
-    if k in (1, "y"):
+    if np.array_equal(along_y, ky) and np.all(along_x == along_x[0]):
         return 1
-    raise TypeError("kxy2 must be a WaveTable from Wavenumber() or an axis")
+    raise ValueError("wavenumber table is neither the plan's kx2 nor its ky2 "
+                     "(Preprocess.py:89-100: integer wavenumbers, Nyquist set to 0)")
 
 
 # ---------------------------------------------------------------- operations

@@ -65,9 +65,11 @@ static int check_hip(alfa_plan* P, const char* where) {
   hipError_t e = P->herr;
   P->herr = hipSuccess;
   if (e != hipSuccess) return set_error(ALFA_EHIP, "%s: %s", where, hipGetErrorString(e));
-  if (P->ws.overflow)
+  if (P->ws.overflow) {
+    P->ws.overflow = false;   // reported once; the next call starts clean
     return set_error(ALFA_ESTATE, "%s: workspace buffer '%s' was not planned", where,
                      P->ws.overflow_name.c_str());
+  }
   return ALFA_OK;
 }
 
@@ -122,7 +124,7 @@ template <class T>
 int filtering_explicit(alfa_plan* P, hipStream_t s, const void* Ffft, void* F, void* Fnew) {
   Ctx<T> c(P, s);
   const auto& d = P->d;
-  if (P->ws.dry) return 0;
+  if (!live(P)) return 0;
   const size_t n = 2 * (size_t)c.nx * c.nyc * c.nz;
   k_fill<T><<<pw_blocks(n), 256, 0, s>>>(n, (T*)Fnew, T(0));
   launch_ok(P);
@@ -317,7 +319,7 @@ int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, 
   ta.RCin_r = (const T*)RCr; ta.RCin_i = (const T*)RCi; ta.fRzr_in = (const T*)fRzr;
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
-  if (!P->ws.dry) {
+  if (live(P)) {
     k_thomas<T><<<c.nx * c.nyc, 64, thomas_lds(c.nz), s>>>(ta);
     launch_ok(P);
   }
@@ -481,7 +483,7 @@ int surface(alfa_plan* P, hipStream_t s, const void* u, const void* v, const voi
   S.psim = (T*)pm; S.psim0 = (T*)pm0; S.psih = (T*)ph; S.psih0 = (T*)ph0; S.fim = (T*)fm; S.fih = (T*)fh;
   Surf<T> sf{(T*)M, (T*)ustar, (T*)qz, (T*)invOB};
   stage_surface<T>(c, S, sf, sfc_val);
-  if (!P->ws.dry && qavg)
+  if (live(P) && qavg)
     (void)hipMemcpyAsync(qavg, P->scal + 1, sizeof(double), hipMemcpyDeviceToDevice, s);
   return 0;
 }
@@ -1316,6 +1318,7 @@ extern "C" int alfa_profile_end(alfa_plan* P, alfa_kernel_stat* out, int max_out
     acc[k].launches += 1;
     acc[k].total_ms += ms;
     acc[k].bytes += rec.bytes;
+    acc[k].ref_bytes += rec.ref_bytes;
   }
   const int n = (int)acc.size() < max_out ? (int)acc.size() : max_out;
   for (int i = 0; i < n; ++i) out[i] = acc[i];

@@ -40,7 +40,8 @@ struct Workspace {
 // record holds the launch's algorithmic bytes (distinct inputs + outputs).
 struct ProfRec {
   std::string name;
-  double bytes;
+  double bytes;       // distinct inputs + outputs of this launch
+  double ref_bytes;   // the same work in the reference's data flow (see ProfScope)
   hipEvent_t a, b;
 };
 struct Prof {
@@ -88,6 +89,12 @@ inline unsigned tile_grid(size_t tiles) {
 
 int set_error(int code, const char* fmt, ...);
 
+// Launches are enqueued only outside the dry run and while every workspace
+// buffer asked for so far exists: a buffer that the plan's dry run did not
+// record makes Workspace::get return null, and no kernel is then launched
+// with it (the entry point reports ALFA_ESTATE instead).
+inline bool live(const alfa_plan* P) { return !P->ws.dry && !P->ws.overflow; }
+
 inline bool launch_ok(alfa_plan* P) {
   if (P->ws.dry) return true;
   hipError_t e = hipGetLastError();
@@ -99,12 +106,18 @@ struct ProfScope {
   alfa_plan* P;
   hipStream_t s;
   long idx = -1;
-  ProfScope(alfa_plan* P_, hipStream_t s_, const std::string& name, double bytes) : P(P_), s(s_) {
+  // ref_bytes < 0: the same as bytes.  The FFT launchers give the bytes of
+  // the reference's data flow instead: each transformed field is one full
+  // read of its input and one full write of its output, a half spectrum
+  // (nx, ny/2+1, nz) being (ny+2)/ny of a field.
+  ProfScope(alfa_plan* P_, hipStream_t s_, const std::string& name, double bytes,
+            double ref_bytes = -1.0)
+      : P(P_), s(s_) {
     Prof& f = P->pf;
     if (!f.on || P->ws.dry) return;
     if (!f.filter.empty() && f.filter != name) return;
     if (f.used + 2 > f.pool.size()) return;
-    ProfRec r{name, bytes, f.pool[f.used], f.pool[f.used + 1]};
+    ProfRec r{name, bytes, ref_bytes < 0 ? bytes : ref_bytes, f.pool[f.used], f.pool[f.used + 1]};
     f.used += 2;
     (void)hipEventRecord(r.a, s);
     idx = (long)f.recs.size();
@@ -165,13 +178,15 @@ template <class T> inline size_t fft_lds(int n) { return (size_t)n * Lines<T>::L
 template <class T, class Loader>
 void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D& fyy,
            const void* tw, int nz, int ncy, int nb, void* const* outs) {
-  if (P->ws.dry) return;
+  if (!live(P)) return;
   PtrBatch pb{};
   for (int b = 0; b < nb; ++b) pb.p[b] = outs[b];
   const double cells = (double)rows * fyy.n * nz;
   ProfScope ps(P, s, std::string("k_y_r2c<") + Loader::NAME + ">",
                Loader::nin(nb) * cells * sizeof(T) +
-                   (double)nb * rows * ncy * nz * sizeof(c2<T>));
+                   (double)nb * rows * ncy * nz * sizeof(c2<T>),
+               (double)nb * rows * nz * ((double)fyy.n * sizeof(T) +
+                                          (double)(fyy.n / 2 + 1) * sizeof(c2<T>)));
   // 16-byte loads need nz to be a multiple of the vector width and aligned
   // buffers; the paired complex stores need 16-byte aligned outputs
   bool vec = nz % Vec16<T>::W == 0 && ld.aligned(nb);
@@ -262,7 +277,7 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
 template <class T>
 void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a_in, int nb, const Fft1D& fin,
             const Fft1D& fout, const void* twin, const void* twout, int rows = -1) {
-  if (P->ws.dry) return;
+  if (!live(P)) return;
   XArgs a = a_in;
   // float lines move in 16-byte pairs: nz even and aligned buffers
   bool vec = a.nz % 2 == 0;
@@ -285,7 +300,7 @@ void x_pass(alfa_plan* P, hipStream_t s, const XArgs& a_in, int nb, const Fft1D&
 template <class T, class Storer>
 void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, int nb, int rows,
            const Fft1D& fyy, const void* tw) {
-  if (P->ws.dry) return;
+  if (!live(P)) return;
   YC2RArgs a = a_in;
   bool vec = a.nz % Vec16<T>::W == 0 && st.aligned(nb);
   for (int b = 0; b < nb; ++b)
@@ -293,7 +308,9 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
   a.vec = vec ? 1 : 0;
   ProfScope ps(P, s, std::string("k_y_c2r<") + Storer::NAME + ">",
                (double)nb * a.nsum * rows * a.ncy_in * a.nz * sizeof(c2<T>) +
-                   (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T));
+                   (double)nb * Storer::NOUT * rows * fyy.n * a.nz * sizeof(T),
+               (double)nb * a.nsum * rows * a.nz *
+                   ((double)(fyy.n / 2 + 1) * sizeof(c2<T>) + (double)fyy.n * sizeof(T)));
   ct_length(fyy.n, [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
     auto launch = [&](auto bs) {
@@ -345,7 +362,7 @@ void plane_sums(alfa_plan* P, hipStream_t s, int ncol, int nz, const F& f, doubl
                 double scale, const char* tag) {
   RedGeom g = red_geom(ncol, nz, esize(P));
   double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * NT * nz * sizeof(double));
-  if (P->ws.dry) return;
+  if (!live(P)) return;
   dim3 grid(g.nbz, g.nbc);
   {
     ProfScope ps(P, s, std::string("k_plane_partial<") + F::NAME + ">",
@@ -365,7 +382,7 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
   const int ncol = nx * ny;
   RedGeom g = red_geom(ncol, nz, esize(P));
   double* part = (double*)P->ws.get(std::string("red_") + tag, (size_t)g.nbc * 2 * nz * sizeof(double));
-  if (P->ws.dry) return;
+  if (!live(P)) return;
   dim3 grid(g.nbz, g.nbc);
   {
     ProfScope ps(P, s, "k_box_clip", 3.0 * ncol * nz * esize(P));
@@ -383,7 +400,7 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
 // n points handled w per thread
 #define PW_LAUNCH_W(P, s, n, w, kern, ...)                                \
   do {                                                                    \
-    if (!(P)->ws.dry) {                                                   \
+    if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, #kern, (double)pw_fields(#kern) * (n) * esize(P)); \
         kern<<<pw_blocks((n) / (w)), 256, 0, s>>>(__VA_ARGS__);           \
@@ -394,7 +411,7 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
 
 #define PW_LAUNCH(P, s, n, kern, ...)                                     \
   do {                                                                    \
-    if (!(P)->ws.dry) {                                                   \
+    if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, #kern, (double)pw_fields(#kern) * (n) * esize(P)); \
         kern<<<pw_blocks(n), 256, 0, s>>>(__VA_ARGS__);                   \
@@ -406,7 +423,7 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
 // Single kernel launch with a profiling record.
 #define KLAUNCH(P, s, name, bytes, kern, grid, block, ...)                \
   do {                                                                    \
-    if (!(P)->ws.dry) {                                                   \
+    if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, name, bytes);                                 \
         kern<<<grid, block, 0, s>>>(__VA_ARGS__);                         \
@@ -418,7 +435,7 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
 // KLAUNCH with dynamic LDS bytes
 #define KLAUNCH_LDS(P, s, name, bytes, kern, grid, block, lds, ...)       \
   do {                                                                    \
-    if (!(P)->ws.dry) {                                                   \
+    if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, name, bytes);                                 \
         kern<<<grid, block, lds, s>>>(__VA_ARGS__);                       \

@@ -942,7 +942,7 @@ __global__ void k_ab2(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ R
                       const T* __restrict__ dpdz, T dt, T th_top, int first,
                       double* cflpart) {
   const int nz = d.nz;
-  float mu = 0.f, mv = 0.f, mw = 0.f;
+  T mu = T(0), mv = T(0), mw = T(0);   // maxima in the field type (exact)
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
     const T ru = Rnu[i] - dpdx[i];
@@ -960,23 +960,23 @@ __global__ void k_ab2(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ R
       const T tn = TH[i] + dt * (T(1.5) * rt - T(0.5) * rpt);
       u[i] = un; v[i] = vn; TH[i] = tn;
       if (z == nz - 2) { u[i + 1] = un; v[i + 1] = vn; TH[i + 1] = tn + th_top; }
-      mu = fmaxf(mu, (float)fabs((double)un));
-      mv = fmaxf(mv, (float)fabs((double)vn));
+      mu = fmax(mu, fabs(un));
+      mv = fmax(mv, fabs(vn));
     }
     if (z == 0 || z == nz - 1) {
       w[i] = T(0);
     } else {
       const T wn = w[i] + dt * (T(1.5) * rw - T(0.5) * rpw);
       w[i] = wn;
-      mw = fmaxf(mw, (float)fabs((double)wn));
+      mw = fmax(mw, fabs(wn));
     }
   }
-  __shared__ float sm[3][256];
+  __shared__ T sm[3][256];
   sm[0][threadIdx.x] = mu; sm[1][threadIdx.x] = mv; sm[2][threadIdx.x] = mw;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s)
-      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmaxf(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0)
@@ -996,7 +996,7 @@ __global__ void k_ab2_v(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__
   constexpr int W = Vec16<T>::W;
   const int nz = d.nz;
   const size_t nv = d.n / W;
-  float mu = 0.f, mv = 0.f, mw = 0.f;
+  T mu = T(0), mv = T(0), mw = T(0);   // maxima in the field type (exact)
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
        t += (size_t)gridDim.x * blockDim.x) {
     const size_t i0 = t * W;
@@ -1024,8 +1024,8 @@ __global__ void k_ab2_v(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__
         uu.v[j] = uu.v[j] + dt * (T(1.5) * ru.v[j] - T(0.5) * rpu.v[j]);
         vv.v[j] = vv.v[j] + dt * (T(1.5) * rv.v[j] - T(0.5) * rpv.v[j]);
         tt.v[j] = tt.v[j] + dt * (T(1.5) * nt.v[j] - T(0.5) * rpt.v[j]);
-        mu = fmaxf(mu, (float)fabs((double)uu.v[j]));
-        mv = fmaxf(mv, (float)fabs((double)vv.v[j]));
+        mu = fmax(mu, fabs(uu.v[j]));
+        mv = fmax(mv, fabs(vv.v[j]));
       } else if (j > 0) {   // z == nz - 1: copies of level nz - 2
         uu.v[j] = uu.v[j - 1];
         vv.v[j] = vv.v[j - 1];
@@ -1035,17 +1035,17 @@ __global__ void k_ab2_v(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__
         ww.v[j] = T(0);
       } else {
         ww.v[j] = ww.v[j] + dt * (T(1.5) * rw.v[j] - T(0.5) * rpw.v[j]);
-        mw = fmaxf(mw, (float)fabs((double)ww.v[j]));
+        mw = fmax(mw, fabs(ww.v[j]));
       }
     }
     vst(u + i0, uu); vst(v + i0, vv); vst(TH + i0, tt); vst(w + i0, ww);
   }
-  __shared__ float sm[3][256];
+  __shared__ T sm[3][256];
   sm[0][threadIdx.x] = mu; sm[1][threadIdx.x] = mv; sm[2][threadIdx.x] = mw;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s)
-      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmaxf(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] = fmax(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0)

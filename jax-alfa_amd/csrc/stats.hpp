@@ -209,7 +209,7 @@ void stats_accumulate(alfa_plan* P, hipStream_t s, const alfa_state* st, double*
     FStatsMomentsQ<T> fq2{(const T*)st->Q, f1.w, m, mq, c.nz};
     plane_sums<2>(P, s, c.ncol, c.nz, fq2, moq, 1.0 / c.ncol, "stats5");
   }
-  if (P->ws.dry) return;
+  if (!live(P)) return;
   StatsProfiles pr{st->Cs2_1D_avg1, st->Cs2_1D_avg2, st->Cs2PrRatio_1D, st->beta1_1D,
                    st->beta2_1D};
   k_stats_add<<<1, 256, 0, s>>>(c.nz, m, mo, sfc, P->scal, pr, g.Ugal, g.T_0, d.z_scale, mq,

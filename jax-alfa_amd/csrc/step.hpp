@@ -348,12 +348,12 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     double* coef = c.D("lcoef", 6 * c.nz);
     FLasddSums<T> f{l1, l2, c.nz, d.L, d.TFR};
     plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasdd");
-    if (!P->ws.dry) {
+    if (live(P)) {
       k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 0, d.TFR, coef);
       k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
       launch_ok(P);
     }
-  } else if (!P->ws.dry) {
+  } else if (live(P)) {
     k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
     launch_ok(P);
   }
@@ -373,7 +373,7 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
   }
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cs2, s2, "box");
-  if (!P->ws.dry) {
+  if (live(P)) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
     if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
@@ -437,12 +437,12 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
     double* coef = c.D("scoef", 6 * c.nz);
     FScalarSums<T> f{l1, l2, c.nz, d.L, d.TFR};
     plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasdd");
-    if (!P->ws.dry) {
+    if (live(P)) {
       k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 1, d.TFR, coef);
       k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
       launch_ok(P);
     }
-  } else if (!P->ws.dry) {
+  } else if (live(P)) {
     k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
     launch_ok(P);
   }
@@ -456,7 +456,7 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
     PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CsPr, s2, "sbox");
-  if (!P->ws.dry) {
+  if (live(P)) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
     if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
@@ -502,12 +502,12 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
     double* coef = c.D("lcoef", 6 * c.nz);
     FWlSums<T> f{l1, l2, c.nz};
     plane_sums<4>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasddwl");
-    if (!P->ws.dry) {
+    if (live(P)) {
       k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 2, d.TFR, coef);
       k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
       launch_ok(P);
     }
-  } else if (!P->ws.dry) {
+  } else if (live(P)) {
     k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
     launch_ok(P);
   }
@@ -516,7 +516,7 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
   PW_LAUNCH(P, c.s, c.F, k_wl_lm<T>, c.dims(), l1, l2, beta, d.L, d.TFR, LM, MM);
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cwl, s2, "box");
-  if (!P->ws.dry) {
+  if (live(P)) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
     if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
@@ -564,12 +564,12 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
     double* coef = c.D("scoef", 6 * c.nz);
     FWlScalarSums<T> f{l1, l2, c.nz};
     plane_sums<4>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasddwl");
-    if (!P->ws.dry) {
+    if (live(P)) {
       k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 2, d.TFR, coef);
       k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
       launch_ok(P);
     }
-  } else if (!P->ws.dry) {
+  } else if (live(P)) {
     k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
     launch_ok(P);
   }
@@ -578,7 +578,7 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
   PW_LAUNCH(P, c.s, c.F, k_wl_scalar_updn<T>, c.dims(), l1, l2, beta, d.L, d.TFR, Tup, Tdn);
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CwlPr, s2, "sbox");
-  if (!P->ws.dry) {
+  if (live(P)) {
     k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
     if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
@@ -1078,7 +1078,7 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
   if (moist)   // AB2_Q (SCL_TimeAdvancement.py:79-99), RHS_Q rotation
     PW_LAUNCH(P, s, c.F, k_ab2_q<T>, c.dims(), SQ, (const T*)RnQ, SRQ, (T)d.dt_nondim,
               (T)(g.q_inversion * d.z_scale * d.dz), first ? 1 : 0);
-  if (!P->ws.dry) {
+  if (live(P)) {
     k_cfl_final<<<1, 256, 0, s>>>(cflpart, AB2_BLOCKS, d.dt_nondim / d.dx, d.dt_nondim / d.dy,
                                   d.dt_nondim / d.dz, P->scal, st->scalars);
     launch_ok(P);

@@ -112,13 +112,35 @@ class Grid:
 # src/operations
 # ---------------------------------------------------------------------------
 
+# FFT backend.  The parity tests use numpy.fft (pocketfft, one thread).  The
+# CPU-baseline timing sets FFT_WORKERS to run the same transforms through
+# scipy.fft (also pocketfft) on that many threads.
+FFT_WORKERS = None
+
+
+def set_fft_workers(n):
+    """None: numpy.fft; n >= 1: scipy.fft with workers=n."""
+    global FFT_WORKERS
+    FFT_WORKERS = None if n is None else int(n)
+
+
+def _rfft2(F):
+    if FFT_WORKERS is None:
+        return np.fft.rfft2(F, axes=(0, 1))
+    import scipy.fft
+    return scipy.fft.rfft2(F, axes=(0, 1), workers=FFT_WORKERS)
+
+
 def FFT(F):
     """src/operations/FFT.py:44-60: rfft2 over axes (0, 1)."""
-    return np.fft.rfft2(F, axes=(0, 1))
+    return _rfft2(F)
 
 
 def irfft2(Fh, s):
-    return np.fft.irfft2(Fh, s=s, axes=(0, 1))
+    if FFT_WORKERS is None:
+        return np.fft.irfft2(Fh, s=s, axes=(0, 1))
+    import scipy.fft
+    return scipy.fft.irfft2(Fh, s=s, axes=(0, 1), workers=FFT_WORKERS)
 
 
 def _cutoff_mask_apply(g, F_fft, cx, cy):
@@ -158,7 +180,7 @@ def Dealias1(g, F_fft):
 
 def FFT_pad(F_pad):
     """src/operations/FFT.py:68-84."""
-    return np.fft.rfft2(F_pad, axes=(0, 1))
+    return _rfft2(F_pad)
 
 
 def Dealias2(g, F_pad_fft):
@@ -228,7 +250,7 @@ def velocityGradients(g, u, v, w, u_fft, v_fft, w_fft, ustar, M_sfc_loc, MOSTfun
 
 def potentialTemperatureGradients(g, TH, ustar, qz_sfc, MOSTfunctions):
     """src/operations/Derivatives.py:107-153 (re-transforms TH, :140)."""
-    THf = np.fft.rfft2(TH, axes=(0, 1))
+    THf = _rfft2(TH)
     dTHdx = Derivxy(g, THf, g.kx2)
     dTHdy = Derivxy(g, THf, g.ky2)
     dTHdz = Derivz_TH(g, TH, ustar, qz_sfc, MOSTfunctions[5])
@@ -555,8 +577,8 @@ def PressureRC(g, u, v, w, RHS_u, RHS_v, RHS_w, Ru_p, Rv_p, Rw_p, divtz):
     nyr = g.ny_rfft
     RC_r = np.zeros((g.nx, nyr, nz + 1))
     RC_i = np.zeros((g.nx, nyr, nz + 1))
-    fbot = np.fft.rfft2(divtz[:, :, 0], axes=(0, 1))
-    ftop = np.fft.rfft2(divtz[:, :, -1], axes=(0, 1))
+    fbot = _rfft2(divtz[:, :, 0])
+    ftop = _rfft2(divtz[:, :, -1])
     RC_r[:, :, 0] = -g.dz * fbot.real
     RC_i[:, :, 0] = -g.dz * fbot.imag
     RC_r[:, :, nz] = -g.dz * ftop.real

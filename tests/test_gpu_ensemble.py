@@ -60,12 +60,50 @@ def test_served_result_is_the_same_alone_and_in_the_mix():
             assert d == 0.0
 
 
+def _solo(spec, steps, prec):
+    cfg, u, v, w, TH, ser = spec.build()
+    les = core.LES(core.Plan(cfg, prec), u, v, w, TH, sfc_series=ser)
+    les.step(steps)
+    torch.cuda.synchronize()
+    out = {f: les.host(f) for f in ("u", "v", "w", "TH")}
+    out["CFL"] = les.diag()["CFL"]
+    les.plan.close()
+    return out
+
+
 def test_small_ensemble_runs():
+    """Four members of three grid sizes stepped together equal each member
+    stepped alone, bit for bit, and the throughput is reported."""
     specs = [MemberSpec("convective", n, 6, seed=i, over=dict(optSurfFlux=0))
              for i, n in enumerate((32, 48, 32, 64))]
-    out = run_ensemble(specs, steps=6, nstreams=2)
-    print(out)
+    out = run_ensemble(specs, steps=6, nstreams=2, keep_fields=True)
+    print({k: v for k, v in out.items() if k != "fields"})
     assert out["members"] == 4 and out["mpoint_steps_per_s"] > 0
+    for spec, got in zip(specs, out["fields"]):
+        ref = _solo(spec, 6, "mixed")
+        for k in ("u", "v", "w", "TH", "CFL"):
+            assert np.array_equal(got[k], ref[k]), (spec.n, k)
+
+
+def test_config3_ensemble_at_full_size():
+    """The batched-ensemble configuration at its stated size: 64 convective
+    members on grids drawn from 48/64/80/96^3 with seeded Ug in [2, 12] m/s
+    and surface heat flux in [0.05, 0.3] K m/s, mixed precision, stepped
+    together (graph replays after the first step).  After 4 steps every
+    member's fields and CFL are finite and bitwise those of the member run
+    alone."""
+    from alfa_amd.ensemble import ensemble_specs
+    specs = ensemble_specs(64, steps=4, seed=0)
+    assert sorted({s.n for s in specs}) == [48, 64, 80, 96]
+    out = run_ensemble(specs, steps=4, keep_fields=True)
+    print({k: v for k, v in out.items() if k != "fields"})
+    for i, (spec, got) in enumerate(zip(specs, out["fields"])):
+        for k in ("u", "v", "w", "TH"):
+            assert np.all(np.isfinite(got[k])), (i, k)
+        assert np.isfinite(got["CFL"]) and got["CFL"] > 0
+        ref = _solo(spec, 4, "mixed")
+        for k in ("u", "v", "w", "TH", "CFL"):
+            assert np.array_equal(got[k], ref[k]), (i, spec.n, k)
 
 
 def test_checkpoint_restart_is_exact(tmp_path):

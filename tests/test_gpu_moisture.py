@@ -90,8 +90,13 @@ def run_both(key, prec, nsteps, with_stats=False):
 
 
 @pytest.mark.parametrize("key", list(CASES))
-def test_moisture_step_fp64(key):
+def test_moisture_step_fp64(key, tmp_path):
     s, les, errs, _, _ = run_both(key, "fp64", 5)
+    # the 3-D snapshot of a moist run carries Q (Main.py:630-631)
+    from alfa_amd import io
+    with np.load(io.save_3d_fields(str(tmp_path), 5, les)) as f:
+        assert sorted(f.keys()) == ["Q", "TH", "u", "v", "w"]
+        np.testing.assert_array_equal(f["Q"], les.host("Q"))
     worst = {k: max(e[k] for e in errs) for k in errs[0]}
     print(key, "fp64 max rel err:", {k: "%.2e" % v for k, v in worst.items()})
     assert max(worst.values()) < 1e-11

@@ -29,6 +29,12 @@ GRIDS = {
     "neutral32": ("neutral", dict()),
     "conv48": ("convective", dict(nx=48, ny=48, nz=32)),
     "stable40": ("stable", dict(nx=40, ny=40, nz=24, optSgs=1)),
+    # the other surface heat-flux variants (SurfaceFlux.py:246, 290, 393):
+    # a prescribed flux series, homogeneous and heterogeneous, and a
+    # prescribed surface temperature over a heterogeneous surface
+    "conv_varflux": ("convective", dict(nx=32, ny=32, nz=16, optSurfBC=1, optSurfFlux=0)),
+    "conv_varflux_het": ("convective", dict(nx=32, ny=32, nz=16, optSurfBC=1, optSurfFlux=1)),
+    "stable_het": ("stable", dict(nx=32, ny=32, nz=16, optSurfFlux=1)),
 }
 
 _plans = {}
@@ -256,6 +262,8 @@ def test_surface(grid, prec):
     md = tuple(P.asdev(m) for m in most)
     fn = {(0, 0): ops.SurfaceFlux_HomogeneousConstantFlux,
           (0, 1): ops.SurfaceFlux_HeterogeneousConstantFlux,
+          (1, 0): ops.SurfaceFlux_HomogeneousVaryingFlux,
+          (1, 1): ops.SurfaceFlux_HeterogeneousVaryingFlux,
           (2, 0): ops.SurfaceFlux_HomogeneousPrescribedTemperature,
           (2, 1): ops.SurfaceFlux_HeterogeneousPrescribedTemperature}[(cfg["optSurfBC"],
                                                                        cfg["optSurfFlux"])]
