@@ -243,6 +243,17 @@ int alfa_derivz_generic_w(alfa_plan*, const void* F, void* dFdz, alfa_stream);
 /* ---- utilities (src/utilities/Utilities.py) ------------------------------ */
 /* PlanarMean (:41): (nx,ny,nz) -> (nz,) double. */
 int alfa_planar_mean(alfa_plan*, const void* F, double* mean, alfa_stream);
+/* alfa_planar_mean without a plan and without any allocation: for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  precision ALFA_FP64 (double F) or ALFA_MIXED (float F); the
+ * same kernels and summation order as alfa_planar_mean, so the means are
+ * bitwise equal.  The partial sums go to a fixed device array of the
+ * library (4 MiB: 2^19 doubles, one per column block and level); a shape
+ * needing more returns ALFA_EINVAL.  Calls that overlap on one GPU share that
+ * array and race. */
+int alfa_planar_mean_noalloc(int precision, int nx, int ny, int nz, const void* F,
+                             double* mean, alfa_stream);
 /* StagGridAvg (:64): (nx,ny,nz) -> (nx,ny,nz-1). */
 int alfa_stag_grid_avg(alfa_plan*, const void* F, void* out, alfa_stream);
 /* Imfilter (:87) applied to every level of an (nx,ny,nz) field. */

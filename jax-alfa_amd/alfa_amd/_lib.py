@@ -138,6 +138,7 @@ _SIGS = {
     "alfa_surface_flux": [_P, _P, _P, _P, C.c_double] + [_P] * 12,
     "alfa_profile_begin": [_P, C.c_char_p, C.c_int],
     "alfa_profile_end": [_P, C.POINTER(AlfaKernelStat), C.c_int, C.POINTER(C.c_int)],
+    "alfa_planar_mean_noalloc": [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P],
 }
 
 _lib = None

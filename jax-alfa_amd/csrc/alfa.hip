@@ -1326,3 +1326,77 @@ extern "C" int alfa_profile_end(alfa_plan* P, alfa_kernel_stat* out, int max_out
   free_prof(P);
   return ALFA_OK;
 }
+
+// ---------------------------------------------------------------------------
+// PlanarMean without a plan and without allocation.  The partial sums of
+// the plane reduction go to `scratch`, a file-scope device array of a fixed
+// size (every GPU that runs this library holds its own copy until the
+// process ends).  Host code never uses the array's name as an address: on
+// the first call on a device (with that device current) its device address
+// is looked up with hipGetSymbolAddress and kept per device.  Every call on
+// one GPU shares that GPU's array: calls that overlap on one GPU (two
+// streams, two threads) race on it.  Every partial is written before it is
+// read, so nothing has to be cleared between calls.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)1 << 19;   // 4 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;   // per device ordinal
+
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(ALFA_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    double* p = nullptr;
+    e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(ALFA_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ALFA_OK;
+}
+
+template <class T>
+static void planar_mean_noalloc_t(int nx, int ny, int nz, const void* F, double* mean,
+                                  double* part, hipStream_t s) {
+  // the same geometry, kernels and summation order as alfa_planar_mean
+  const int ncol = nx * ny;
+  const RedGeom g = red_geom(ncol, nz, sizeof(T));
+  FSum1<T> f{(const T*)F, nz, 0.0};
+  k_plane_partial<1, FSum1<T>><<<dim3(g.nbz, g.nbc), RED_BS, 0, s>>>(g, f, part);
+  k_plane_final<<<(nz + 3) / 4, 256, 0, s>>>(g.nbc, 1, nz, part, mean, 1.0 / ncol);
+}
+
+extern "C" int alfa_planar_mean_noalloc(int precision, int nx, int ny, int nz, const void* F,
+                                        double* mean, alfa_stream stream) {
+  if (precision != ALFA_FP64 && precision != ALFA_MIXED)
+    return set_error(ALFA_EINVAL, "alfa_planar_mean_noalloc: precision %d", precision);
+  if (nx <= 0 || ny <= 0 || nz <= 0 || (long long)nx * ny > (1LL << 30))
+    return set_error(ALFA_EINVAL, "alfa_planar_mean_noalloc: bad shape (%d, %d, %d)", nx, ny, nz);
+  if (!F || !mean) return set_error(ALFA_EINVAL, "alfa_planar_mean_noalloc: null argument");
+  const RedGeom g = red_geom(nx * ny, nz, precision == ALFA_FP64 ? 8 : 4);
+  const size_t need = (size_t)g.nbc * nz;
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    return set_error(ALFA_EINVAL,
+                     "alfa_planar_mean_noalloc: (%d, %d, %d) needs %zu partial sums, more than "
+                     "the %zu of its fixed scratch (use alfa_planar_mean with a plan)",
+                     nx, ny, nz, need, NOALLOC_SCRATCH_DOUBLES);
+  double* part = nullptr;
+  int rc = noalloc_scratch(&part);
+  if (rc != ALFA_OK) return rc;
+  hipStream_t s = STREAM(stream);
+  if (precision == ALFA_FP64) planar_mean_noalloc_t<double>(nx, ny, nz, F, mean, part, s);
+  else planar_mean_noalloc_t<float>(nx, ny, nz, F, mean, part, s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(ALFA_EHIP, "alfa_planar_mean_noalloc: %s", hipGetErrorString(e));
+  return ALFA_OK;
+}
