@@ -271,17 +271,21 @@ def main():
     if not args.no_ensemble:
         from alfa_amd import ensemble as E
         specs = E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps, seed=0)
-        ens = E.run_ensemble(specs)
-        # a second, identical run beside the first: the first streams of a
-        # process ran ensembles about 25% slower, which the runner avoids by
-        # setting them aside; the two figures should now agree
-        rep = E.run_ensemble(E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps,
-                                              seed=0))
-        ens["repeat_mpoint_steps_per_s"] = rep["mpoint_steps_per_s"]
+        # shared launches: one launch per kernel for all members of a grid
+        # size, the whole ensemble step one graph replay
+        ens = E.run_ensemble_grouped(specs)
+        torch.cuda.empty_cache()
+        # beside it, the members stepped one by one (a graph per member step
+        # on 4 streams)
+        solo = E.run_ensemble(E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps,
+                                               seed=0))
+        ens["per_member_launches"] = {k: solo[k] for k in ("mpoint_steps_per_s", "wall_s",
+                                                           "streams")}
         torch.cuda.empty_cache()
         ens["workload"] = (f"{args.ensemble_members} convective members, grids from "
-                           f"48/64/80/96^3, seeded Ug and surface flux, mixed, one hipGraph "
-                           f"per member step, {ens['streams']} streams")
+                           f"48/64/80/96^3, seeded Ug and surface flux, mixed, "
+                           f"{args.ensemble_steps} steps; members of one grid size share every "
+                           f"launch (alfa_group), the ensemble step is one hipGraph")
         log(f"[bench] ensemble: {ens}")
         reqs = E.serving_requests(args.serve_requests, rate=2.0, steps=tuple(args.serve_steps),
                                   seed=0)

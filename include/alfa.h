@@ -168,6 +168,30 @@ int alfa_step(alfa_plan* plan, const alfa_state* st, int iteration, double sfc_v
 int alfa_batch_step(alfa_plan* const* plans, const alfa_state* states, int n,
                     const int* iterations, const double* sfc_vals, alfa_stream stream);
 
+/* Member group: n runs of one configuration shape (same grid, precision,
+ * options and FFT tiles; their physical constants, forcing profiles and
+ * initial fields may differ) advanced by shared launches.  Each kernel of
+ * the step is enqueued once for the whole group; the group's members are
+ * the z index of the launch grid and every member reads its own arguments
+ * (its state, workspace, constants) from a device table.  Each member's
+ * fields are bitwise those of alfa_step on that member alone.
+ * sfc_vals[m]: device pointer to member m's surface value of the step (as
+ * alfa_step_sfc_dev; NULL allowed where optSurfBC is 0).  The group keeps
+ * the plan and state pointers; they must outlive it.
+ * alfa_group_step builds the member tables of a kind of step (first step,
+ * dynamic or static SGS step, forcing and tiles as set) the first time it
+ * meets it: that call allocates and copies (not capturable); later steps of
+ * that kind enqueue kernels only and can be captured into a hipGraph.
+ * alfa_group_launches: kernels per group step of that kind (-1 before its
+ * first step).  Replaces, for ensembles, the per-run loop of
+ * src/Main.py:156-540 (the reference runs one case per process). */
+typedef struct alfa_group alfa_group;
+int alfa_group_create(alfa_plan* const* plans, const alfa_state* states,
+                      const double* const* sfc_vals, int n, alfa_group** out);
+int alfa_group_step(alfa_group* group, int iteration, alfa_stream stream);
+int alfa_group_launches(const alfa_group* group, int iteration, int* n);
+int alfa_group_destroy(alfa_group* group);
+
 /* ---- time-varying forcing (src/Main.py:167-188) --------------------------
  * Sets the geostrophic wind profiles Ug, Vg (optGeoWind >= 1:
  * GeoWind_U/V[iteration-1], Initialization.py:173-254) and the large-scale

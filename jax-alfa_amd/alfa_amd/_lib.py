@@ -139,6 +139,10 @@ _SIGS = {
     "alfa_profile_begin": [_P, C.c_char_p, C.c_int],
     "alfa_profile_end": [_P, C.POINTER(AlfaKernelStat), C.c_int, C.POINTER(C.c_int)],
     "alfa_planar_mean_noalloc": [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P],
+    "alfa_group_create": [_P, _P, _P, C.c_int, _P],
+    "alfa_group_step": [_P, C.c_int, _P],
+    "alfa_group_launches": [_P, C.c_int, C.POINTER(C.c_int)],
+    "alfa_group_destroy": [_P],
 }
 
 _lib = None

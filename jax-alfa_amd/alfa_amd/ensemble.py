@@ -210,12 +210,182 @@ class Member:
         self.plan.close()
 
 
+# configuration entries that decide a step's launch sequence: members that
+# agree in these (and in grid and precision) can share launches
+_SHAPE_KEYS = ("FGR", "optSgs", "dynamicSGS_call_time", "optSurfBC", "optSurfFlux",
+               "optMoisture", "optMoistureSurfBC", "optBuoyancy", "optDamping",
+               "optPressureSolver", "istep")
+
+
+def shape_key(m):
+    return (m.spec.n, m.plan.precision) + tuple(m.cfg.get(k) for k in _SHAPE_KEYS)
+
+
+class MemberGroup:
+    """Members of one configuration shape advanced by shared launches
+    (alfa_group): every kernel of the step runs once for the whole group,
+    each member reading its own arguments, so a member's fields are bitwise
+    those of its run alone."""
+
+    def __init__(self, members):
+        self.members = members
+        n = len(members)
+        lib = members[0].plan.lib
+        self.lib = lib
+        plans = (C.c_void_p * n)(*[m.plan.handle.value for m in members])
+        states = (_lib.AlfaState * n)(*[m.les.state for m in members])
+        sfc = (C.c_void_p * n)(*[m.sfc.data_ptr() for m in members])
+        self._h = C.c_void_p()
+        _lib.check(lib.alfa_group_create(C.cast(plans, C.c_void_p), C.cast(states, C.c_void_p),
+                                         C.cast(sfc, C.c_void_p), n, C.byref(self._h)),
+                   "alfa_group_create")
+
+    @property
+    def iteration(self):
+        return self.members[0].les.iteration
+
+    def set_surface(self):
+        """Each member's surface value of the coming step, on the device."""
+        for m in self.members:
+            if m.les.sfc_series is not None:
+                m.sfc.fill_(float(m.les.sfc_series[m.les.iteration - 1]))
+
+    def enqueue(self):
+        """Enqueue one group step on the current stream (no host sync)."""
+        _lib.check(self.lib.alfa_group_step(self._h, int(self.iteration), core.current_stream()),
+                   "alfa_group_step")
+
+    def advance(self):
+        for m in self.members:
+            m.les.iteration += 1
+            m.done += 1
+
+    def launches(self):
+        n = C.c_int(-1)
+        _lib.check(self.lib.alfa_group_launches(self._h, int(self.iteration), C.byref(n)),
+                   "alfa_group_launches")
+        return n.value
+
+    def close(self):
+        if self._h.value:
+            self.lib.alfa_group_destroy(self._h)
+            self._h = C.c_void_p()
+
+
+def run_ensemble_grouped(specs: List[MemberSpec], steps=None, precision="mixed", graphs=True,
+                         warmup=3, keep_fields=False, parallel=True):
+    """Config 3 with shared launches: members are grouped by configuration
+    shape (grid, precision, options) and each group's step is one launch
+    per kernel over its members; after one eager step of each kind the
+    whole ensemble step (every group) is captured into one hipGraph per
+    kind and replayed.  parallel: each group on its own stream, forked from
+    and joined back into the step's stream (parallel branches of the graph),
+    so that the small groups' launches overlap the large ones.  Returns the
+    run_ensemble figures."""
+    torch = core._torch()
+    stream = member_streams(1)[0]
+    t0 = time.perf_counter()
+    members = [Member(s, precision, stream, graphs=False) for s in specs]
+    by_key = {}
+    for m in members:
+        by_key.setdefault(shape_key(m), []).append(m)
+    with torch.cuda.stream(stream):
+        # largest groups first: their branches start first
+        groups = [MemberGroup(ms) for ms in sorted(by_key.values(),
+                                                     key=lambda ms: -sum(m.points for m in ms))]
+    side = member_streams(len(groups)) if parallel and len(groups) > 1 else None
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    nsteps = steps or specs[0].steps
+    graphs_by_kind = {}
+
+    def kind():
+        # the step kind of every group (first step, dynamic or static SGS):
+        # a captured graph replays each group's program of that kind
+        out = []
+        for g in groups:
+            m = g.members[0]
+            it = m.les.iteration
+            out.append((it == m.cfg.get("istep", 1), m._dynamic(it)))
+        return tuple(out)
+
+    def enqueue_all():
+        if side is None:
+            for g in groups:
+                g.enqueue()
+            return
+        fork = torch.cuda.Event()
+        fork.record(stream)
+        joins = []
+        for g, st in zip(groups, side):
+            st.wait_event(fork)
+            with torch.cuda.stream(st):
+                g.enqueue()
+            e = torch.cuda.Event()
+            e.record(st)
+            joins.append(e)
+        for e in joins:
+            stream.wait_event(e)
+
+    def one_step(capture):
+        with torch.cuda.stream(stream):
+            for g in groups:
+                g.set_surface()
+            k = kind()
+            if not capture or not graphs:
+                enqueue_all()
+            elif k in graphs_by_kind:
+                graphs_by_kind[k].replay()
+            else:
+                gr = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(gr, stream=stream):
+                    enqueue_all()
+                graphs_by_kind[k] = gr
+                gr.replay()
+            for g in groups:
+                g.advance()
+
+    # untimed: the first step of each kind builds its member tables (eager),
+    # the next one is captured
+    warmup = max(1, min(max(3, warmup), nsteps - 1))
+    seen = set()
+    for _ in range(warmup):
+        k = kind()
+        one_step(capture=k in seen)
+        seen.add(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps - warmup):
+        k = kind()
+        one_step(capture=k in seen)
+        seen.add(k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pts = sum(m.points for m in members) * (nsteps - warmup)
+    out = dict(members=len(members), steps_timed=nsteps - warmup, wall_s=wall, setup_s=t_setup,
+               mpoint_steps_per_s=pts / wall / 1e6, groups=len(groups),
+               streams=len(side) if side else 1,
+               launches_per_step=sum(g.launches() for g in groups), graphs=graphs,
+               grids={str(n): sum(1 for s in specs if s.n == n) for n in sorted({s.n for s in specs})},
+               hbm_bytes=sum(m.bytes for m in members))
+    if keep_fields:
+        out["fields"] = [dict({f: m.les.host(f) for f in ("u", "v", "w", "TH")},
+                              CFL=m.les.diag()["CFL"]) for m in members]
+    graphs_by_kind.clear()
+    for g in groups:
+        g.close()
+    for m in members:
+        m.close()
+    return out
+
+
 def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mixed",
                  graphs=True, log=None, warmup=2, keep_fields=False):
     """Step all members together for `steps` steps (default: each spec's
-    own length, all equal in config 3).  Returns throughput figures, and
-    with keep_fields each member's final u, v, w, TH and CFL (host copies)
-    under "fields"."""
+    own length, all equal in config 3), each member with its own launches
+    and step graph on one of nstreams streams.  Returns throughput figures,
+    and with keep_fields each member's final u, v, w, TH and CFL (host
+    copies) under "fields"."""
     torch = core._torch()
     streams = member_streams(nstreams)
     t0 = time.perf_counter()

@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 #include "step.hpp"
@@ -126,7 +127,7 @@ int filtering_explicit(alfa_plan* P, hipStream_t s, const void* Ffft, void* F, v
   const auto& d = P->d;
   if (!live(P)) return 0;
   const size_t n = 2 * (size_t)c.nx * c.nyc * c.nz;
-  k_fill<T><<<pw_blocks(n), 256, 0, s>>>(n, (T*)Fnew, T(0));
+  launch_k<1024>(P, s, dim3(pw_blocks(n)), dim3(256), 0, KBODY(k_fill<T>), n, (T*)Fnew, T(0));
   launch_ok(P);
   // masked spectrum at full pitch: only the first cut0_y columns are
   // processed (grid), columns >= cut0_y keep the zeros written above
@@ -320,7 +321,7 @@ int thomas_solve(alfa_plan* P, hipStream_t s, const void* RCr, const void* RCi, 
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
   if (live(P)) {
-    k_thomas<T><<<c.nx * c.nyc, 64, thomas_lds(c.nz), s>>>(ta);
+    launch_k<64>(P, s, dim3(c.nx * c.nyc), dim3(64), thomas_lds(c.nz), KBODY(k_thomas<T>), ta);
     launch_ok(P);
   }
   XArgs xa{};
@@ -751,15 +752,15 @@ extern "C" int alfa_state_init(alfa_plan* P, const alfa_state* st, alfa_stream s
   const size_t F = (size_t)P->nx * P->ny * P->nz, pl = (size_t)P->nx * P->ny;
   DISPATCH(P, "alfa_state_init", {
     void* zero3[4] = {st->RHS_u, st->RHS_v, st->RHS_w, st->RHS_TH};
-    for (void* z : zero3) k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)z, T(0));
-    k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)st->Cs2_3D, (T)P->cfg.Cs2);
-    k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)st->Cs2PrRatio_3D, (T)P->cfg.Cs2PrRatio);
+    for (void* z : zero3) launch_k<1024>(P, s, dim3(pw_blocks(F)), dim3(256), 0, KBODY(k_fill<T>), F, (T*)z, T(0));
+    launch_k<1024>(P, s, dim3(pw_blocks(F)), dim3(256), 0, KBODY(k_fill<T>), F, (T*)st->Cs2_3D, (T)P->cfg.Cs2);
+    launch_k<1024>(P, s, dim3(pw_blocks(F)), dim3(256), 0, KBODY(k_fill<T>), F, (T*)st->Cs2PrRatio_3D, (T)P->cfg.Cs2PrRatio);
     void* zero2[4] = {st->psi_m, st->psi_m0, st->psi_h, st->psi_h0};
-    for (void* z : zero2) k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)z, T(0));
-    k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)st->fi_m, T(1));
-    k_fill<T><<<pw_blocks(pl), 256, 0, s>>>(pl, (T*)st->fi_h, T(1));
+    for (void* z : zero2) launch_k<1024>(P, s, dim3(pw_blocks(pl)), dim3(256), 0, KBODY(k_fill<T>), pl, (T*)z, T(0));
+    launch_k<1024>(P, s, dim3(pw_blocks(pl)), dim3(256), 0, KBODY(k_fill<T>), pl, (T*)st->fi_m, T(1));
+    launch_k<1024>(P, s, dim3(pw_blocks(pl)), dim3(256), 0, KBODY(k_fill<T>), pl, (T*)st->fi_h, T(1));
     if (P->cfg.optMoisture >= 1 && st->RHS_Q)   // RHS_Q_previous = 0 (Main.py:73)
-      k_fill<T><<<pw_blocks(F), 256, 0, s>>>(F, (T*)st->RHS_Q, T(0));
+      launch_k<1024>(P, s, dim3(pw_blocks(F)), dim3(256), 0, KBODY(k_fill<T>), F, (T*)st->RHS_Q, T(0));
     launch_ok(P);
   });
 }
@@ -964,7 +965,7 @@ extern "C" int alfa_set_forcing(alfa_plan* P, const double* Ug, const double* Vg
   if (!P) return set_error(ALFA_EINVAL, "alfa_set_forcing: null plan");
   if (!Ug && !Vg && !Uadv && !Vadv && !THadv && !Qadv) return ALFA_OK;
   hipStream_t s = STREAM(stream);
-  k_set_forcing<<<(P->nz + 255) / 256, 256, 0, s>>>(P->nz, Ug, Vg, Uadv, Vadv, THadv, Qadv,
+  launch_k<1024>(P, s, dim3((P->nz + 255) / 256), dim3(256), 0, KBODY(k_set_forcing), P->nz, Ug, Vg, Uadv, Vadv, THadv, Qadv,
                                                      P->prof);
   launch_ok(P);
   if (Uadv || Vadv || THadv || Qadv) P->adv_on = true;
@@ -995,6 +996,171 @@ extern "C" int alfa_batch_step(alfa_plan* const* plans, const alfa_state* states
     rc = check_hip(P, "alfa_batch_step");
     if (rc) return rc;
   }
+  return ALFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Member groups: runs of one configuration shape stepped by shared launches.
+// Each member's step is recorded once per step kind (first step; dynamic or
+// static SGS step; forcing on or off; FFT tiles) and the records of all
+// members must match launch for launch.  The packed arguments of every
+// launch are laid out member by member in one device table, uploaded once
+// per kind; a group step then enqueues each kernel once, as k_member over
+// the group (blockIdx.z = member), with no host-device traffic, so it can be
+// captured into a hipGraph after one step of each kind.
+// ---------------------------------------------------------------------------
+struct GroupProgram {
+  std::vector<LaunchRec> recs;   // member 0's records (identity, shape, launcher)
+  std::vector<size_t> off;       // table offset of each launch
+  void* dev = nullptr;
+};
+
+struct alfa_group {
+  std::vector<alfa_plan*> plans;
+  std::vector<alfa_state> states;
+  std::vector<const double*> sfc;
+  int prec = 0;
+  std::map<long long, GroupProgram> progs;
+};
+
+static bool step_dynamic(const alfa_plan* P, int it) {
+  const auto& g = P->cfg;
+  return (g.optSgs >= 1 && g.optSgs <= 4) &&
+         (it == g.istep || (g.dynamicSGS_call_time > 0 && it % g.dynamicSGS_call_time == 0));
+}
+
+static long long step_kind(const alfa_plan* P, int it) {
+  long long k = (it == P->cfg.istep ? 1 : 0) | (step_dynamic(P, it) ? 2 : 0) |
+                (P->adv_on ? 4 : 0);
+  k |= (long long)P->tile_r2c << 3;
+  k |= (long long)P->tile_x << 13;
+  k |= (long long)P->tile_c2r << 23;
+  return k;
+}
+
+static void free_group(alfa_group* G) {
+  if (!G) return;
+  for (auto& kv : G->progs)
+    if (kv.second.dev) (void)hipFree(kv.second.dev);
+  delete G;
+}
+
+extern "C" int alfa_group_create(alfa_plan* const* plans, const alfa_state* states,
+                                 const double* const* sfc_vals, int n, alfa_group** out) {
+  if (!out || n <= 0 || !plans || !states)
+    return set_error(ALFA_EINVAL, "alfa_group_create: empty or null member table");
+  *out = nullptr;
+  alfa_group* G = new alfa_group();
+  G->prec = plans[0] ? plans[0]->prec : 0;
+  for (int m = 0; m < n; ++m) {
+    alfa_plan* P = plans[m];
+    if (!P) { free_group(G); return set_error(ALFA_EINVAL, "alfa_group_create: member %d has no plan", m); }
+    if (P->prec != G->prec || P->nx != plans[0]->nx || P->ny != plans[0]->ny ||
+        P->nz != plans[0]->nz) {
+      free_group(G);
+      return set_error(ALFA_EINVAL, "alfa_group_create: member %d differs from member 0 in "
+                                    "grid or precision", m);
+    }
+    if (int rc = check_moist_state(P, &states[m], "alfa_group_create")) { free_group(G); return rc; }
+    const double* sv = sfc_vals ? sfc_vals[m] : nullptr;
+    if (!sv && P->cfg.optSurfBC != 0) {
+      free_group(G);
+      return set_error(ALFA_EINVAL, "alfa_group_create: member %d (optSurfBC %d) needs a "
+                                    "device surface value", m, P->cfg.optSurfBC);
+    }
+    G->plans.push_back(P);
+    G->states.push_back(states[m]);
+    G->sfc.push_back(sv);
+  }
+  *out = G;
+  return ALFA_OK;
+}
+
+template <class T>
+static int group_record(alfa_group* G, int it, hipStream_t s, GroupProgram& pr) {
+  const int M = (int)G->plans.size();
+  std::vector<std::vector<LaunchRec>> recs(M);
+  for (int m = 0; m < M; ++m) {
+    alfa_plan* P = G->plans[m];
+    P->rec = &recs[m];
+    step_t<T>(P, &G->states[m], it, 0.0, s, G->sfc[m]);
+    P->rec = nullptr;
+    if (int rc = check_hip(P, "alfa_group_step (recording)")) return rc;
+  }
+  for (int m = 1; m < M; ++m) {
+    bool same = recs[m].size() == recs[0].size();
+    for (size_t r = 0; same && r < recs[0].size(); ++r) {
+      const LaunchRec &a = recs[0][r], &b = recs[m][r];
+      same = a.id == b.id && a.grid.x == b.grid.x && a.grid.y == b.grid.y &&
+             a.grid.z == b.grid.z && a.block.x == b.block.x && a.block.y == b.block.y &&
+             a.block.z == b.block.z && a.lds == b.lds && a.pack.size() == b.pack.size();
+    }
+    if (!same)
+      return set_error(ALFA_EINVAL, "alfa_group_step: member %d's step launches differ from "
+                                    "member 0's (the members' configurations must match in "
+                                    "everything but their physical constants)", m);
+  }
+  // tables: launch r at a 256-byte aligned offset, one pack per member at
+  // the pack's own size (its array stride in the kernel)
+  size_t total = 0;
+  pr.off.resize(recs[0].size());
+  for (size_t r = 0; r < recs[0].size(); ++r) {
+    total = (total + 255) / 256 * 256;
+    pr.off[r] = total;
+    total += (size_t)M * recs[0][r].pack.size();
+  }
+  std::vector<unsigned char> host(total ? total : 1, 0);
+  for (size_t r = 0; r < recs[0].size(); ++r) {
+    const size_t ps = recs[0][r].pack.size();
+    for (int m = 0; m < M; ++m)
+      std::memcpy(host.data() + pr.off[r] + (size_t)m * ps, recs[m][r].pack.data(), ps);
+  }
+  hipError_t e = hipMalloc(&pr.dev, host.size());
+  if (e == hipSuccess) e = hipMemcpy(pr.dev, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (pr.dev) (void)hipFree(pr.dev);
+    pr.dev = nullptr;
+    return set_error(ALFA_EHIP, "alfa_group_step: member tables: %s", hipGetErrorString(e));
+  }
+  pr.recs = std::move(recs[0]);
+  return ALFA_OK;
+}
+
+extern "C" int alfa_group_step(alfa_group* G, int iteration, alfa_stream stream) {
+  if (!G) return set_error(ALFA_EINVAL, "alfa_group_step: null group");
+  hipStream_t s = STREAM(stream);
+  const long long kind = step_kind(G->plans[0], iteration);
+  for (size_t m = 1; m < G->plans.size(); ++m)
+    if (step_kind(G->plans[m], iteration) != kind)
+      return set_error(ALFA_EINVAL, "alfa_group_step: member %zu takes another kind of step "
+                                    "at iteration %d (first/dynamic/forcing/tiles)", m, iteration);
+  auto it = G->progs.find(kind);
+  if (it == G->progs.end()) {
+    GroupProgram pr;
+    const int rc = G->prec == ALFA_FP64 ? group_record<double>(G, iteration, s, pr)
+                                        : group_record<float>(G, iteration, s, pr);
+    if (rc != ALFA_OK) return rc;
+    it = G->progs.emplace(kind, std::move(pr)).first;
+  }
+  const GroupProgram& pr = it->second;
+  const int M = (int)G->plans.size();
+  for (size_t r = 0; r < pr.recs.size(); ++r)
+    pr.recs[r].member((const unsigned char*)pr.dev + pr.off[r], M, s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(ALFA_EHIP, "alfa_group_step: %s", hipGetErrorString(e));
+  return ALFA_OK;
+}
+
+extern "C" int alfa_group_launches(const alfa_group* G, int iteration, int* n) {
+  if (!G || !n) return set_error(ALFA_EINVAL, "alfa_group_launches: null argument");
+  auto it = G->progs.find(step_kind(G->plans[0], iteration));
+  *n = it == G->progs.end() ? -1 : (int)it->second.recs.size();
+  return ALFA_OK;
+}
+
+extern "C" int alfa_group_destroy(alfa_group* G) {
+  if (G) (void)hipDeviceSynchronize();
+  free_group(G);
   return ALFA_OK;
 }
 
@@ -1083,7 +1249,7 @@ extern "C" int alfa_compute_beta(alfa_plan* P, const double* coeffs, int n, doub
                                  alfa_stream s) {
   if (!P) return set_error(ALFA_EINVAL, "alfa_compute_beta: null plan");
   if (n <= 0) return set_error(ALFA_EINVAL, "alfa_compute_beta: n must be positive");
-  k_beta<<<(n + 7) / 8, 64, 0, STREAM(s)>>>(coeffs, n, beta);
+  launch_k<1024>(P, STREAM(s), dim3((n + 7) / 8), dim3(64), 0, KBODY(k_beta), coeffs, n, beta);
   launch_ok(P);
   return check_hip(P, "alfa_compute_beta");
 }
@@ -1261,7 +1427,7 @@ extern "C" int alfa_wall(alfa_plan* P, const void* u, const void* v, const void*
   DISPATCH(P, "alfa_wall", {
     const int ncol = P->nx * P->ny;
     const double logm = std::log(0.5 * P->d.dz * P->d.z_scale / P->cfg.z0m);
-    k_wall_plane<T><<<pw_blocks(ncol), 256, 0, STREAM(s)>>>(ncol, P->nz, (const T*)u, (const T*)v,
+    launch_k<1024>(P, STREAM(s), dim3(pw_blocks(ncol)), dim3(256), 0, KBODY(k_wall_plane<T>), ncol, P->nz, (const T*)u, (const T*)v,
                                                             (const T*)M, (const T*)pm,
                                                             (const T*)pm0, logm, 0.4, P->cfg.Ugal,
                                                             (T*)txz, (T*)tyz);
@@ -1371,8 +1537,10 @@ static void planar_mean_noalloc_t(int nx, int ny, int nz, const void* F, double*
   const int ncol = nx * ny;
   const RedGeom g = red_geom(ncol, nz, sizeof(T));
   FSum1<T> f{(const T*)F, nz, 0.0};
-  k_plane_partial<1, FSum1<T>><<<dim3(g.nbz, g.nbc), RED_BS, 0, s>>>(g, f, part);
-  k_plane_final<<<(nz + 3) / 4, 256, 0, s>>>(g.nbc, 1, nz, part, mean, 1.0 / ncol);
+  k_solo<RED_BS><<<dim3(g.nbz, g.nbc), RED_BS, 0, s>>>(KBODY(k_plane_partial<1, FSum1<T>>), g, f,
+                                                       part);
+  k_solo<1024><<<(nz + 3) / 4, 256, 0, s>>>(KBODY(k_plane_final), g.nbc, 1, nz,
+                                            (const double*)part, mean, 1.0 / ncol);
 }
 
 extern "C" int alfa_planar_mean_noalloc(int precision, int nx, int ny, int nz, const void* F,

@@ -150,3 +150,62 @@ constexpr int MAXB = 24;
 
 __host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
+
+// ----------------------------------------------------------------------------
+// Kernel entry points.  Every kernel body of the library is a __device__
+// function; it runs as one of two __global__ entry points:
+//   k_solo   : one run (one plan), the arguments passed by value;
+//   k_member : several runs of the same shape in one launch, blockIdx.z
+//              selecting the run, whose arguments the launch reads from a
+//              device table of ArgPacks (one per run, in run order).
+// Both call the same body with the same argument values, so a run's results
+// are bitwise the same launched alone or with others.  The body sees the
+// same blockIdx.x/y and gridDim.x/y either way.  BS is the workgroup-size
+// bound the body was written for (1024, the compiler's default, where none
+// was stated).
+// ----------------------------------------------------------------------------
+template <class... A> struct ArgPack;
+template <> struct ArgPack<> {
+  template <class F, class... B>
+  __device__ __forceinline__ void call(F f, const B&... b) const {
+    [[clang::always_inline]] f(b...);
+  }
+};
+template <class H, class... R> struct ArgPack<H, R...> {
+  H h;
+  ArgPack<R...> r;
+  template <class F, class... B>
+  __device__ __forceinline__ void call(F f, const B&... b) const { r.call(f, b..., h); }
+};
+inline ArgPack<> make_pack() { return {}; }
+template <class H, class... R> ArgPack<H, R...> make_pack(H h, R... r) {
+  ArgPack<H, R...> p;
+  p.h = h;
+  p.r = make_pack(r...);
+  return p;
+}
+
+// (the body is always inlined into the entry point: a call would pass the
+// structures through memory and cost registers)
+template <int BS, class F, class... A>
+__global__ void __launch_bounds__(BS) k_solo(F f, A... a) {
+  [[clang::always_inline]] f(a...);
+}
+
+// The table is read through the constant address space, as k_solo reads its
+// kernel arguments: uniform reads of it become scalar loads into SGPRs
+// instead of per-lane vector loads into VGPRs.
+template <int BS, class F, class Pk>
+__global__ void __launch_bounds__(BS) k_member(F f, const Pk* __restrict__ tab) {
+  using CPk = const __attribute__((address_space(4))) Pk;
+  const Pk& p = *(const Pk*)((CPk*)tab + blockIdx.z);
+  [[clang::always_inline]] p.call(f);
+}
+
+// the body of kernel `kern` as a function object for k_solo / k_member.  The
+// arguments are forwarded by reference and the bodies take their structures
+// by const reference, so a structure indexed at run time (a field table, a
+// radix plan) is read in place, from the kernel's argument segment (k_solo)
+// or from the device table (k_member), never copied to private memory.
+#define KBODY(...) \
+  [] __device__(const auto&... a_) { [[clang::always_inline]] __VA_ARGS__(a_...); }

@@ -480,9 +480,9 @@ struct TileGrid {
 // evaluated only where the store reads it.
 // ----------------------------------------------------------------------------
 template <class T, class Loader, int MAXN, int NCT, int BS, bool BAND = false>
-__global__ void __launch_bounds__(BS)
-k_y_r2c(Loader ld, Fft1D fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, PtrBatch outb,
-        TileGrid tg, int vec) {
+__device__ __forceinline__ void
+k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, const PtrBatch& outb,
+        const TileGrid& tg, int vec) {
   constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
@@ -704,9 +704,9 @@ __device__ __forceinline__ void x_band_mid(c2<T>* buf, const c2<T>* tw, const XO
 // forward spectrum is kept in registers and every op (mask / derivative /
 // dealias map, optional inverse) is built from it in turn.
 template <class T, int MAXN, bool MULTI, int NCT, int BS, bool BAND = false>
-__global__ void __launch_bounds__(BS)
-k_x(XArgs a, Fft1D fin, Fft1D fout, const c2<T>* __restrict__ tw_in,
-    const c2<T>* __restrict__ tw_out, TileGrid tg) {
+__device__ __forceinline__ void
+k_x(const XArgs& a, const Fft1D& fin, const Fft1D& fout, const c2<T>* __restrict__ tw_in,
+    const c2<T>* __restrict__ tw_out, const TileGrid& tg) {
   constexpr int L = Lines<T>::L * BS / FFT_BS;
   // spectrum elements per thread in the multi-op path: rounded up, since
   // MAXN * L need not be a multiple of BS (96 double lines at 256 threads)
@@ -914,8 +914,8 @@ __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC
 // BAND (compile-time lengths with ncy_in <= N/4, chosen by the launcher):
 // the first inverse pass reads only the nonzero modes.
 template <class T, class Storer, int MAXN, int NCT, int BS, bool FUSED = false, bool BAND = false>
-__global__ void __launch_bounds__(BS)
-k_y_c2r(Storer st, YC2RArgs a, Fft1D fy, const c2<T>* __restrict__ tw, TileGrid tg) {
+__device__ __forceinline__ void
+k_y_c2r(const Storer& st, const YC2RArgs& a, const Fft1D& fy, const c2<T>* __restrict__ tw, const TileGrid& tg) {
   constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;

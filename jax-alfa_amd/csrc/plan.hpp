@@ -3,6 +3,7 @@
 #pragma once
 #include <type_traits>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -52,6 +53,19 @@ struct Prof {
   std::vector<ProfRec> recs;
 };
 
+// One launch of a step recorded for a member group (alfa_group_step): the
+// k_member instance it runs (its identity), the launch shape, this run's
+// packed arguments, and a function that launches k_member over a table of
+// `runs` packs.
+struct LaunchRec {
+  const void* id;
+  dim3 grid, block;
+  size_t lds;
+  size_t align;
+  std::vector<unsigned char> pack;
+  std::function<void(const void* tab, int runs, hipStream_t s)> member;
+};
+
 struct alfa_plan {
   alfa_config cfg;
   alfa_derived d;
@@ -75,6 +89,9 @@ struct alfa_plan {
   // 512; the lines per tile scale with them): the build's defaults, or the
   // choices of alfa_plan_autotune / alfa_plan_set_tiles
   int tile_x = 256, tile_c2r = 512, tile_r2c = 256;
+  // set while a member group records this plan's step: launches are
+  // appended here instead of being enqueued
+  std::vector<LaunchRec>* rec = nullptr;
 };
 
 // Grid of the tiled FFT kernels: one workgroup per tile, rounded up to whole
@@ -100,6 +117,30 @@ inline bool launch_ok(alfa_plan* P) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess && P->herr == hipSuccess) P->herr = e;
   return e == hipSuccess;
+}
+
+// Launch body f (a KBODY) with arguments a: enqueued as k_solo, or, while
+// the plan records for a member group, appended to its record list.
+template <int BS, class F, class... A>
+void launch_k(alfa_plan* P, hipStream_t s, dim3 grid, dim3 block, size_t lds, F f, A... a) {
+  if (P->rec) {
+    using Pk = ArgPack<A...>;
+    const Pk pk = make_pack(a...);
+    LaunchRec r;
+    r.id = reinterpret_cast<const void*>(&k_member<BS, F, Pk>);
+    r.grid = grid;
+    r.block = block;
+    r.lds = lds;
+    r.align = alignof(Pk);
+    r.pack.resize(sizeof(Pk));
+    std::memcpy(r.pack.data(), &pk, sizeof(Pk));
+    r.member = [f, grid, block, lds](const void* tab, int runs, hipStream_t st) {
+      k_member<BS, F, Pk><<<dim3(grid.x, grid.y, runs), block, lds, st>>>(f, (const Pk*)tab);
+    };
+    P->rec->push_back(std::move(r));
+    return;
+  }
+  k_solo<BS><<<grid, block, lds, s>>>(f, a...);
 }
 
 struct ProfScope {
@@ -203,13 +244,11 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
       // evaluated only at the outputs the store reads
       if constexpr (NCT > 0) {
         if (ALFA_BAND && ncy <= NCT / last_radix(NCT)) {
-          k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS, true>
-              <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
+          launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS, true>), ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
           return;
         }
       }
-      k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>
-          <<<grid, BS, lds, s>>>(ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
+      launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>), ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
     };
     if constexpr (NCT > 0) {
       if (P->tile_r2c == 512) launch(std::integral_constant<int, 512>{});
@@ -223,7 +262,7 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
 
 // one x launch at BS threads (NCT, MAXN: the plan's length constants)
 template <class T, bool MULTI, int BS, class NCTT, class MAXNT>
-void launch_kx_bs(hipStream_t s, const XArgs& a, int rows, int nb, const Fft1D& fin,
+void launch_kx_bs(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, const Fft1D& fin,
                   const Fft1D& fout, const void* twin, const void* twout, NCTT, MAXNT) {
   constexpr int NCT = NCTT::value;
   using maxn_t = MAXNT;
@@ -242,13 +281,11 @@ void launch_kx_bs(hipStream_t s, const XArgs& a, int rows, int nb, const Fft1D& 
     if (ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops == 1 && o.inverse &&
         o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 4 &&
         o.cx <= NCT / last_radix(NCT)) {
-      k_x<T, maxn_t::value, MULTI, NCT, BS, true>
-          <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+      launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_x<T, maxn_t::value, MULTI, NCT, BS, true>), a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
       return;
     }
   }
-  k_x<T, maxn_t::value, MULTI, NCT, BS>
-      <<<grid, BS, lds, s>>>(a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
+  launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_x<T, maxn_t::value, MULTI, NCT, BS>), a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
 }
 
 template <class T, bool MULTI>
@@ -261,11 +298,11 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
     constexpr int NCT = decltype(nct)::value;
     if constexpr (NCT > 0 && !MULTI) {
       if (P->tile_x == 512)
-        launch_kx_bs<T, MULTI, 512>(s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
+        launch_kx_bs<T, MULTI, 512>(P, s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
       else
-        launch_kx_bs<T, MULTI, 256>(s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
+        launch_kx_bs<T, MULTI, 256>(P, s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
     } else {
-      launch_kx_bs<T, MULTI, FFT_BS>(s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
+      launch_kx_bs<T, MULTI, FFT_BS>(P, s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
     }
   };
   if (n > 0) ct_length(n, go);
@@ -330,21 +367,17 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
         constexpr bool fuse = !(sizeof(T) == 4 && std::is_same<Storer, StoreDiv<T>>::value);
         if (fuse && a.vec) {
           if (band)
-            k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true, true>
-                <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+            launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true, true>), st, a, fyy, (const c2<T>*)tw, tg);
           else
-            k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>
-                <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+            launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, true>), st, a, fyy, (const c2<T>*)tw, tg);
           return;
         }
         if (band) {
-          k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, false, true>
-              <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+          launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS, false, true>), st, a, fyy, (const c2<T>*)tw, tg);
           return;
         }
       }
-      k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>
-          <<<grid, BS, lds, s>>>(st, a, fyy, (const c2<T>*)tw, tg);
+      launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>), st, a, fyy, (const c2<T>*)tw, tg);
     };
     if constexpr (NCT > 0) {
       if (P->tile_c2r == 256) launch(std::integral_constant<int, 256>{});
@@ -367,10 +400,10 @@ void plane_sums(alfa_plan* P, hipStream_t s, int ncol, int nz, const F& f, doubl
   {
     ProfScope ps(P, s, std::string("k_plane_partial<") + F::NAME + ">",
                  (double)F::NFIELDS * ncol * nz * esize(P));
-    k_plane_partial<NT, F><<<grid, RED_BS, 0, s>>>(g, f, part);
+    launch_k<RED_BS>(P, s, dim3(grid), dim3(RED_BS), 0, KBODY(k_plane_partial<NT, F>), g, f, part);
   }
   launch_ok(P);
-  k_plane_final<<<(NT * nz + 3) / 4, 256, 0, s>>>(g.nbc, NT, nz, part, out, scale);
+  launch_k<1024>(P, s, dim3((NT * nz + 3) / 4), dim3(256), 0, KBODY(k_plane_final), g.nbc, NT, nz, part, out, scale);
   launch_ok(P);
 }
 
@@ -388,12 +421,12 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
     ProfScope ps(P, s, "k_box_clip", 3.0 * ncol * nz * esize(P));
     // 16-byte form where the levels and buffers allow it (same partials)
     if (vec_ok<T>(nz, {num, den, out}))
-      k_box_clip_v<T><<<grid, RED_BS / Vec16<T>::W, 0, s>>>(g, num, den, out, nx, ny, part);
+      launch_k<RED_BS>(P, s, dim3(grid), dim3(RED_BS / Vec16<T>::W), 0, KBODY(k_box_clip_v<T>), g, num, den, out, nx, ny, part);
     else
-      k_box_clip<T><<<grid, RED_BS, 0, s>>>(g, num, den, out, nx, ny, part);
+      launch_k<RED_BS>(P, s, dim3(grid), dim3(RED_BS), 0, KBODY(k_box_clip<T>), g, num, den, out, nx, ny, part);
   }
   launch_ok(P);
-  k_plane_final<<<(2 * nz + 3) / 4, 256, 0, s>>>(g.nbc, 2, nz, part, sums, 1.0);
+  launch_k<1024>(P, s, dim3((2 * nz + 3) / 4), dim3(256), 0, KBODY(k_plane_final), g.nbc, 2, nz, part, sums, 1.0);
   launch_ok(P);
 }
 
@@ -403,7 +436,7 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
     if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, #kern, (double)pw_fields(#kern) * (n) * esize(P)); \
-        kern<<<pw_blocks((n) / (w)), 256, 0, s>>>(__VA_ARGS__);           \
+        launch_k<1024>(P, s, dim3(pw_blocks((n) / (w))), dim3(256), 0, KBODY(kern), __VA_ARGS__); \
       }                                                                   \
       launch_ok(P);                                                       \
     }                                                                     \
@@ -414,31 +447,32 @@ void box_clip_sums(alfa_plan* P, hipStream_t s, int nx, int ny, int nz, const T*
     if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, #kern, (double)pw_fields(#kern) * (n) * esize(P)); \
-        kern<<<pw_blocks(n), 256, 0, s>>>(__VA_ARGS__);                   \
+        launch_k<1024>(P, s, dim3(pw_blocks(n)), dim3(256), 0, KBODY(kern), __VA_ARGS__); \
       }                                                                   \
       launch_ok(P);                                                       \
     }                                                                     \
   } while (0)
 
-// Single kernel launch with a profiling record.
+// Single kernel launch with a profiling record (a body written for any
+// workgroup size up to 1024).
 #define KLAUNCH(P, s, name, bytes, kern, grid, block, ...)                \
   do {                                                                    \
     if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, name, bytes);                                 \
-        kern<<<grid, block, 0, s>>>(__VA_ARGS__);                         \
+        launch_k<1024>(P, s, dim3(grid), dim3(block), 0, KBODY(kern), __VA_ARGS__); \
       }                                                                   \
       launch_ok(P);                                                       \
     }                                                                     \
   } while (0)
 
-// KLAUNCH with dynamic LDS bytes
-#define KLAUNCH_LDS(P, s, name, bytes, kern, grid, block, lds, ...)       \
+// KLAUNCH with dynamic LDS bytes and the body's workgroup-size bound BS
+#define KLAUNCH_LDS(P, s, name, bytes, BS, kern, grid, block, lds, ...)   \
   do {                                                                    \
     if (live(P)) {                                                   \
       {                                                                   \
         ProfScope ps_(P, s, name, bytes);                                 \
-        kern<<<grid, block, lds, s>>>(__VA_ARGS__);                       \
+        launch_k<BS>(P, s, dim3(grid), dim3(block), lds, KBODY(kern), __VA_ARGS__); \
       }                                                                   \
       launch_ok(P);                                                       \
     }                                                                     \

@@ -17,7 +17,7 @@ struct PwDims {
 
 // Derivz_M + Derivz_TH (src/operations/Derivatives.py:190-273).
 template <class T>
-__global__ void k_derivz(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_derivz(const PwDims& d, const T* __restrict__ u, const T* __restrict__ v,
                          const T* __restrict__ w, const T* __restrict__ TH,
                          const T* __restrict__ ustar, const T* __restrict__ M,
                          const T* __restrict__ fim, const T* __restrict__ fih,
@@ -54,7 +54,7 @@ __global__ void k_derivz(PwDims d, const T* __restrict__ u, const T* __restrict_
 // 16-byte accesses (nz % W == 0, arrays 16-byte aligned): the same
 // arithmetic, the neighbour levels across the vector edge read as scalars.
 template <class T>
-__global__ void k_derivz_v(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_derivz_v(const PwDims& d, const T* __restrict__ u, const T* __restrict__ v,
                            const T* __restrict__ w, const T* __restrict__ TH,
                            const T* __restrict__ ustar, const T* __restrict__ M,
                            const T* __restrict__ fim, const T* __restrict__ fih,
@@ -107,7 +107,7 @@ __global__ void k_derivz_v(PwDims d, const T* __restrict__ u, const T* __restric
 
 // Derivz_Generic_uvp (Derivatives.py:310) / Derivz_Generic_w (:340).
 template <class T>
-__global__ void k_derivz_generic(PwDims d, const T* __restrict__ F, T* out, T idz, int wnode) {
+__device__ __forceinline__ void k_derivz_generic(const PwDims& d, const T* __restrict__ F, T* out, T idz, int wnode) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
@@ -118,7 +118,7 @@ __global__ void k_derivz_generic(PwDims d, const T* __restrict__ F, T* out, T id
 
 // StagGridAvg (Utilities.py:64): out is (ncol, nz-1).
 template <class T>
-__global__ void k_stag_avg(PwDims d, const T* __restrict__ F, T* out) {
+__device__ __forceinline__ void k_stag_avg(const PwDims& d, const T* __restrict__ F, T* out) {
   const int nz = d.nz;
   const size_t n = d.n / nz * (nz - 1);
   PW_LOOP(n) {
@@ -132,7 +132,7 @@ __global__ void k_stag_avg(PwDims d, const T* __restrict__ F, T* out) {
 // Advection_NoDealias (src/pde/NSE_AdvectionTerms.py:163-238); on the padded
 // grid it is the product stage of Advection_Dealias (:65-155).
 template <class T>
-__global__ void k_advection(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_advection(const PwDims& d, const T* __restrict__ u, const T* __restrict__ v,
                             const T* __restrict__ w, const T* __restrict__ dudy,
                             const T* __restrict__ dudz, const T* __restrict__ dvdx,
                             const T* __restrict__ dvdz, const T* __restrict__ dwdx,
@@ -167,7 +167,7 @@ __global__ void k_advection(PwDims d, const T* __restrict__ u, const T* __restri
 // ScalarAdvection_NoDealias (src/pde/SCL_AdvectionTerms.py:114-150); product
 // stage of the dealiased variant (:57-106) on the padded grid.
 template <class T>
-__global__ void k_scalar_adv(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_scalar_adv(const PwDims& d, const T* __restrict__ u, const T* __restrict__ v,
                              const T* __restrict__ w, const T* __restrict__ sx,
                              const T* __restrict__ sy, const T* __restrict__ sz, T* out) {
   const int nz = d.nz;
@@ -273,7 +273,7 @@ __device__ __forceinline__ AdvOut<T> adv_point(size_t i, int z, size_t c, int nz
 // BuoyancyOpt1/2 (src/pde/NSE_BuoyancyTerms.py:49-150), dry; bar = plane
 // mean of TH + T_0 (double).
 template <class T>
-__global__ void k_buoyancy(PwDims d, const T* __restrict__ TH, const double* __restrict__ bar,
+__device__ __forceinline__ void k_buoyancy(const PwDims& d, const T* __restrict__ TH, const double* __restrict__ bar,
                            T* out, double T0, double g, int opt) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
@@ -327,7 +327,7 @@ __device__ __forceinline__ StrainOut<T> strains_point(size_t i, int z, size_t c,
 
 // StrainsUVPnodes_NoDealias: writes S12, S13, S23 and |S|.
 template <class T>
-__global__ void k_strains_uvp(PwDims d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
+__device__ __forceinline__ void k_strains_uvp(const PwDims& d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
                               const T* __restrict__ dwdx, const T* __restrict__ dudy,
                               const T* __restrict__ dvdy, const T* __restrict__ dwdy,
                               const T* __restrict__ dudz, const T* __restrict__ dvdz,
@@ -357,7 +357,7 @@ template <class T> struct DzAdvStrainArgs {
   T *Cx, *Cy, *Cz, *sadv, *S12, *S13, *S23, *S;
 };
 template <class T>
-__global__ void k_dz_adv_strains(PwDims d, DzAdvStrainArgs<T> a) {
+__device__ __forceinline__ void k_dz_adv_strains(const PwDims& d, const DzAdvStrainArgs<T>& a) {
   const int nz = d.nz;
   const DzUV<T, true> uz{a.u, a.fim, a.ustar, a.M, a.Ugal, a.idz, a.half_dz_vonk, nz};
   const DzUV<T, false> vz{a.v, a.fim, a.ustar, a.M, T(0), a.idz, a.half_dz_vonk, nz};
@@ -384,7 +384,7 @@ __global__ void k_dz_adv_strains(PwDims d, DzAdvStrainArgs<T> a) {
 // adv_point / strains_point run on the windows: the same arithmetic as the
 // scalar kernel.
 template <class T>
-__global__ void k_dz_adv_strains_v(PwDims d, DzAdvStrainArgs<T> a) {
+__device__ __forceinline__ void k_dz_adv_strains_v(const PwDims& d, const DzAdvStrainArgs<T>& a) {
   constexpr int W = Vec16<T>::W;
   const int nz = d.nz;
   const size_t nv = d.n / W;
@@ -436,7 +436,7 @@ __global__ void k_dz_adv_strains_v(PwDims d, DzAdvStrainArgs<T> a) {
 // |S_w| always; the diagonal/S12 w-node components only when S11w != NULL
 // (the dealiased branch transforms all six).
 template <class T>
-__global__ void k_strains_w(PwDims d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
+__device__ __forceinline__ void k_strains_w(const PwDims& d, const T* __restrict__ dudx, const T* __restrict__ dvdx,
                             const T* __restrict__ dwdx, const T* __restrict__ dudy,
                             const T* __restrict__ dvdy, const T* __restrict__ dwdy,
                             const T* __restrict__ dudz, const T* __restrict__ dvdz,
@@ -469,7 +469,7 @@ __global__ void k_strains_w(PwDims d, const T* __restrict__ dudx, const T* __res
 
 // |S| from six components (the padded magnitudes of StrainRates.py:127-133).
 template <class T>
-__global__ void k_smag(PwDims d, const T* a, const T* b, const T* c, const T* e,
+__device__ __forceinline__ void k_smag(const PwDims& d, const T* a, const T* b, const T* c, const T* e,
                        const T* f, const T* g, T* out) {
   PW_LOOP(d.n) out[i] = smag<T>(a[i], b[i], c[i], e[i], f[i], g[i]);
 }
@@ -478,7 +478,7 @@ __global__ void k_smag(PwDims d, const T* a, const T* b, const T* c, const T* e,
 // Dealias2: tij = -2 L^2 Cs2 |S| Sij, top level 0.  With S == NULL the
 // Wong-Lilly form (SGSStresses_WL.py:70-141): m2L2 = -2 L^(4/3), no |S|.
 template <class T>
-__global__ void k_stress_uvp(PwDims d, const T* __restrict__ S11, const T* __restrict__ S22,
+__device__ __forceinline__ void k_stress_uvp(const PwDims& d, const T* __restrict__ S11, const T* __restrict__ S22,
                              const T* __restrict__ S33, const T* __restrict__ S12,
                              const T* __restrict__ S, const T* __restrict__ Cs2, T m2L2,
                              T* txx, T* tyy, T* tzz, T* txy) {
@@ -495,7 +495,7 @@ __global__ void k_stress_uvp(PwDims d, const T* __restrict__ S11, const T* __res
 // stress of Wall() (:63-90) when `wall` is set, else 0 (padded branch).
 // Sw == NULL: Wong-Lilly (SGSStresses_WL.py:146-239).
 template <class T>
-__global__ void k_stress_w(PwDims d, const T* __restrict__ S13w, const T* __restrict__ S23w,
+__device__ __forceinline__ void k_stress_w(const PwDims& d, const T* __restrict__ S13w, const T* __restrict__ S23w,
                            const T* __restrict__ Sw, const T* __restrict__ Cs2, T m2L2,
                            T* txz, T* tyz, int wall, const T* __restrict__ u,
                            const T* __restrict__ v, const T* __restrict__ M,
@@ -527,7 +527,7 @@ __global__ void k_stress_w(PwDims d, const T* __restrict__ S13w, const T* __rest
 // SGSStresses_SM.py:240-241), and qz level 0 = surface flux
 // (ScalarSGSFluxes_SM.py:175).
 template <class T>
-__global__ void k_level0(int ncol, int nz, T* txz, T* tyz, const T* __restrict__ u,
+__device__ __forceinline__ void k_level0(int ncol, int nz, T* txz, T* tyz, const T* __restrict__ u,
                          const T* __restrict__ v, const T* __restrict__ M,
                          const T* __restrict__ psim, const T* __restrict__ psim0,
                          double logz1z0, double vonk, double Ugal, T* qz,
@@ -548,7 +548,7 @@ __global__ void k_level0(int ncol, int nz, T* txz, T* tyz, const T* __restrict__
 // Dealias2; qz level 0 = qsfc when qsfc != NULL, else 0.  S == Sw == NULL:
 // Wong-Lilly (ScalarSGSFluxes_WL.py:55-201) with m2L2 = -L^(4/3).
 template <class T>
-__global__ void k_scalar_flux(PwDims d, const T* __restrict__ dx, const T* __restrict__ dy,
+__device__ __forceinline__ void k_scalar_flux(const PwDims& d, const T* __restrict__ dx, const T* __restrict__ dy,
                               const T* __restrict__ dz, const T* __restrict__ S,
                               const T* __restrict__ Sw, const T* __restrict__ CsPr, T m2L2,
                               T* qx, T* qy, T* qz, const T* __restrict__ qsfc) {
@@ -578,7 +578,7 @@ template <class T> struct SgsFusedArgs {
 };
 
 template <class T>
-__global__ void k_sgs_fused(PwDims d, SgsFusedArgs<T> a) {
+__device__ __forceinline__ void k_sgs_fused(const PwDims& d, const SgsFusedArgs<T>& a) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
@@ -623,7 +623,7 @@ __global__ void k_sgs_fused(PwDims d, SgsFusedArgs<T> a) {
 // 16-byte accesses (nz % W == 0, all arrays 16-byte aligned); the lower
 // neighbours of the first level come from one scalar load per array.
 template <class T>
-__global__ void k_sgs_fused_v(PwDims d, SgsFusedArgs<T> a) {
+__device__ __forceinline__ void k_sgs_fused_v(const PwDims& d, const SgsFusedArgs<T>& a) {
   constexpr int W = Vec16<T>::W;
   const int nz = d.nz;
   const size_t nv = d.n / W;
@@ -695,7 +695,7 @@ __global__ void k_sgs_fused_v(PwDims d, SgsFusedArgs<T> a) {
 // RHS_Momentum (src/pde/NSE_AllTerms.py:48-104) + RHS_Scalar
 // (src/pde/SCL_AllTerms.py:48-82), optAdvection = 0.  Profiles are double.
 template <class T>
-__global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_rhs(const PwDims& d, const T* __restrict__ u, const T* __restrict__ v,
                       const T* __restrict__ w, const T* __restrict__ TH,
                       const T* __restrict__ Cx, const T* __restrict__ Cy,
                       const T* __restrict__ Cz, const T* __restrict__ buoy,
@@ -758,7 +758,7 @@ __global__ void k_rhs(PwDims d, const T* __restrict__ u, const T* __restrict__ v
 // thread and 16-byte accesses (nz % W == 0, arrays 16-byte aligned).  Same
 // arithmetic, element by element.
 template <class T>
-__global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_rhs_v(const PwDims& d, const T* __restrict__ u, const T* __restrict__ v,
                         const T* __restrict__ w, const T* __restrict__ TH,
                         const T* __restrict__ Cx, const T* __restrict__ Cy,
                         const T* __restrict__ Cz, const T* __restrict__ dtx,
@@ -832,7 +832,7 @@ __global__ void k_rhs_v(PwDims d, const T* __restrict__ u, const T* __restrict__
 
 // Per-iteration forcing profiles into the plan's profile table: Ug, Vg
 // (slots 0, 1) and Uadv, Vadv, THadv, Qadv (slots 4-7); NULL keeps a slot.
-__global__ void k_set_forcing(int nz, const double* __restrict__ Ug, const double* __restrict__ Vg,
+__device__ __forceinline__ void k_set_forcing(int nz, const double* __restrict__ Ug, const double* __restrict__ Vg,
                               const double* __restrict__ Ua, const double* __restrict__ Va,
                               const double* __restrict__ Ta, const double* __restrict__ Qa,
                               double* prof) {
@@ -849,7 +849,7 @@ __global__ void k_set_forcing(int nz, const double* __restrict__ Ug, const doubl
 // AB2_uvw (NSE_TimeAdvancement.py:45-76) / AB2_TH (SCL_TimeAdvancement.py:
 // 46-71) alone, in place; the thread of level nz-2 writes the top level.
 template <class T>
-__global__ void k_ab2_plain(PwDims d, T* F, const T* __restrict__ R, const T* __restrict__ Rp,
+__device__ __forceinline__ void k_ab2_plain(const PwDims& d, T* F, const T* __restrict__ R, const T* __restrict__ Rp,
                             T dt, int kind, T th_top) {
   // kind 0: u/v (top copies nz-2), 1: w (0 at bottom and top), 2: TH
   const int nz = d.nz;
@@ -871,7 +871,7 @@ __global__ void k_ab2_plain(PwDims d, T* F, const T* __restrict__ R, const T* __
 // into the previous-step buffer (Main.py:164-165); level nz-1 becomes
 // level nz-2 plus q_top = q_inversion dz.
 template <class T>
-__global__ void k_ab2_q(PwDims d, T* Q, const T* __restrict__ R, T* Rp, T dt, T q_top,
+__device__ __forceinline__ void k_ab2_q(const PwDims& d, T* Q, const T* __restrict__ R, T* Rp, T dt, T q_top,
                         int first) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
@@ -888,7 +888,7 @@ __global__ void k_ab2_q(PwDims d, T* Q, const T* __restrict__ R, T* Rp, T dt, T 
 
 // 3x3 periodic box mean of every level (Utilities.py:87-115).
 template <class T>
-__global__ void k_imfilter(PwDims d, int nx, int ny, const T* __restrict__ F, T* out) {
+__device__ __forceinline__ void k_imfilter(const PwDims& d, int nx, int ny, const T* __restrict__ F, T* out) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
     const int z = (int)(i % nz);
@@ -906,7 +906,7 @@ __global__ void k_imfilter(PwDims d, int nx, int ny, const T* __restrict__ F, T*
 
 // Wall stress planes (SGSStresses_SM.py:63-90).
 template <class T>
-__global__ void k_wall_plane(int ncol, int nz, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void k_wall_plane(int ncol, int nz, const T* __restrict__ u, const T* __restrict__ v,
                              const T* __restrict__ M, const T* __restrict__ psim,
                              const T* __restrict__ psim0, double logz1z0, double vonk,
                              double Ugal, T* txz, T* tyz) {
@@ -921,7 +921,7 @@ __global__ void k_wall_plane(int ncol, int nz, const T* __restrict__ u, const T*
 
 // Real part of a spectrum (fRz_real of NSE_PressureTerms.py:182).
 template <class T>
-__global__ void k_real_part(size_t n, const c2<T>* __restrict__ a, T* out) {
+__device__ __forceinline__ void k_real_part(size_t n, const c2<T>* __restrict__ a, T* out) {
   PW_LOOP(n) out[i] = a[i].x;
 }
 
@@ -935,7 +935,7 @@ __global__ void k_real_part(size_t n, const c2<T>* __restrict__ a, T* out) {
 // level (u, v copy; TH + inversion*dz), which no other thread touches.
 // Per-block max |u|,|v|,|w| of the new fields go to cflpart[3*block + k].
 template <class T>
-__global__ void k_ab2(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ Rnu,
+__device__ __forceinline__ void k_ab2(const PwDims& d, T* u, T* v, T* w, T* TH, const T* __restrict__ Rnu,
                       const T* __restrict__ Rnv, const T* __restrict__ Rnw,
                       const T* __restrict__ RnT, T* Rpu, T* Rpv, T* Rpw, T* RpT,
                       const T* __restrict__ dpdx, const T* __restrict__ dpdy,
@@ -987,7 +987,7 @@ __global__ void k_ab2(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ R
 // == 0, arrays 16-byte aligned); the top level copies level nz-2 from the
 // same vector.  Same arithmetic and CFL maxima.
 template <class T>
-__global__ void k_ab2_v(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__ Rnu,
+__device__ __forceinline__ void k_ab2_v(const PwDims& d, T* u, T* v, T* w, T* TH, const T* __restrict__ Rnu,
                         const T* __restrict__ Rnv, const T* __restrict__ Rnw,
                         const T* __restrict__ RnT, T* Rpu, T* Rpv, T* Rpw, T* RpT,
                         const T* __restrict__ dpdx, const T* __restrict__ dpdy,
@@ -1053,11 +1053,11 @@ __global__ void k_ab2_v(PwDims d, T* u, T* v, T* w, T* TH, const T* __restrict__
 }
 
 template <class T>
-__global__ void k_copy(size_t n, const T* __restrict__ a, T* b) {
+__device__ __forceinline__ void k_copy(size_t n, const T* __restrict__ a, T* b) {
   PW_LOOP(n) b[i] = a[i];
 }
 
 template <class T>
-__global__ void k_fill(size_t n, T* a, T val) {
+__device__ __forceinline__ void k_fill(size_t n, T* a, T val) {
   PW_LOOP(n) a[i] = val;
 }

@@ -34,7 +34,7 @@ __device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_u
 __device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
 
 template <class T>
-__global__ void __launch_bounds__(64) k_thomas(ThomasArgs<T> a) {
+__device__ __forceinline__ void k_thomas(const ThomasArgs<T>& a) {
   extern __shared__ double th_smem[];
   const int col = blockIdx.x;
   const int lane = threadIdx.x;

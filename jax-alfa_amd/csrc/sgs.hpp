@@ -49,7 +49,7 @@ __device__ __forceinline__ void red_block(int& bz, int& bc) {
 }
 
 template <int NT, class F>
-__global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double* part) {
+__device__ __forceinline__ void k_plane_partial(const RedGeom& g, const F& f, double* part) {
   __shared__ double sm[NT][RED_BS];
   int bz, bc;
   red_block(bz, bc);
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(RED_BS) k_plane_partial(RedGeom g, F f, double
 
 // out[t*nz + z] = sum_b part[b][t][z] * scale; one wavefront per output,
 // lane-strided partial sums and a fixed shuffle tree (deterministic).
-__global__ void k_plane_final(int nbc, int nt, int nz, const double* __restrict__ part,
+__device__ __forceinline__ void k_plane_final(int nbc, int nt, int nz, const double* __restrict__ part,
                               double* out, double scale) {
   const int i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
@@ -195,7 +195,7 @@ __device__ __forceinline__ void lasdd_lm_point(const MomLevel<T>& l1, size_t i, 
 }
 
 template <class T>
-__global__ void k_lasdd_lm(PwDims d, MomLevel<T> l1, const double* __restrict__ beta, double L,
+__device__ __forceinline__ void k_lasdd_lm(const PwDims& d, const MomLevel<T>& l1, const double* __restrict__ beta, double L,
                            double TFR, T* LM, T* MM, T* Shat) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
@@ -213,7 +213,7 @@ __global__ void k_lasdd_lm(PwDims d, MomLevel<T> l1, const double* __restrict__ 
 // point function run on them (same arithmetic).  Two, not 16 bytes' worth:
 // four floats of 21 fields would not fit the registers.
 template <class T>
-__global__ void k_lasdd_lm_v(PwDims d, MomLevel<T> l1, const double* __restrict__ beta,
+__device__ __forceinline__ void k_lasdd_lm_v(const PwDims& d, const MomLevel<T>& l1, const double* __restrict__ beta,
                              double L, double TFR, T* LM, T* MM, T* Shat) {
   constexpr int W = 2;
   const int nz = d.nz;
@@ -257,7 +257,7 @@ __global__ void k_lasdd_lm_v(PwDims d, MomLevel<T> l1, const double* __restrict_
 // |S| of the level-2 filtered strains (DynamicSGS_LASDD_SM.py:240), kept for
 // the scalar model.
 template <class T>
-__global__ void k_smag_level(PwDims d, MomLevel<T> l, T* out) {
+__device__ __forceinline__ void k_smag_level(const PwDims& d, const MomLevel<T>& l, T* out) {
   PW_LOOP(d.n) {
     double s[6];
 #pragma unroll
@@ -333,7 +333,7 @@ __device__ __forceinline__ void scalar_updn_point(const ScaLevel<T>& l1, size_t 
 }
 
 template <class T>
-__global__ void k_scalar_updn(PwDims d, ScaLevel<T> l1, const double* __restrict__ beta,
+__device__ __forceinline__ void k_scalar_updn(const PwDims& d, const ScaLevel<T>& l1, const double* __restrict__ beta,
                               double L, double TFR, T* Tup, T* Tdn) {
   const int nz = d.nz;
   PW_LOOP(d.n) {
@@ -346,7 +346,7 @@ __global__ void k_scalar_updn(PwDims d, ScaLevel<T> l1, const double* __restrict
 // per thread, the 14 fields as 2-element vectors in registers and the point
 // function run on them (same arithmetic).
 template <class T>
-__global__ void k_scalar_updn_v(PwDims d, ScaLevel<T> l1, const double* __restrict__ beta,
+__device__ __forceinline__ void k_scalar_updn_v(const PwDims& d, const ScaLevel<T>& l1, const double* __restrict__ beta,
                                 double L, double TFR, T* Tup, T* Tdn) {
   constexpr int W = 2;
   const int nz = d.nz;
@@ -430,7 +430,7 @@ template <class T> struct FWlSums {
 
 // M_ij = 2 L^(4/3) (S^_ij - TFR^(4/3) beta S^^_ij), LM, MM (LASDD_WL.py:284-301)
 template <class T>
-__global__ void k_wl_lm(PwDims d, WlMom<T> l1, WlMom<T> l2, const double* __restrict__ beta,
+__device__ __forceinline__ void k_wl_lm(const PwDims& d, const WlMom<T>& l1, const WlMom<T>& l2, const double* __restrict__ beta,
                         double L, double TFR, T* LM, T* MM) {
   const int nz = d.nz;
   const double T1 = 2.0 * pow(L, 4.0 / 3.0), T2 = 2.0 * pow(TFR * L, 4.0 / 3.0);
@@ -485,7 +485,7 @@ template <class T> struct FWlScalarSums {
 
 // T_up, T_dn of ScalarLASDD_WL.py:252-268; l2 needs only the gradient hats
 template <class T>
-__global__ void k_wl_scalar_updn(PwDims d, WlSca<T> l1, WlSca<T> l2,
+__device__ __forceinline__ void k_wl_scalar_updn(const PwDims& d, const WlSca<T>& l1, const WlSca<T>& l2,
                                  const double* __restrict__ beta, double L, double TFR, T* Tup,
                                  T* Tdn) {
   const int nz = d.nz;
@@ -519,8 +519,8 @@ __global__ void k_wl_scalar_updn(PwDims d, WlSca<T> l1, WlSca<T> l2,
 // loads one new column of the 3x3 box (3 values per field) instead of nine.
 // Partials in the layout of k_plane_partial (NT = 2: sqrt(C), C).
 template <class T>
-__global__ void __launch_bounds__(RED_BS)
-k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* __restrict__ out,
+__device__ __forceinline__ void
+k_box_clip(const RedGeom& g, const T* __restrict__ num, const T* __restrict__ den, T* __restrict__ out,
            int nx, int ny, double* part) {
   __shared__ double sm[2][RED_BS];
   int bz, bc;
@@ -585,8 +585,8 @@ k_box_clip(RedGeom g, const T* __restrict__ num, const T* __restrict__ den, T* _
 // takes W consecutive levels of the columns that W threads of k_box_clip
 // take, in the same order, so the partial sums are those of k_box_clip.
 template <class T>
-__global__ void __launch_bounds__(RED_BS)
-k_box_clip_v(RedGeom g, const T* __restrict__ num, const T* __restrict__ den,
+__device__ __forceinline__ void
+k_box_clip_v(const RedGeom& g, const T* __restrict__ num, const T* __restrict__ den,
              T* __restrict__ out, int nx, int ny, double* part) {
   constexpr int W = Vec16<T>::W;
   __shared__ double sm[2][RED_BS];
@@ -720,7 +720,7 @@ __device__ dcx laguerre(const double* p, double guess, bool* ok) {
   return x;
 }
 
-__global__ void k_beta(const double* __restrict__ coef, int n, double* beta) {
+__device__ __forceinline__ void k_beta(const double* __restrict__ coef, int n, double* beta) {
   const double guesses[8] = {0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0, 4.5};
   __shared__ double best[64];
   const int lv = threadIdx.x / 8, g = threadIdx.x % 8;
@@ -746,7 +746,7 @@ __global__ void k_beta(const double* __restrict__ coef, int n, double* beta) {
 // 296-302 scalar) from the plane means m[10][nz]; coef (6, nz).
 // mode 0: SM momentum, 1: SM scalar, 2: Wong-Lilly (momentum or scalar,
 // LASDD_WL.py:255-274 from m[4][nz] = a1, a3, a6, a8).
-__global__ void k_quintic(const double* __restrict__ m, int nz, int scalar, double TFR,
+__device__ __forceinline__ void k_quintic(const double* __restrict__ m, int nz, int scalar, double TFR,
                           double* coef) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nz) return;
@@ -788,7 +788,7 @@ __global__ void k_quintic(const double* __restrict__ m, int nz, int scalar, doub
 }
 
 // Profiles: avg1 = mean(sqrt C)^2, avg2 = mean(C) from the box-clip sums.
-__global__ void k_profiles(const double* __restrict__ s, int nz, double inv_n, double* avg1,
+__device__ __forceinline__ void k_profiles(const double* __restrict__ s, int nz, double inv_n, double* avg1,
                            double* avg2) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nz) return;
@@ -799,12 +799,12 @@ __global__ void k_profiles(const double* __restrict__ s, int nz, double inv_n, d
 
 // device copy of n doubles (a kernel rather than a memcpy node, so that
 // captured steps hold kernels only)
-__global__ void k_copy_d(int n, const double* __restrict__ src, double* __restrict__ dst) {
+__device__ __forceinline__ void k_copy_d(int n, const double* __restrict__ src, double* __restrict__ dst) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) dst[k] = src[k];
 }
 
-__global__ void k_fill_d(int n, double* a, double v) {
+__device__ __forceinline__ void k_fill_d(int n, double* a, double v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) a[k] = v;
 }

@@ -124,9 +124,9 @@ struct StatsProfiles {
 
 // acc += the sample (Statistics.py:158-305): means, gradients / z_scale,
 // moments, SGS profiles, surface scalars and the coefficient profiles.
-__global__ void k_stats_add(int nz, const double* __restrict__ m, const double* __restrict__ mo,
+__device__ __forceinline__ void k_stats_add(int nz, const double* __restrict__ m, const double* __restrict__ mo,
                             const double* __restrict__ sfc, const double* __restrict__ scal,
-                            StatsProfiles pr, double Ugal, double T0, double z_scale,
+                            const StatsProfiles& pr, double Ugal, double T0, double z_scale,
                             const double* __restrict__ mq, const double* __restrict__ moq,
                             double* acc) {
   for (int z = threadIdx.x; z < nz; z += blockDim.x) {
@@ -212,7 +212,7 @@ void stats_accumulate(alfa_plan* P, hipStream_t s, const alfa_state* st, double*
   if (!live(P)) return;
   StatsProfiles pr{st->Cs2_1D_avg1, st->Cs2_1D_avg2, st->Cs2PrRatio_1D, st->beta1_1D,
                    st->beta2_1D};
-  k_stats_add<<<1, 256, 0, s>>>(c.nz, m, mo, sfc, P->scal, pr, g.Ugal, g.T_0, d.z_scale, mq,
+  launch_k<1024>(P, s, dim3(1), dim3(256), 0, KBODY(k_stats_add), c.nz, m, mo, sfc, P->scal, pr, g.Ugal, g.T_0, d.z_scale, mq,
                                 moq, acc);
   launch_ok(P);
 }

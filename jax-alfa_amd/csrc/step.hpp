@@ -273,7 +273,7 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   {
     // 3 spectra in, 2 spectra out, the double pivot table
     const double spb = (double)c.nx * c.nyc * c.nz * sizeof(c2<T>);
-    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 1.0 * c.nx * c.nyc * (c.nz + 1) * 8.0,
+    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 1.0 * c.nx * c.nyc * (c.nz + 1) * 8.0, 64,
                 k_thomas<T>, c.nx * c.nyc, 64, thomas_lds(c.nz), ta);
   }
   XArgs xa{};
@@ -349,12 +349,12 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     FLasddSums<T> f{l1, l2, c.nz, d.L, d.TFR};
     plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasdd");
     if (live(P)) {
-      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 0, d.TFR, coef);
-      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 0, d.TFR, coef);
+      launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
       launch_ok(P);
     }
   } else if (live(P)) {
-    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
   T* LM = c.W("LM");
@@ -374,9 +374,9 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cs2, s2, "box");
   if (live(P)) {
-    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
-    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
+    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -438,12 +438,12 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
     FScalarSums<T> f{l1, l2, c.nz, d.L, d.TFR};
     plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasdd");
     if (live(P)) {
-      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 1, d.TFR, coef);
-      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 1, d.TFR, coef);
+      launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
       launch_ok(P);
     }
   } else if (live(P)) {
-    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
   T* Tup = c.W("Tup");
@@ -457,9 +457,9 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CsPr, s2, "sbox");
   if (live(P)) {
-    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
-    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
+    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -503,12 +503,12 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
     FWlSums<T> f{l1, l2, c.nz};
     plane_sums<4>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasddwl");
     if (live(P)) {
-      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 2, d.TFR, coef);
-      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 2, d.TFR, coef);
+      launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
       launch_ok(P);
     }
   } else if (live(P)) {
-    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
   T* LM = c.W("LM");
@@ -517,9 +517,9 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cwl, s2, "box");
   if (live(P)) {
-    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, avg1, avg2);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, avg1, avg2);
     launch_ok(P);
-    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
+    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -565,12 +565,12 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
     FWlScalarSums<T> f{l1, l2, c.nz};
     plane_sums<4>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasddwl");
     if (live(P)) {
-      k_quintic<<<(c.nz + 63) / 64, 64, 0, c.s>>>(sums, c.nz, 2, d.TFR, coef);
-      k_beta<<<(c.nz + 7) / 8, 64, 0, c.s>>>(coef, c.nz, beta);
+      launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 2, d.TFR, coef);
+      launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
       launch_ok(P);
     }
   } else if (live(P)) {
-    k_fill_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, 1.0);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
   T* Tup = c.W("Tup");
@@ -579,9 +579,9 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CwlPr, s2, "sbox");
   if (live(P)) {
-    k_profiles<<<(c.nz + 255) / 256, 256, 0, c.s>>>(s2, c.nz, 1.0 / c.ncol, nullptr, prof);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, nullptr, prof);
     launch_ok(P);
-    if (beta_out) k_copy_d<<<(c.nz + 255) / 256, 256, 0, c.s>>>(c.nz, beta, beta_out);
+    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -757,7 +757,7 @@ template <class T> struct FSum4 {
   }
 };
 
-__global__ void k_cfl_final(const double* __restrict__ part, int nb, double cx, double cy,
+__device__ __forceinline__ void k_cfl_final(const double* __restrict__ part, int nb, double cx, double cy,
                             double cz, double* scal, double* out) {
   __shared__ double sm[3][256];
   double m[3] = {0, 0, 0};
@@ -1079,7 +1079,7 @@ int step_t(alfa_plan* P, const alfa_state* st, int it, double sfc_val, hipStream
     PW_LAUNCH(P, s, c.F, k_ab2_q<T>, c.dims(), SQ, (const T*)RnQ, SRQ, (T)d.dt_nondim,
               (T)(g.q_inversion * d.z_scale * d.dz), first ? 1 : 0);
   if (live(P)) {
-    k_cfl_final<<<1, 256, 0, s>>>(cflpart, AB2_BLOCKS, d.dt_nondim / d.dx, d.dt_nondim / d.dy,
+    launch_k<1024>(P, s, dim3(1), dim3(256), 0, KBODY(k_cfl_final), cflpart, AB2_BLOCKS, d.dt_nondim / d.dx, d.dt_nondim / d.dy,
                                   d.dt_nondim / d.dz, P->scal, st->scalars);
     launch_ok(P);
   }

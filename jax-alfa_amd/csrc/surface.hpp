@@ -66,7 +66,7 @@ template <class T> __device__ __forceinline__ void surf_col(const SurfArgs<T>& a
 }
 
 template <class T>
-__global__ void __launch_bounds__(SURF_T) k_surf_sums(SurfArgs<T> a, double* part) {
+__device__ __forceinline__ void k_surf_sums(const SurfArgs<T>& a, double* part) {
   __shared__ double sm[SURF_T];
   double sM = 0, sT = 0;
   for (int c = blockIdx.x * SURF_T + threadIdx.x; c < a.ncol; c += gridDim.x * SURF_T) {
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(SURF_T) k_surf_sums(SurfArgs<T> a, double* par
 
 // prescribed-temperature flux partials (optSurfBC = 2)
 template <class T>
-__global__ void __launch_bounds__(SURF_T) k_surf_q(SurfArgs<T> a, const double* part,
+__device__ __forceinline__ void k_surf_q(const SurfArgs<T>& a, const double* part,
                                                    double* part2) {
   __shared__ double sm[SURF_T];
   const double invn = 1.0 / a.ncol;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(SURF_T) k_surf_q(SurfArgs<T> a, const double* 
 }
 
 template <class T>
-__global__ void __launch_bounds__(SURF_T) k_surf_final(SurfArgs<T> a, const double* part,
+__device__ __forceinline__ void k_surf_final(const SurfArgs<T>& a, const double* part,
                                                        const double* part2, double* part3) {
   __shared__ double sm[SURF_T];
   const double invn = 1.0 / a.ncol;
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(SURF_T) k_surf_final(SurfArgs<T> a, const doub
   }
 }
 
-__global__ void __launch_bounds__(SURF_T) k_surf_scal(int nb, int ncol, const double* part3,
+__device__ __forceinline__ void k_surf_scal(int nb, int ncol, const double* part3,
                                                       double* scal) {
   __shared__ double sm[SURF_T];
   const double su = sum_parts(part3, nb, 3, sm);
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(SURF_T) k_surf_scal(int nb, int ncol, const do
 // diagnosed from the prescribed surface Q through MOST with the heat
 // stability functions (homogeneous: against the plane mean of Q at level 0).
 template <class T>
-__global__ void k_surf_moist(int ncol, int nz, int mode, int hetero, const T* __restrict__ Q,
+__device__ __forceinline__ void k_surf_moist(int ncol, int nz, int mode, int hetero, const T* __restrict__ Q,
                              const T* __restrict__ ustar, const T* __restrict__ psih,
                              const T* __restrict__ psih0, const double* __restrict__ qavg0,
                              const double* __restrict__ sval, double flux, double vonk,

@@ -18,7 +18,8 @@ if not torch.cuda.is_available():
     pytest.skip("no GPU", allow_module_level=True)
 
 from alfa_amd import core  # noqa: E402
-from alfa_amd.ensemble import MemberSpec, Member, Server, run_ensemble  # noqa: E402
+from alfa_amd.ensemble import (MemberSpec, Member, Server, run_ensemble,  # noqa: E402
+                               run_ensemble_grouped)
 
 
 @pytest.mark.parametrize("case,prec", [("stable", "mixed"), ("convective", "fp64"),
@@ -168,3 +169,44 @@ def test_batch_step_equals_member_steps():
     for a, b in zip(batch, alone):
         for k in ("u", "v", "w", "TH"):
             assert np.array_equal(a.host(k), b.host(k)), k
+
+
+def test_config3_grouped_ensemble_at_full_size():
+    """The same 64-member configuration advanced by shared launches
+    (alfa_group: one launch per kernel for all members of a grid size, the
+    whole ensemble step in one graph per step kind): every member bitwise
+    equal to its run alone after 5 steps, and far fewer launches per step
+    than 64 separate steps."""
+    from alfa_amd.ensemble import ensemble_specs
+    specs = ensemble_specs(64, steps=5, seed=0)
+    out = run_ensemble_grouped(specs, steps=5, keep_fields=True)
+    print({k: v for k, v in out.items() if k != "fields"})
+    assert out["groups"] == 4
+    for i, (spec, got) in enumerate(zip(specs, out["fields"])):
+        for k in ("u", "v", "w", "TH"):
+            assert np.all(np.isfinite(got[k])), (i, k)
+        ref = _solo(spec, 5, "mixed")
+        for k in ("u", "v", "w", "TH", "CFL"):
+            assert np.array_equal(got[k], ref[k]), (i, spec.n, k)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_grouped_members_of_mixed_shapes(prec):
+    """Groups of several shapes at once: stable members with a cooling
+    surface series (each its own rate), dealiased neutral members, and
+    members whose dynamic and static SGS steps alternate (two step kinds,
+    two graphs); each member bitwise its run alone after 7 steps."""
+    specs = [MemberSpec("stable", 32, 9, seed=1, cooling_K_per_h=0.2),
+             MemberSpec("stable", 32, 9, seed=2, cooling_K_per_h=0.6, over=dict(Ug2=6.0)),
+             MemberSpec("neutral", 32, 9, seed=3, over=dict(Ug2=8.0)),
+             MemberSpec("neutral", 32, 9, seed=4, over=dict(Ug2=12.0)),
+             MemberSpec("convective", 48, 9, seed=5, over=dict(dynamicSGS_call_time=2)),
+             MemberSpec("convective", 48, 9, seed=6, over=dict(dynamicSGS_call_time=2,
+                                                               SensibleHeatFlux=0.1))]
+    out = run_ensemble_grouped(specs, steps=7, precision=prec, keep_fields=True)
+    print({k: v for k, v in out.items() if k != "fields"})
+    assert out["groups"] == 3
+    for i, (spec, got) in enumerate(zip(specs, out["fields"])):
+        ref = _solo(spec, 7, prec)
+        for k in ("u", "v", "w", "TH", "CFL"):
+            assert np.array_equal(got[k], ref[k]), (i, spec.case, k)
