@@ -195,6 +195,28 @@ def cpu_baseline():
                              workload="neutral 32^3, LAD-SM, FGR=1 (dealiased), dt=0.5 s"))
 
 
+def copy_probe(mib):
+    """3 device copies of a mib-MiB buffer: a plain copy kernel of known size
+    beside the measured kernels, to scale the HBM traffic counters."""
+    if mib <= 0:
+        return
+    import torch
+    nel = mib * 2 ** 20 // 4
+    src = torch.ones(nel, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    # the same source read as 64-byte rows (16 floats of every 32), the access
+    # width of the FFT kernels' tiles, into a half-size destination
+    half = torch.empty(nel // 2, dtype=torch.float32, device="cuda")
+    rows = src.view(-1, 32)[:, :16]
+    for _ in range(3):
+        half.view(-1, 16).copy_(rows)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: 3 copies of {mib} MiB, 3 copies of its 64-byte rows "
+        f"({mib // 2} MiB read, {mib // 2} MiB written)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,12 +233,18 @@ def main():
                     help="run-length range of served requests (the serving configuration's "
                          "200-2000)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-developed", action="store_true",
+                    help="skip the mixed-vs-FP64 comparison at a developed state (64^3, 1 h)")
     ap.add_argument("--tiles", default=None, metavar="R2C,X,C2R",
                     help="threads per workgroup of the y R2C, x and y C2R FFT kernels (256 or "
                          "512 each) instead of the plan's autotuner, e.g. under a profiler, "
                          "whose trace would also hold the autotuner's trial steps")
     ap.add_argument("--kernels-json", default=None, metavar="PATH",
                     help="write the per-kernel tables (HIP-event times, bytes) to PATH")
+    ap.add_argument("--skip-mixed", action="store_true",
+                    help="profiling: time only the FP64 step (the JSON line then carries it)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="profiling: run only the ensemble and serving workloads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 3 times (a plain copy "
                          "kernel of known size, to scale HBM traffic counters)")
@@ -232,6 +260,23 @@ def main():
     tiles = [int(t) for t in args.tiles.split(",")] if args.tiles else None
     if tiles and (len(tiles) != 3 or any(t not in (256, 512) for t in tiles)):
         ap.error("--tiles takes three of 256 or 512, e.g. 256,256,512")
+    if args.ensemble_only:
+        from alfa_amd import ensemble as E
+        specs = E.ensemble_specs(args.ensemble_members, steps=args.ensemble_steps, seed=0)
+        ens = E.run_ensemble_grouped(specs)
+        log(f"[bench] ensemble: {ens}")
+        print(json.dumps({"ensemble": ens}), flush=True)
+        return
+    if args.skip_mixed:
+        fp64 = run_gpu("fp64", n, args.steps, args.warmup, find_dominant=True, tiles=tiles)
+        log(f"[bench] fp64: {fp64['ms']:.3f} ms/step, {fp64['mpts']:.1f} Mpoint-steps/s")
+        for r in (fp64["table"] or [])[:25]:
+            bw = r["bytes"] / (r["total_ms"] * 1e-3) / 1e9 if r["total_ms"] > 0 else 0
+            log(f"  {r['name']:40s} {r['launches']:5d} launches {r['total_ms'] / 2:9.3f} ms/step "
+                f"{bw:8.0f} GB/s")
+        copy_probe(args.copy_probe)
+        print(json.dumps({"fp64_ms_per_step": fp64["ms"], "value": fp64["mpts"]}), flush=True)
+        return
     mixed = run_gpu("mixed", n, args.steps, args.warmup, tiles=tiles)
     log(f"[bench] mixed: {mixed['ms']:.3f} ms/step, {mixed['mpts']:.1f} Mpoint-steps/s, "
         f"CFL {mixed['cfl']:.3f}")
@@ -296,15 +341,14 @@ def main():
                            f"{args.serve_steps[0]}-{args.serve_steps[1]} steps each, mixed, "
                            f"96 GiB HBM budget")
         log(f"[bench] serving: {srv}")
-    if args.copy_probe > 0:
-        nel = args.copy_probe * 2 ** 20 // 4
-        src = torch.ones(nel, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] copy probe: 3 copies of {args.copy_probe} MiB")
-        del src, dst
+    developed = None
+    if not args.no_developed:
+        # the coefficient error at a developed state, against the variability
+        # of an FP64 run perturbed at float rounding
+        from alfa_amd import stats as S
+        developed = S.developed_state_comparison(n=64, hours=1.0, avg_min=30.0)
+        log(f"[bench] developed state: {developed}")
+    copy_probe(args.copy_probe)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": f"Mpoint-steps/s at {n}^3 (FP32-mixed vs FP64)",
@@ -334,6 +378,7 @@ def main():
         # the FP64 GPU path is held to the FP64 oracle at <= 1e-11 by the parity
         # tests; at 256^3 it stands in for the oracle, which would take hours
         "error_mixed_vs_fp64": errors,
+        "error_mixed_vs_fp64_developed": developed,
         "roofline": roof,
         "ensemble": ens,
         "serving": srv,

@@ -107,3 +107,68 @@ def run(les: LES, nsteps: int, SampleInterval: int, OutputInterval: int, outdir:
             paths.append(stats.save(outdir, it))
             stats.reset()
     return paths
+
+
+def developed_state_comparison(n: int = 64, hours: float = 1.0, avg_min: float = 30.0,
+                               samples: int = 60, case: str = "stable") -> dict:
+    """Mixed precision against FP64 at a developed state.
+
+    Three runs of case `case` at n^3 (dt scaled with the grid spacing from
+    the case's base grid) for `hours` of simulated time: mixed, FP64, and
+    FP64 with the initial theta perturbed by 2^-24 relative (float
+    rounding).  Each averages the statistics of `ComputeStats` (on the
+    device) over the last `avg_min` minutes.  After the flow has become
+    turbulent the runs are decorrelated, so the time averages differ by
+    their finite-sample variability; the perturbed FP64 run measures that
+    variability, and a mixed run without a precision bias differs from FP64
+    by about as much.  Returns, for both comparisons, the largest
+    difference of the mean wind profile (relative to the largest mean wind
+    speed), the theta profile (relative to the absolute theta), the
+    resolved-TKE profile and the dynamic-coefficient profiles Cs2 and
+    Cs2/Pr (relative to their largest value), on the levels whose resolved
+    TKE is at least 1% of its largest.
+    """
+    from . import cases, core
+    base = {"neutral": 32, "convective": 128, "stable": 256}[case]
+    dt = cases.CASES[case]["dt"] * base / n
+    nsteps = int(round(hours * 3600.0 / dt))
+    navg = int(round(avg_min * 60.0 / dt))
+    every = max(1, navg // samples)
+    res = {}
+    for run in ("mixed", "fp64", "fp64_perturbed"):
+        cfg, u, v, w, TH = cases.initial_fields(case, nx=n, ny=n, nz=n, dt=dt)
+        if run == "fp64_perturbed":
+            rng = np.random.default_rng(7)
+            TH = TH + 2.0 ** -24 * (TH + cfg["T_0"]) * rng.standard_normal(TH.shape)
+        ser = cases.surface_series(cfg, nsteps + 2)
+        les = core.LES(core.Plan(cfg, "mixed" if run == "mixed" else "fp64"), u, v, w, TH,
+                       sfc_series=ser)
+        st = Statistics(les)
+        for k in range(nsteps):
+            les.step(1)
+            if k >= nsteps - navg and (nsteps - 1 - k) % every == 0:
+                st.sample()
+        res[run] = st.averaged()
+        les.plan.close()
+    ref = res["fp64"]
+    tke = ref["u2"] + ref["v2"] + ref["w2"]
+    turb = tke >= 0.01 * np.max(tke)
+    speed = float(np.max(np.hypot(ref["U"], ref["V"])))
+
+    def compare(a):
+        out = {"mean_wind_profile": float(max(np.max(np.abs(a["U"] - ref["U"])),
+                                              np.max(np.abs(a["V"] - ref["V"]))) / speed),
+               "mean_theta_profile": float(np.max(np.abs(a["TH"] - ref["TH"]) /
+                                                  np.abs(ref["TH"]))),
+               "tke_profile": float(np.max(np.abs(a["u2"] + a["v2"] + a["w2"] - tke)) /
+                                    np.max(tke))}
+        for k in ("Cs2_2", "Cs2PrRatio"):
+            d = np.abs(a[k] - ref[k])[turb]
+            out[k + "_profile"] = float(np.max(d) / max(np.max(np.abs(ref[k][turb])), 1e-300))
+        return {k: float("%.3g" % v) for k, v in out.items()}
+
+    return {"case": case, "grid": n, "dt_s": dt, "steps": nsteps, "simulated_h": hours,
+            "average_over_min": avg_min, "samples_per_run": len(range(0, navg, every)),
+            "turbulent_levels": int(turb.sum()),
+            "mixed_vs_fp64": compare(res["mixed"]),
+            "fp64_perturbed_vs_fp64": compare(res["fp64_perturbed"])}
