@@ -656,16 +656,17 @@ extern "C" int alfa_plan_create(const alfa_config* cfg, int precision, alfa_plan
   if (e == hipSuccess) P->tw_mx = tw(P->mx);
   if (e == hipSuccess) P->tw_my = tw(P->my);
   // Thomas factors (ThomasPressureInit, NSE_PressureTerms_Thomas.py:151-214)
+  // the pivots depend on kx only through kx^2: one row per (|kx|, ky)
   const int N = nz + 1, nyc = P->nyc;
-  const size_t ncolc = (size_t)nx * nyc;
+  const size_t ncolc = (size_t)(nx / 2 + 1) * nyc;
   std::vector<double> hb(ncolc * N, 0.0);   // pivots; the kernel forms the multipliers
   {
     const double idz2 = 1.0 / (d.dz * d.dz);
     std::vector<double> a(N, idz2), c(N, idz2), b(N);
     a[0] = 0; a[nz] = -1; c[0] = 1; c[nz] = 0;
-    for (int i = 0; i < nx; ++i)
+    for (int i = 0; i <= nx / 2; ++i)
       for (int j = 0; j < nyc; ++j) {
-        const double kr = (i < nx / 2) ? i : i - nx, kc = j;
+        const double kr = i, kc = j;
         const double bb = -(kr * kr + kc * kc + 2.0 / (d.dz * d.dz));
         for (int k = 0; k < N; ++k) b[k] = bb;
         if (i == 0 && j == 0)

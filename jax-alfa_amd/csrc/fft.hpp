@@ -18,6 +18,7 @@
 //             store functor (fused epilogues).
 #pragma once
 #include "common.hpp"
+#include <type_traits>
 
 // ----------------------------------------------------------------------------
 // small DFTs; sign = -1 forward, +1 inverse
@@ -475,10 +476,20 @@ struct TileGrid {
 // ncy_out columns of its half spectrum, (rows, ncy_out, nz) complex.  With
 // vec set (nz a multiple of the vector width, 16-byte aligned buffers) every
 // thread loads 16 bytes of z at a time through ld.vload and stores the two
-// complex values of one packed line together.  BAND (compile-time lengths
-// with ncy_out <= N/RL, chosen by the launcher): the last radix pass is
-// evaluated only where the store reads it.
+// complex values of one packed line together.  The compile-time-length
+// kernels (NCT > 0) are built for vec only (the launcher sends other cases
+// to the generic kernel), which keeps the element-wise path and its
+// registers out of them.  BAND (compile-time lengths with ncy_out <= N/RL,
+// chosen by the launcher): the last radix pass is evaluated only where the
+// store reads it.
 // ----------------------------------------------------------------------------
+// number of field kinds of a loader with per-kind load loops (with_kind /
+// vload_k), 0 for loaders with one vload for every field
+template <class L, class = void> struct LoaderKinds { static constexpr int value = 0; };
+template <class L> struct LoaderKinds<L, std::void_t<decltype(L::KINDS)>> {
+  static constexpr int value = L::KINDS;
+};
+
 template <class T, class Loader, int MAXN, int NCT, int BS, bool BAND = false>
 __device__ __forceinline__ void
 k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, const PtrBatch& outb,
@@ -493,13 +504,24 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
   T* bufr = reinterpret_cast<T*>(buf);
   TwStage<T, BS, (NCT > 0 ? NCT : MAXN)> tws;
   tws.load(tw, n, stw);
+  const bool vv = NCT > 0 || vec;
   int zt, x, b;
   for (int it = 0; tg.tile(it, zt, x, b); ++it) {
     __syncthreads();
-    if (vec) {
-      for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
-        const int y = e / ZT, z = zt * ZT + (e - y * ZT);
-        vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
+    if (vv) {
+      if constexpr (LoaderKinds<Loader>::value > 0) {
+        ld.with_kind(b, [&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
+            const int y = e / ZT, z = zt * ZT + (e - y * ZT);
+            vst(bufr + e, z < nz ? ld.template vload_k<K>(b, x, y, z) : vzero<T>());
+          }
+        });
+      } else {
+        for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
+          const int y = e / ZT, z = zt * ZT + (e - y * ZT);
+          vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
+        }
       }
     } else {
       for (int i = threadIdx.x; i < nEl; i += BS) {
@@ -529,7 +551,7 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
       const c2<T> r0 = mkc<T>(T(0.5) * (Zk.x + Zn.x), T(0.5) * (Zk.y - Zn.y));
       const c2<T> r1 = mkc<T>(T(0.5) * (Zk.y + Zn.y), T(-0.5) * (Zk.x - Zn.x));
       c2<T>* o = out + ((size_t)x * ncy_out + k) * nz + z;
-      if (vec) {
+      if (vv) {
         cst2<T>(o, r0, r1);
       } else {
         o[0] = r0;

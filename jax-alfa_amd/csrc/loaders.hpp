@@ -2,6 +2,7 @@
 // while the y-transform loads its tile, so they never exist in HBM.
 #pragma once
 #include "common.hpp"
+#include <type_traits>
 
 // The 21 momentum fields of DynamicSGS_LASDD_SM.py:184-233:
 //   0..2  u_, v_, w_ (w_ = w averaged to uvp nodes, :187-190)
@@ -47,66 +48,98 @@ template <class T> struct LoadLasdd {
       default: return vv * ww;
     }
   }
-  // W consecutive z values from z (z % W == 0, z + W <= nz)
-  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
-    constexpr int W = Vec16<T>::W;
+  // 16-byte loads.  The field of a tile is uniform over the workgroup, so
+  // the kernel picks the field's kind once per tile (with_kind) and runs a
+  // load loop compiled for that kind (vload_k): no per-element branching on
+  // the field.  Kinds: 0 u_, 1 v_, 2 w_, 3 uu, 4 vv, 5 ww, 6 uv, 7 uw, 8 vw,
+  // 9 S_ij, 10 |S| S_ij.
+  static constexpr int KINDS = 11;
+  template <class F> __device__ __forceinline__ void with_kind(int b, F&& f) const {
     b += b0;
-    const size_t i = ((size_t)x * ny + y) * nz + z;
-    Vec16<T> r;
-    if (b >= 9) {
-      r = vld(S[(b - 9) % 6] + i);
-      if (b >= 15) {
-        const Vec16<T> m = vld(Sm + i);
-#pragma unroll
-        for (int j = 0; j < W; ++j) r.v[j] *= m.v[j];
-      }
-      return r;
+    switch (b < 9 ? b : (b < 15 ? 9 : 10)) {
+      case 0: f(std::integral_constant<int, 0>{}); break;
+      case 1: f(std::integral_constant<int, 1>{}); break;
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      case 5: f(std::integral_constant<int, 5>{}); break;
+      case 6: f(std::integral_constant<int, 6>{}); break;
+      case 7: f(std::integral_constant<int, 7>{}); break;
+      case 8: f(std::integral_constant<int, 8>{}); break;
+      case 9: f(std::integral_constant<int, 9>{}); break;
+      default: f(std::integral_constant<int, 10>{}); break;
     }
-    // only the inputs field b needs (b is uniform over the workgroup)
-    Vec16<T> uu = vzero<T>(), vv = vzero<T>(), ww = vzero<T>();
-    if (b == 0 || b == 3 || b == 6 || b == 7) {
-      uu = vld(u + i);
-      if (shu) {
+  }
+  // a velocity with its level shift (W consecutive z from z, z % W == 0)
+  __device__ __forceinline__ Vec16<T> vshift(const T* f, const double* sh, size_t i, int z) const {
+    Vec16<T> r = vld(f + i);
+    if (sh) {
 #pragma unroll
-        for (int j = 0; j < W; ++j) uu.v[j] = (T)((double)uu.v[j] - shu[z + j]);
-      }
-    }
-    if (b == 1 || b == 4 || b == 6 || b == 8) {
-      vv = vld(v + i);
-      if (shv) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) vv.v[j] = (T)((double)vv.v[j] - shv[z + j]);
-      }
-    }
-    if (b == 2 || b == 5 || b == 7 || b == 8) {
-      // w averaged to uvp nodes: needs w[z .. z+W]
-      const Vec16<T> w0 = vld(w + i);
-      const T wn = (z + W < nz) ? w[i + W] : T(0);
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const int zz = z + j;
-        const T a = w0.v[j], c = (j + 1 < W) ? w0.v[j + 1] : wn;
-        if (zz == 0) ww.v[j] = T(0.5) * c;
-        else if (zz == nz - 1) ww.v[j] = (j > 0) ? w0.v[j - 1] : w[i - 1];
-        else ww.v[j] = T(0.5) * (a + c);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      const T a = uu.v[j], c = vv.v[j], e = ww.v[j];
-      switch (b) {
-        case 0: r.v[j] = a; break;
-        case 1: r.v[j] = c; break;
-        case 2: r.v[j] = e; break;
-        case 3: r.v[j] = a * a; break;
-        case 4: r.v[j] = c * c; break;
-        case 5: r.v[j] = e * e; break;
-        case 6: r.v[j] = a * c; break;
-        case 7: r.v[j] = a * e; break;
-        default: r.v[j] = c * e; break;
-      }
+      for (int j = 0; j < Vec16<T>::W; ++j) r.v[j] = (T)((double)r.v[j] - sh[z + j]);
     }
     return r;
+  }
+  // w averaged to uvp nodes: needs w[z .. z+W]
+  __device__ __forceinline__ Vec16<T> vwbar(size_t i, int z) const {
+    constexpr int W = Vec16<T>::W;
+    const Vec16<T> w0 = vld(w + i);
+    const T wn = (z + W < nz) ? w[i + W] : T(0);
+    Vec16<T> r;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int zz = z + j;
+      const T a = w0.v[j], c = (j + 1 < W) ? w0.v[j + 1] : wn;
+      if (zz == 0) r.v[j] = T(0.5) * c;
+      else if (zz == nz - 1) r.v[j] = (j > 0) ? w0.v[j - 1] : w[i - 1];
+      else r.v[j] = T(0.5) * (a + c);
+    }
+    return r;
+  }
+  template <int K> __device__ __forceinline__ Vec16<T> vload_k(int b, int x, int y, int z) const {
+    constexpr int W = Vec16<T>::W;
+    const size_t i = ((size_t)x * ny + y) * nz + z;
+    Vec16<T> r;
+    if constexpr (K == 0) return vshift(u, shu, i, z);
+    else if constexpr (K == 1) return vshift(v, shv, i, z);
+    else if constexpr (K == 2) return vwbar(i, z);
+    else if constexpr (K == 3) {
+      const Vec16<T> a = vshift(u, shu, i, z);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * a.v[j];
+      return r;
+    } else if constexpr (K == 4) {
+      const Vec16<T> a = vshift(v, shv, i, z);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * a.v[j];
+      return r;
+    } else if constexpr (K == 5) {
+      const Vec16<T> a = vwbar(i, z);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * a.v[j];
+      return r;
+    } else if constexpr (K == 6) {
+      const Vec16<T> a = vshift(u, shu, i, z), c = vshift(v, shv, i, z);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * c.v[j];
+      return r;
+    } else if constexpr (K == 7) {
+      const Vec16<T> a = vshift(u, shu, i, z), c = vwbar(i, z);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * c.v[j];
+      return r;
+    } else if constexpr (K == 8) {
+      const Vec16<T> a = vshift(v, shv, i, z), c = vwbar(i, z);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * c.v[j];
+      return r;
+    } else if constexpr (K == 9) {
+      return vld(S[b + b0 - 9] + i);
+    } else {
+      const Vec16<T> a = vld(S[b + b0 - 15] + i), m = vld(Sm + i);
+#pragma unroll
+      for (int j = 0; j < W; ++j) r.v[j] = a.v[j] * m.v[j];
+      return r;
+    }
   }
   bool aligned(int) const {
     for (int n = 0; n < 6; ++n)
@@ -176,41 +209,57 @@ template <class T> struct LoadScalar {
     }
     return r;
   }
-  __device__ __forceinline__ Vec16<T> vload(int b, int x, int y, int z) const {
+  __device__ __forceinline__ Vec16<T> vshift(const T* f, const double* sh, size_t i, int z) const {
+    Vec16<T> r = vld(f + i);
+    if (sh) {
+#pragma unroll
+      for (int j = 0; j < Vec16<T>::W; ++j) r.v[j] = (T)((double)r.v[j] - sh[z + j]);
+    }
+    return r;
+  }
+  // 16-byte loads, one load loop per kind of field (as LoadLasdd): 0 TH_,
+  // 1 u_TH, 2 v_TH, 3 w_TH, 4 dTHdx or dTHdy, 6 THz, 7 |S| dTHdx or |S|
+  // dTHdy, 9 |S| THz
+  static constexpr int KINDS = 8;
+  template <class F> __device__ __forceinline__ void with_kind(int b, F&& f) const {
+    b += b0;
+    switch (b) {
+      case 0: f(std::integral_constant<int, 0>{}); break;
+      case 1: f(std::integral_constant<int, 1>{}); break;
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: case 5: f(std::integral_constant<int, 4>{}); break;
+      case 6: f(std::integral_constant<int, 6>{}); break;
+      case 7: case 8: f(std::integral_constant<int, 7>{}); break;
+      default: f(std::integral_constant<int, 9>{}); break;
+    }
+  }
+  template <int K> __device__ __forceinline__ Vec16<T> vload_k(int b, int x, int y, int z) const {
     constexpr int W = Vec16<T>::W;
     b += b0;
     const size_t i = ((size_t)x * ny + y) * nz + z;
-    Vec16<T> r;
-    if (b >= 4) {
-      const int d = (b - 4) % 3;
-      r = d == 0 ? vld(dx + i) : d == 1 ? vld(dy + i) : stag(dz, i, z, false);
-      if (b >= 7) {
-        const Vec16<T> m = vld(Sm + i);
+    if constexpr (K == 4) return vld((b == 4 ? dx : dy) + i);
+    else if constexpr (K == 6) return stag(dz, i, z, false);
+    else if constexpr (K == 7 || K == 9) {
+      Vec16<T> r = K == 7 ? vld((b == 7 ? dx : dy) + i) : stag(dz, i, z, false);
+      const Vec16<T> m = vld(Sm + i);
 #pragma unroll
-        for (int j = 0; j < W; ++j) r.v[j] *= m.v[j];
-      }
+      for (int j = 0; j < W; ++j) r.v[j] *= m.v[j];
       return r;
-    }
-    Vec16<T> th = vld(TH + i);
-    if (sht) {
-#pragma unroll
-      for (int j = 0; j < W; ++j) th.v[j] = (T)((double)th.v[j] - sht[z + j]);
-    }
-    if (b == 0) return th;
-    Vec16<T> f;
-    if (b == 3) {
-      f = w_uvp ? vld(w + i) : stag(w, i, z, true);
     } else {
-      f = vld((b == 1 ? u : v) + i);
-      const double* sh = b == 1 ? shu : shv;
-      if (sh) {
+      const Vec16<T> th = vshift(TH, sht, i, z);
+      if constexpr (K == 0) return th;
+      else {
+        Vec16<T> f;
+        if constexpr (K == 3) f = w_uvp ? vld(w + i) : stag(w, i, z, true);
+        else if constexpr (K == 1) f = vshift(u, shu, i, z);
+        else f = vshift(v, shv, i, z);
+        Vec16<T> r;
 #pragma unroll
-        for (int j = 0; j < W; ++j) f.v[j] = (T)((double)f.v[j] - sh[z + j]);
+        for (int j = 0; j < W; ++j) r.v[j] = f.v[j] * th.v[j];
+        return r;
       }
     }
-#pragma unroll
-    for (int j = 0; j < W; ++j) r.v[j] = f.v[j] * th.v[j];
-    return r;
   }
   bool aligned(int) const {
     return al16(u) && al16(v) && al16(w) && al16(TH) && al16(dx) && al16(dy) && al16(dz) &&

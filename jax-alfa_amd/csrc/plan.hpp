@@ -73,7 +73,7 @@ struct alfa_plan {
   int nx, ny, nz, nyc, mx, my, myc;
   Fft1D fx, fy, fmx, fmy;
   void *tw_x = nullptr, *tw_y = nullptr, *tw_mx = nullptr, *tw_my = nullptr;
-  double* th_b = nullptr;   // Thomas pivots (nx*nyc, nz+1)
+  double* th_b = nullptr;   // Thomas pivots (nx/2+1, nyc, nz+1): row (|kx|, ky)
   double* prof = nullptr;                     // Ug, Vg, rf, rs, Uadv, Vadv, THadv, Qadv: 8*nz
   bool adv_on = false;                        // advection forcing set (alfa_set_forcing)
   const double* fadv() const { return adv_on ? prof + 4 * (size_t)nz : nullptr; }
@@ -232,7 +232,10 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
   // buffers; the paired complex stores need 16-byte aligned outputs
   bool vec = nz % Vec16<T>::W == 0 && ld.aligned(nb);
   for (int b = 0; b < nb; ++b) vec = vec && al16(outs[b]);
-  ct_length(fyy.n, [&](auto nct, auto maxn) {
+  // the compile-time-length kernels are built for 16-byte accesses only:
+  // without them the generic kernel of the length's register tier runs
+  const int nsel = vec ? fyy.n : (fyy.n <= 256 ? 1 : FFT_MAXN);
+  ct_length(nsel, [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
     auto launch = [&](auto bs) {
       constexpr int BS = decltype(bs)::value;

@@ -17,7 +17,9 @@
 template <class T> struct ThomasArgs {
   const c2<T>* fRx; const c2<T>* fRy; const c2<T>* fRz;   // (nx, nyc, nz)
   const c2<T>* fbt;                                        // (nx, nyc, 2)
-  const double* b;   // (ncol, nz+1) pivots of ThomasPressureInit's elimination
+  // (nx/2+1, nyc, nz+1) pivots of ThomasPressureInit's elimination; they
+  // depend on kx only through kx^2, so row (|kx|, ky) serves kx and -kx
+  const double* b;
   c2<T>* phat; c2<T>* dpz;                                 // (nx, nyc, nz)
   T* RCr; T* RCi;             // optional out (ncol, nz+1): RC before masking
   const T* RCin_r; const T* RCin_i;   // optional in (ncol, nz+1): RC given
@@ -27,7 +29,11 @@ template <class T> struct ThomasArgs {
 };
 
 constexpr int TH_MAXN = 520;   // nz + 1 <= 520
-// dynamic LDS of one column: 4 arrays of nz + 1 doubles
+// levels of a lane whose right-hand-side loads are issued together
+#ifndef ALFA_TH_TU
+#define ALFA_TH_TU 3
+#endif
+// dynamic LDS of one column: 3 arrays of nz + 1 doubles
 inline size_t thomas_lds(int nz) { return 3 * (size_t)(nz + 1) * sizeof(double); }
 
 __device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
@@ -47,30 +53,59 @@ __device__ __forceinline__ void k_thomas(const ThomasArgs<T>& a) {
   const double kc = (double)j;
   const bool special = (i == 0 && j == 0) || (2 * i == a.nx) || (2 * j == a.ny);
   const size_t cb = (size_t)col * nz;
+  const int ia = (i <= a.nx / 2) ? i : a.nx - i;
+  const double* brow = a.b + ((size_t)ia * a.nyc + j) * N;
   // right-hand side of the column (NSE_PressureTerms.py:150-180)
-  for (int k = lane; k < N; k += 64) {
-    double rr, ri;
-    if (a.RCin_r) {
-      rr = (double)a.RCin_r[(size_t)col * N + k];
-      ri = (double)a.RCin_i[(size_t)col * N + k];
-    } else if (k == 0 || k == nz) {
-      const c2<T> f = a.fbt[(size_t)col * 2 + (k == 0 ? 0 : 1)];
-      rr = -a.dz * (double)f.x;
-      ri = -a.dz * (double)f.y;
-    } else {
-      const c2<T> X = a.fRx[cb + k - 1], Y = a.fRy[cb + k - 1];
-      const c2<T> Z1 = a.fRz[cb + k], Z0 = a.fRz[cb + k - 1];
-      double hr = -kr * (double)X.y;
-      hr -= kc * (double)Y.y;
-      double hi = kr * (double)X.x;
-      hi += kc * (double)Y.x;
-      rr = hr + ((double)Z1.x - (double)Z0.x) / a.dz;
-      ri = hi + ((double)Z1.y - (double)Z0.y) / a.dz;
+  if (a.RCin_r) {
+    for (int k = lane; k < N; k += 64) {
+      const double rr = (double)a.RCin_r[(size_t)col * N + k];
+      const double ri = (double)a.RCin_i[(size_t)col * N + k];
+      if (a.RCr) { a.RCr[(size_t)col * N + k] = (T)rr; a.RCi[(size_t)col * N + k] = (T)ri; }
+      dr[k] = special ? 0.0 : rr;
+      di[k] = special ? 0.0 : ri;
+      sb[k] = brow[k];
     }
-    if (a.RCr) { a.RCr[(size_t)col * N + k] = (T)rr; a.RCi[(size_t)col * N + k] = (T)ri; }
-    dr[k] = special ? 0.0 : rr;
-    di[k] = special ? 0.0 : ri;
-    sb[k] = a.b[(size_t)col * N + k];
+  } else {
+    // the levels of a lane TU at a time, every load of the batch issued
+    // first (indices clamped into the column, so the loads need no branch):
+    // one memory round trip per batch instead of one per level
+    constexpr int TU = ALFA_TH_TU;
+    const c2<T> fb0 = a.fbt[(size_t)col * 2], fb1 = a.fbt[(size_t)col * 2 + 1];
+    for (int kb = lane; kb < N; kb += 64 * TU) {
+      c2<T> X[TU], Y[TU], Z1[TU], Z0[TU];
+      double bv[TU];
+#pragma unroll
+      for (int t = 0; t < TU; ++t) {
+        const int k = kb + 64 * t;
+        const int kq = k < 1 ? 1 : (k > nz - 1 ? nz - 1 : k);   // an interior level
+        bv[t] = brow[k < N ? k : N - 1];
+        X[t] = a.fRx[cb + kq - 1]; Y[t] = a.fRy[cb + kq - 1];
+        Z1[t] = a.fRz[cb + kq]; Z0[t] = a.fRz[cb + kq - 1];
+      }
+#pragma unroll
+      for (int t = 0; t < TU; ++t) {
+        const int k = kb + 64 * t;
+        if (k < N) {
+          double rr, ri;
+          if (k == 0 || k == nz) {
+            const c2<T> f = k == 0 ? fb0 : fb1;
+            rr = -a.dz * (double)f.x;
+            ri = -a.dz * (double)f.y;
+          } else {
+            double hr = -kr * (double)X[t].y;
+            hr -= kc * (double)Y[t].y;
+            double hi = kr * (double)X[t].x;
+            hi += kc * (double)Y[t].x;
+            rr = hr + ((double)Z1[t].x - (double)Z0[t].x) / a.dz;
+            ri = hi + ((double)Z1[t].y - (double)Z0[t].y) / a.dz;
+          }
+          if (a.RCr) { a.RCr[(size_t)col * N + k] = (T)rr; a.RCi[(size_t)col * N + k] = (T)ri; }
+          dr[k] = special ? 0.0 : rr;
+          di[k] = special ? 0.0 : ri;
+          sb[k] = bv[t];
+        }
+      }
+    }
   }
   __syncthreads();
   if (special) {
@@ -144,14 +179,21 @@ __device__ __forceinline__ void k_thomas(const ThomasArgs<T>& a) {
     dr[k] = yr;
     di[k] = yi;
   }
-  // back substitution: x_k = (y_k - c_k x_{k+1}) / b_k   (:135-142)
+  // back substitution: x_k = (y_k - c_k x_{k+1}) / b_k   (:135-142).  The
+  // quotients -c_k/b_k and y_k/b_k are formed once, kept in the lane's own
+  // levels of sb, dr and di, and read again by the replay below.
   A = 1.0; Br = 0.0; Bi = 0.0;
   for (int k = k1 - 1; k >= k0; --k) {
     const double ck = (k == 0) ? 1.0 : (k == nz ? 0.0 : a.c_in);
-    const double al = (k == N - 1) ? 0.0 : -ck / sb[k];
+    const double pk = sb[k];
+    const double al = (k == N - 1) ? 0.0 : -ck / pk;
+    const double qr = dr[k] / pk, qi = di[k] / pk;
     A *= al;
-    Br = al * Br + dr[k] / sb[k];
-    Bi = al * Bi + di[k] / sb[k];
+    Br = al * Br + qr;
+    Bi = al * Bi + qi;
+    sb[k] = al;
+    dr[k] = qr;
+    di[k] = qi;
   }
   for (int off = 1; off < 64; off <<= 1) {
     const double A2 = shfl_down_d(A, off), R2 = shfl_down_d(Br, off), I2 = shfl_down_d(Bi, off);
@@ -164,10 +206,9 @@ __device__ __forceinline__ void k_thomas(const ThomasArgs<T>& a) {
   double xr = shfl_down_d(Br, 1), xi = shfl_down_d(Bi, 1);
   if (lane == 63) { xr = 0.0; xi = 0.0; }
   for (int k = k1 - 1; k >= k0; --k) {
-    const double ck = (k == 0) ? 1.0 : (k == nz ? 0.0 : a.c_in);
-    const double al = (k == N - 1) ? 0.0 : -ck / sb[k];
-    xr = al * xr + dr[k] / sb[k];
-    xi = al * xi + di[k] / sb[k];
+    const double al = sb[k];
+    xr = al * xr + dr[k];
+    xi = al * xi + di[k];
     dr[k] = xr;
     di[k] = xi;
   }

@@ -271,9 +271,9 @@ void stage_pressure_from_spectra(Ctx<T>& c, c2<T>* fR[3], c2<T>* fbt, T* dpdx, T
   ta.nx = c.nx; ta.ny = c.ny; ta.nyc = c.nyc; ta.nz = c.nz;
   ta.dz = d.dz; ta.idz = 1.0 / d.dz; ta.c_in = 1.0 / (d.dz * d.dz);
   {
-    // 3 spectra in, 2 spectra out, the double pivot table
+    // 3 spectra in, 2 spectra out, the double pivot table (rows of |kx|)
     const double spb = (double)c.nx * c.nyc * c.nz * sizeof(c2<T>);
-    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 1.0 * c.nx * c.nyc * (c.nz + 1) * 8.0, 64,
+    KLAUNCH_LDS(P, c.s, "k_thomas", 5.0 * spb + 1.0 * (c.nx / 2 + 1) * c.nyc * (c.nz + 1) * 8.0, 64,
                 k_thomas<T>, c.nx * c.nyc, 64, thomas_lds(c.nz), ta);
   }
   XArgs xa{};
