@@ -176,14 +176,14 @@ def cpu_baseline():
     with one worker per core of the process's CPU share (at most 16, the
     share of one GPU), the rest of the NumPy work on one thread.  Samples:
     the stable layer of the GPU workload (same physics: LAD-SM, FGR=2,
-    prescribed cooling surface) at 128^3 for 2 steps, and the neutral 32^3
+    prescribed cooling surface) at 128^3 for 10 steps, and the neutral 32^3
     configuration for its 100 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import alfa_oracle as O
     workers = min(len(os.sched_getaffinity(0)), 16)
     O.set_fft_workers(workers)
     try:
-        v128, k128, t128 = _oracle_rate(O, "stable", dict(nx=128, ny=128, nz=128), 2, 30.0)
+        v128, k128, t128 = _oracle_rate(O, "stable", dict(nx=128, ny=128, nz=128), 10, 30.0)
         v32, k32, t32 = _oracle_rate(O, "neutral", {}, 100, 30.0)
     finally:
         O.set_fft_workers(None)

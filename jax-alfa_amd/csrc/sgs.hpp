@@ -605,27 +605,31 @@ k_box_clip_v(const RedGeom& g, const T* __restrict__ num, const T* __restrict__ 
     const int c1 = imin(c0 + chunk, g.ncol);
     const T inv9 = T(1.0 / 9.0);
     Vec16<T> a0, a1, a2, b0, b1, b2;   // column sums at y-1, y, y+1
-    int xm = 0, x = -1, xp = 0;
+    // the x rows x-1, x, x+1 of the two fields at level z; the chunk walks
+    // y with one division at its start and rebases the rows when it crosses
+    // into the next x
+    const T *nm = nullptr, *n0 = nullptr, *np = nullptr, *dm = nullptr, *d0 = nullptr, *dp = nullptr;
     auto col = [&](int yy, Vec16<T>& sa, Vec16<T>& sb) {
-      const size_t jm = ((size_t)xm * ny + yy) * nz + z;
-      const size_t j0 = ((size_t)x * ny + yy) * nz + z;
-      const size_t jp = ((size_t)xp * ny + yy) * nz + z;
-      const Vec16<T> nm = vld(num + jm), n0 = vld(num + j0), np = vld(num + jp);
-      const Vec16<T> dm = vld(den + jm), d0 = vld(den + j0), dp = vld(den + jp);
+      const size_t o = (size_t)yy * nz;
+      const Vec16<T> vm = vld(nm + o), v0 = vld(n0 + o), vp = vld(np + o);
+      const Vec16<T> wm = vld(dm + o), w0 = vld(d0 + o), wp = vld(dp + o);
 #pragma unroll
       for (int j = 0; j < W; ++j) {
-        sa.v[j] = nm.v[j] + n0.v[j] + np.v[j];
-        sb.v[j] = dm.v[j] + d0.v[j] + dp.v[j];
+        sa.v[j] = vm.v[j] + v0.v[j] + vp.v[j];
+        sb.v[j] = wm.v[j] + w0.v[j] + wp.v[j];
       }
     };
+    int x = c0 / ny, y = c0 - x * ny;
+    bool fresh = true;
     for (int c = c0; c < c1; ++c) {
-      const int cx = c / ny, y = c - cx * ny;
       const int yp = (y + 1 == ny) ? 0 : y + 1;
-      if (cx != x || c == c0) {
-        x = cx;
-        xm = (x + nx - 1) % nx;
-        xp = (x + 1) % nx;
-        col((y + ny - 1) % ny, a0, b0);
+      if (fresh) {
+        fresh = false;
+        const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x + 1 == nx) ? 0 : x + 1;
+        const size_t pl = (size_t)ny * nz;
+        nm = num + xm * pl + z; n0 = num + x * pl + z; np = num + xp * pl + z;
+        dm = den + xm * pl + z; d0 = den + x * pl + z; dp = den + xp * pl + z;
+        col((y == 0) ? ny - 1 : y - 1, a0, b0);
         col(y, a1, b1);
       } else {
         a0 = a1; b0 = b1;
@@ -644,6 +648,7 @@ k_box_clip_v(const RedGeom& g, const T* __restrict__ num, const T* __restrict__ 
         s2[j] += (double)r;
       }
       vst(out + (size_t)c * nz + z, o);
+      if (++y == ny) { y = 0; ++x; fresh = true; }
     }
   }
 #pragma unroll
