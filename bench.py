@@ -108,12 +108,13 @@ def error_report(mx, fp):
     return r
 
 
-def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None, tiles=None):
+def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None, tiles=None, fusion=True):
     import torch
     from alfa_amd import cases, core
     cfg, u, v, w, TH = cases.initial_fields("stable", nx=n, ny=n, nz=n)
     ser = cases.surface_series(cfg, warmup + steps + 8)
     P = core.Plan(cfg, prec)
+    P.set_fusion(fusion)
     les = core.LES(P, u, v, w, TH, sfc_series=ser)
     del u, v, w, TH
     log(f"[bench] {prec} {n}^3 plan ready, workspace {P.derived['workspace_bytes'] / 2**30:.2f} GiB")
@@ -239,6 +240,9 @@ def main():
                     help="threads per workgroup of the y R2C, x and y C2R FFT kernels (256 or "
                          "512 each) instead of the plan's autotuner, e.g. under a profiler, "
                          "whose trace would also hold the autotuner's trial steps")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="run the LAD dynamic procedure's separate test-filter C2R and "
+                         "contraction kernels instead of the fused ones (A/B comparisons)")
     ap.add_argument("--kernels-json", default=None, metavar="PATH",
                     help="write the per-kernel tables (HIP-event times, bytes) to PATH")
     ap.add_argument("--skip-mixed", action="store_true",
@@ -277,7 +281,7 @@ def main():
         copy_probe(args.copy_probe)
         print(json.dumps({"fp64_ms_per_step": fp64["ms"], "value": fp64["mpts"]}), flush=True)
         return
-    mixed = run_gpu("mixed", n, args.steps, args.warmup, tiles=tiles)
+    mixed = run_gpu("mixed", n, args.steps, args.warmup, tiles=tiles, fusion=not args.no_fusion)
     log(f"[bench] mixed: {mixed['ms']:.3f} ms/step, {mixed['mpts']:.1f} Mpoint-steps/s, "
         f"CFL {mixed['cfl']:.3f}")
     for r in (mixed["table"] or [])[:25]:
@@ -375,6 +379,9 @@ def main():
         # threads per workgroup of the y R2C, x and y C2R kernels, as the
         # plan's autotuner chose them after the warmup
         "fft_tiles": {"mixed": mixed["tiles"], "fp64": None if fp64 is None else fp64["tiles"]},
+        # the mixed step's LAD procedures: level-1 test-filter inverse
+        # transforms fused with their contractions (FP64 runs them separately)
+        "fused_lad_contraction": not args.no_fusion,
         # the FP64 GPU path is held to the FP64 oracle at <= 1e-11 by the parity
         # tests; at 256^3 it stands in for the oracle, which would take hours
         "error_mixed_vs_fp64": errors,

@@ -148,6 +148,19 @@ int alfa_plan_autotune(alfa_plan* plan, const alfa_state* st, int iteration, dou
                        alfa_stream stream, int chosen[3]);
 int alfa_plan_set_tiles(alfa_plan* plan, const int tiles[3]);
 int alfa_plan_get_tiles(const alfa_plan* plan, int tiles[3]);
+/* Kernel fusion of the LAD dynamic procedure (no reference counterpart; the
+ * reference forms every filtered field, DynamicSGS_LASDD_SM.py:184-233 and
+ * DynamicSGS_ScalarLASDD_SM.py:179-224, then contracts them, :327-355 and
+ * :241-312).  With fusion on (the default) a float plan with optSgs = 3
+ * inverts the level-1 test-filtered fields and contracts them in one kernel
+ * per procedure, so the filtered fields other than u_hat never reach HBM;
+ * off, the separate transform and pointwise kernels run.  The two agree to
+ * float rounding (their double accumulations are ordered differently).
+ * FP64 plans and other models always run the separate kernels.
+ *   alfa_plan_set_fusion: on = 1 (fused) or 0 (separate kernels).
+ *   alfa_plan_get_fusion: reads the setting. */
+int alfa_plan_set_fusion(alfa_plan* plan, int on);
+int alfa_plan_get_fusion(const alfa_plan* plan, int* on);
 /* Initial state values of Main.py:87-111 (RHS_prev = 0, Cs2 fields =
  * Cs2/Cs2PrRatio, psi = 0, fi = 1) written into caller buffers. */
 int alfa_state_init(alfa_plan* plan, const alfa_state* st, alfa_stream stream);

@@ -85,6 +85,8 @@ _SIGS = {
     "alfa_plan_autotune": [_P, C.POINTER(AlfaState), C.c_int, C.c_double, _P, _P],
     "alfa_plan_set_tiles": [_P, _P],
     "alfa_plan_get_tiles": [_P, _P],
+    "alfa_plan_set_fusion": [_P, C.c_int],
+    "alfa_plan_get_fusion": [_P, _P],
     "alfa_state_init": [_P, C.POINTER(AlfaState), _P],
     "alfa_step": [_P, C.POINTER(AlfaState), C.c_int, C.c_double, _P],
     "alfa_step_sfc_dev": [_P, C.POINTER(AlfaState), C.c_int, _P, _P],

@@ -72,6 +72,19 @@ class Plan:
         _lib.check(self.lib.alfa_plan_set_tiles(self._p, t), "alfa_plan_set_tiles")
         return self.tiles()
 
+    def fusion(self):
+        """Whether the LAD dynamic procedure runs its fused transform +
+        contraction kernels (float plans, optSgs = 3)."""
+        v = C.c_int()
+        _lib.check(self.lib.alfa_plan_get_fusion(self._p, C.byref(v)), "alfa_plan_get_fusion")
+        return bool(v.value)
+
+    def set_fusion(self, on):
+        """Select the fused (True) or separate (False) LAD kernels; the
+        results agree to float rounding."""
+        _lib.check(self.lib.alfa_plan_set_fusion(self._p, 1 if on else 0), "alfa_plan_set_fusion")
+        return self.fusion()
+
     def autotune(self, les=None, stream=None):
         """Time each FFT kernel at both tile choices on this device and keep
         the faster (alfa_plan_autotune); returns the kept choices.  With an

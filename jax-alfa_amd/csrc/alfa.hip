@@ -809,6 +809,18 @@ extern "C" int alfa_plan_set_tiles(alfa_plan* P, const int tiles[3]) {
   return ALFA_OK;
 }
 
+extern "C" int alfa_plan_set_fusion(alfa_plan* P, int on) {
+  if (!P) return set_error(ALFA_EINVAL, "alfa_plan_set_fusion: null plan");
+  P->fuse_contract = on ? 1 : 0;
+  return ALFA_OK;
+}
+
+extern "C" int alfa_plan_get_fusion(const alfa_plan* P, int* on) {
+  if (!P || !on) return set_error(ALFA_EINVAL, "alfa_plan_get_fusion: null argument");
+  *on = P->fuse_contract;
+  return ALFA_OK;
+}
+
 extern "C" int alfa_plan_get_tiles(const alfa_plan* P, int tiles[3]) {
   if (!P || !tiles) return set_error(ALFA_EINVAL, "alfa_plan_get_tiles: null argument");
   tiles[0] = P->tile_r2c;
@@ -1036,6 +1048,7 @@ static long long step_kind(const alfa_plan* P, int it) {
   k |= (long long)P->tile_r2c << 3;
   k |= (long long)P->tile_x << 13;
   k |= (long long)P->tile_c2r << 23;
+  k |= (long long)P->fuse_contract << 33;
   return k;
 }
 

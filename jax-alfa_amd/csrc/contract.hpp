@@ -1,0 +1,575 @@
+// Level-1 test-filter inverse transforms fused with the pointwise contraction
+// of the dynamic procedure (LAD models, optSgs = 3: one test-filter level,
+// beta given per level).
+//
+// Unfused, the y C2R of the test filter writes every filtered field to HBM
+// (21 momentum fields, DynamicSGS_LASDD_SM.py:184-233; 10 scalar fields,
+// DynamicSGS_ScalarLASDD_SM.py:179-224) and a pointwise kernel reads them all
+// back to form LM, MM, |S_hat| (:235, :327-355) or T_up, T_dn
+// (DynamicSGS_ScalarLASDD_SM.py:241-312).  Here one workgroup inverts every
+// field of one tile in turn and folds each into per-point state held in
+// registers, so only the fields that later stages read reach HBM.
+//
+// Tile: BS = 256 threads, L lines (2L z levels, two per complex line) of one
+// x row, with L = BS R / N so that the last radix pass (radix R, stride N/R)
+// is one butterfly per thread and field: thread (j, p) ends every field
+// holding rows y = j + r N/R (r < R) of the level pair (z0 + 2p, z0 + 2p + 1),
+// i.e. the same 2R points for every field.  Fields go through the passes in
+// pairs, lines 0..L-1 and L..2L-1 of one LDS buffer: the radix-8 passes then
+// have a butterfly for every thread, and each barrier serves two fields.  The
+// passes are those of k_y_c2r (band first pass, compile-time radix plan) on
+// the same values, so each field's float values are the ones the unfused
+// kernel would store.
+//
+// Momentum (LAD): the six S_hat_ij are inverted first (their squares give
+// |S_hat|), then u_hat_i (kept; stored with the outputs at the end of the
+// tile: the scalar procedure reads them), then per two components ij:
+// S_hat_ij again, (u_i u_j)_hat and (|S| S_ij)_hat, which complete L_ij and
+// M_ij; LM accumulates in double and MM (a sum of squares) in float.
+// Inverting S_hat_ij twice (27 transforms, 14 pair steps, instead of 21)
+// keeps six fields per point out of the registers.
+// Scalar: TH_hat, then per direction k: (u_k TH)_hat, (dTH/dx_k)_hat and
+// (|S| dTH/dx_k)_hat (5 pair steps), with u_hat_k and |S_hat| read from HBM.  With
+// D_k = sdh_k - TFR^2 beta |S_hat| dh_k, the reference's
+//   T_up = L^2 a - L^2 TFR^2 beta b,  T_dn = L^4 c - 2 L^4 TFR^2 beta d + L^4 TFR^4 beta^2 e
+// (a..e the sums of DynamicSGS_ScalarLASDD_SM.py:241-302 at level 1) are
+// T_up = L^2 sum_k LT_k D_k and T_dn = L^4 sum_k D_k^2, which this kernel
+// accumulates directly (T_up in double, the sum of squares in float).
+#pragma once
+#include <type_traits>
+#include "fft.hpp"
+#include "loaders.hpp"
+#include "plan.hpp"
+
+// geometry of the fused kernels at compile-time length N: L lines per field,
+// LL = 2L lines per pair
+template <int N, int BS> struct CtrGeom {
+  static constexpr int R = N > 0 ? last_radix(N) : 1;
+  static constexpr int NS = N > 0 ? N / R : 1;
+  static constexpr int L = BS / NS;
+  static constexpr int LL = 2 * L;
+  static constexpr bool OK = N > 0 && (BS % NS == 0) && R <= 5 && radix_of(N) == 8;
+};
+
+// An empty asm that the compiler must assume changes v: values derived from
+// a float held across the field loops (its double, products with it) are then
+// formed where they are used instead of being hoisted out of the loop and
+// held at twice the registers.
+template <class T> __device__ __forceinline__ void ctr_keep(T& v) { asm volatile("" : "+v"(v)); }
+
+// spectrum tile of one field, (k < kmax, line p < L) as the two levels
+// (z, z + 1); a null field loads nothing
+template <class T, int L, int BS, int IN_IT>
+__device__ __forceinline__ void ctr_load(const c2<T>* __restrict__ in, int ncy_in, int nz, int kmax,
+                                         int x, int z0, c2<T> (&X1)[IN_IT], c2<T> (&X2)[IN_IT]) {
+#pragma unroll
+  for (int q = 0; q < IN_IT; ++q) {
+    const int e = threadIdx.x + q * BS;
+    const int k = e / L, p = e % L, z = z0 + 2 * p;
+    X1[q] = mkc<T>(0, 0);
+    X2[q] = mkc<T>(0, 0);
+    if (in != nullptr && k < kmax && z < nz)
+      cld2<T>(in + ((size_t)x * ncy_in + k) * nz + z, X1[q], X2[q]);
+  }
+}
+
+// a loaded field into lines line0 .. line0+L-1 of the LL-line buffer, packed
+// as k_y_c2r packs it (imaginary parts of ky = 0 and the Nyquist column
+// dropped)
+template <class T, int N, int L, int LL, int BS, int IN_IT>
+__device__ __forceinline__ void ctr_stage(c2<T>* buf, const c2<T> (&X1)[IN_IT],
+                                          const c2<T> (&X2)[IN_IT], int kmax, int line0) {
+  constexpr int half = N / 2;
+#pragma unroll
+  for (int q = 0; q < IN_IT; ++q) {
+    const int e = threadIdx.x + q * BS;
+    const int k = e / L, p = line0 + e % L;
+    if (k < kmax) {
+      c2<T> a = X1[q], b = X2[q];
+      if (k == 0 || k == half) { a.y = T(0); b.y = T(0); }
+      buf[k * LL + p] = mkc<T>(a.x - b.y, a.y + b.x);
+      if (k > 0 && k < half) buf[(N - k) * LL + p] = mkc<T>(a.x + b.y, b.x - a.y);
+    }
+  }
+}
+
+// every pass but the last over the LL lines, then the last (radix R) from
+// LDS into registers: o[h][r][0], o[h][r][1] = levels z, z + 1 of row
+// j + r NS of field h of the pair, times sc
+template <class T, int N, int BS, bool BAND>
+__device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int kmax, T sc,
+                                            T (&o)[2][CtrGeom<N, BS>::R][2]) {
+  using G = CtrGeom<N, BS>;
+  constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL;
+  if constexpr (BAND) {
+    band_first_pass_inv<T, LL, N, BS>(buf, kmax);
+    FftCTMid<T, LL, N, 8, N / 8, BS>::run(buf, stw, +1);
+  } else {
+    FftCTMid<T, LL, N, 1, N, BS>::run(buf, stw, +1);
+  }
+  const int j = threadIdx.x / L, p = threadIdx.x % L;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    c2<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      c2<T> e = buf[(j + r * NS) * LL + h * L + p];
+      if (r > 0) {
+        c2<T> w = stw[j * r];
+        w.y = -w.y;
+        e = cmul<T>(e, w);
+      }
+      v[r] = e;
+    }
+    Dft<T, R>::run(v, +1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      o[h][r][0] = v[r].x * sc;
+      o[h][r][1] = v[r].y * sc;
+    }
+  }
+}
+
+// The pair-step driver of both kernels.  A kernel walks its tiles with
+// first() / next() and, per tile, calls step(s, use) for its NP pair steps in
+// order (from its own phase loops, so each phase keeps only its own state
+// live).  Step s stages the loaded spectra of pair s (in[2s], in[2s+1], the
+// second possibly null), issues the loads of pair s + 1 (or of the next
+// tile's pair 0), inverts both fields and hands their values to use(o).
+template <class T, int N, int BS, bool BAND, int NP>
+struct CtrDriver {
+  using G = CtrGeom<N, BS>;
+  static constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL, ZT = 2 * L;
+  static constexpr int KM = BAND ? N / 4 : N / 2 + 1;
+  static constexpr int IN_IT = (KM * L + BS - 1) / BS;
+  const c2<T>* const* in;
+  const TileGrid& tg;
+  c2<T>* buf;
+  c2<T>* stw;
+  int ncy_in, nz, kmax;
+  T sc;
+  int zt = 0, x = 0, it = 0, zt2 = 0, x2 = 0;
+  bool more = false;
+  c2<T> A1[IN_IT], A2[IN_IT], B1[IN_IT], B2[IN_IT];
+
+  __device__ __forceinline__ CtrDriver(const c2<T>* const* in_, int ncy_in_, int nz_, double scale,
+                                       const c2<T>* __restrict__ tw, const TileGrid& tg_)
+      : in(in_), tg(tg_), ncy_in(ncy_in_), nz(nz_) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    buf = reinterpret_cast<c2<T>*>(smem_raw);
+    stw = buf + N * LL;
+    kmax = imin(ncy_in, N / 2 + 1);
+    sc = (T)scale;
+    for (int i = threadIdx.x; i < N; i += BS) stw[i] = tw[i];
+  }
+  __device__ __forceinline__ void load_pair(int s, int xr, int ztr) {
+    ctr_load<T, L, BS, IN_IT>(in[2 * s], ncy_in, nz, kmax, xr, ztr * ZT, A1, A2);
+    ctr_load<T, L, BS, IN_IT>(in[2 * s + 1], ncy_in, nz, kmax, xr, ztr * ZT, B1, B2);
+  }
+  // the workgroup's first tile (false: none)
+  __device__ __forceinline__ bool first() {
+    int b;
+    if (!tg.tile(0, zt, x, b)) return false;
+    load_pair(0, x, zt);
+    return true;
+  }
+  // after a tile's last step: move to the next tile (false: none)
+  __device__ __forceinline__ bool next() {
+    if (!more) return false;
+    zt = zt2;
+    x = x2;
+    ++it;
+    return true;
+  }
+  template <class Use> __device__ __forceinline__ void step(int s, Use&& use) {
+    __syncthreads();
+    if constexpr (!BAND) {
+      for (int i = threadIdx.x; i < N * LL; i += BS) {
+        const int k = i / LL;
+        if (k >= kmax && k <= N - kmax) buf[i] = mkc<T>(0, 0);
+      }
+    }
+    ctr_stage<T, N, L, LL, BS, IN_IT>(buf, A1, A2, kmax, 0);
+    ctr_stage<T, N, L, LL, BS, IN_IT>(buf, B1, B2, kmax, L);
+    if (s + 1 < NP) {
+      load_pair(s + 1, x, zt);
+    } else {
+      int b;
+      more = tg.tile(it + 1, zt2, x2, b);
+      if (more) load_pair(0, x2, zt2);
+    }
+    __syncthreads();
+    T o[2][R][2];
+    ctr_invert2<T, N, BS, BAND>(buf, stw, kmax, sc, o);
+    use(o);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// momentum (LAD)
+// ---------------------------------------------------------------------------
+constexpr int LAD_NP = 14;
+// field of slot h of pair step s, as an index of the 21 LoadLasdd fields
+// (0..2 u_hat, 3..8 products uu vv ww uv uw vw, 9..14 S_ij, 15..20 |S| S_ij),
+// -1 for the empty slot
+__host__ __device__ constexpr int lad_field(int s, int h) {
+  return s < 3 ? 9 + 2 * s + h
+       : s == 3 ? h
+       : s == 4 ? (h == 0 ? 2 : -1)
+       : ((s - 5) % 3 == 0 ? 9 : (s - 5) % 3 == 1 ? 3 : 15) + 2 * ((s - 5) / 3) + h;
+}
+
+template <class T> struct LadArgs {
+  const c2<T>* in[2 * LAD_NP];   // x-filtered level-1 spectra, pair by pair
+  const double* beta;            // (nz)
+  T* uh[3];                      // u_hat, v_hat, w_hat
+  T* LM; T* MM; T* Shat;
+  int ncy_in, ny, nz;
+  double scale;                  // 1 / (nx ny)
+  double T1, T2;                 // 2 L^2, 2 (TFR L)^2
+};
+
+template <class T, int N, int BS, bool BAND>
+__device__ __forceinline__ void k_c2r_lad(const LadArgs<T>& a, const c2<T>* __restrict__ tw,
+                                          const TileGrid& tg) {
+  using D = CtrDriver<T, N, BS, BAND, LAD_NP>;
+  constexpr int R = D::R, NS = D::NS, L = D::L;
+  const int nz = a.nz;
+  const int j = threadIdx.x / L, p = threadIdx.x % L;
+  D d(a.in, a.ncy_in, nz, a.scale, tw, tg);
+  if (!d.first()) return;
+  do {
+    const int z = d.zt * D::ZT + 2 * p;
+    const bool zin = z < nz;
+    const double bz[2] = {zin ? a.beta[z] : 0.0, zin ? a.beta[z + 1] : 0.0};
+    // |S_hat| = sqrt(2 S_ij S_ij) from the six S_hat_ij (summed in double,
+    // kept as the float the unfused path stores)
+    double sm[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r) { sm[r][0] = 0.0; sm[r][1] = 0.0; }
+#pragma unroll 1
+    for (int s = 0; s < 3; ++s) {
+      // components 2s, 2s + 1; the off-diagonal ones (3, 4, 5) count twice
+      const double w0 = s < 2 ? 1.0 : 2.0, w1 = s < 1 ? 1.0 : 2.0;
+      d.step(s, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const double s0 = o[0][r][q], s1 = o[1][r][q];
+            sm[r][q] += w0 * (s0 * s0);
+            sm[r][q] += w1 * (s1 * s1);
+          }
+      });
+    }
+    T smf[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) smf[r][q] = (T)sqrt(2.0 * sm[r][q]);
+    // u_hat, v_hat, w_hat
+    T uh[3][R][2];
+    d.step(3, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { uh[0][r][q] = o[0][r][q]; uh[1][r][q] = o[1][r][q]; }
+    });
+    d.step(4, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) uh[2][r][q] = o[0][r][q];
+    });
+    // per two components (uu vv | ww uv | uw vw): S_hat_ij, (u_i u_j)_hat and
+    // (|S| S_ij)_hat complete L_ij and M_ij.  LM (a signed sum) accumulates
+    // in double; MM, a sum of squares, in the field type, each term added in
+    // double (at most 6 roundings of a sum of positive terms).
+    double lm[R][2];
+    T mm[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) { lm[r][q] = 0.0; mm[r][q] = T(0); }
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      T sh[2][R][2], ph[2][R][2];
+      d.step(5 + 3 * m, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < R; ++r) { sh[h][r][0] = o[h][r][0]; sh[h][r][1] = o[h][r][1]; }
+      });
+      d.step(6 + 3 * m, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < R; ++r) { ph[h][r][0] = o[h][r][0]; ph[h][r][1] = o[h][r][1]; }
+      });
+      d.step(7 + 3 * m, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          // component n = 2m + h of uu vv ww uv uw vw
+          const int n = 2 * m + h;
+          const int ia = n < 3 ? n : (n < 5 ? 0 : 1);
+          const int ib = n < 3 ? n : (n == 3 ? 1 : 2);
+          const double wgt = n < 3 ? 1.0 : 2.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+              const T ta = ia == 0 ? uh[0][r][q] : (ia == 1 ? uh[1][r][q] : uh[2][r][q]);
+              const T tb = ib == 0 ? uh[0][r][q] : (ib == 1 ? uh[1][r][q] : uh[2][r][q]);
+              T smv = smf[r][q];
+              ctr_keep(smv);
+              const double Lij = (double)ph[h][r][q] - (double)ta * (double)tb;
+              const double M = a.T1 * (double)o[h][r][q] -
+                               a.T2 * bz[q] * (double)smv * (double)sh[h][r][q];
+              lm[r][q] += wgt * (Lij * M);
+              mm[r][q] = (T)((double)mm[r][q] + wgt * (M * M));
+            }
+        }
+      });
+    }
+    if (zin) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const size_t i = ((size_t)d.x * a.ny + j + r * NS) * nz + z;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          *reinterpret_cast<c2<T>*>(a.uh[k] + i) = mkc<T>(uh[k][r][0], uh[k][r][1]);
+        *reinterpret_cast<c2<T>*>(a.LM + i) = mkc<T>((T)lm[r][0], (T)lm[r][1]);
+        *reinterpret_cast<c2<T>*>(a.MM + i) = mkc<T>(mm[r][0], mm[r][1]);
+        *reinterpret_cast<c2<T>*>(a.Shat + i) = mkc<T>(smf[r][0], smf[r][1]);
+      }
+    }
+  } while (d.next());
+}
+
+// ---------------------------------------------------------------------------
+// scalar (LAD)
+// ---------------------------------------------------------------------------
+constexpr int SCA_NP = 5;
+// field of slot h of pair step s, as an index of the 10 LoadScalar fields
+// (0 TH_hat, 1..3 (u_k TH)_hat, 4..6 dTH/dx_k hats, 7..9 |S| dTH/dx_k hats):
+// (TH, uTH_0), (dTH_0, SdTH_0), (uTH_1, dTH_1), (SdTH_1, uTH_2), (dTH_2, SdTH_2)
+__host__ __device__ constexpr int sca_field(int s, int h) {
+  return s == 0 ? (h == 0 ? 0 : 1)
+       : s == 1 ? (h == 0 ? 4 : 7)
+       : s == 2 ? (h == 0 ? 2 : 5)
+       : s == 3 ? (h == 0 ? 8 : 3)
+       : (h == 0 ? 6 : 9);
+}
+
+template <class T> struct ScaArgs {
+  const c2<T>* in[2 * SCA_NP];   // x-filtered level-1 spectra, pair by pair
+  const double* beta;            // (nz)
+  const T* uh[3];                // the momentum procedure's u_hat, v_hat, w_hat
+  const T* Sm;                   // and |S_hat|
+  T* Tup; T* Tdn;
+  int ncy_in, ny, nz;
+  double scale;                  // 1 / (nx ny)
+  double L2, L4, T2;             // L^2, L^4, TFR^2
+};
+
+template <class T, int N, int BS, bool BAND>
+__device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __restrict__ tw,
+                                          const TileGrid& tg) {
+  using D = CtrDriver<T, N, BS, BAND, SCA_NP>;
+  constexpr int R = D::R, NS = D::NS, L = D::L;
+  const int nz = a.nz;
+  const int j = threadIdx.x / L, p = threadIdx.x % L;
+  D d(a.in, a.ncy_in, nz, a.scale, tw, tg);
+  if (!d.first()) return;
+  do {
+    const int z = d.zt * D::ZT + 2 * p;
+    const bool zin = z < nz;
+    const double bz[2] = {zin ? a.beta[z] : 0.0, zin ? a.beta[z + 1] : 0.0};
+    // the values of a momentum-procedure field at the thread's points
+    auto point_load = [&](const T* f, T (&v)[R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        c2<T> t = mkc<T>(0, 0);
+        if (zin) t = *reinterpret_cast<const c2<T>*>(f + ((size_t)d.x * a.ny + j + r * NS) * nz + z);
+        v[r][0] = t.x;
+        v[r][1] = t.y;
+      }
+    };
+    T smh[R][2], um[R][2], th[R][2], uth[R][2], dh[R][2];
+    point_load(a.Sm, smh);
+    point_load(a.uh[0], um);
+    // T_up (a signed sum) accumulates in double, T_dn (a sum of squares) in
+    // the field type, each term added in double
+    double up[R][2];
+    T dn[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) { up[r][q] = 0.0; dn[r][q] = T(0); }
+    // direction k: T_up += LT D, T_dn += D^2 from (u_k TH)_hat = ut, the
+    // momentum u_hat_k = u, dTH/dx_k hat = dd and |S| dTH/dx_k hat = sd
+    auto fold = [&](const T (&ut)[R][2], const T (&u)[R][2], const T (&dd)[R][2],
+                    const T (&sd)[R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          T thv = th[r][q], smv = smh[r][q];
+          ctr_keep(thv);
+          ctr_keep(smv);
+          const double LT = (double)ut[r][q] - (double)u[r][q] * (double)thv;
+          const double Dk = (double)sd[r][q] - a.T2 * bz[q] * (double)smv * (double)dd[r][q];
+          up[r][q] += LT * Dk;
+          dn[r][q] = (T)((double)dn[r][q] + Dk * Dk);
+        }
+    };
+    d.step(0, [&](const T (&o)[2][R][2]) {     // TH_hat, (u TH)_hat
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { th[r][q] = o[0][r][q]; uth[r][q] = o[1][r][q]; }
+    });
+    d.step(1, [&](const T (&o)[2][R][2]) {     // dTH/dx, |S| dTH/dx: direction x
+      fold(uth, um, o[0], o[1]);
+    });
+    point_load(a.uh[1], um);
+    d.step(2, [&](const T (&o)[2][R][2]) {     // (v TH)_hat, dTH/dy
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { uth[r][q] = o[0][r][q]; dh[r][q] = o[1][r][q]; }
+    });
+    d.step(3, [&](const T (&o)[2][R][2]) {     // |S| dTH/dy: direction y; (w TH)_hat
+      fold(uth, um, dh, o[0]);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) uth[r][q] = o[1][r][q];
+    });
+    point_load(a.uh[2], um);
+    d.step(4, [&](const T (&o)[2][R][2]) {     // dTH/dz, |S| dTH/dz: direction z
+      fold(uth, um, o[0], o[1]);
+    });
+    if (zin) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const size_t i = ((size_t)d.x * a.ny + j + r * NS) * nz + z;
+        *reinterpret_cast<c2<T>*>(a.Tup + i) =
+            mkc<T>((T)(a.L2 * up[r][0]), (T)(a.L2 * up[r][1]));
+        *reinterpret_cast<c2<T>*>(a.Tdn + i) =
+            mkc<T>((T)(a.L4 * (double)dn[r][0]), (T)(a.L4 * (double)dn[r][1]));
+      }
+    }
+  } while (d.next());
+}
+
+// ---------------------------------------------------------------------------
+// launchers: false when the fused form does not apply (double fields, a
+// length without a compile-time plan or with a last radix above 5, nz not a
+// multiple of 4, unaligned buffers); the caller then runs the unfused C2R
+// and pointwise kernel.
+// ---------------------------------------------------------------------------
+constexpr int CTR_BS = 256;
+
+template <class T, class Args, class K>
+bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_fields,
+                  int ref_fields, int ref_pw_fields, int rows, const Fft1D& fyy, const void* tw,
+                  const char* name, K kern_sel) {
+  if constexpr (sizeof(T) != 4) {
+    return false;
+  } else {
+    if (a.nz % 4 != 0) return false;
+    bool ok = false;
+    ct_length(fyy.n, [&](auto nct, auto) {
+      constexpr int NCT = decltype(nct)::value;
+      if constexpr (CtrGeom<NCT, CTR_BS>::OK) ok = true;
+    });
+    if (!ok) return false;
+    if (!live(P)) return true;
+    const double F = (double)rows * fyy.n * a.nz;
+    ProfScope ps(P, s, name,
+                 (double)nin * rows * a.ncy_in * a.nz * sizeof(c2<T>) + (double)nout_fields * F * sizeof(T),
+                 (double)ref_fields * rows * a.nz *
+                         ((double)(fyy.n / 2 + 1) * sizeof(c2<T>) + (double)fyy.n * sizeof(T)) +
+                     (double)ref_pw_fields * F * sizeof(T));
+    ct_length(fyy.n, [&](auto nct, auto) {
+      constexpr int NCT = decltype(nct)::value;
+      if constexpr (CtrGeom<NCT, CTR_BS>::OK) {
+        constexpr int L = CtrGeom<NCT, CTR_BS>::L, LL = CtrGeom<NCT, CTR_BS>::LL;
+        TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, 1, 0};
+        const unsigned grid = tile_grid((size_t)tg.ntz * rows);
+        const size_t lds = (size_t)NCT * LL * sizeof(c2<T>) + (size_t)NCT * sizeof(c2<T>);
+        const bool band = ALFA_BAND && a.ncy_in <= NCT / 4;
+        if (band) kern_sel(std::integral_constant<int, NCT>{}, std::true_type{}, grid, lds, tg);
+        else kern_sel(std::integral_constant<int, NCT>{}, std::false_type{}, grid, lds, tg);
+      }
+    });
+    launch_ok(P);
+    return true;
+  }
+}
+
+// LM, MM, |S_hat| and u_hat from the 21 x-filtered level-1 spectra sp[]
+// (LoadLasdd order); false if the fused form does not apply
+template <class T>
+bool c2r_lad(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const double* beta,
+             T* const* uh, T* LM, T* MM, T* Shat, int rows, int ny, int nz, const Fft1D& fyy,
+             const void* tw, double L, double TFR) {
+  LadArgs<T> a{};
+  for (int s = 0; s < LAD_NP; ++s)
+    for (int h = 0; h < 2; ++h) a.in[2 * s + h] = lad_field(s, h) < 0 ? nullptr : sp[lad_field(s, h)];
+  a.beta = beta;
+  for (int k = 0; k < 3; ++k) a.uh[k] = uh[k];
+  a.LM = LM; a.MM = MM; a.Shat = Shat;
+  a.ncy_in = ncy_in; a.ny = ny; a.nz = nz;
+  a.scale = 1.0 / ((double)rows * ny);
+  a.T1 = 2.0 * L * L;
+  a.T2 = 2.0 * (TFR * L) * (TFR * L);
+  for (void* q : {(void*)uh[0], (void*)uh[1], (void*)uh[2], (void*)LM, (void*)MM, (void*)Shat})
+    if (!al16(q)) return false;
+  for (int b = 0; b < 21; ++b)
+    if (!al16(sp[b])) return false;
+  // reference data flow: 21 irfft2 outputs, then the contraction's 21 reads
+  // and 3 writes
+  return c2r_contract<T>(P, s, a, 27, 6, 21, 24, rows, fyy, tw, "k_y_c2r<LadContract>",
+                         [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
+                           constexpr int NCT = decltype(nct)::value;
+                           constexpr bool B = decltype(band)::value;
+                           launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
+                                            KBODY(k_c2r_lad<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                         });
+}
+
+// T_up, T_dn from the 10 x-filtered level-1 scalar spectra sp[] (LoadScalar
+// order), the momentum hats uh[] and |S_hat|; false if the fused form does
+// not apply
+template <class T>
+bool c2r_sca(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const double* beta,
+             const T* const* uh, const T* Sm, T* Tup, T* Tdn, int rows, int ny, int nz,
+             const Fft1D& fyy, const void* tw, double L, double TFR) {
+  ScaArgs<T> a{};
+  for (int s = 0; s < SCA_NP; ++s)
+    for (int h = 0; h < 2; ++h) a.in[2 * s + h] = sp[sca_field(s, h)];
+  a.beta = beta;
+  for (int k = 0; k < 3; ++k) a.uh[k] = uh[k];
+  a.Sm = Sm; a.Tup = Tup; a.Tdn = Tdn;
+  a.ncy_in = ncy_in; a.ny = ny; a.nz = nz;
+  a.scale = 1.0 / ((double)rows * ny);
+  a.L2 = L * L;
+  a.L4 = a.L2 * a.L2;
+  a.T2 = TFR * TFR;
+  for (const void* q : {(const void*)uh[0], (const void*)uh[1], (const void*)uh[2],
+                        (const void*)Sm, (const void*)Tup, (const void*)Tdn})
+    if (!al16(q)) return false;
+  for (int b = 0; b < 10; ++b)
+    if (!al16(sp[b])) return false;
+  // reference data flow: 10 irfft2 outputs, then the contraction's 14 reads
+  // and 2 writes
+  return c2r_contract<T>(P, s, a, 10, 6, 10, 16, rows, fyy, tw, "k_y_c2r<ScalarContract>",
+                         [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
+                           constexpr int NCT = decltype(nct)::value;
+                           constexpr bool B = decltype(band)::value;
+                           launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
+                                            KBODY(k_c2r_sca<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                         });
+}

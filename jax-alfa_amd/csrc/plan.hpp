@@ -89,6 +89,10 @@ struct alfa_plan {
   // 512; the lines per tile scale with them): the build's defaults, or the
   // choices of alfa_plan_autotune / alfa_plan_set_tiles
   int tile_x = 256, tile_c2r = 512, tile_r2c = 256;
+  // LAD models in float: the level-1 test-filter inverse transforms fused
+  // with the LM/MM and T_up/T_dn contractions (1, the default) or the
+  // separate C2R and pointwise kernels (0); alfa_plan_set_fusion
+  int fuse_contract = 1;
   // set while a member group records this plan's step: launches are
   // appended here instead of being enqueued
   std::vector<LaunchRec>* rec = nullptr;

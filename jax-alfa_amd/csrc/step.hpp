@@ -6,6 +6,7 @@
 #include "plan.hpp"
 #include "pressure.hpp"
 #include "surface.hpp"
+#include "contract.hpp"
 
 namespace alfa {
 
@@ -111,17 +112,17 @@ void dealias2(Ctx<T>& c, int nb, const T* const* in, T* const* out) {
 }
 
 // Test filters of nb virtual fields at one or two levels
-// (Filtering_Level1/2 of FFT(field), Filtering.py:97-165).
+// (Filtering_Level1/2 of FFT(field), Filtering.py:97-165), forward half: the
+// y R2C and the x pass, leaving each level's filtered spectrum (inverse in x,
+// pruned in y) in s1[b] / s2[b].
 template <class T, class Loader>
-void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level1, bool level2,
-                 const std::string& tag, T* const* out1, T* const* out2) {
+void test_filter_fwd(Ctx<T>& c, const Loader& ld, int nb, bool level1, bool level2,
+                     const std::string& tag, c2<T>** s1, c2<T>** s2) {
   alfa_plan* P = c.P;
   const auto& d = P->d;
   // the level-2 cut-off lies inside the level-1 one: the forward transform
   // is pruned to the widest level asked for
   const int ncy = level1 ? d.cut1_y : d.cut2_y;
-  c2<T>* s1[MAXB];
-  c2<T>* s2[MAXB];
   for (int b = 0; b < nb; ++b) {
     s1[b] = c.S(tag + (level1 ? "1_" : "2_") + std::to_string(b));
     if (level1 && level2) s2[b] = c.S(tag + "2_" + std::to_string(b));
@@ -139,6 +140,14 @@ void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level1, bool level2,
   }
   xa.nops = nops; xa.ncy_in = ncy; xa.nz = c.nz; xa.ny_full = c.ny;
   x_pass<T>(P, c.s, xa, nb, P->fx, P->fx, P->tw_x, P->tw_x);
+}
+
+// inverse half: the y C2R of each level's spectra into out1[b] / out2[b]
+template <class T>
+void test_filter_inv(Ctx<T>& c, int nb, bool level1, bool level2, c2<T>* const* s1,
+                     c2<T>* const* s2, T* const* out1, T* const* out2) {
+  alfa_plan* P = c.P;
+  const auto& d = P->d;
   const double sc = 1.0 / ((double)c.nx * c.ny);
   for (int lev = 1; lev <= 2; ++lev) {
     if ((lev == 1 && !level1) || (lev == 2 && !level2)) continue;
@@ -150,6 +159,15 @@ void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level1, bool level2,
     st.ny = c.ny; st.nz = c.nz;
     y_c2r<T>(P, c.s, st, ya, nb, c.nx, P->fy, P->tw_y);
   }
+}
+
+template <class T, class Loader>
+void test_filter(Ctx<T>& c, const Loader& ld, int nb, bool level1, bool level2,
+                 const std::string& tag, T* const* out1, T* const* out2) {
+  c2<T>* s1[MAXB];
+  c2<T>* s2[MAXB];
+  test_filter_fwd<T>(c, ld, nb, level1, level2, tag, s1, s2);
+  test_filter_inv<T>(c, nb, level1, level2, s1, s2, out1, out2);
 }
 
 // Pointers of the carried state of one run.
@@ -335,38 +353,50 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     f1[b] = c.W("lf1_" + std::to_string(b));
     f2[b] = lasdd ? c.W("lf2_" + std::to_string(b)) : nullptr;
   }
-  test_filter<T>(c, ld, 21, true, lasdd, "ls", f1, f2);
+  c2<T>* sp1[MAXB];
+  c2<T>* sp2[MAXB];
+  test_filter_fwd<T>(c, ld, 21, true, lasdd, "ls", sp1, sp2);
+  T* LM = c.W("LM");
+  T* MM = c.W("MM");
+  T* Shat = c.W("Shat");
+  double* beta = c.D("beta1", c.nz);
+  if (!lasdd && live(P)) {
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
+    launch_ok(P);
+  }
+  // LAD (beta = 1, one level): the level-1 inverse transforms and the LM/MM
+  // contraction in one kernel, where it applies (u_hat and |S_hat| are still
+  // written: the scalar procedure reads them)
+  const bool fused = !lasdd && P->fuse_contract &&
+                     c2r_lad<T>(P, c.s, sp1, d.cut1_y, beta, f1, LM, MM, Shat, c.nx, c.ny, c.nz,
+                                P->fy, P->tw_y, d.L, d.TFR);
   MomLevel<T> l1{}, l2{};
   for (int k = 0; k < 3; ++k) { l1.uh[k] = f1[k]; l2.uh[k] = f2[k]; }
   for (int n = 0; n < 6; ++n) {
     l1.ph[n] = f1[3 + n]; l1.Sh[n] = f1[9 + n]; l1.SSh[n] = f1[15 + n];
     l2.ph[n] = f2[3 + n]; l2.Sh[n] = f2[9 + n]; l2.SSh[n] = f2[15 + n];
   }
-  double* beta = c.D("beta1", c.nz);
-  if (lasdd) {
-    double* sums = c.D("lsums", 10 * c.nz);
-    double* coef = c.D("lcoef", 6 * c.nz);
-    FLasddSums<T> f{l1, l2, c.nz, d.L, d.TFR};
-    plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasdd");
-    if (live(P)) {
-      launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 0, d.TFR, coef);
-      launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
-      launch_ok(P);
+  if (!fused) {
+    test_filter_inv<T>(c, 21, true, lasdd, sp1, sp2, f1, f2);
+    if (lasdd) {
+      double* sums = c.D("lsums", 10 * c.nz);
+      double* coef = c.D("lcoef", 6 * c.nz);
+      FLasddSums<T> f{l1, l2, c.nz, d.L, d.TFR};
+      plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "lasdd");
+      if (live(P)) {
+        launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 0, d.TFR, coef);
+        launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
+        launch_ok(P);
+      }
     }
-  } else if (live(P)) {
-    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
-    launch_ok(P);
+    if (vec_ok<T>(c.nz, {l1.uh[0], l1.uh[1], l1.uh[2], l1.ph[0], l1.ph[1], l1.ph[2], l1.ph[3],
+                         l1.ph[4], l1.ph[5], l1.Sh[0], l1.Sh[1], l1.Sh[2], l1.Sh[3], l1.Sh[4],
+                         l1.Sh[5], l1.SSh[0], l1.SSh[1], l1.SSh[2], l1.SSh[3], l1.SSh[4],
+                         l1.SSh[5], LM, MM, Shat}))
+      PW_LAUNCH_W(P, c.s, c.F, 2, k_lasdd_lm_v<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
+    else
+      PW_LAUNCH(P, c.s, c.F, k_lasdd_lm<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
   }
-  T* LM = c.W("LM");
-  T* MM = c.W("MM");
-  T* Shat = c.W("Shat");
-  if (vec_ok<T>(c.nz, {l1.uh[0], l1.uh[1], l1.uh[2], l1.ph[0], l1.ph[1], l1.ph[2], l1.ph[3],
-                       l1.ph[4], l1.ph[5], l1.Sh[0], l1.Sh[1], l1.Sh[2], l1.Sh[3], l1.Sh[4],
-                       l1.Sh[5], l1.SSh[0], l1.SSh[1], l1.SSh[2], l1.SSh[3], l1.SSh[4],
-                       l1.SSh[5], LM, MM, Shat}))
-    PW_LAUNCH_W(P, c.s, c.F, 2, k_lasdd_lm_v<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
-  else
-    PW_LAUNCH(P, c.s, c.F, k_lasdd_lm<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
   if (lasdd) {
     T* Shatd = c.W("Shatd");
     PW_LAUNCH(P, c.s, c.F, k_smag_level<T>, c.dims(), l2, Shatd);
@@ -420,7 +450,9 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
     f1[b] = c.W("sf1_" + std::to_string(b));
     f2[b] = lasdd ? c.W("sf2_" + std::to_string(b)) : nullptr;
   }
-  test_filter<T>(c, ld, 10, true, lasdd, "ss", f1, f2);
+  c2<T>* sp1[MAXB];
+  c2<T>* sp2[MAXB];
+  test_filter_fwd<T>(c, ld, 10, true, lasdd, "ss", sp1, sp2);
   ScaLevel<T> l1{}, l2{};
   l1.th = f1[0]; l2.th = f2[0];
   for (int k = 0; k < 3; ++k) {
@@ -432,28 +464,37 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
   l1.Sm = mh.Shat;
   l2.Sm = mh.Shatd;
   double* beta = c.D("beta2", c.nz);
-  if (lasdd) {
-    double* sums = c.D("ssums", 10 * c.nz);
-    double* coef = c.D("scoef", 6 * c.nz);
-    FScalarSums<T> f{l1, l2, c.nz, d.L, d.TFR};
-    plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasdd");
-    if (live(P)) {
-      launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 1, d.TFR, coef);
-      launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
-      launch_ok(P);
-    }
-  } else if (live(P)) {
+  T* Tup = c.W("Tup");
+  T* Tdn = c.W("Tdn");
+  if (!lasdd && live(P)) {
     launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
-  T* Tup = c.W("Tup");
-  T* Tdn = c.W("Tdn");
-  if (vec_ok<T>(c.nz, {l1.th, l1.Sm, l1.uth[0], l1.uth[1], l1.uth[2], l1.dh[0], l1.dh[1],
-                       l1.dh[2], l1.sdh[0], l1.sdh[1], l1.sdh[2], l1.uh[0], l1.uh[1], l1.uh[2],
-                       Tup, Tdn}))
-    PW_LAUNCH_W(P, c.s, c.F, 2, k_scalar_updn_v<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
-  else
-    PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
+  // LAD (beta = 1, one level): the level-1 inverse transforms and the
+  // T_up/T_dn contraction in one kernel, where it applies
+  const bool fused = !lasdd && P->fuse_contract &&
+                     c2r_sca<T>(P, c.s, sp1, d.cut1_y, beta, l1.uh, l1.Sm, Tup, Tdn, c.nx, c.ny,
+                                c.nz, P->fy, P->tw_y, d.L, d.TFR);
+  if (!fused) {
+    test_filter_inv<T>(c, 10, true, lasdd, sp1, sp2, f1, f2);
+    if (lasdd) {
+      double* sums = c.D("ssums", 10 * c.nz);
+      double* coef = c.D("scoef", 6 * c.nz);
+      FScalarSums<T> f{l1, l2, c.nz, d.L, d.TFR};
+      plane_sums<10>(P, c.s, c.ncol, c.nz, f, sums, 1.0 / c.ncol, "slasdd");
+      if (live(P)) {
+        launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 1, d.TFR, coef);
+        launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
+        launch_ok(P);
+      }
+    }
+    if (vec_ok<T>(c.nz, {l1.th, l1.Sm, l1.uth[0], l1.uth[1], l1.uth[2], l1.dh[0], l1.dh[1],
+                         l1.dh[2], l1.sdh[0], l1.sdh[1], l1.sdh[2], l1.uh[0], l1.uh[1], l1.uh[2],
+                         Tup, Tdn}))
+      PW_LAUNCH_W(P, c.s, c.F, 2, k_scalar_updn_v<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
+    else
+      PW_LAUNCH(P, c.s, c.F, k_scalar_updn<T>, c.dims(), l1, beta, d.L, d.TFR, Tup, Tdn);
+  }
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CsPr, s2, "sbox");
   if (live(P)) {
