@@ -267,19 +267,22 @@ __device__ __forceinline__ void k_c2r_lad(const LadArgs<T>& a, const c2<T>* __re
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int q = 0; q < 2; ++q) smf[r][q] = (T)sqrt(2.0 * sm[r][q]);
-    // u_hat, v_hat, w_hat
-    T uh[3][R][2];
+    // u_hat, v_hat, w_hat, held in LDS after the pair buffer and the
+    // twiddles ([3][R][2][BS]: each thread's own values; registers held
+    // across the whole tile would cost a wave per SIMD)
+    T* uhs = reinterpret_cast<T*>(d.stw + N);
+    auto UH = [&](int k, int r, int q) -> T& { return uhs[((k * R + r) * 2 + q) * BS + threadIdx.x]; };
     d.step(3, [&](const T (&o)[2][R][2]) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) { uh[0][r][q] = o[0][r][q]; uh[1][r][q] = o[1][r][q]; }
+        for (int q = 0; q < 2; ++q) { UH(0, r, q) = o[0][r][q]; UH(1, r, q) = o[1][r][q]; }
     });
     d.step(4, [&](const T (&o)[2][R][2]) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) uh[2][r][q] = o[0][r][q];
+        for (int q = 0; q < 2; ++q) UH(2, r, q) = o[0][r][q];
     });
     // per two components (uu vv | ww uv | uw vw): S_hat_ij, (u_i u_j)_hat and
     // (|S| S_ij)_hat complete L_ij and M_ij.  LM (a signed sum) accumulates
@@ -318,8 +321,7 @@ __device__ __forceinline__ void k_c2r_lad(const LadArgs<T>& a, const c2<T>* __re
           for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-              const T ta = ia == 0 ? uh[0][r][q] : (ia == 1 ? uh[1][r][q] : uh[2][r][q]);
-              const T tb = ib == 0 ? uh[0][r][q] : (ib == 1 ? uh[1][r][q] : uh[2][r][q]);
+              const T ta = UH(ia, r, q), tb = UH(ib, r, q);
               T smv = smf[r][q];
               ctr_keep(smv);
               const double Lij = (double)ph[h][r][q] - (double)ta * (double)tb;
@@ -337,7 +339,7 @@ __device__ __forceinline__ void k_c2r_lad(const LadArgs<T>& a, const c2<T>* __re
         const size_t i = ((size_t)d.x * a.ny + j + r * NS) * nz + z;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          *reinterpret_cast<c2<T>*>(a.uh[k] + i) = mkc<T>(uh[k][r][0], uh[k][r][1]);
+          *reinterpret_cast<c2<T>*>(a.uh[k] + i) = mkc<T>(UH(k, r, 0), UH(k, r, 1));
         *reinterpret_cast<c2<T>*>(a.LM + i) = mkc<T>((T)lm[r][0], (T)lm[r][1]);
         *reinterpret_cast<c2<T>*>(a.MM + i) = mkc<T>(mm[r][0], mm[r][1]);
         *reinterpret_cast<c2<T>*>(a.Shat + i) = mkc<T>(smf[r][0], smf[r][1]);
@@ -395,8 +397,20 @@ __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __re
         v[r][1] = t.y;
       }
     };
-    T smh[R][2], um[R][2], th[R][2], uth[R][2], dh[R][2];
-    point_load(a.Sm, smh);
+    // TH_hat and |S_hat| at the thread's points, held in LDS after the pair
+    // buffer and the twiddles ([2][R][2][BS])
+    T* sts = reinterpret_cast<T*>(d.stw + N);
+    auto TH_ = [&](int r, int q) -> T& { return sts[((0 * R + r) * 2 + q) * BS + threadIdx.x]; };
+    auto SM_ = [&](int r, int q) -> T& { return sts[((1 * R + r) * 2 + q) * BS + threadIdx.x]; };
+    T um[R][2], uth[R][2], dh[R][2];
+    {
+      T smh[R][2];
+      point_load(a.Sm, smh);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) SM_(r, q) = smh[r][q];
+    }
     point_load(a.uh[0], um);
     // T_up (a signed sum) accumulates in double, T_dn (a sum of squares) in
     // the field type, each term added in double
@@ -414,11 +428,18 @@ __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __re
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          T thv = th[r][q], smv = smh[r][q];
+          // each operand pinned here, so the conversions to double are
+          // made point by point instead of all at once
+          T thv = TH_(r, q), smv = SM_(r, q), utv = ut[r][q], uv = u[r][q], sdv = sd[r][q],
+            ddv = dd[r][q];
           ctr_keep(thv);
           ctr_keep(smv);
-          const double LT = (double)ut[r][q] - (double)u[r][q] * (double)thv;
-          const double Dk = (double)sd[r][q] - a.T2 * bz[q] * (double)smv * (double)dd[r][q];
+          ctr_keep(utv);
+          ctr_keep(uv);
+          ctr_keep(sdv);
+          ctr_keep(ddv);
+          const double LT = (double)utv - (double)uv * (double)thv;
+          const double Dk = (double)sdv - a.T2 * bz[q] * (double)smv * (double)ddv;
           up[r][q] += LT * Dk;
           dn[r][q] = (T)((double)dn[r][q] + Dk * Dk);
         }
@@ -427,7 +448,7 @@ __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __re
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) { th[r][q] = o[0][r][q]; uth[r][q] = o[1][r][q]; }
+        for (int q = 0; q < 2; ++q) { TH_(r, q) = o[0][r][q]; uth[r][q] = o[1][r][q]; }
     });
     d.step(1, [&](const T (&o)[2][R][2]) {     // dTH/dx, |S| dTH/dx: direction x
       fold(uth, um, o[0], o[1]);
@@ -471,10 +492,11 @@ __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __re
 // ---------------------------------------------------------------------------
 constexpr int CTR_BS = 256;
 
+// extra_lds: fields of per-thread point state the kernel holds in LDS
 template <class T, class Args, class K>
 bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_fields,
                   int ref_fields, int ref_pw_fields, int rows, const Fft1D& fyy, const void* tw,
-                  const char* name, K kern_sel) {
+                  const char* name, int extra_lds, K kern_sel) {
   if constexpr (sizeof(T) != 4) {
     return false;
   } else {
@@ -498,7 +520,8 @@ bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_
         constexpr int L = CtrGeom<NCT, CTR_BS>::L, LL = CtrGeom<NCT, CTR_BS>::LL;
         TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, 1, 0};
         const unsigned grid = tile_grid((size_t)tg.ntz * rows);
-        const size_t lds = (size_t)NCT * LL * sizeof(c2<T>) + (size_t)NCT * sizeof(c2<T>);
+        const size_t lds = (size_t)NCT * LL * sizeof(c2<T>) + (size_t)NCT * sizeof(c2<T>) +
+                           (size_t)extra_lds * CtrGeom<NCT, CTR_BS>::R * 2 * CTR_BS * sizeof(T);
         const bool band = ALFA_BAND && a.ncy_in <= NCT / 4;
         if (band) kern_sel(std::integral_constant<int, NCT>{}, std::true_type{}, grid, lds, tg);
         else kern_sel(std::integral_constant<int, NCT>{}, std::false_type{}, grid, lds, tg);
@@ -531,7 +554,7 @@ bool c2r_lad(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const do
     if (!al16(sp[b])) return false;
   // reference data flow: 21 irfft2 outputs, then the contraction's 21 reads
   // and 3 writes
-  return c2r_contract<T>(P, s, a, 27, 6, 21, 24, rows, fyy, tw, "k_y_c2r<LadContract>",
+  return c2r_contract<T>(P, s, a, 27, 6, 21, 24, rows, fyy, tw, "k_y_c2r<LadContract>", 3,
                          [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
                            constexpr int NCT = decltype(nct)::value;
                            constexpr bool B = decltype(band)::value;
@@ -565,7 +588,7 @@ bool c2r_sca(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const do
     if (!al16(sp[b])) return false;
   // reference data flow: 10 irfft2 outputs, then the contraction's 14 reads
   // and 2 writes
-  return c2r_contract<T>(P, s, a, 10, 6, 10, 16, rows, fyy, tw, "k_y_c2r<ScalarContract>",
+  return c2r_contract<T>(P, s, a, 10, 6, 10, 16, rows, fyy, tw, "k_y_c2r<ScalarContract>", 2,
                          [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
                            constexpr int NCT = decltype(nct)::value;
                            constexpr bool B = decltype(band)::value;
