@@ -15,8 +15,10 @@
 // is one butterfly per thread and field: thread (j, p) ends every field
 // holding rows y = j + r N/R (r < R) of the level pair (z0 + 2p, z0 + 2p + 1),
 // i.e. the same 2R points for every field.  Fields go through the passes in
-// pairs, lines 0..L-1 and L..2L-1 of one LDS buffer: the radix-8 passes then
-// have a butterfly for every thread, and each barrier serves two fields.  The
+// pairs, interleaved in one LDS buffer of 2L lines (line p of field h at
+// 2p + h): the radix-8 passes then have a butterfly for every thread, each
+// barrier serves two fields, and the pair's staging writes and last-pass
+// reads are single 16-byte accesses on contiguous rows.  The
 // passes are those of k_y_c2r (band first pass, compile-time radix plan) on
 // the same values, so each field's float values are the ones the unfused
 // kernel would store.
@@ -73,29 +75,37 @@ __device__ __forceinline__ void ctr_load(const c2<T>* __restrict__ in, int ncy_i
   }
 }
 
-// a loaded field into lines line0 .. line0+L-1 of the LL-line buffer, packed
-// as k_y_c2r packs it (imaginary parts of ky = 0 and the Nyquist column
-// dropped)
+// the loaded pair into the LL-line buffer, packed as k_y_c2r packs a field
+// (imaginary parts of ky = 0 and the Nyquist column dropped).  The two fields
+// are interleaved, field h of line p at line 2p + h, so each (k, p) of the
+// pair is one 16-byte LDS store: the lanes of a wave write contiguous rows.
 template <class T, int N, int L, int LL, int BS, int IN_IT>
-__device__ __forceinline__ void ctr_stage(c2<T>* buf, const c2<T> (&X1)[IN_IT],
-                                          const c2<T> (&X2)[IN_IT], int kmax, int line0) {
+__device__ __forceinline__ void ctr_stage2(c2<T>* buf, const c2<T> (&A1)[IN_IT],
+                                           const c2<T> (&A2)[IN_IT], const c2<T> (&B1)[IN_IT],
+                                           const c2<T> (&B2)[IN_IT], int kmax) {
   constexpr int half = N / 2;
+  auto pk = [](c2<T> a, c2<T> b, int k, c2<T>& lo, c2<T>& hi) {
+    if (k == 0 || k == half) { a.y = T(0); b.y = T(0); }
+    lo = mkc<T>(a.x - b.y, a.y + b.x);
+    hi = mkc<T>(a.x + b.y, b.x - a.y);
+  };
 #pragma unroll
   for (int q = 0; q < IN_IT; ++q) {
     const int e = threadIdx.x + q * BS;
-    const int k = e / L, p = line0 + e % L;
+    const int k = e / L, p = e % L;
     if (k < kmax) {
-      c2<T> a = X1[q], b = X2[q];
-      if (k == 0 || k == half) { a.y = T(0); b.y = T(0); }
-      buf[k * LL + p] = mkc<T>(a.x - b.y, a.y + b.x);
-      if (k > 0 && k < half) buf[(N - k) * LL + p] = mkc<T>(a.x + b.y, b.x - a.y);
+      c2<T> al, ah, bl, bh;
+      pk(A1[q], A2[q], k, al, ah);
+      pk(B1[q], B2[q], k, bl, bh);
+      cst2<T>(buf + k * LL + 2 * p, al, bl);
+      if (k > 0 && k < half) cst2<T>(buf + (N - k) * LL + 2 * p, ah, bh);
     }
   }
 }
 
 // every pass but the last over the LL lines, then the last (radix R) from
 // LDS into registers: o[h][r][0], o[h][r][1] = levels z, z + 1 of row
-// j + r NS of field h of the pair, times sc
+// j + r NS of field h of the pair (line 2p + h), times sc
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int kmax, T sc,
                                             T (&o)[2][CtrGeom<N, BS>::R][2]) {
@@ -108,24 +118,28 @@ __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int km
     FftCTMid<T, LL, N, 1, N, BS>::run(buf, stw, +1);
   }
   const int j = threadIdx.x / L, p = threadIdx.x % L;
+  // line p of both fields (lines 2p, 2p + 1): one 16-byte read per r
+  c2<T> v[2][R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    c2<T> e0, e1;
+    cld2<T>(buf + (j + r * NS) * LL + 2 * p, e0, e1);
+    if (r > 0) {
+      c2<T> w = stw[j * r];
+      w.y = -w.y;
+      e0 = cmul<T>(e0, w);
+      e1 = cmul<T>(e1, w);
+    }
+    v[0][r] = e0;
+    v[1][r] = e1;
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    c2<T> v[R];
+    Dft<T, R>::run(v[h], +1);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      c2<T> e = buf[(j + r * NS) * LL + h * L + p];
-      if (r > 0) {
-        c2<T> w = stw[j * r];
-        w.y = -w.y;
-        e = cmul<T>(e, w);
-      }
-      v[r] = e;
-    }
-    Dft<T, R>::run(v, +1);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      o[h][r][0] = v[r].x * sc;
-      o[h][r][1] = v[r].y * sc;
+      o[h][r][0] = v[h][r].x * sc;
+      o[h][r][1] = v[h][r].y * sc;
     }
   }
 }
@@ -189,8 +203,7 @@ struct CtrDriver {
         if (k >= kmax && k <= N - kmax) buf[i] = mkc<T>(0, 0);
       }
     }
-    ctr_stage<T, N, L, LL, BS, IN_IT>(buf, A1, A2, kmax, 0);
-    ctr_stage<T, N, L, LL, BS, IN_IT>(buf, B1, B2, kmax, L);
+    ctr_stage2<T, N, L, LL, BS, IN_IT>(buf, A1, A2, B1, B2, kmax);
     if (s + 1 < NP) {
       load_pair(s + 1, x, zt);
     } else {
