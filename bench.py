@@ -41,6 +41,14 @@ A100_REF = {"Mpoint_steps_per_s": 202.1, "ms_per_step": 83.0,
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
+# --case: (description, domain and model, optSgs, FGR) of the timed step
+CASE_TEXT = {
+    "stable": ("GABLS1-like stable boundary layer", "400^3 m, LAD-SM, FGR=2, dt=0.05 s", 3, 2),
+    "convective": ("convective boundary layer", "5120^2 x 2048 m, LASDD-SM, FGR=2, dt=0.2 s", 1, 2),
+    "neutral": ("neutral boundary layer", "2000^2 x 1000 m, LAD-SM, FGR=1 (dealiased), dt=0.5 s",
+                3, 1),
+}
+
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
@@ -108,10 +116,11 @@ def error_report(mx, fp):
     return r
 
 
-def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None, tiles=None, fusion=True):
+def run_gpu(prec, n, steps, warmup, find_dominant=True, dominant=None, tiles=None, fusion=True,
+            case="stable"):
     import torch
     from alfa_amd import cases, core
-    cfg, u, v, w, TH = cases.initial_fields("stable", nx=n, ny=n, nz=n)
+    cfg, u, v, w, TH = cases.initial_fields(case, nx=n, ny=n, nz=n)
     ser = cases.surface_series(cfg, warmup + steps + 8)
     P = core.Plan(cfg, prec)
     P.set_fusion(fusion)
@@ -240,6 +249,9 @@ def main():
                     help="threads per workgroup of the y R2C, x and y C2R FFT kernels (256 or "
                          "512 each) instead of the plan's autotuner, e.g. under a profiler, "
                          "whose trace would also hold the autotuner's trial steps")
+    ap.add_argument("--case", default="stable", choices=("stable", "convective", "neutral"),
+                    help="the seeded case of the timed step (default: the stable layer of the "
+                         "headline workload; convective runs the LASDD model)")
     ap.add_argument("--no-fusion", action="store_true",
                     help="run the LAD dynamic procedure's separate test-filter C2R and "
                          "contraction kernels instead of the fused ones (A/B comparisons)")
@@ -281,7 +293,8 @@ def main():
         copy_probe(args.copy_probe)
         print(json.dumps({"fp64_ms_per_step": fp64["ms"], "value": fp64["mpts"]}), flush=True)
         return
-    mixed = run_gpu("mixed", n, args.steps, args.warmup, tiles=tiles, fusion=not args.no_fusion)
+    mixed = run_gpu("mixed", n, args.steps, args.warmup, tiles=tiles, fusion=not args.no_fusion,
+                    case=args.case)
     log(f"[bench] mixed: {mixed['ms']:.3f} ms/step, {mixed['mpts']:.1f} Mpoint-steps/s, "
         f"CFL {mixed['cfl']:.3f}")
     for r in (mixed["table"] or [])[:25]:
@@ -292,7 +305,8 @@ def main():
     errors = None
     if not args.no_fp64:
         # same step count as the mixed run, so the fields can be compared
-        fp64 = run_gpu("fp64", n, args.steps, args.warmup, find_dominant=True, tiles=tiles)
+        fp64 = run_gpu("fp64", n, args.steps, args.warmup, find_dominant=True, tiles=tiles,
+                       case=args.case)
         log(f"[bench] fp64: {fp64['ms']:.3f} ms/step, {fp64['mpts']:.1f} Mpoint-steps/s")
         errors = error_report(mixed["stats"], fp64["stats"])
         errors["after_steps"] = mixed["total_steps"]
@@ -366,10 +380,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 fields, f64 reductions/pressure/beta (mixed)",
-        "data": "synthetic: seeded GABLS1-like stable layer (cases.initial_fields('stable'))",
-        "config": {"workload": f"stable boundary layer {n}^3, 400^3 m, LAD-SM, FGR=2, "
-                               f"Thomas pressure, dt=0.05 s, mixed precision",
-                   "grid": [n, n, n], "optSgs": 3, "FGR": 2, "precision": "mixed"},
+        "data": (f"synthetic: seeded {CASE_TEXT[args.case][0]} "
+                 f"(cases.initial_fields('{args.case}'))"),
+        "config": {"workload": f"{CASE_TEXT[args.case][0]} {n}^3, {CASE_TEXT[args.case][1]}, "
+                               f"Thomas pressure, mixed precision",
+                   "grid": [n, n, n], "optSgs": CASE_TEXT[args.case][2],
+                   "FGR": CASE_TEXT[args.case][3], "precision": "mixed"},
         "hbm_frac_step": round(mixed["mpts"] * 1e6 * B_ALG["mixed"] / (HBM_PEAK_GBS * 1e9), 4),
         "b_alg_bytes_per_point_step": B_ALG["mixed"],
         "fp64": None if fp64 is None else {

@@ -148,15 +148,18 @@ int alfa_plan_autotune(alfa_plan* plan, const alfa_state* st, int iteration, dou
                        alfa_stream stream, int chosen[3]);
 int alfa_plan_set_tiles(alfa_plan* plan, const int tiles[3]);
 int alfa_plan_get_tiles(const alfa_plan* plan, int tiles[3]);
-/* Kernel fusion of the LAD dynamic procedure (no reference counterpart; the
+/* Kernel fusion of the dynamic procedures (no reference counterpart; the
  * reference forms every filtered field, DynamicSGS_LASDD_SM.py:184-233 and
- * DynamicSGS_ScalarLASDD_SM.py:179-224, then contracts them, :327-355 and
- * :241-312).  With fusion on (the default) a float plan with optSgs = 3
- * inverts the level-1 test-filtered fields and contracts them in one kernel
- * per procedure, so the filtered fields other than u_hat never reach HBM;
- * off, the separate transform and pointwise kernels run.  The two agree to
+ * DynamicSGS_ScalarLASDD_SM.py:179-224, then reduces and contracts them,
+ * :266-355 and :241-312).  With fusion on (the default) a float plan with
+ * optSgs = 3 (LAD) inverts the level-1 test-filtered fields and contracts
+ * them in one kernel per procedure; with optSgs = 1 (LASDD) one kernel per
+ * level and procedure first forms the plane sums that give beta the same
+ * way.  The filtered fields other than u_hat never reach HBM.  Off, the
+ * separate transform, reduction and pointwise kernels run.  The two agree to
  * float rounding (their double accumulations are ordered differently).
- * FP64 plans and other models always run the separate kernels.
+ * FP64 plans, the Wong-Lilly models and grids without a compile-time
+ * transform length always run the separate kernels.
  *   alfa_plan_set_fusion: on = 1 (fused) or 0 (separate kernels).
  *   alfa_plan_get_fusion: reads the setting. */
 int alfa_plan_set_fusion(alfa_plan* plan, int on);

@@ -492,7 +492,7 @@ int surface(alfa_plan* P, hipStream_t s, const void* u, const void* v, const voi
 }  // namespace ops
 
 // Run every operation once in dry mode so that its named buffers are planned.
-template <class T> static void plan_dry_run(alfa_plan* P) {
+template <class T> static void plan_dry_run_once(alfa_plan* P) {
   hipStream_t s = nullptr;
   void* f = (void*)(uintptr_t)0x1000;
   alfa_state st{};
@@ -541,6 +541,18 @@ template <class T> static void plan_dry_run(alfa_plan* P) {
   ops::div_flux_sgs<T>(P, s, nine, f, f, f, f, three, f, o >= 1 && o <= 4, f, nullptr, nullptr, f,
                        f);
   ops::scalar_fluxes<T>(P, s, nine, three, f, f, three_o);
+}
+
+// Every buffer of the step and the operations, under both settings of
+// alfa_plan_set_fusion (the fused and the separate LAD/LASDD kernels use
+// different scratch), so the setting can change at any time.
+template <class T> static void plan_dry_run(alfa_plan* P) {
+  const int fuse = P->fuse_contract;
+  for (int f : {1, 0}) {
+    P->fuse_contract = f;
+    plan_dry_run_once<T>(P);
+  }
+  P->fuse_contract = fuse;
 }
 
 template <class T> static int allocate_workspace(alfa_plan* P) {

@@ -1,6 +1,8 @@
-// Level-1 test-filter inverse transforms fused with the pointwise contraction
-// of the dynamic procedure (LAD models, optSgs = 3: one test-filter level,
-// beta given per level).
+// Test-filter inverse transforms fused with what the dynamic procedures do
+// with the filtered fields: the pointwise contractions (LM/MM, T_up/T_dn) of
+// level 1, and for LASDD (optSgs = 1) the plane sums of both levels that give
+// beta (the sums kernels at the end of this file); LAD (optSgs = 3) has
+// beta = 1 and one level.
 //
 // Unfused, the y C2R of the test filter writes every filtered field to HBM
 // (21 momentum fields, DynamicSGS_LASDD_SM.py:184-233; 10 scalar fields,
@@ -498,12 +500,308 @@ __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __re
 }
 
 // ---------------------------------------------------------------------------
+// plane sums of the LASDD procedures (optSgs = 1), one test-filter level per
+// launch: the momentum a1..e2 of DynamicSGS_LASDD_SM.py:266-312 and the scalar
+// a2..e4 of DynamicSGS_ScalarLASDD_SM.py:241-302, which the beta quintics
+// need before any contraction.  The kernels invert the level's filtered
+// spectra as the contraction kernels do and add each point's five terms of
+// the level into per-thread sums (over the thread's R rows, per level of its
+// z pair); the workgroup reduces those over the tile's rows (a fixed xor
+// shuffle tree inside each wave, then the waves in order) and writes one
+// partial per (x row, term, z).  k_plane_final adds the partials in x-row
+// order, so the sums are reproducible run to run.
+// ---------------------------------------------------------------------------
+template <class T, int N, int BS>
+__device__ __forceinline__ void ctr_tile_sums(c2<T>* buf, const double (&acc)[5][2],
+                                              double* __restrict__ part, int x, int z0, int nz,
+                                              int nt, int tbase, int tstride, const double (&c)[5]) {
+  constexpr int L = CtrGeom<N, BS>::L, NW = BS / 64;
+  double v[5][2];
+#pragma unroll
+  for (int t = 0; t < 5; ++t)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      double s = acc[t][q];
+      // lanes j*L + p of a wave hold rows j of level pair p: add over j
+#pragma unroll
+      for (int off = L; off < 64; off <<= 1) s += __shfl_xor(s, off, 64);
+      v[t][q] = s;
+    }
+  // the transform buffer is free once every lane has left the last pass
+  double* red = reinterpret_cast<double*>(buf);   // [NW][5][2][L]
+  __syncthreads();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane < L) {
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) red[((wave * 5 + t) * 2 + q) * L + lane] = v[t][q];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 5 * 2 * L; i += BS) {
+    const int t = i / (2 * L), q = (i / L) % 2, pl = i % L;
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sum += red[((w * 5 + t) * 2 + q) * L + pl];
+    const int z = z0 + 2 * pl + q;
+    if (z < nz) part[((size_t)x * nt + tbase + t * tstride) * nz + z] = c[t] * sum;
+  }
+}
+
+template <class T> struct LasddSumArgs {
+  const c2<T>* in[2 * LAD_NP];   // the level's x-filtered spectra, pair by pair
+  T* uh[3];                      // level 2: u_hat, v_hat, w_hat out (else null)
+  T* Shat;                       // level 2: |S_hat| out (else null)
+  double* part;                  // (nx, 10, nz) partial sums
+  int ncy_in, ny, nz, tbase, tstride;
+  double scale;                  // 1 / (nx ny)
+  // factors of the level's terms <L:|S|S>, |S^| <L:S^>, <|S|S:|S|S>,
+  // |S^|^2 <S^:S^>, |S^| <S^:|S|S>
+  double c[5];
+};
+
+template <class T, int N, int BS, bool BAND>
+__device__ __forceinline__ void k_c2r_lasdd_sums(const LasddSumArgs<T>& a,
+                                                 const c2<T>* __restrict__ tw, const TileGrid& tg) {
+  using D = CtrDriver<T, N, BS, BAND, LAD_NP>;
+  constexpr int R = D::R, NS = D::NS, L = D::L;
+  const int nz = a.nz;
+  const int j = threadIdx.x / L, p = threadIdx.x % L;
+  D d(a.in, a.ncy_in, nz, a.scale, tw, tg);
+  if (!d.first()) return;
+  do {
+    const int z = d.zt * D::ZT + 2 * p;
+    const bool zin = z < nz;
+    double acc[5][2];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) { acc[t][0] = 0.0; acc[t][1] = 0.0; }
+    // S_ij S_ij (summed in double) and |S_hat|
+    double sm[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r) { sm[r][0] = 0.0; sm[r][1] = 0.0; }
+#pragma unroll 1
+    for (int s = 0; s < 3; ++s) {
+      const double w0 = s < 2 ? 1.0 : 2.0, w1 = s < 1 ? 1.0 : 2.0;
+      d.step(s, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const double s0 = o[0][r][q], s1 = o[1][r][q];
+            sm[r][q] += w0 * (s0 * s0);
+            sm[r][q] += w1 * (s1 * s1);
+          }
+      });
+    }
+    T smf[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        smf[r][q] = (T)sqrt(2.0 * sm[r][q]);
+        // |S^|^2 <S^:S^> = 2 <S^:S^>^2
+        acc[3][q] += 2.0 * (sm[r][q] * sm[r][q]);
+      }
+    // u_hat, v_hat, w_hat, held in LDS after the pair buffer and the twiddles
+    T* uhs = reinterpret_cast<T*>(d.stw + N);
+    auto UH = [&](int k, int r, int q) -> T& { return uhs[((k * R + r) * 2 + q) * BS + threadIdx.x]; };
+    d.step(3, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { UH(0, r, q) = o[0][r][q]; UH(1, r, q) = o[1][r][q]; }
+    });
+    d.step(4, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) UH(2, r, q) = o[0][r][q];
+    });
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      T sh[2][R][2], ph[2][R][2];
+      d.step(5 + 3 * m, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < R; ++r) { sh[h][r][0] = o[h][r][0]; sh[h][r][1] = o[h][r][1]; }
+      });
+      d.step(6 + 3 * m, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < R; ++r) { ph[h][r][0] = o[h][r][0]; ph[h][r][1] = o[h][r][1]; }
+      });
+      d.step(7 + 3 * m, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int n = 2 * m + h;
+          const int ia = n < 3 ? n : (n < 5 ? 0 : 1);
+          const int ib = n < 3 ? n : (n == 3 ? 1 : 2);
+          const double wgt = n < 3 ? 1.0 : 2.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+              T shv = smf[r][q], ta = UH(ia, r, q), tb = UH(ib, r, q), sv = sh[h][r][q],
+                pv = ph[h][r][q], ssv = o[h][r][q];
+              ctr_keep(shv);
+              ctr_keep(sv);
+              ctr_keep(pv);
+              const double Lij = (double)pv - (double)ta * (double)tb;
+              const double S = sv, SS = ssv, Sh = shv;
+              acc[0][q] += wgt * (Lij * SS);
+              acc[1][q] += wgt * Sh * (Lij * S);
+              acc[2][q] += wgt * (SS * SS);
+              acc[4][q] += wgt * Sh * (S * SS);
+            }
+        }
+      });
+    }
+    if (a.uh[0] != nullptr && zin) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const size_t i = ((size_t)d.x * a.ny + j + r * NS) * nz + z;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          *reinterpret_cast<c2<T>*>(a.uh[k] + i) = mkc<T>(UH(k, r, 0), UH(k, r, 1));
+        *reinterpret_cast<c2<T>*>(a.Shat + i) = mkc<T>(smf[r][0], smf[r][1]);
+      }
+    }
+    ctr_tile_sums<T, N, BS>(d.buf, acc, a.part, d.x, d.zt * D::ZT, nz, 10, a.tbase, a.tstride, a.c);
+  } while (d.next());
+}
+
+template <class T> struct ScaSumArgs {
+  const c2<T>* in[2 * SCA_NP];   // the level's x-filtered scalar spectra, pair by pair
+  const T* uh[3];                // the momentum u_hat, v_hat, w_hat of the level
+  const T* Sm;                   // and its |S_hat|
+  double* part;                  // (nx, 10, nz) partial sums
+  int ncy_in, ny, nz, tbase, tstride;
+  double scale;                  // 1 / (nx ny)
+  // factors of the level's terms <LT.SdT>, <LT.|S^| dT>, <SdT.SdT>,
+  // <SdT.|S^| dT>, <(|S^| dT)^2>
+  double c[5];
+};
+
+template <class T, int N, int BS, bool BAND>
+__device__ __forceinline__ void k_c2r_sca_sums(const ScaSumArgs<T>& a,
+                                               const c2<T>* __restrict__ tw, const TileGrid& tg) {
+  using D = CtrDriver<T, N, BS, BAND, SCA_NP>;
+  constexpr int R = D::R, NS = D::NS, L = D::L;
+  const int nz = a.nz;
+  const int j = threadIdx.x / L, p = threadIdx.x % L;
+  D d(a.in, a.ncy_in, nz, a.scale, tw, tg);
+  if (!d.first()) return;
+  do {
+    const int z = d.zt * D::ZT + 2 * p;
+    const bool zin = z < nz;
+    auto point_load = [&](const T* f, T (&v)[R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        c2<T> t = mkc<T>(0, 0);
+        if (zin) t = *reinterpret_cast<const c2<T>*>(f + ((size_t)d.x * a.ny + j + r * NS) * nz + z);
+        v[r][0] = t.x;
+        v[r][1] = t.y;
+      }
+    };
+    // TH_hat and |S_hat| in LDS after the pair buffer and the twiddles
+    T* sts = reinterpret_cast<T*>(d.stw + N);
+    auto TH_ = [&](int r, int q) -> T& { return sts[((0 * R + r) * 2 + q) * BS + threadIdx.x]; };
+    auto SM_ = [&](int r, int q) -> T& { return sts[((1 * R + r) * 2 + q) * BS + threadIdx.x]; };
+    T um[R][2], uth[R][2], dh[R][2];
+    {
+      T smh[R][2];
+      point_load(a.Sm, smh);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) SM_(r, q) = smh[r][q];
+    }
+    point_load(a.uh[0], um);
+    double acc[5][2];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) { acc[t][0] = 0.0; acc[t][1] = 0.0; }
+    // direction k: the five terms of sca_terms from (u_k TH)_hat = ut, the
+    // momentum u_hat_k = u, dTH/dx_k hat = dd and |S| dTH/dx_k hat = sd
+    auto fold = [&](const T (&ut)[R][2], const T (&u)[R][2], const T (&dd)[R][2],
+                    const T (&sd)[R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          T thv = TH_(r, q), smv = SM_(r, q), utv = ut[r][q], uv = u[r][q], sdv = sd[r][q],
+            ddv = dd[r][q];
+          ctr_keep(thv);
+          ctr_keep(smv);
+          ctr_keep(utv);
+          ctr_keep(uv);
+          ctr_keep(sdv);
+          ctr_keep(ddv);
+          const double LT = (double)utv - (double)uv * (double)thv;
+          const double S = sdv, Smd = (double)smv * (double)ddv;
+          acc[0][q] += LT * S;
+          acc[1][q] += LT * Smd;
+          acc[2][q] += S * S;
+          acc[3][q] += S * Smd;
+          acc[4][q] += Smd * Smd;
+        }
+    };
+    d.step(0, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { TH_(r, q) = o[0][r][q]; uth[r][q] = o[1][r][q]; }
+    });
+    d.step(1, [&](const T (&o)[2][R][2]) { fold(uth, um, o[0], o[1]); });
+    point_load(a.uh[1], um);
+    d.step(2, [&](const T (&o)[2][R][2]) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { uth[r][q] = o[0][r][q]; dh[r][q] = o[1][r][q]; }
+    });
+    d.step(3, [&](const T (&o)[2][R][2]) {
+      fold(uth, um, dh, o[0]);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) uth[r][q] = o[1][r][q];
+    });
+    point_load(a.uh[2], um);
+    d.step(4, [&](const T (&o)[2][R][2]) { fold(uth, um, o[0], o[1]); });
+    ctr_tile_sums<T, N, BS>(d.buf, acc, a.part, d.x, d.zt * D::ZT, nz, 10, a.tbase, a.tstride, a.c);
+  } while (d.next());
+}
+
+// ---------------------------------------------------------------------------
 // launchers: false when the fused form does not apply (double fields, a
 // length without a compile-time plan or with a last radix above 5, nz not a
 // multiple of 4, unaligned buffers); the caller then runs the unfused C2R
 // and pointwise kernel.
 // ---------------------------------------------------------------------------
 constexpr int CTR_BS = 256;
+
+// whether the fused kernels serve a plan of element type T, transform length
+// n (y) and nz levels; buffers must also be 16-byte aligned (ctr_aligned)
+template <class T> inline bool ctr_applies(int nz, int n) {
+  if constexpr (sizeof(T) != 4) {
+    return false;
+  } else {
+    if (nz % 4 != 0) return false;
+    bool ok = false;
+    ct_length(n, [&](auto nct, auto) {
+      constexpr int NCT = decltype(nct)::value;
+      if constexpr (CtrGeom<NCT, CTR_BS>::OK) ok = true;
+    });
+    return ok;
+  }
+}
+inline bool ctr_aligned(std::initializer_list<const void*> ps) {
+  for (const void* q : ps)
+    if (q != nullptr && !al16(q)) return false;
+  return true;
+}
 
 // extra_lds: fields of per-thread point state the kernel holds in LDS
 template <class T, class Args, class K>
@@ -608,4 +906,71 @@ bool c2r_sca(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const do
                            launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
                                             KBODY(k_c2r_sca<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
                          });
+}
+
+// The level's five LASDD plane-sum terms into the partials part (nx, 10, nz)
+// from its 21 x-filtered spectra sp[] (LoadLasdd order, pitch ncy_in);
+// level 2 also writes u_hat (uh_out) and |S_hat| (Shat_out), which the scalar
+// procedure reads.  The caller has checked ctr_applies and the alignment.
+template <class T>
+void c2r_lasdd_sums(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, double* part,
+                    int level, T* const* uh_out, T* Shat_out, int rows, int ny, int nz,
+                    const Fft1D& fyy, const void* tw, double L, double TFR) {
+  LasddSumArgs<T> a{};
+  for (int st = 0; st < LAD_NP; ++st)
+    for (int h = 0; h < 2; ++h)
+      a.in[2 * st + h] = lad_field(st, h) < 0 ? nullptr : sp[lad_field(st, h)];
+  for (int k = 0; k < 3; ++k) a.uh[k] = uh_out ? uh_out[k] : nullptr;
+  a.Shat = Shat_out;
+  a.part = part;
+  a.ncy_in = ncy_in; a.ny = ny; a.nz = nz;
+  a.tbase = level - 1; a.tstride = 2;
+  a.scale = 1.0 / ((double)rows * ny);
+  // FLasddSums (DynamicSGS_LASDD_SM.py:266-312): level 1 a1 c1 e1 g1 i1 (the
+  // even terms), level 2 the odd ones
+  const double L2v = L * L, T2 = TFR * TFR, T4 = T2 * T2, T8 = T4 * T4, k2 = 2.0 * L2v;
+  const double c1[5] = {k2, k2 * T2, k2 * k2, 4.0 * L2v * L2v * T4, 8.0 * L2v * L2v * T2};
+  const double c2v[5] = {k2, k2 * T4, k2 * k2, 4.0 * L2v * L2v * T8, 8.0 * L2v * L2v * T4};
+  for (int t = 0; t < 5; ++t) a.c[t] = level == 1 ? c1[t] : c2v[t];
+  // reference data flow: the level's 21 irfft2 outputs and their reads by the
+  // plane sums
+  c2r_contract<T>(P, s, a, 27, uh_out ? 4 : 0, 21, 21, rows, fyy, tw, "k_y_c2r<LasddSums>", 3,
+                  [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
+                    constexpr int NCT = decltype(nct)::value;
+                    constexpr bool B = decltype(band)::value;
+                    launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
+                                     KBODY(k_c2r_lasdd_sums<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                  });
+}
+
+// The level's five scalar LASDD plane-sum terms into part (nx, 10, nz) from
+// its 10 x-filtered scalar spectra sp[] (LoadScalar order) and the level's
+// momentum u_hat and |S_hat|.  The caller has checked ctr_applies and the
+// alignment.
+template <class T>
+void c2r_sca_sums(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, double* part,
+                  int level, const T* const* uh, const T* Sm, int rows, int ny, int nz,
+                  const Fft1D& fyy, const void* tw, double L, double TFR) {
+  ScaSumArgs<T> a{};
+  for (int st = 0; st < SCA_NP; ++st)
+    for (int h = 0; h < 2; ++h) a.in[2 * st + h] = sp[sca_field(st, h)];
+  for (int k = 0; k < 3; ++k) a.uh[k] = uh[k];
+  a.Sm = Sm;
+  a.part = part;
+  a.ncy_in = ncy_in; a.ny = ny; a.nz = nz;
+  a.tbase = 5 * (level - 1); a.tstride = 1;
+  a.scale = 1.0 / ((double)rows * ny);
+  // FScalarSums (DynamicSGS_ScalarLASDD_SM.py:241-302): a2..e2 at level 1,
+  // a4..e4 at level 2
+  const double L2 = L * L, L4 = L2 * L2, T2 = TFR * TFR, T4 = T2 * T2, T8 = T4 * T4;
+  const double c1[5] = {L2, -L2 * T2, L4, -2.0 * L4 * T2, L4 * T4};
+  const double c2v[5] = {L2, -L2 * T4, L4, -2.0 * L4 * T4, L4 * T8};
+  for (int t = 0; t < 5; ++t) a.c[t] = level == 1 ? c1[t] : c2v[t];
+  c2r_contract<T>(P, s, a, 10, 0, 10, 14, rows, fyy, tw, "k_y_c2r<ScalarSums>", 2,
+                  [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
+                    constexpr int NCT = decltype(nct)::value;
+                    constexpr bool B = decltype(band)::value;
+                    launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
+                                     KBODY(k_c2r_sca_sums<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                  });
 }

@@ -364,12 +364,34 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
-  // LAD (beta = 1, one level): the level-1 inverse transforms and the LM/MM
-  // contraction in one kernel, where it applies (u_hat and |S_hat| are still
-  // written: the scalar procedure reads them)
-  const bool fused = !lasdd && P->fuse_contract &&
-                     c2r_lad<T>(P, c.s, sp1, d.cut1_y, beta, f1, LM, MM, Shat, c.nx, c.ny, c.nz,
-                                P->fy, P->tw_y, d.L, d.TFR);
+  // Fused form, where it applies: the level-1 inverse transforms and the
+  // LM/MM contraction in one kernel (u_hat and |S_hat| are still written: the
+  // scalar procedure reads them).  LASDD first takes the plane sums of both
+  // levels the same way (level 2 also writing u_hat and |S_hat| of its
+  // level), then beta, then the contraction with beta.
+  bool fused = false;
+  T* Shatd = lasdd ? c.W("Shatd") : nullptr;
+  if (P->fuse_contract && ctr_applies<T>(c.nz, c.ny) &&
+      ctr_aligned({f1[0], f1[1], f1[2], LM, MM, Shat, Shatd}) &&
+      (!lasdd || ctr_aligned({f2[0], f2[1], f2[2]}))) {
+    if (lasdd) {
+      double* part = c.D("lpart", (size_t)c.nx * 10 * c.nz);
+      double* sums = c.D("lsums", 10 * c.nz);
+      double* coef = c.D("lcoef", 6 * c.nz);
+      c2r_lasdd_sums<T>(P, c.s, sp1, d.cut1_y, part, 1, nullptr, nullptr, c.nx, c.ny, c.nz, P->fy,
+                        P->tw_y, d.L, d.TFR);
+      c2r_lasdd_sums<T>(P, c.s, sp2, d.cut2_y, part, 2, f2, Shatd, c.nx, c.ny, c.nz, P->fy,
+                        P->tw_y, d.L, d.TFR);
+      if (live(P)) {
+        launch_k<1024>(P, c.s, dim3((10 * c.nz + 3) / 4), dim3(256), 0, KBODY(k_plane_final), c.nx, 10, c.nz, part, sums, 1.0 / c.ncol);
+        launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 0, d.TFR, coef);
+        launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
+        launch_ok(P);
+      }
+    }
+    fused = c2r_lad<T>(P, c.s, sp1, d.cut1_y, beta, f1, LM, MM, Shat, c.nx, c.ny, c.nz, P->fy,
+                       P->tw_y, d.L, d.TFR);
+  }
   MomLevel<T> l1{}, l2{};
   for (int k = 0; k < 3; ++k) { l1.uh[k] = f1[k]; l2.uh[k] = f2[k]; }
   for (int n = 0; n < 6; ++n) {
@@ -397,10 +419,7 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
     else
       PW_LAUNCH(P, c.s, c.F, k_lasdd_lm<T>, c.dims(), l1, beta, d.L, d.TFR, LM, MM, Shat);
   }
-  if (lasdd) {
-    T* Shatd = c.W("Shatd");
-    PW_LAUNCH(P, c.s, c.F, k_smag_level<T>, c.dims(), l2, Shatd);
-  }
+  if (lasdd && !fused) PW_LAUNCH(P, c.s, c.F, k_smag_level<T>, c.dims(), l2, Shatd);
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cs2, s2, "box");
   if (live(P)) {
@@ -470,11 +489,31 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
     launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_fill_d), c.nz, beta, 1.0);
     launch_ok(P);
   }
-  // LAD (beta = 1, one level): the level-1 inverse transforms and the
-  // T_up/T_dn contraction in one kernel, where it applies
-  const bool fused = !lasdd && P->fuse_contract &&
-                     c2r_sca<T>(P, c.s, sp1, d.cut1_y, beta, l1.uh, l1.Sm, Tup, Tdn, c.nx, c.ny,
-                                c.nz, P->fy, P->tw_y, d.L, d.TFR);
+  // Fused form, where it applies: the level-1 inverse transforms and the
+  // T_up/T_dn contraction in one kernel; LASDD first takes the plane sums of
+  // both levels the same way, then beta
+  bool fused = false;
+  if (P->fuse_contract && ctr_applies<T>(c.nz, c.ny) &&
+      ctr_aligned({l1.uh[0], l1.uh[1], l1.uh[2], l1.Sm, Tup, Tdn}) &&
+      (!lasdd || ctr_aligned({l2.uh[0], l2.uh[1], l2.uh[2], l2.Sm}))) {
+    if (lasdd) {
+      double* part = c.D("spart", (size_t)c.nx * 10 * c.nz);
+      double* sums = c.D("ssums", 10 * c.nz);
+      double* coef = c.D("scoef", 6 * c.nz);
+      c2r_sca_sums<T>(P, c.s, sp1, d.cut1_y, part, 1, l1.uh, l1.Sm, c.nx, c.ny, c.nz, P->fy,
+                      P->tw_y, d.L, d.TFR);
+      c2r_sca_sums<T>(P, c.s, sp2, d.cut2_y, part, 2, l2.uh, l2.Sm, c.nx, c.ny, c.nz, P->fy,
+                      P->tw_y, d.L, d.TFR);
+      if (live(P)) {
+        launch_k<1024>(P, c.s, dim3((10 * c.nz + 3) / 4), dim3(256), 0, KBODY(k_plane_final), c.nx, 10, c.nz, part, sums, 1.0 / c.ncol);
+        launch_k<1024>(P, c.s, dim3((c.nz + 63) / 64), dim3(64), 0, KBODY(k_quintic), sums, c.nz, 1, d.TFR, coef);
+        launch_k<1024>(P, c.s, dim3((c.nz + 7) / 8), dim3(64), 0, KBODY(k_beta), coef, c.nz, beta);
+        launch_ok(P);
+      }
+    }
+    fused = c2r_sca<T>(P, c.s, sp1, d.cut1_y, beta, l1.uh, l1.Sm, Tup, Tdn, c.nx, c.ny, c.nz,
+                       P->fy, P->tw_y, d.L, d.TFR);
+  }
   if (!fused) {
     test_filter_inv<T>(c, 10, true, lasdd, sp1, sp2, f1, f2);
     if (lasdd) {

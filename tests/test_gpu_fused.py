@@ -1,6 +1,8 @@
-"""The fused LAD test-filter kernels (level-1 inverse transforms + the
-LM/MM and T_up/T_dn contractions in one kernel each) against the separate
-y C2R and pointwise kernels they replace, on the same plan configuration.
+"""The fused test-filter kernels against the separate y C2R and pointwise
+kernels they replace, on the same plan configuration: for LAD (optSgs = 3)
+the level-1 inverse transforms + the LM/MM and T_up/T_dn contractions in one
+kernel each; for LASDD (optSgs = 1) also the plane sums of both levels (one
+kernel per level and procedure) that give beta before the contractions.
 
 Both paths transform each filtered field by the same passes, so their float
 field values agree bitwise; the contractions differ only in how their double
@@ -35,6 +37,15 @@ FUSED_GRIDS = {
     "stable192": ("stable", dict(nx=32, ny=192, nz=16)),
     "stable256": ("stable", dict(nx=64, ny=256, nz=24)),               # the benchmark's length
     "neutral256": ("neutral", dict(nx=32, ny=256, nz=12)),             # FGR = 1 at 256
+}
+LASDD_GRIDS = {
+    "convective48": ("convective", dict(nx=48, ny=48, nz=48)),         # LASDD-SM, band first pass
+    "convective80": ("convective", dict(nx=48, ny=80, nz=16)),
+    "convective96": ("convective", dict(nx=96, ny=96, nz=32)),
+    "convective128": ("convective", dict(nx=64, ny=128, nz=16)),
+    "convective192": ("convective", dict(nx=32, ny=192, nz=16)),
+    "convective256": ("convective", dict(nx=64, ny=256, nz=24)),
+    "neutral48_lasdd": ("neutral", dict(nx=48, ny=48, nz=32, optSgs=1)),   # FGR = 1
 }
 UNFUSED_GRIDS = {
     "stable40": ("stable", dict(nx=40, ny=40, nz=40)),                 # no compile-time plan
@@ -116,6 +127,71 @@ def test_fused_coefficients_match_on_stirred_fields(key):
         assert e[k] < 1e-4, (k, e[k])
     for r in (a, b):
         r.plan.close()
+
+
+@pytest.mark.parametrize("key", list(LASDD_GRIDS))
+def test_fused_lasdd_matches_separate_kernels(key):
+    """LASDD (beta from the plane sums of both test-filter levels): from
+    stirred fields, both betas within 1e-4 of the separate kernels' after the
+    first step (the sums differ only in the order of their double additions;
+    beta is a root the reference's Laguerre iteration finds to 1e-6) and the
+    coefficient fields within 1e-4 of their largest values; after 4 steps the
+    prognostic fields within float rounding of each other."""
+    name, over = LASDD_GRIDS[key]
+    a, b = pair(name, over, 4, stirred=True)
+    a.step(1)
+    b.step(1)
+    torch.cuda.synchronize()
+    db = {k: float(np.max(np.abs(a.prof[k].cpu().numpy() - b.prof[k].cpu().numpy())))
+          for k in ("beta1_1D", "beta2_1D")}
+    e1 = {k: rel_err(a.host(k), b.host(k)) for k in ("Cs2_3D", "Cs2PrRatio_3D")}
+    print(key, "LASDD step 1 fused vs separate: beta max diff",
+          {k: "%.1e" % v for k, v in db.items()}, "coefficients", {k: "%.1e" % v for k, v in e1.items()},
+          "beta ranges %.3f..%.3f" % (float(a.prof["beta1_1D"].min()), float(a.prof["beta1_1D"].max())))
+    for k, v in db.items():
+        assert v < 1e-4, (k, v)
+    for k, v in e1.items():
+        assert v < 1e-4, (k, v)
+    a.step(3)
+    b.step(3)
+    torch.cuda.synchronize()
+    fa, fb = a.fields(), b.fields()
+    e = {k: rel_err(fa[k], fb[k]) for k in ("u", "v", "w", "TH")}
+    print(key, "LASDD 4 steps fused vs separate:", {k: "%.1e" % v for k, v in e.items()})
+    for k in ("u", "v", "w", "TH"):
+        assert e[k] < 1e-5, (k, e[k])
+    for r in (a, b):
+        r.plan.close()
+
+
+def test_fused_lasdd_mixed_profiles_vs_oracle():
+    """Mixed precision with the fused LASDD kernels (convective 48^3): mean
+    wind and potential temperature profiles within 1e-4 relative of the FP64
+    oracle after 20 steps."""
+    name, over = LASDD_GRIDS["convective48"]
+    cfg, u, v, w, TH = cases.initial_fields(name, **over)
+    nsteps = 20
+    ser = cases.surface_series(cfg, nsteps + 2)
+    g = O.Grid(cfg)
+    s = O.initial_state(g, u, v, w, TH)
+    th = O.ThomasPressureInit(g)
+    les = core.LES(core.Plan(cfg, "mixed"), u, v, w, TH, sfc_series=ser)
+    for _ in range(nsteps):
+        s = O.step(g, s, th, None if ser is None else ser[s.iteration - 1])
+    les.step(nsteps)
+    f = les.fields()
+    ug = {k: f[k].mean(axis=(0, 1)) for k in ("u", "v", "TH")}
+    uo = {k: getattr(s, k).mean(axis=(0, 1)) for k in ("u", "v", "TH")}
+    speed = np.max(np.hypot(uo["u"] + cfg["Ugal"], uo["v"]))
+    for k in ("u", "v"):
+        e = np.max(np.abs(ug[k] - uo[k])) / speed
+        print("convective48 LASDD", k, "fused mixed profile err / max mean wind speed %.2e" % e)
+        assert e < 1e-4
+    e = np.max(np.abs(ug["TH"] - uo["TH"])) / np.max(np.abs(uo["TH"] + cfg["T_0"]))
+    print("convective48 LASDD TH fused mixed profile rel err %.2e" % e)
+    assert e < 1e-4
+    db = np.max(np.abs(les.prof["beta1_1D"].cpu().numpy() - s.diag["beta1_1D"]))
+    print("convective48 LASDD beta1 after 20 steps: max diff to the oracle %.2e" % db)
 
 
 @pytest.mark.parametrize("key", list(UNFUSED_GRIDS))
