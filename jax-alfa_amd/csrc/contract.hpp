@@ -45,10 +45,29 @@
 #include "loaders.hpp"
 #include "plan.hpp"
 
+// Radix of the fused kernels' last pass at length N: that of the
+// compile-time plan, except 64 (8 x 8), which the fused kernels factor as
+// 8 x 4 x 2 so that a thread holds 4 points per field instead of 16.
+__host__ __device__ constexpr int ctr_last_radix(int n) { return n == 64 ? 2 : last_radix(n); }
+
+// The passes between the first (radix 8, stride 1) and the last (radix RL):
+// radix_of(M / RL) at each step, which for every length but 64 is the
+// compile-time plan's own sequence (the same passes as k_y_c2r's).
+template <class T, int LPB, int N, int NS, int M, int RL, int BS>
+struct CtrMid {
+  static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
+    if constexpr (M != RL) {
+      constexpr int R = radix_of(M / RL);
+      fft_pass_ct<T, R, LPB, N, NS, BS>(buf, tw, sign);
+      CtrMid<T, LPB, N, NS * R, M / R, RL, BS>::run(buf, tw, sign);
+    }
+  }
+};
+
 // geometry of the fused kernels at compile-time length N: L lines per field,
 // LL = 2L lines per pair
 template <int N, int BS> struct CtrGeom {
-  static constexpr int R = N > 0 ? last_radix(N) : 1;
+  static constexpr int R = N > 0 ? ctr_last_radix(N) : 1;
   static constexpr int NS = N > 0 ? N / R : 1;
   static constexpr int L = BS / NS;
   static constexpr int LL = 2 * L;
@@ -113,12 +132,9 @@ __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int km
                                             T (&o)[2][CtrGeom<N, BS>::R][2]) {
   using G = CtrGeom<N, BS>;
   constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL;
-  if constexpr (BAND) {
-    band_first_pass_inv<T, LL, N, BS>(buf, kmax);
-    FftCTMid<T, LL, N, 8, N / 8, BS>::run(buf, stw, +1);
-  } else {
-    FftCTMid<T, LL, N, 1, N, BS>::run(buf, stw, +1);
-  }
+  if constexpr (BAND) band_first_pass_inv<T, LL, N, BS>(buf, kmax);
+  else fft_pass_ct<T, 8, LL, N, 1, BS>(buf, stw, +1);
+  CtrMid<T, LL, N, 8, N / 8, R, BS>::run(buf, stw, +1);
   const int j = threadIdx.x / L, p = threadIdx.x % L;
   // line p of both fields (lines 2p, 2p + 1): one 16-byte read per r
   c2<T> v[2][R];

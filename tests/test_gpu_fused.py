@@ -4,12 +4,13 @@ the level-1 inverse transforms + the LM/MM and T_up/T_dn contractions in one
 kernel each; for LASDD (optSgs = 1) also the plane sums of both levels (one
 kernel per level and procedure) that give beta before the contractions.
 
-Both paths transform each filtered field by the same passes, so their float
-field values agree bitwise; the contractions differ only in how their double
+Both paths transform each filtered field by the same passes (but for length
+64, which the fused kernels factor as 8 x 4 x 2 instead of 8 x 8), so their
+float field values agree bitwise; the contractions differ only in how their double
 arithmetic is ordered (and MM, T_dn are accumulated in float), so LM, MM,
 T_up and T_dn agree to a few float roundings and the runs stay within float
 rounding of each other.  The grids cover every transform length the fused
-kernels serve (48, 80, 96, 128, 192, 256), the band-limited first pass
+kernels serve (48, 64, 80, 96, 128, 192, 256), the band-limited first pass
 (FGR = 2) and the full one (FGR = 1), and the grids where the fused form
 does not apply (a length without a compile-time plan, nz not a multiple of
 4), where both settings must give bitwise the same run.
@@ -31,6 +32,7 @@ from alfa_amd import cases, core  # noqa: E402
 FUSED_GRIDS = {
     "stable48": ("stable", dict(nx=48, ny=48, nz=48)),                 # band first pass, radix 3 last
     "neutral48": ("neutral", dict(nx=48, ny=48, nz=32)),               # FGR = 1: full first pass
+    "stable64": ("stable", dict(nx=48, ny=64, nz=16)),                 # 8 x 4 x 2 in the fused kernels
     "stable80": ("stable", dict(nx=48, ny=80, nz=16)),                 # radix-5 last pass
     "stable96": ("stable", dict(nx=96, ny=96, nz=32)),
     "stable128": ("stable", dict(nx=64, ny=128, nz=16)),               # radix-2 last pass
@@ -40,6 +42,7 @@ FUSED_GRIDS = {
 }
 LASDD_GRIDS = {
     "convective48": ("convective", dict(nx=48, ny=48, nz=48)),         # LASDD-SM, band first pass
+    "convective64": ("convective", dict(nx=64, ny=64, nz=32)),
     "convective80": ("convective", dict(nx=48, ny=80, nz=16)),
     "convective96": ("convective", dict(nx=96, ny=96, nz=32)),
     "convective128": ("convective", dict(nx=64, ny=128, nz=16)),
