@@ -12,7 +12,7 @@
 // field of one tile in turn and folds each into per-point state held in
 // registers, so only the fields that later stages read reach HBM.
 //
-// Tile: BS = 256 threads, L lines (2L z levels, two per complex line) of one
+// Tile: BS = 256 threads (320 for the scalar kernels at length 80), L lines (2L z levels, two per complex line) of one
 // x row, with L = BS R / N so that the last radix pass (radix R, stride N/R)
 // is one butterfly per thread and field: thread (j, p) ends every field
 // holding rows y = j + r N/R (r < R) of the level pair (z0 + 2p, z0 + 2p + 1),
@@ -45,21 +45,31 @@
 #include "loaders.hpp"
 #include "plan.hpp"
 
-// Radix of the fused kernels' last pass at length N: that of the
-// compile-time plan, except 64 (8 x 8), which the fused kernels factor as
-// 8 x 4 x 2 so that a thread holds 4 points per field instead of 16.
-__host__ __device__ constexpr int ctr_last_radix(int n) { return n == 64 ? 2 : last_radix(n); }
+// Radix of the fused kernels' last pass at length N with BS threads: that of
+// the compile-time plan, except 64 (8 x 8), which the fused kernels factor
+// as 8 x 4 x 2, and 80 (8 x 2 x 5) at 320 threads, factored 8 x 5 x 2, so
+// that a thread holds 4 points per field instead of 16 or 10.
+__host__ __device__ constexpr int ctr_last_radix(int n, int bs) {
+  return n == 64 || (n == 80 && bs == 320) ? 2 : last_radix(n);
+}
+// threads per workgroup of the fused kernels at length N: 256, but 320 (8
+// lines of the radix-2 last pass) for the scalar kernels at 80, whose state
+// at 10 points per thread needs 250+ VGPRs (1 wave per SIMD; 119-140 at 4).
+// Measured on the 64-member ensemble: the scalar contraction and sums at 80
+// run 15-23% faster at 320; the momentum ones 34-47% slower (twice the
+// tiles, half the threads idle in the first pass), so they keep 256.
+__host__ __device__ constexpr int ctr_bs(int n, bool scalar) { return scalar && n == 80 ? 320 : 256; }
 
 // The passes between the first (radix 8, stride 1) and the last (radix RL):
 // radix_of(M / RL) at each step, which for every length but 64 is the
 // compile-time plan's own sequence (the same passes as k_y_c2r's).
-template <class T, int LPB, int N, int NS, int M, int RL, int BS>
+template <class T, int LPB, int N, int NS, int M, int RL, int BS, bool SWZ = false>
 struct CtrMid {
   static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
     if constexpr (M != RL) {
       constexpr int R = radix_of(M / RL);
-      fft_pass_ct<T, R, LPB, N, NS, BS>(buf, tw, sign);
-      CtrMid<T, LPB, N, NS * R, M / R, RL, BS>::run(buf, tw, sign);
+      fft_pass_ct<T, R, LPB, N, NS, BS, SWZ>(buf, tw, sign);
+      CtrMid<T, LPB, N, NS * R, M / R, RL, BS, SWZ && NS == 1>::run(buf, tw, sign);
     }
   }
 };
@@ -67,7 +77,7 @@ struct CtrMid {
 // geometry of the fused kernels at compile-time length N: L lines per field,
 // LL = 2L lines per pair
 template <int N, int BS> struct CtrGeom {
-  static constexpr int R = N > 0 ? ctr_last_radix(N) : 1;
+  static constexpr int R = N > 0 ? ctr_last_radix(N, BS) : 1;
   static constexpr int NS = N > 0 ? N / R : 1;
   static constexpr int L = BS / NS;
   static constexpr int LL = 2 * L;
@@ -127,14 +137,18 @@ __device__ __forceinline__ void ctr_stage2(c2<T>* buf, const c2<T> (&A1)[IN_IT],
 // every pass but the last over the LL lines, then the last (radix R) from
 // LDS into registers: o[h][r][0], o[h][r][1] = levels z, z + 1 of row
 // j + r NS of field h of the pair (line 2p + h), times sc
-template <class T, int N, int BS, bool BAND>
+// (SWZ_OK: the first pass's output is swizzled, where a middle pass reads it)
+template <class T, int N, int BS, bool BAND, bool SWZ_OK>
 __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int kmax, T sc,
                                             T (&o)[2][CtrGeom<N, BS>::R][2]) {
   using G = CtrGeom<N, BS>;
   constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL;
-  if constexpr (BAND) band_first_pass_inv<T, LL, N, BS>(buf, kmax);
-  else fft_pass_ct<T, 8, LL, N, 1, BS>(buf, stw, +1);
-  CtrMid<T, LL, N, 8, N / 8, R, BS>::run(buf, stw, +1);
+  // (with LL >= 16 a 16-lane group of the first pass's writes covers one
+  // row: no conflict to remove)
+  constexpr bool SWZ = SWZ_OK && N / 8 != R && LL <= 8;
+  if constexpr (BAND) band_first_pass_inv<T, LL, N, BS, SWZ>(buf, kmax);
+  else fft_pass_ct<T, 8, LL, N, 1, BS, SWZ>(buf, stw, +1);
+  CtrMid<T, LL, N, 8, N / 8, R, BS, SWZ>::run(buf, stw, +1);
   const int j = threadIdx.x / L, p = threadIdx.x % L;
   // line p of both fields (lines 2p, 2p + 1): one 16-byte read per r
   c2<T> v[2][R];
@@ -168,7 +182,7 @@ __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int km
 // live).  Step s stages the loaded spectra of pair s (in[2s], in[2s+1], the
 // second possibly null), issues the loads of pair s + 1 (or of the next
 // tile's pair 0), inverts both fields and hands their values to use(o).
-template <class T, int N, int BS, bool BAND, int NP>
+template <class T, int N, int BS, bool BAND, int NP, bool SWZ = false>
 struct CtrDriver {
   using G = CtrGeom<N, BS>;
   static constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL, ZT = 2 * L;
@@ -231,7 +245,7 @@ struct CtrDriver {
     }
     __syncthreads();
     T o[2][R][2];
-    ctr_invert2<T, N, BS, BAND>(buf, stw, kmax, sc, o);
+    ctr_invert2<T, N, BS, BAND, SWZ>(buf, stw, kmax, sc, o);
     use(o);
   }
 };
@@ -263,7 +277,7 @@ template <class T> struct LadArgs {
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void k_c2r_lad(const LadArgs<T>& a, const c2<T>* __restrict__ tw,
                                           const TileGrid& tg) {
-  using D = CtrDriver<T, N, BS, BAND, LAD_NP>;
+  using D = CtrDriver<T, N, BS, BAND, LAD_NP, true>;
   constexpr int R = D::R, NS = D::NS, L = D::L;
   const int nz = a.nz;
   const int j = threadIdx.x / L, p = threadIdx.x % L;
@@ -408,7 +422,7 @@ template <class T> struct ScaArgs {
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __restrict__ tw,
                                           const TileGrid& tg) {
-  using D = CtrDriver<T, N, BS, BAND, SCA_NP>;
+  using D = CtrDriver<T, N, BS, BAND, SCA_NP, true>;
   constexpr int R = D::R, NS = D::NS, L = D::L;
   const int nz = a.nz;
   const int j = threadIdx.x / L, p = threadIdx.x % L;
@@ -703,7 +717,7 @@ template <class T> struct ScaSumArgs {
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void k_c2r_sca_sums(const ScaSumArgs<T>& a,
                                                const c2<T>* __restrict__ tw, const TileGrid& tg) {
-  using D = CtrDriver<T, N, BS, BAND, SCA_NP>;
+  using D = CtrDriver<T, N, BS, BAND, SCA_NP, true>;
   constexpr int R = D::R, NS = D::NS, L = D::L;
   const int nz = a.nz;
   const int j = threadIdx.x / L, p = threadIdx.x % L;
@@ -796,8 +810,6 @@ __device__ __forceinline__ void k_c2r_sca_sums(const ScaSumArgs<T>& a,
 // multiple of 4, unaligned buffers); the caller then runs the unfused C2R
 // and pointwise kernel.
 // ---------------------------------------------------------------------------
-constexpr int CTR_BS = 256;
-
 // whether the fused kernels serve a plan of element type T, transform length
 // n (y) and nz levels; buffers must also be 16-byte aligned (ctr_aligned)
 template <class T> inline bool ctr_applies(int nz, int n) {
@@ -808,7 +820,8 @@ template <class T> inline bool ctr_applies(int nz, int n) {
     bool ok = false;
     ct_length(n, [&](auto nct, auto) {
       constexpr int NCT = decltype(nct)::value;
-      if constexpr (CtrGeom<NCT, CTR_BS>::OK) ok = true;
+      if constexpr (CtrGeom<NCT, ctr_bs(NCT, false)>::OK && CtrGeom<NCT, ctr_bs(NCT, true)>::OK)
+        ok = true;
     });
     return ok;
   }
@@ -819,8 +832,9 @@ inline bool ctr_aligned(std::initializer_list<const void*> ps) {
   return true;
 }
 
-// extra_lds: fields of per-thread point state the kernel holds in LDS
-template <class T, class Args, class K>
+// extra_lds: fields of per-thread point state the kernel holds in LDS;
+// SCALAR: the scalar procedure's kernels (threads per workgroup ctr_bs)
+template <class T, bool SCALAR, class Args, class K>
 bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_fields,
                   int ref_fields, int ref_pw_fields, int rows, const Fft1D& fyy, const void* tw,
                   const char* name, int extra_lds, K kern_sel) {
@@ -831,7 +845,7 @@ bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_
     bool ok = false;
     ct_length(fyy.n, [&](auto nct, auto) {
       constexpr int NCT = decltype(nct)::value;
-      if constexpr (CtrGeom<NCT, CTR_BS>::OK) ok = true;
+      if constexpr (CtrGeom<NCT, ctr_bs(NCT, SCALAR)>::OK) ok = true;
     });
     if (!ok) return false;
     if (!live(P)) return true;
@@ -843,12 +857,13 @@ bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_
                      (double)ref_pw_fields * F * sizeof(T));
     ct_length(fyy.n, [&](auto nct, auto) {
       constexpr int NCT = decltype(nct)::value;
-      if constexpr (CtrGeom<NCT, CTR_BS>::OK) {
-        constexpr int L = CtrGeom<NCT, CTR_BS>::L, LL = CtrGeom<NCT, CTR_BS>::LL;
+      constexpr int BS = ctr_bs(NCT, SCALAR);
+      if constexpr (CtrGeom<NCT, BS>::OK) {
+        constexpr int L = CtrGeom<NCT, BS>::L, LL = CtrGeom<NCT, BS>::LL;
         TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, 1, 0};
         const unsigned grid = tile_grid((size_t)tg.ntz * rows);
         const size_t lds = (size_t)NCT * LL * sizeof(c2<T>) + (size_t)NCT * sizeof(c2<T>) +
-                           (size_t)extra_lds * CtrGeom<NCT, CTR_BS>::R * 2 * CTR_BS * sizeof(T);
+                           (size_t)extra_lds * CtrGeom<NCT, BS>::R * 2 * BS * sizeof(T);
         const bool band = ALFA_BAND && a.ncy_in <= NCT / 4;
         if (band) kern_sel(std::integral_constant<int, NCT>{}, std::true_type{}, grid, lds, tg);
         else kern_sel(std::integral_constant<int, NCT>{}, std::false_type{}, grid, lds, tg);
@@ -881,12 +896,13 @@ bool c2r_lad(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const do
     if (!al16(sp[b])) return false;
   // reference data flow: 21 irfft2 outputs, then the contraction's 21 reads
   // and 3 writes
-  return c2r_contract<T>(P, s, a, 27, 6, 21, 24, rows, fyy, tw, "k_y_c2r<LadContract>", 3,
+  return c2r_contract<T, false>(P, s, a, 27, 6, 21, 24, rows, fyy, tw, "k_y_c2r<LadContract>", 3,
                          [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
                            constexpr int NCT = decltype(nct)::value;
                            constexpr bool B = decltype(band)::value;
-                           launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
-                                            KBODY(k_c2r_lad<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                           constexpr int BS = ctr_bs(NCT, false);
+                           launch_k<BS>(P, s, dim3(grid), dim3(BS), lds,
+                                        KBODY(k_c2r_lad<T, NCT, BS, B>), a, (const c2<T>*)tw, tg);
                          });
 }
 
@@ -915,12 +931,13 @@ bool c2r_sca(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, const do
     if (!al16(sp[b])) return false;
   // reference data flow: 10 irfft2 outputs, then the contraction's 14 reads
   // and 2 writes
-  return c2r_contract<T>(P, s, a, 10, 6, 10, 16, rows, fyy, tw, "k_y_c2r<ScalarContract>", 2,
+  return c2r_contract<T, true>(P, s, a, 10, 6, 10, 16, rows, fyy, tw, "k_y_c2r<ScalarContract>", 2,
                          [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
                            constexpr int NCT = decltype(nct)::value;
                            constexpr bool B = decltype(band)::value;
-                           launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
-                                            KBODY(k_c2r_sca<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                           constexpr int BS = ctr_bs(NCT, true);
+                           launch_k<BS>(P, s, dim3(grid), dim3(BS), lds,
+                                        KBODY(k_c2r_sca<T, NCT, BS, B>), a, (const c2<T>*)tw, tg);
                          });
 }
 
@@ -950,12 +967,13 @@ void c2r_lasdd_sums(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, d
   for (int t = 0; t < 5; ++t) a.c[t] = level == 1 ? c1[t] : c2v[t];
   // reference data flow: the level's 21 irfft2 outputs and their reads by the
   // plane sums
-  c2r_contract<T>(P, s, a, 27, uh_out ? 4 : 0, 21, 21, rows, fyy, tw, "k_y_c2r<LasddSums>", 3,
+  c2r_contract<T, false>(P, s, a, 27, uh_out ? 4 : 0, 21, 21, rows, fyy, tw, "k_y_c2r<LasddSums>", 3,
                   [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
                     constexpr int NCT = decltype(nct)::value;
                     constexpr bool B = decltype(band)::value;
-                    launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
-                                     KBODY(k_c2r_lasdd_sums<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                    constexpr int BS = ctr_bs(NCT, false);
+                    launch_k<BS>(P, s, dim3(grid), dim3(BS), lds,
+                                 KBODY(k_c2r_lasdd_sums<T, NCT, BS, B>), a, (const c2<T>*)tw, tg);
                   });
 }
 
@@ -982,11 +1000,12 @@ void c2r_sca_sums(alfa_plan* P, hipStream_t s, c2<T>* const* sp, int ncy_in, dou
   const double c1[5] = {L2, -L2 * T2, L4, -2.0 * L4 * T2, L4 * T4};
   const double c2v[5] = {L2, -L2 * T4, L4, -2.0 * L4 * T4, L4 * T8};
   for (int t = 0; t < 5; ++t) a.c[t] = level == 1 ? c1[t] : c2v[t];
-  c2r_contract<T>(P, s, a, 10, 0, 10, 14, rows, fyy, tw, "k_y_c2r<ScalarSums>", 2,
+  c2r_contract<T, true>(P, s, a, 10, 0, 10, 14, rows, fyy, tw, "k_y_c2r<ScalarSums>", 2,
                   [&](auto nct, auto band, unsigned grid, size_t lds, const TileGrid& tg) {
                     constexpr int NCT = decltype(nct)::value;
                     constexpr bool B = decltype(band)::value;
-                    launch_k<CTR_BS>(P, s, dim3(grid), dim3(CTR_BS), lds,
-                                     KBODY(k_c2r_sca_sums<T, NCT, CTR_BS, B>), a, (const c2<T>*)tw, tg);
+                    constexpr int BS = ctr_bs(NCT, true);
+                    launch_k<BS>(P, s, dim3(grid), dim3(BS), lds,
+                                 KBODY(k_c2r_sca_sums<T, NCT, BS, B>), a, (const c2<T>*)tw, tg);
                   });
 }

@@ -208,7 +208,23 @@ __host__ __device__ constexpr int radix_of(int m) {
   return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : 5;
 }
 
-template <class T, int R, int LPB, int N, int NS, int BS>
+// Bank-conflict-free layout of the radix-8 first pass's output.  The first
+// pass writes output m of butterfly j to row 8j + m.  The LDS serves a 64-bit
+// access 16 lanes at a time (a 128-bit one 8 at a time); with rows of L <= 8
+// 8-byte lines those lanes span several j, whose rows 8j + m lie a multiple
+// of the bank count apart, so they all hit the same banks.  Swizzled, row 8j
+// + m is stored at 8j + (m ^ (j & 3)): consecutive j land in different banks.
+// The pass that reads the first pass's output maps its rows the same way.
+// Rows stay inside their aligned group of 8, so every access that covers
+// whole groups of consecutive rows keeps its banks.  The per-lane row map
+// costs about ten VGPRs; measured at 256^3 it pays only in the fused
+// contraction kernels of the LAD and scalar procedures (2-5% faster) and
+// costs the k_x, y C2R and LASDD plane-sum kernels occupancy (3-21%
+// slower), so only the former use it.
+__device__ __forceinline__ int swz_row(int row) { return row ^ ((row >> 3) & 3); }
+// SWZ: the first pass's output is swizzled (written so by the NS = 1 pass,
+// read so by the NS = 8 pass)
+template <class T, int R, int LPB, int N, int NS, int BS, bool SWZ = false>
 __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sign) {
   constexpr int NR = N / R;
   constexpr int TOTAL = NR * LPB;
@@ -223,7 +239,8 @@ __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sig
       const int jm = j % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        c2<T> a = buf[(j + r * NR) * LPB + p];
+        const int row = j + r * NR;
+        c2<T> a = buf[((SWZ && NS == 8) ? swz_row(row) : row) * LPB + p];
         if (NS > 1 && r > 0) {
           c2<T> w = tw[jm * r * TSTEP];
           if (sign > 0) w.y = -w.y;
@@ -243,7 +260,10 @@ __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sig
       const int jm = j % NS;
       const int base = (j / NS) * NS * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(base + r * NS) * LPB + p] = v[q][r];
+      for (int r = 0; r < R; ++r) {
+        const int row = base + r * NS;
+        buf[((SWZ && NS == 1) ? swz_row(row) : row) * LPB + p] = v[q][r];
+      }
     }
   }
   __syncthreads();
@@ -358,7 +378,7 @@ __host__ __device__ constexpr bool band_r(int r) { return r < 2 || r > 5; }
 // nonzero elements of buf are read.  The butterfly is the full radix-8 one
 // with zeros in place, so the result is that of the full pass.  Ends with a
 // barrier, like a full pass.
-template <class T, int L, int N, int BS>
+template <class T, int L, int N, int BS, bool SWZ = false>
 __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
   constexpr int NR1 = N / 8, TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
   c2<T> v[IT][8];
@@ -385,7 +405,7 @@ __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = v[q][m];
+      for (int m = 0; m < 8; ++m) buf[(SWZ ? swz_row(8 * j + m) : 8 * j + m) * L + p] = v[q][m];
     }
   }
   __syncthreads();
@@ -505,30 +525,9 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
   TwStage<T, BS, (NCT > 0 ? NCT : MAXN)> tws;
   tws.load(tw, n, stw);
   const bool vv = NCT > 0 || vec;
-  int zt, x, b;
-  for (int it = 0; tg.tile(it, zt, x, b); ++it) {
-    __syncthreads();
-    if (vv) {
-      if constexpr (LoaderKinds<Loader>::value > 0) {
-        ld.with_kind(b, [&](auto kc) {
-          constexpr int K = decltype(kc)::value;
-          for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
-            const int y = e / ZT, z = zt * ZT + (e - y * ZT);
-            vst(bufr + e, z < nz ? ld.template vload_k<K>(b, x, y, z) : vzero<T>());
-          }
-        });
-      } else {
-        for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
-          const int y = e / ZT, z = zt * ZT + (e - y * ZT);
-          vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
-        }
-      }
-    } else {
-      for (int i = threadIdx.x; i < nEl; i += BS) {
-        const int y = i / ZT, z = zt * ZT + (i - y * ZT);
-        bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
-      }
-    }
+  // the transform of the staged tile (zt, x) of field b and the store of its
+  // first ncy_out columns
+  auto finish = [&](int zt, int x, int b) {
     tws.store(stw);
     __syncthreads();
     // band (compile-time lengths, ncy_out <= N/RL): the last pass is
@@ -558,6 +557,70 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
         if (z + 1 < nz) o[1] = r1;
       }
     }
+  };
+  int zt, x, b;
+  if constexpr (NCT > 0) {
+    // Compile-time lengths: thread t stages the NV vectors e0 + q*BS*W of the
+    // tile, all at the same z (BS*W is a multiple of the tile width ZT).
+    // Loaders of stored fields issue all NV loads at once, before the barrier
+    // that waits for the previous tile's output reads.  Loaders that form
+    // products (more registers per vector) load one vector at a time after
+    // the barrier: measured at 256^3, holding their NV vectors costs more in
+    // occupancy than the overlap gains.
+    constexpr int NEL = NCT * ZT, NV = (NEL + BS * W - 1) / (BS * W), YS = BS * W / ZT;
+    static_assert((BS * W) % ZT == 0, "a thread's vectors share one z");
+    const int e0 = threadIdx.x * W, y0 = e0 / ZT, zo = e0 - y0 * ZT;
+    auto in_tile = [&](int q) { return NEL % (BS * W) == 0 || e0 + q * BS * W < NEL; };
+    for (int it = 0; tg.tile(it, zt, x, b); ++it) {
+      if constexpr (LoaderKinds<Loader>::value > 0) {
+        __syncthreads();
+        ld.with_kind(b, [&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+#pragma unroll 1
+          for (int e = e0; e < NEL; e += BS * W) {
+            const int y = e / ZT, ze = zt * ZT + (e - y * ZT);
+            vst(bufr + e, ze < nz ? ld.template vload_k<K>(b, x, y, ze) : vzero<T>());
+          }
+        });
+      } else {
+        const int z = zt * ZT + zo;
+        Vec16<T> rq[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+          if (in_tile(q)) rq[q] = z < nz ? ld.vload(b, x, y0 + q * YS, z) : vzero<T>();
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+          if (in_tile(q)) vst(bufr + e0 + q * BS * W, rq[q]);
+      }
+      finish(zt, x, b);
+    }
+  } else {
+  for (int it = 0; tg.tile(it, zt, x, b); ++it) {
+    __syncthreads();
+    if (vv) {
+      if constexpr (LoaderKinds<Loader>::value > 0) {
+        ld.with_kind(b, [&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
+            const int y = e / ZT, z = zt * ZT + (e - y * ZT);
+            vst(bufr + e, z < nz ? ld.template vload_k<K>(b, x, y, z) : vzero<T>());
+          }
+        });
+      } else {
+        for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
+          const int y = e / ZT, z = zt * ZT + (e - y * ZT);
+          vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
+        }
+      }
+    } else {
+      for (int i = threadIdx.x; i < nEl; i += BS) {
+        const int y = i / ZT, z = zt * ZT + (i - y * ZT);
+        bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
+      }
+    }
+    finish(zt, x, b);
+  }
   }
 }
 

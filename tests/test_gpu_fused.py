@@ -5,8 +5,9 @@ kernel each; for LASDD (optSgs = 1) also the plane sums of both levels (one
 kernel per level and procedure) that give beta before the contractions.
 
 Both paths transform each filtered field by the same passes (but for length
-64, which the fused kernels factor as 8 x 4 x 2 instead of 8 x 8), so their
-float field values agree bitwise; the contractions differ only in how their double
+64, which the fused kernels factor as 8 x 4 x 2 instead of 8 x 8, and 80 in
+the scalar kernels, 8 x 5 x 2 instead of 8 x 2 x 5), so their float field
+values agree bitwise; the contractions differ only in how their double
 arithmetic is ordered (and MM, T_dn are accumulated in float), so LM, MM,
 T_up and T_dn agree to a few float roundings and the runs stay within float
 rounding of each other.  The grids cover every transform length the fused
@@ -33,7 +34,7 @@ FUSED_GRIDS = {
     "stable48": ("stable", dict(nx=48, ny=48, nz=48)),                 # band first pass, radix 3 last
     "neutral48": ("neutral", dict(nx=48, ny=48, nz=32)),               # FGR = 1: full first pass
     "stable64": ("stable", dict(nx=48, ny=64, nz=16)),                 # 8 x 4 x 2 in the fused kernels
-    "stable80": ("stable", dict(nx=48, ny=80, nz=16)),                 # radix-5 last pass
+    "stable80": ("stable", dict(nx=48, ny=80, nz=16)),                 # radix 5 (LAD), 2 (scalar, 320 threads)
     "stable96": ("stable", dict(nx=96, ny=96, nz=32)),
     "stable128": ("stable", dict(nx=64, ny=128, nz=16)),               # radix-2 last pass
     "stable192": ("stable", dict(nx=32, ny=192, nz=16)),
