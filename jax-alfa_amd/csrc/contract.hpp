@@ -63,13 +63,13 @@ __host__ __device__ constexpr int ctr_bs(int n, bool scalar) { return scalar && 
 // The passes between the first (radix 8, stride 1) and the last (radix RL):
 // radix_of(M / RL) at each step, which for every length but 64 is the
 // compile-time plan's own sequence (the same passes as k_y_c2r's).
-template <class T, int LPB, int N, int NS, int M, int RL, int BS, bool SWZ = false>
+template <class T, int LPB, int N, int NS, int M, int RL, int BS, int LAY = 0>
 struct CtrMid {
   static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
     if constexpr (M != RL) {
       constexpr int R = radix_of(M / RL);
-      fft_pass_ct<T, R, LPB, N, NS, BS, SWZ>(buf, tw, sign);
-      CtrMid<T, LPB, N, NS * R, M / R, RL, BS, SWZ && NS == 1>::run(buf, tw, sign);
+      fft_pass_ct<T, R, LPB, N, NS, BS, LAY>(buf, tw, sign);
+      CtrMid<T, LPB, N, NS * R, M / R, RL, BS, NS == 1 ? LAY : 0>::run(buf, tw, sign);
     }
   }
 };
@@ -82,6 +82,11 @@ template <int N, int BS> struct CtrGeom {
   static constexpr int L = BS / NS;
   static constexpr int LL = 2 * L;
   static constexpr bool OK = N > 0 && (BS % NS == 0) && R <= 5 && radix_of(N) == 8;
+  // first-pass layout: padded at rows of 8 lines (64 bytes), where the first
+  // pass's writes would conflict and a middle pass reads them; with 16 or
+  // more lines a 16-lane group of its writes covers one row
+  static constexpr int LAY = (N > 0 && N / 8 != R && LL <= 8) ? 2 : 0;
+  static constexpr int ROWS = N > 0 ? lay_rows(N, LAY) : 1;   // rows of the pair buffer
 };
 
 // An empty asm that the compiler must assume changes v: values derived from
@@ -137,18 +142,14 @@ __device__ __forceinline__ void ctr_stage2(c2<T>* buf, const c2<T> (&A1)[IN_IT],
 // every pass but the last over the LL lines, then the last (radix R) from
 // LDS into registers: o[h][r][0], o[h][r][1] = levels z, z + 1 of row
 // j + r NS of field h of the pair (line 2p + h), times sc
-// (SWZ_OK: the first pass's output is swizzled, where a middle pass reads it)
-template <class T, int N, int BS, bool BAND, bool SWZ_OK>
+template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int kmax, T sc,
                                             T (&o)[2][CtrGeom<N, BS>::R][2]) {
   using G = CtrGeom<N, BS>;
-  constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL;
-  // (with LL >= 16 a 16-lane group of the first pass's writes covers one
-  // row: no conflict to remove)
-  constexpr bool SWZ = SWZ_OK && N / 8 != R && LL <= 8;
-  if constexpr (BAND) band_first_pass_inv<T, LL, N, BS, SWZ>(buf, kmax);
-  else fft_pass_ct<T, 8, LL, N, 1, BS, SWZ>(buf, stw, +1);
-  CtrMid<T, LL, N, 8, N / 8, R, BS, SWZ>::run(buf, stw, +1);
+  constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL, LAY = G::LAY;
+  if constexpr (BAND) band_first_pass_inv<T, LL, N, BS, LAY>(buf, kmax);
+  else fft_pass_ct<T, 8, LL, N, 1, BS, LAY>(buf, stw, +1);
+  CtrMid<T, LL, N, 8, N / 8, R, BS, LAY>::run(buf, stw, +1);
   const int j = threadIdx.x / L, p = threadIdx.x % L;
   // line p of both fields (lines 2p, 2p + 1): one 16-byte read per r
   c2<T> v[2][R];
@@ -182,7 +183,7 @@ __device__ __forceinline__ void ctr_invert2(c2<T>* buf, const c2<T>* stw, int km
 // live).  Step s stages the loaded spectra of pair s (in[2s], in[2s+1], the
 // second possibly null), issues the loads of pair s + 1 (or of the next
 // tile's pair 0), inverts both fields and hands their values to use(o).
-template <class T, int N, int BS, bool BAND, int NP, bool SWZ = false>
+template <class T, int N, int BS, bool BAND, int NP>
 struct CtrDriver {
   using G = CtrGeom<N, BS>;
   static constexpr int R = G::R, NS = G::NS, L = G::L, LL = G::LL, ZT = 2 * L;
@@ -203,7 +204,7 @@ struct CtrDriver {
       : in(in_), tg(tg_), ncy_in(ncy_in_), nz(nz_) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     buf = reinterpret_cast<c2<T>*>(smem_raw);
-    stw = buf + N * LL;
+    stw = buf + G::ROWS * LL;
     kmax = imin(ncy_in, N / 2 + 1);
     sc = (T)scale;
     for (int i = threadIdx.x; i < N; i += BS) stw[i] = tw[i];
@@ -245,7 +246,7 @@ struct CtrDriver {
     }
     __syncthreads();
     T o[2][R][2];
-    ctr_invert2<T, N, BS, BAND, SWZ>(buf, stw, kmax, sc, o);
+    ctr_invert2<T, N, BS, BAND>(buf, stw, kmax, sc, o);
     use(o);
   }
 };
@@ -277,7 +278,7 @@ template <class T> struct LadArgs {
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void k_c2r_lad(const LadArgs<T>& a, const c2<T>* __restrict__ tw,
                                           const TileGrid& tg) {
-  using D = CtrDriver<T, N, BS, BAND, LAD_NP, true>;
+  using D = CtrDriver<T, N, BS, BAND, LAD_NP>;
   constexpr int R = D::R, NS = D::NS, L = D::L;
   const int nz = a.nz;
   const int j = threadIdx.x / L, p = threadIdx.x % L;
@@ -422,7 +423,7 @@ template <class T> struct ScaArgs {
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void k_c2r_sca(const ScaArgs<T>& a, const c2<T>* __restrict__ tw,
                                           const TileGrid& tg) {
-  using D = CtrDriver<T, N, BS, BAND, SCA_NP, true>;
+  using D = CtrDriver<T, N, BS, BAND, SCA_NP>;
   constexpr int R = D::R, NS = D::NS, L = D::L;
   const int nz = a.nz;
   const int j = threadIdx.x / L, p = threadIdx.x % L;
@@ -717,7 +718,7 @@ template <class T> struct ScaSumArgs {
 template <class T, int N, int BS, bool BAND>
 __device__ __forceinline__ void k_c2r_sca_sums(const ScaSumArgs<T>& a,
                                                const c2<T>* __restrict__ tw, const TileGrid& tg) {
-  using D = CtrDriver<T, N, BS, BAND, SCA_NP, true>;
+  using D = CtrDriver<T, N, BS, BAND, SCA_NP>;
   constexpr int R = D::R, NS = D::NS, L = D::L;
   const int nz = a.nz;
   const int j = threadIdx.x / L, p = threadIdx.x % L;
@@ -862,7 +863,7 @@ bool c2r_contract(alfa_plan* P, hipStream_t s, const Args& a, int nin, int nout_
         constexpr int L = CtrGeom<NCT, BS>::L, LL = CtrGeom<NCT, BS>::LL;
         TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, 1, 0};
         const unsigned grid = tile_grid((size_t)tg.ntz * rows);
-        const size_t lds = (size_t)NCT * LL * sizeof(c2<T>) + (size_t)NCT * sizeof(c2<T>) +
+        const size_t lds = (size_t)CtrGeom<NCT, BS>::ROWS * LL * sizeof(c2<T>) + (size_t)NCT * sizeof(c2<T>) +
                            (size_t)extra_lds * CtrGeom<NCT, BS>::R * 2 * BS * sizeof(T);
         const bool band = ALFA_BAND && a.ncy_in <= NCT / 4;
         if (band) kern_sel(std::integral_constant<int, NCT>{}, std::true_type{}, grid, lds, tg);

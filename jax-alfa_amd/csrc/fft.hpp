@@ -208,24 +208,41 @@ __host__ __device__ constexpr int radix_of(int m) {
   return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : 5;
 }
 
-// Bank-conflict-free layout of the radix-8 first pass's output.  The first
-// pass writes output m of butterfly j to row 8j + m.  The LDS serves a 64-bit
-// access 16 lanes at a time (a 128-bit one 8 at a time); with rows of L <= 8
-// 8-byte lines those lanes span several j, whose rows 8j + m lie a multiple
-// of the bank count apart, so they all hit the same banks.  Swizzled, row 8j
-// + m is stored at 8j + (m ^ (j & 3)): consecutive j land in different banks.
-// The pass that reads the first pass's output maps its rows the same way.
-// Rows stay inside their aligned group of 8, so every access that covers
-// whole groups of consecutive rows keeps its banks.  The per-lane row map
-// costs about ten VGPRs; measured at 256^3 it pays only in the fused
-// contraction kernels of the LAD and scalar procedures (2-5% faster) and
-// costs the k_x, y C2R and LASDD plane-sum kernels occupancy (3-21%
-// slower), so only the former use it.
-__device__ __forceinline__ int swz_row(int row) { return row ^ ((row >> 3) & 3); }
-// SWZ: the first pass's output is swizzled (written so by the NS = 1 pass,
-// read so by the NS = 8 pass)
-template <class T, int R, int LPB, int N, int NS, int BS, bool SWZ = false>
+// Layouts of the radix-8 first pass's output (LAY).  The first pass writes
+// output m of butterfly j to row 8j + m.  The LDS serves a 64-bit access 16
+// lanes at a time (a 128-bit one 8 at a time); with rows of at most 64 bytes
+// those lanes span several j, whose rows 8j + m lie a multiple of the bank
+// count apart, so they all hit the same banks (measured: 0.3 of the LDS
+// cycles of the y R2C and x kernels, 0.8 of the fused contraction kernels'
+// are conflict cycles).  The pass that reads the first pass's output maps
+// its rows the same way; later passes use plain rows.
+//   LAY 0: plain, row r at r.
+//   LAY 2: padded, row r at r + r/8 (a spare row after each group of 8):
+//          consecutive j land in different banks, and the addresses keep
+//          their constant offsets, so it costs no VGPRs; the buffer needs
+//          n/8 more rows.  (A swizzle within each group of 8 rows, m ^ (j &
+//          3), needs no extra LDS but a per-lane row map: about ten more
+//          VGPRs, which cost most of these kernels occupancy.)
+template <int LAY> __device__ __forceinline__ int lay_row(int row) {
+  return LAY == 2 ? row + (row >> 3) : row;
+}
+// rows of a transform buffer of length n under layout lay
+__host__ __device__ constexpr int lay_rows(int n, int lay) { return lay == 2 ? n + n / 8 : n; }
+// a plan of length n whose radix-8 first pass is read by a middle pass
+// (FftCTMid from NS = 1 runs both), or by any second pass (FftCT)
+__host__ __device__ constexpr bool mid_second(int n) {
+  return n > 8 && radix_of(n) == 8 && n / 8 != radix_of(n / 8);
+}
+__host__ __device__ constexpr bool ct_second(int n) { return n > 8 && radix_of(n) == 8; }
+// the layout of the y R2C and x kernels at compile-time length n with lines
+// of rowb bytes: padded where the first pass's writes conflict
+__host__ __device__ constexpr int fft_lay(int n, int rowb) { return (mid_second(n) && rowb <= 64) ? 2 : 0; }
+
+// LAY: the first pass's output layout (written so by the NS = 1 pass, read
+// so by the NS = 8 pass)
+template <class T, int R, int LPB, int N, int NS, int BS, int LAY = 0>
 __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sign) {
+  static_assert(LAY == 0 || NS != 1 || R == 8, "the first-pass layouts are for a radix-8 first pass");
   constexpr int NR = N / R;
   constexpr int TOTAL = NR * LPB;
   constexpr int MAXIT = (TOTAL + BS - 1) / BS;
@@ -239,8 +256,10 @@ __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sig
       const int jm = j % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int row = j + r * NR;
-        c2<T> a = buf[((SWZ && NS == 8) ? swz_row(row) : row) * LPB + p];
+        int row = j + r * NR;
+        if constexpr (LAY == 2 && NS == 8 && NR % 8 == 0) row = j + (j >> 3) + r * (NR + NR / 8);
+        else if constexpr (LAY != 0 && NS == 8) row = lay_row<LAY>(row);
+        c2<T> a = buf[row * LPB + p];
         if (NS > 1 && r > 0) {
           c2<T> w = tw[jm * r * TSTEP];
           if (sign > 0) w.y = -w.y;
@@ -261,37 +280,39 @@ __device__ __forceinline__ void fft_pass_ct(c2<T>* buf, const c2<T>* tw, int sig
       const int base = (j / NS) * NS * R + jm;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int row = base + r * NS;
-        buf[((SWZ && NS == 1) ? swz_row(row) : row) * LPB + p] = v[q][r];
+        int row = base + r * NS;
+        if constexpr (LAY == 2 && NS == 1) row = base + (base >> 3) + r;   // base = 8j
+        else if constexpr (LAY != 0 && NS == 1) row = lay_row<LAY>(row);
+        buf[row * LPB + p] = v[q][r];
       }
     }
   }
   __syncthreads();
 }
 
-template <class T, int LPB, int N, int NS, int M, int BS>
+template <class T, int LPB, int N, int NS, int M, int BS, int LAY = 0>
 struct FftCT {
   static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
     constexpr int R = radix_of(M);
-    fft_pass_ct<T, R, LPB, N, NS, BS>(buf, tw, sign);
-    FftCT<T, LPB, N, NS * R, M / R, BS>::run(buf, tw, sign);
+    fft_pass_ct<T, R, LPB, N, NS, BS, LAY>(buf, tw, sign);
+    FftCT<T, LPB, N, NS * R, M / R, BS, NS == 1 ? LAY : 0>::run(buf, tw, sign);
   }
 };
-template <class T, int LPB, int N, int NS, int BS>
-struct FftCT<T, LPB, N, NS, 1, BS> {
+template <class T, int LPB, int N, int NS, int BS, int LAY>
+struct FftCT<T, LPB, N, NS, 1, BS, LAY> {
   static __device__ __forceinline__ void run(c2<T>*, const c2<T>*, int) {}
 };
 
 // the passes of a compile-time plan after the first and before the last
 // (kernels that fuse the first pass into their loads and the last pass into
 // their stores run these in between)
-template <class T, int LPB, int N, int NS, int M, int BS>
+template <class T, int LPB, int N, int NS, int M, int BS, int LAY = 0>
 struct FftCTMid {
   static __device__ __forceinline__ void run(c2<T>* buf, const c2<T>* tw, int sign) {
     constexpr int R = radix_of(M);
     if constexpr (M != R) {
-      fft_pass_ct<T, R, LPB, N, NS, BS>(buf, tw, sign);
-      FftCTMid<T, LPB, N, NS * R, M / R, BS>::run(buf, tw, sign);
+      fft_pass_ct<T, R, LPB, N, NS, BS, LAY>(buf, tw, sign);
+      FftCTMid<T, LPB, N, NS * R, M / R, BS, NS == 1 ? LAY : 0>::run(buf, tw, sign);
     }
   }
 };
@@ -378,7 +399,7 @@ __host__ __device__ constexpr bool band_r(int r) { return r < 2 || r > 5; }
 // nonzero elements of buf are read.  The butterfly is the full radix-8 one
 // with zeros in place, so the result is that of the full pass.  Ends with a
 // barrier, like a full pass.
-template <class T, int L, int N, int BS, bool SWZ = false>
+template <class T, int L, int N, int BS, int LAY = 0>
 __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
   constexpr int NR1 = N / 8, TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
   c2<T> v[IT][8];
@@ -405,7 +426,8 @@ __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) buf[(SWZ ? swz_row(8 * j + m) : 8 * j + m) * L + p] = v[q][m];
+      for (int m = 0; m < 8; ++m)
+        buf[(LAY == 2 ? 9 * j + m : lay_row<LAY>(8 * j + m)) * L + p] = v[q][m];
     }
   }
   __syncthreads();
@@ -413,9 +435,9 @@ __device__ __forceinline__ void band_first_pass_inv(c2<T>* buf, int kmax) {
 
 // transform of LPB lines by BS threads: compile-time length NCT, or f.n when
 // NCT == 0 (the generic passes run with FFT_BS threads)
-template <class T, int LPB, int MAXN, int NCT, int BS = FFT_BS>
+template <class T, int LPB, int MAXN, int NCT, int BS = FFT_BS, int LAY = 0>
 __device__ __forceinline__ void fft_any(c2<T>* buf, const c2<T>* tw, const Fft1D& f, int sign) {
-  if constexpr (NCT > 0) FftCT<T, LPB, NCT, 1, NCT, BS>::run(buf, tw, sign);
+  if constexpr (NCT > 0) FftCT<T, LPB, NCT, 1, NCT, BS, ct_second(NCT) ? LAY : 0>::run(buf, tw, sign);
   else fft_lines<T, LPB, MAXN>(buf, tw, f, sign);
 }
 
@@ -517,10 +539,12 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
   constexpr int L = Lines<T>::L * BS / FFT_BS;
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
+  // padded first-pass rows where its writes would conflict
+  constexpr int LAY = NCT > 0 ? fft_lay(NCT, L * (int)sizeof(c2<T>)) : 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = NCT > 0 ? NCT : fy.n, nEl = n * ZT;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
-  c2<T>* stw = buf + n * L;
+  c2<T>* stw = buf + lay_rows(n, LAY) * L;
   T* bufr = reinterpret_cast<T*>(buf);
   TwStage<T, BS, (NCT > 0 ? NCT : MAXN)> tws;
   tws.load(tw, n, stw);
@@ -533,10 +557,10 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
     // band (compile-time lengths, ncy_out <= N/RL): the last pass is
     // evaluated only at the outputs k < ncy_out and n - k that the store reads
     if constexpr (BAND) {
-      FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw, -1);
+      FftCTMid<T, L, NCT, 1, NCT, BS, LAY>::run(buf, stw, -1);
       fwd_last_pass_band<T, L, NCT, BS>(buf, stw, ncy_out);
     } else {
-      fft_any<T, L, MAXN, NCT, BS>(buf, stw, fy, -1);
+      fft_any<T, L, MAXN, NCT, BS, LAY>(buf, stw, fy, -1);
     }
     // line p holds z0+2p (real part) and z0+2p+1 (imaginary part)
     c2<T>* out = reinterpret_cast<c2<T>*>(outb.p[b]);
@@ -742,7 +766,7 @@ __device__ __forceinline__ void x_store(const c2<T>* buf, void* dstp, int ncy, i
 // outputs at the modes k = j + r N/8 (r = 0, 1, 6, 7) the mask keeps, applies
 // the op and runs the radix-8 inverse butterfly j on them (zeros elsewhere,
 // as the full path has after the mask).  Ends with a barrier.
-template <class T, int L, int N, int BS>
+template <class T, int L, int N, int BS, int LAY = 0>
 __device__ __forceinline__ void x_band_mid(c2<T>* buf, const c2<T>* tw, const XOp& op, int kyv) {
   constexpr int RL = last_radix(N), NSL = N / RL, NR1 = N / 8;
   constexpr int TOT = NR1 * L, IT = (TOT + BS - 1) / BS;
@@ -777,7 +801,8 @@ __device__ __forceinline__ void x_band_mid(c2<T>* buf, const c2<T>* tw, const XO
     if (TOT % BS == 0 || it < TOT) {
       const int j = it / L, p = it % L;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) buf[(8 * j + m) * L + p] = v[q][m];
+      for (int m = 0; m < 8; ++m)
+        buf[(LAY == 2 ? 9 * j + m : lay_row<LAY>(8 * j + m)) * L + p] = v[q][m];
     }
   }
   __syncthreads();
@@ -799,14 +824,16 @@ k_x(const XArgs& a, const Fft1D& fin, const Fft1D& fout, const c2<T>* __restrict
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nin = NCT > 0 ? NCT : fin.n;   // NCT is set only when fin.n == fout.n
   const int nbuf = NCT > 0 ? NCT : imax(nin, fout.n);
+  // padded first-pass rows where its writes would conflict
+  constexpr int LAY = NCT > 0 ? fft_lay(NCT, L * (int)sizeof(c2<T>)) : 0;
   c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
-  c2<T>* stw_in = buf + nbuf * L;
-  c2<T>* stw_out = stw_in + nin;
+  c2<T>* stw_in = buf + lay_rows(nbuf, LAY) * L;
+  c2<T>* stw_out = stw_in + nin;   // (lengths without a compile-time plan only)
   const int nz = a.nz;
   const int nEl = nin * L;
   TwStage<T, BS, (NCT > 0 ? NCT : MAXN)> tws;
   tws.load(tw_in, nin, stw_in);
-  if (MULTI)
+  if (MULTI && NCT == 0)
     for (int i = threadIdx.x; i < fout.n; i += BS) stw_out[i] = tw_out[i];
   int zt, ky, b;
   for (int it = 0; tg.tile(it, zt, ky, b); ++it) {
@@ -819,18 +846,18 @@ k_x(const XArgs& a, const Fft1D& fin, const Fft1D& fout, const c2<T>* __restrict
     if constexpr (BAND) {
       // (chosen by the launcher) x-space input, one op with the identity
       // map, mask c <= min(N/4, N/RL), inverse transform
-      FftCTMid<T, L, NCT, 1, NCT, BS>::run(buf, stw_in, -1);
-      x_band_mid<T, L, NCT, BS>(buf, stw_in, a.op[0], kyv);
-      FftCT<T, L, NCT, 8, NCT / 8, BS>::run(buf, stw_in, +1);
+      FftCTMid<T, L, NCT, 1, NCT, BS, LAY>::run(buf, stw_in, -1);
+      x_band_mid<T, L, NCT, BS, LAY>(buf, stw_in, a.op[0], kyv);
+      FftCT<T, L, NCT, 8, NCT / 8, BS, LAY>::run(buf, stw_in, +1);
       x_store<T, L, BS>(buf, a.dst[b][0], a.op[0].ncy, nz, nin, zt, ky, a.vec);
       continue;
     }
-    if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, -1);
+    if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS, LAY>(buf, stw_in, fin, -1);
     if (!MULTI) {
       const XOp op = a.op[0];
       for (int i = threadIdx.x; i < nEl; i += BS) buf[i] = x_apply<T>(op, buf[i], i / L, nin, kyv);
       __syncthreads();
-      if (op.inverse) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, +1);
+      if (op.inverse) fft_any<T, L, MAXN, NCT, BS, LAY>(buf, stw_in, fin, +1);
       x_store<T, L, BS>(buf, a.dst[b][0], op.ncy, nz, nin, zt, ky, a.vec);
     } else {
       c2<T> spec[MAXE];
@@ -869,7 +896,7 @@ k_x(const XArgs& a, const Fft1D& fin, const Fft1D& fout, const c2<T>* __restrict
         }
         __syncthreads();
         if (op.inverse) {
-          if (NCT > 0 || op.map == XMAP_ID) fft_any<T, L, MAXN, NCT, BS>(buf, stw_in, fin, +1);
+          if (NCT > 0 || op.map == XMAP_ID) fft_any<T, L, MAXN, NCT, BS, LAY>(buf, stw_in, fin, +1);
           else fft_any<T, L, MAXN, NCT, BS>(buf, stw_out, fout, +1);
         }
         x_store<T, L, BS>(buf, a.dst[b][o], op.ncy, nz, nout, zt, ky, a.vec);

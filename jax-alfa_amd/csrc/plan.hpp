@@ -246,7 +246,9 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
       constexpr int L = Lines<T>::L * BS / FFT_BS;
       TileGrid tg{(nz + 2 * L - 1) / (2 * L), rows, nb, Loader::BATCH_FAST};
       const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
-      const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
+      // (the kernel's first-pass layout: padded rows where they conflict)
+      constexpr int LAY = NCT > 0 ? fft_lay(NCT, L * (int)sizeof(c2<T>)) : 0;
+      const size_t lds = (size_t)lay_rows(fyy.n, LAY) * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
       // pruned output within the stride of the last radix pass: that pass is
       // evaluated only at the outputs the store reads
       if constexpr (NCT > 0) {
@@ -277,8 +279,11 @@ void launch_kx_bs(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb,
   TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
   const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
   const int nbuf = imax(fin.n, fout.n);
-  const size_t lds =
-      (size_t)nbuf * L * sizeof(c2<T>) + (size_t)(fin.n + fout.n) * sizeof(c2<T>);
+  // the kernel's first-pass layout (padded rows where they conflict); at a
+  // compile-time length only the input twiddles are staged
+  constexpr int LAY = NCT > 0 ? fft_lay(NCT, L * (int)sizeof(c2<T>)) : 0;
+  const size_t lds = (size_t)lay_rows(nbuf, LAY) * L * sizeof(c2<T>) +
+                     (size_t)(NCT > 0 ? fin.n : fin.n + fout.n) * sizeof(c2<T>);
   // one masked op between the forward and inverse transforms, with the
   // kept modes within twice the stride of the radix-8 first pass and within
   // the stride of the last pass: band passes (float only: the double
