@@ -128,8 +128,9 @@ int alfa_plan_destroy(alfa_plan* plan);
 int alfa_plan_derived(const alfa_plan* plan, alfa_derived* out);
 const char* alfa_last_error(void);
 /* Tile choices of the compile-time-length FFT kernels (transform lengths
- * 48, 64, 80, 96, 128, 192, 256): threads per workgroup of the y R2C, x and y C2R
- * kernels, in that order, each 256 or 512.  The choice changes timing only:
+ * 128, 192, 256; lengths 48 to 96 always run 128-thread kernels): threads
+ * per workgroup of the y R2C, x and y C2R kernels, in that order, each 256
+ * or 512.  The choice changes timing only:
  * every line is transformed by the same operations, so results are bitwise
  * equal.  No counterpart in the reference (jnp.fft.rfft2/irfft2 leave such
  * choices to XLA's cuFFT plans); they exist for callers that reuse a plan

@@ -77,6 +77,18 @@ constexpr int FFT_MAXRAD = 8;
 #define ALFA_BAND 1
 #endif
 template <class T> struct Lines { static constexpr int L = ALFA_LPB * 8 / (int)sizeof(c2<T>); };
+// Lines per tile of an FFT kernel with BS threads: L at 256 threads, in
+// proportion above (512 threads: 128-byte rows); below 256 (the 128-thread
+// kernels of lengths up to FFT_SMALL_N) the rows stay 64 bytes and each
+// thread takes more of the passes' items.
+template <class T> constexpr int fft_lines(int bs) { return Lines<T>::L * (bs > FFT_BS ? bs / FFT_BS : 1); }
+// Compile-time lengths up to this run their y R2C, x and y C2R kernels on
+// 128 threads: at 256 most of a short transform's threads idle in its
+// passes (length 96 at 8 lines: 96 radix-8 butterflies for 256 threads),
+// while 128-thread workgroups fit twice as many tiles per CU.  Measured on
+// one-grid ensembles (convective, mixed): 48^3 +8.5%, 64^3 +10%, 80^3 +4%,
+// 96^3 +3%, but 128^3 -2%, which keeps the plan's tile choice.
+constexpr int FFT_SMALL_N = 96;
 
 struct Fft1D {
   int n;

@@ -536,7 +536,7 @@ template <class T, class Loader, int MAXN, int NCT, int BS, bool BAND = false>
 __device__ __forceinline__ void
 k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz, int ncy_out, const PtrBatch& outb,
         const TileGrid& tg, int vec) {
-  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int L = fft_lines<T>(BS);
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
   // padded first-pass rows where its writes would conflict
@@ -817,7 +817,7 @@ template <class T, int MAXN, bool MULTI, int NCT, int BS, bool BAND = false>
 __device__ __forceinline__ void
 k_x(const XArgs& a, const Fft1D& fin, const Fft1D& fout, const c2<T>* __restrict__ tw_in,
     const c2<T>* __restrict__ tw_out, const TileGrid& tg) {
-  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int L = fft_lines<T>(BS);
   // spectrum elements per thread in the multi-op path: rounded up, since
   // MAXN * L need not be a multiple of BS (96 double lines at 256 threads)
   constexpr int MAXE = (MAXN * L + BS - 1) / BS;
@@ -957,7 +957,7 @@ template <class T, class Storer, int N, int BS>
 __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC2RArgs& a,
                                                       const c2<T>* buf, const c2<T>* stw, int z0,
                                                       int x, int b) {
-  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int L = fft_lines<T>(BS);
   constexpr int R = last_radix(N), NS = N / R, TOT = NS * L, IT = (TOT + BS - 1) / BS;
   const T sc = (T)a.scale;
   if constexpr (sizeof(T) == 4 && L % 2 == 0 && ALFA_C2R_PAIRS) {
@@ -1028,7 +1028,7 @@ __device__ __forceinline__ void y_c2r_last_pass_store(const Storer& st, const YC
 template <class T, class Storer, int MAXN, int NCT, int BS, bool FUSED = false, bool BAND = false>
 __device__ __forceinline__ void
 k_y_c2r(const Storer& st, const YC2RArgs& a, const Fft1D& fy, const c2<T>* __restrict__ tw, const TileGrid& tg) {
-  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int L = fft_lines<T>(BS);
   constexpr int ZT = 2 * L;
   constexpr int W = Vec16<T>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];

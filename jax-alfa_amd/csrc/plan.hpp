@@ -243,7 +243,7 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
     constexpr int NCT = decltype(nct)::value;
     auto launch = [&](auto bs) {
       constexpr int BS = decltype(bs)::value;
-      constexpr int L = Lines<T>::L * BS / FFT_BS;
+      constexpr int L = fft_lines<T>(BS);
       TileGrid tg{(nz + 2 * L - 1) / (2 * L), rows, nb, Loader::BATCH_FAST};
       const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
       // (the kernel's first-pass layout: padded rows where they conflict)
@@ -259,7 +259,9 @@ void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D&
       }
       launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_r2c<T, Loader, decltype(maxn)::value, NCT, BS>), ld, fyy, (const c2<T>*)tw, nz, ncy, pb, tg, vec ? 1 : 0);
     };
-    if constexpr (NCT > 0) {
+    if constexpr (NCT > 0 && NCT <= FFT_SMALL_N) {
+      launch(std::integral_constant<int, 128>{});
+    } else if constexpr (NCT > 0) {
       if (P->tile_r2c == 512) launch(std::integral_constant<int, 512>{});
       else launch(std::integral_constant<int, 256>{});
     } else {
@@ -275,7 +277,7 @@ void launch_kx_bs(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb,
                   const Fft1D& fout, const void* twin, const void* twout, NCTT, MAXNT) {
   constexpr int NCT = NCTT::value;
   using maxn_t = MAXNT;
-  constexpr int L = Lines<T>::L * BS / FFT_BS;
+  constexpr int L = fft_lines<T>(BS);
   TileGrid tg{(a.nz + L - 1) / L, rows, nb, 0};
   const unsigned grid = tile_grid((size_t)tg.ntz * tg.rows * nb);
   const int nbuf = imax(fin.n, fout.n);
@@ -308,7 +310,9 @@ void launch_kx(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb, co
   const int tier = imax(fin.n, fout.n);
   auto go = [&](auto nct, auto maxn) {
     constexpr int NCT = decltype(nct)::value;
-    if constexpr (NCT > 0 && !MULTI) {
+    if constexpr (NCT > 0 && NCT <= FFT_SMALL_N) {
+      launch_kx_bs<T, MULTI, 128>(P, s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
+    } else if constexpr (NCT > 0 && !MULTI) {
       if (P->tile_x == 512)
         launch_kx_bs<T, MULTI, 512>(P, s, a, rows, nb, fin, fout, twin, twout, nct, maxn);
       else
@@ -365,7 +369,7 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
     auto launch = [&](auto bs) {
       // compile-time lengths: tile_c2r threads and lines in proportion
       constexpr int BS = decltype(bs)::value;
-      constexpr int L = Lines<T>::L * BS / FFT_BS;
+      constexpr int L = fft_lines<T>(BS);
       TileGrid tg{(a.nz + 2 * L - 1) / (2 * L), rows, nb, 0};
       const unsigned grid = tile_grid((size_t)tg.ntz * rows * nb);
       const size_t lds = (size_t)fyy.n * L * sizeof(c2<T>) + (size_t)fyy.n * sizeof(c2<T>);
@@ -391,7 +395,9 @@ void y_c2r(alfa_plan* P, hipStream_t s, const Storer& st, const YC2RArgs& a_in, 
       }
       launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_y_c2r<T, Storer, decltype(maxn)::value, NCT, BS>), st, a, fyy, (const c2<T>*)tw, tg);
     };
-    if constexpr (NCT > 0) {
+    if constexpr (NCT > 0 && NCT <= FFT_SMALL_N) {
+      launch(std::integral_constant<int, 128>{});
+    } else if constexpr (NCT > 0) {
       if (P->tile_c2r == 256) launch(std::integral_constant<int, 256>{});
       else launch(std::integral_constant<int, 512>{});
     } else {

@@ -1,7 +1,8 @@
 """Tile choices of the compile-time-length FFT kernels (alfa_plan_set_tiles,
 alfa_plan_autotune): 256 or 512 threads per workgroup for the y R2C, x and
-y C2R kernels.  Every transform line goes through the same operations at
-either size, so whole steps must agree bitwise across the choices."""
+y C2R kernels of lengths 128 to 256 (lengths up to 96 always run their
+128-thread kernels).  Every transform line goes through the same operations
+at either size, so whole steps must agree bitwise across the choices."""
 import ctypes as C
 
 import numpy as np
@@ -16,9 +17,12 @@ if not torch.cuda.is_available():
 from alfa_amd import _lib, cases, core  # noqa: E402
 
 # compile-time lengths 64 x 96 (FGR=2: band-limited passes at the filter
-# cut-offs) and 128 x 128 at FGR=1 (dealiasing pads to 192 x 192)
+# cut-offs; 128-thread kernels whatever the choice), 64 x 256 (the y
+# kernels at the benchmark's length) and 128 x 128 at FGR=1 (dealiasing pads
+# to 192 x 192)
 SETUPS = {
     "convective64x96": ("convective", dict(nx=64, ny=96, nz=16)),
+    "stable64x256": ("stable", dict(nx=64, ny=256, nz=16)),
     "neutral128": ("neutral", dict(nx=128, ny=128, nz=16)),
 }
 
