@@ -12,7 +12,7 @@
 // field of one tile in turn and folds each into per-point state held in
 // registers, so only the fields that later stages read reach HBM.
 //
-// Tile: BS = 256 threads (320 for the scalar kernels at length 80), L lines (2L z levels, two per complex line) of one
+// Tile: BS = 256 threads (others at lengths 48 and 80, ctr_bs), L lines (2L z levels, two per complex line) of one
 // x row, with L = BS R / N so that the last radix pass (radix R, stride N/R)
 // is one butterfly per thread and field: thread (j, p) ends every field
 // holding rows y = j + r N/R (r < R) of the level pair (z0 + 2p, z0 + 2p + 1),
@@ -52,13 +52,20 @@
 __host__ __device__ constexpr int ctr_last_radix(int n, int bs) {
   return n == 64 || (n == 80 && bs == 320) ? 2 : last_radix(n);
 }
-// threads per workgroup of the fused kernels at length N: 256, but 320 (8
-// lines of the radix-2 last pass) for the scalar kernels at 80, whose state
-// at 10 points per thread needs 250+ VGPRs (1 wave per SIMD; 119-140 at 4).
-// Measured on the 64-member ensemble: the scalar contraction and sums at 80
-// run 15-23% faster at 320; the momentum ones 34-47% slower (twice the
-// tiles, half the threads idle in the first pass), so they keep 256.
-__host__ __device__ constexpr int ctr_bs(int n, bool scalar) { return scalar && n == 80 ? 320 : 256; }
+// threads per workgroup of the fused kernels at length N: 256, except
+//  - 320 (8 lines of the radix-2 last pass) for the scalar kernels at 80,
+//    whose state at 10 points per thread needs 250+ VGPRs (1 wave per SIMD;
+//    119-140 at 4).  Measured on the 64-member ensemble: the scalar
+//    contraction and sums at 80 run 15-23% faster at 320; the momentum ones
+//    34-47% slower (twice the tiles, half the threads idle in the first
+//    pass);
+//  - 128 for the momentum kernels at 80 and all of them at 48 (8 lines, 16
+//    levels per tile instead of 16 lines, 32 levels): grids of 48 or 80
+//    levels then fill their tiles exactly instead of computing 17-25% of
+//    padding levels.
+__host__ __device__ constexpr int ctr_bs(int n, bool scalar) {
+  return n == 48 ? 128 : n == 80 ? (scalar ? 320 : 128) : 256;
+}
 
 // The passes between the first (radix 8, stride 1) and the last (radix RL):
 // radix_of(M / RL) at each step, which for every length but 64 is the
