@@ -136,8 +136,8 @@ const char* alfa_last_error(void);
  * choices to XLA's cuFFT plans); they exist for callers that reuse a plan
  * for many steps (ensembles, serving).
  *   alfa_plan_autotune: times each kernel's two choices, one kernel at a
- *   time, on `stream`, keeps the faster and writes the kept choices to
- *   `chosen` (nullable).  With a state `st`, it times whole steps
+ *   time, on `stream`, switches to the other choice where it is at least 1%
+ *   faster and writes the kept choices to `chosen` (nullable).  With a state `st`, it times whole steps
  *   (iterations `iteration`, `iteration`+1, ..., surface value `sfc_val`)
  *   on a scratch copy of it: the caller's state is not modified.  With st
  *   NULL it times a forward transform and three inverses (an x derivative,

@@ -949,7 +949,11 @@ int autotune_tiles(alfa_plan* P, hipStream_t s, const alfa_state* st, int iterat
           rc = w.time(e0, e1, ms);
           t[c] += ms;
         }
-      *knob[k] = rc == ALFA_OK ? (t[1] < t[0] ? 512 : 256) : keep;
+      // the other size must win by 1%: whole steps at either size are
+      // often within run-to-run noise, and the kept choice then does not
+      // flip from run to run
+      const float t_keep = t[keep == 512 ? 1 : 0], t_other = t[keep == 512 ? 0 : 1];
+      *knob[k] = (rc == ALFA_OK && t_other < 0.99f * t_keep) ? (keep == 512 ? 256 : 512) : keep;
     }
     e = hipStreamSynchronize(s);
   }
