@@ -620,31 +620,31 @@ k_y_r2c(const Loader& ld, const Fft1D& fy, const c2<T>* __restrict__ tw, int nz,
       finish(zt, x, b);
     }
   } else {
-  for (int it = 0; tg.tile(it, zt, x, b); ++it) {
-    __syncthreads();
-    if (vv) {
-      if constexpr (LoaderKinds<Loader>::value > 0) {
-        ld.with_kind(b, [&](auto kc) {
-          constexpr int K = decltype(kc)::value;
+    for (int it = 0; tg.tile(it, zt, x, b); ++it) {
+      __syncthreads();
+      if (vv) {
+        if constexpr (LoaderKinds<Loader>::value > 0) {
+          ld.with_kind(b, [&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
+              const int y = e / ZT, z = zt * ZT + (e - y * ZT);
+              vst(bufr + e, z < nz ? ld.template vload_k<K>(b, x, y, z) : vzero<T>());
+            }
+          });
+        } else {
           for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
             const int y = e / ZT, z = zt * ZT + (e - y * ZT);
-            vst(bufr + e, z < nz ? ld.template vload_k<K>(b, x, y, z) : vzero<T>());
+            vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
           }
-        });
+        }
       } else {
-        for (int e = threadIdx.x * W; e < nEl; e += BS * W) {
-          const int y = e / ZT, z = zt * ZT + (e - y * ZT);
-          vst(bufr + e, z < nz ? ld.vload(b, x, y, z) : vzero<T>());
+        for (int i = threadIdx.x; i < nEl; i += BS) {
+          const int y = i / ZT, z = zt * ZT + (i - y * ZT);
+          bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
         }
       }
-    } else {
-      for (int i = threadIdx.x; i < nEl; i += BS) {
-        const int y = i / ZT, z = zt * ZT + (i - y * ZT);
-        bufr[i] = (z < nz) ? ld(b, x, y, z) : T(0);
-      }
+      finish(zt, x, b);
     }
-    finish(zt, x, b);
-  }
   }
 }
 
