@@ -843,13 +843,46 @@ k_x(const XArgs& a, const Fft1D& fin, const Fft1D& fout, const c2<T>* __restrict
     tws.store(stw_in);
     __syncthreads();
     const int kyv = (2 * ky == a.ny_full) ? 0 : ky;
-    if constexpr (BAND) {
+    if constexpr (BAND && !MULTI) {
       // (chosen by the launcher) x-space input, one op with the identity
       // map, mask c <= min(N/4, N/RL), inverse transform
       FftCTMid<T, L, NCT, 1, NCT, BS, LAY>::run(buf, stw_in, -1);
       x_band_mid<T, L, NCT, BS, LAY>(buf, stw_in, a.op[0], kyv);
       FftCT<T, L, NCT, 8, NCT / 8, BS, LAY>::run(buf, stw_in, +1);
       x_store<T, L, BS>(buf, a.dst[b][0], a.op[0].ncy, nz, nin, zt, ky, a.vec);
+      continue;
+    }
+    if constexpr (BAND && MULTI) {
+      // (chosen by the launcher) the same for several such ops: the forward
+      // passes but the last run once, and their output, held in registers,
+      // feeds each op's band middle (last forward pass at the kept modes,
+      // op, first inverse pass) and inverse passes
+      constexpr int ME = (NCT * L + BS - 1) / BS;
+      FftCTMid<T, L, NCT, 1, NCT, BS, LAY>::run(buf, stw_in, -1);
+      c2<T> mid[ME];
+#pragma unroll
+      for (int q = 0; q < ME; ++q) {
+        const int i = threadIdx.x + q * BS;
+        mid[q] = (NCT * L % BS == 0 || i < NCT * L) ? buf[i] : mkc<T>(0, 0);
+      }
+      bool used = false;   // buf no longer holds the forward passes' output
+      for (int o = 0; o < a.nops; ++o) {
+        const XOp op = a.op[o];
+        if (ky >= op.ncy) continue;
+        if (used) {
+          __syncthreads();   // the previous op's store has read buf
+#pragma unroll
+          for (int q = 0; q < ME; ++q) {
+            const int i = threadIdx.x + q * BS;
+            if (NCT * L % BS == 0 || i < NCT * L) buf[i] = mid[q];
+          }
+          __syncthreads();
+        }
+        x_band_mid<T, L, NCT, BS, LAY>(buf, stw_in, op, kyv);
+        FftCT<T, L, NCT, 8, NCT / 8, BS, LAY>::run(buf, stw_in, +1);
+        x_store<T, L, BS>(buf, a.dst[b][o], op.ncy, nz, nin, zt, ky, a.vec);
+        used = true;
+      }
       continue;
     }
     if (!a.in_spectral) fft_any<T, L, MAXN, NCT, BS, LAY>(buf, stw_in, fin, -1);

@@ -286,15 +286,19 @@ void launch_kx_bs(alfa_plan* P, hipStream_t s, const XArgs& a, int rows, int nb,
   constexpr int LAY = NCT > 0 ? fft_lay(NCT, L * (int)sizeof(c2<T>)) : 0;
   const size_t lds = (size_t)lay_rows(nbuf, LAY) * L * sizeof(c2<T>) +
                      (size_t)(NCT > 0 ? fin.n : fin.n + fout.n) * sizeof(c2<T>);
-  // one masked op between the forward and inverse transforms, with the
+  // masked ops between the forward and inverse transforms, each with its
   // kept modes within twice the stride of the radix-8 first pass and within
   // the stride of the last pass: band passes (float only: the double
   // kernel needs 14 more VGPRs for them and loses a wave per SIMD)
-  if constexpr (NCT > 0 && !MULTI && sizeof(T) == 4) {
-    const XOp& o = a.op[0];
-    if (ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops == 1 && o.inverse &&
-        o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 4 &&
-        o.cx <= NCT / last_radix(NCT)) {
+  if constexpr (NCT > 0 && sizeof(T) == 4) {
+    bool band = ALFA_BAND && radix_of(NCT) == 8 && !a.in_spectral && a.nops >= 1 &&
+                (MULTI || a.nops == 1);
+    for (int k = 0; k < a.nops && band; ++k) {
+      const XOp& o = a.op[k];
+      band = o.inverse && o.map == XMAP_ID && o.nout == NCT && o.cx >= 1 && o.cx <= NCT / 4 &&
+             o.cx <= NCT / last_radix(NCT);
+    }
+    if (band) {
       launch_k<BS>(P, s, dim3(grid), dim3(BS), lds, KBODY(k_x<T, maxn_t::value, MULTI, NCT, BS, true>), a, fin, fout, (const c2<T>*)twin, (const c2<T>*)twout, tg);
       return;
     }
