@@ -217,8 +217,6 @@ template <class F> inline void ct_length(int n, F&& f) {
   }
 }
 
-template <class T> inline size_t fft_lds(int n) { return (size_t)n * Lines<T>::L * sizeof(c2<T>) + (size_t)n * sizeof(c2<T>); }
-
 // y R2C of nb fields, rows x [0, rows), transform length fyy.n, output pitch ncy.
 template <class T, class Loader>
 void y_r2c(alfa_plan* P, hipStream_t s, const Loader& ld, int rows, const Fft1D& fyy,
