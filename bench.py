@@ -201,7 +201,7 @@ def cpu_baseline():
                 sample=f"FP64 NumPy/SciPy oracle, stable layer 128^3 (LAD-SM, FGR=2), {k128} "
                        f"steps in {t128:.1f} s from the initial state; scipy.fft workers="
                        f"{workers}, other NumPy work single-threaded",
-                config0=dict(value=round(v32, 4), steps=k32, seconds=round(t32, 1),
+                neutral32=dict(value=round(v32, 4), steps=k32, seconds=round(t32, 1),
                              workload="neutral 32^3, LAD-SM, FGR=1 (dealiased), dt=0.5 s"))
 
 

@@ -67,7 +67,7 @@ class MemberSpec:
 
 
 def ensemble_specs(nmembers=64, sizes=(48, 64, 80, 96), steps=500, seed=0):
-    """Config 3: convective physics, per-member Ug in [2, 12] m/s and surface
+    """The ensemble workload: convective physics, per-member Ug in [2, 12] m/s and surface
     heat flux in [0.05, 0.3] K m/s (homogeneous flux), grids drawn from sizes."""
     rng = np.random.default_rng(seed)
     out = []
@@ -81,7 +81,7 @@ def ensemble_specs(nmembers=64, sizes=(48, 64, 80, 96), steps=500, seed=0):
 
 
 def serving_requests(nreq=40, rate=2.0, sizes=(32, 64, 128, 192), steps=(200, 2000), seed=0):
-    """Config 4: a Poisson stream (rate per second) of requests mixing the
+    """The serving workload: a Poisson stream (rate per second) of requests mixing the
     neutral, convective and stable setups with seeded forcing."""
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -274,7 +274,7 @@ class MemberGroup:
 
 def run_ensemble_grouped(specs: List[MemberSpec], steps=None, precision="mixed", graphs=True,
                          warmup=3, keep_fields=False, parallel=True):
-    """Config 3 with shared launches: members are grouped by configuration
+    """An ensemble with shared launches: members are grouped by configuration
     shape (grid, precision, options) and each group's step is one launch
     per kernel over its members; after one eager step of each kind the
     whole ensemble step (every group) is captured into one hipGraph per
@@ -382,7 +382,7 @@ def run_ensemble_grouped(specs: List[MemberSpec], steps=None, precision="mixed",
 def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mixed",
                  graphs=True, log=None, warmup=2, keep_fields=False):
     """Step all members together for `steps` steps (default: each spec's
-    own length, all equal in config 3), each member with its own launches
+    own length, all equal in the ensemble workload), each member with its own launches
     and step graph on one of nstreams streams.  Returns throughput figures,
     and with keep_fields each member's final u, v, w, TH and CFL (host
     copies) under "fields"."""

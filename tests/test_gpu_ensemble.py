@@ -86,7 +86,7 @@ def test_small_ensemble_runs():
             assert np.array_equal(got[k], ref[k]), (spec.n, k)
 
 
-def test_config3_ensemble_at_full_size():
+def test_ensemble_64_members_at_full_size():
     """The batched-ensemble configuration at its stated size: 64 convective
     members on grids drawn from 48/64/80/96^3 with seeded Ug in [2, 12] m/s
     and surface heat flux in [0.05, 0.3] K m/s, mixed precision, stepped
@@ -171,7 +171,7 @@ def test_batch_step_equals_member_steps():
             assert np.array_equal(a.host(k), b.host(k)), k
 
 
-def test_config3_grouped_ensemble_at_full_size():
+def test_grouped_ensemble_64_members_at_full_size():
     """The same 64-member configuration advanced by shared launches
     (alfa_group: one launch per kernel for all members of a grid size, the
     whole ensemble step in one graph per step kind): every member bitwise

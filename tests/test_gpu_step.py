@@ -61,7 +61,7 @@ CASES = {
 
 def branch_report(les, s):
     """The discrete decisions of the step where a rounding difference can
-    switch branches (SURVEY: Cs2 clip at 0/1 and |MM| < 1e-10, the beta root
+    switch branches (the Cs2 clip at 0/1 and |MM| < 1e-10, the beta root
     chosen, the stability branch): points whose clipped coefficient is zero
     on one side only, levels whose beta differs by more than the root
     solver's tolerance, and a stability-branch disagreement."""
@@ -159,7 +159,7 @@ def test_step_fp64_bench_lengths():
 
 @pytest.mark.parametrize("key", ["neutral32", "convective32", "stable32", "convective_wl"])
 def test_step_fp64_100(key):
-    """The north star's bar on the physics of the configurations: every
+    """FP64 parity over 100 steps on the physics of the seeded cases: every
     prognostic field within 1e-11 of the oracle (relative to its own largest
     magnitude) after each of 100 steps: LAD dealiased (neutral), LASDD-SM with
     the beta root (convective), LAD over a prescribed cooling surface
