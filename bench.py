@@ -243,6 +243,8 @@ def main():
                     help="run-length range of served requests (the serving configuration's "
                          "200-2000)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serve-compare", action="store_true",
+                    help="skip the serving run with per-request launches beside the grouped one")
     ap.add_argument("--no-developed", action="store_true",
                     help="skip the mixed-vs-FP64 comparison at a developed state (64^3, 1 h)")
     ap.add_argument("--tiles", default=None, metavar="R2C,X,C2R",
@@ -352,12 +354,21 @@ def main():
         log(f"[bench] ensemble: {ens}")
         reqs = E.serving_requests(args.serve_requests, rate=2.0, steps=tuple(args.serve_steps),
                                   seed=0)
+        # admitted requests of one configuration shape share launches
+        # (regrouped between rounds); beside it, every request on its own
+        # graph replays
         server = E.Server(hbm_budget_bytes=96 << 30, precision="mixed")
         srv = server.serve(reqs)
+        torch.cuda.empty_cache()
+        if not args.no_serve_compare:
+            solo = E.Server(hbm_budget_bytes=96 << 30, precision="mixed", grouped=False).serve(reqs)
+            srv["per_request_launches"] = {k: solo[k] for k in ("mpoint_steps_per_s", "p50_s",
+                                                                "p99_s", "wall_s")}
+            torch.cuda.empty_cache()
         srv["workload"] = (f"Poisson stream (2/s) of {args.serve_requests} requests, grids from "
                            f"32/64/128/192^3, neutral/convective/stable with seeded forcing, "
                            f"{args.serve_steps[0]}-{args.serve_steps[1]} steps each, mixed, "
-                           f"96 GiB HBM budget")
+                           f"96 GiB HBM budget; admitted requests of one shape share launches")
         log(f"[bench] serving: {srv}")
     developed = None
     if not args.no_developed:

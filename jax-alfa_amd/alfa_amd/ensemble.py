@@ -17,6 +17,10 @@ state tensors; members of any grid size share the device.
   steps every admitted member in rounds (continuous batching: a member that
   arrives joins the next round, a member that finishes leaves), and reports
   completion-latency percentiles and aggregate throughput.
+* Members of one configuration shape share launches (``alfa_group``): one
+  launch per kernel for all of them, each reading its own arguments from a
+  device table (``run_ensemble_grouped``; the server regroups its admitted
+  requests between rounds).
 
 Results do not depend on the mix: every kernel is deterministic (fixed
 reduction orders) and each member has its own workspace, so a member's
@@ -418,18 +422,108 @@ def run_ensemble(specs: List[MemberSpec], steps=None, nstreams=4, precision="mix
     return out
 
 
+class _ServedGroup:
+    """Admitted requests of one configuration shape and SGS phase, advanced
+    by shared launches (MemberGroup) for the rounds in which the set does not
+    change: the first step of each step kind runs eagerly (it records the
+    group's launch tables), the second is captured into a hipGraph, later
+    ones replay it."""
+
+    def __init__(self, members, stream, graphs):
+        torch = core._torch()
+        self.members = members
+        self.stream = stream
+        self.use_graphs = graphs
+        self.graphs = {}
+        self.seen = set()
+        with torch.cuda.stream(stream):
+            self.group = MemberGroup(members)
+
+    def step(self, n):
+        torch = core._torch()
+        g = self.group
+        with torch.cuda.stream(self.stream):
+            for _ in range(n):
+                g.set_surface()
+                m = self.members[0]
+                kind = m._dynamic(m.les.iteration)
+                if not self.use_graphs or kind not in self.seen:
+                    g.enqueue()
+                    self.seen.add(kind)
+                elif kind in self.graphs:
+                    self.graphs[kind].replay()
+                else:
+                    gr = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(gr, stream=self.stream):
+                        g.enqueue()
+                    self.graphs[kind] = gr
+                    gr.replay()
+                g.advance()
+
+    def close(self):
+        self.graphs.clear()
+        self.group.close()
+
+
+def _sgs_phase(m):
+    """Members of one shape whose iterations agree modulo the dynamic-SGS
+    period take the same kind of step at every later iteration."""
+    c = m.cfg
+    ct = int(c.get("dynamicSGS_call_time", 1) or 1)
+    return m.les.iteration % ct if c["optSgs"] >= 1 and ct > 1 else 0
+
+
 class Server:
-    """Continuous-batching request loop with HBM-budget admission."""
+    """Continuous-batching request loop with HBM-budget admission.
+
+    grouped (default): between rounds the admitted requests are partitioned
+    by configuration shape (grid, precision, options) and SGS phase, and each
+    part advances by shared launches (one launch per kernel for all its
+    members, alfa_group) on one of the streams; a part is kept, with its
+    launch tables and graphs, for as long as its set of requests does not
+    change.  A request's first step runs alone and eagerly.  Otherwise every
+    admitted request advances by its own graph replays on one of the
+    streams.  Either way a request's fields are bitwise those of its run
+    alone."""
 
     def __init__(self, hbm_budget_bytes, precision="mixed", nstreams=4, round_steps=10,
-                 graphs=True):
+                 graphs=True, grouped=True):
         torch = core._torch()
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
         self.streams = member_streams(nstreams)
         self.round_steps = round_steps
         self.graphs = graphs
+        self.grouped = grouped
         self.results = {}
+        self._parts = {}
+        self._formed = 0
+        self._largest = 0
+
+    def _round_grouped(self, active):
+        """One round: first steps of new requests, then every part of the
+        admitted set advances min(round_steps, its shortest remainder)."""
+        new = [m for m in active.values() if m.done == 0]
+        for m in new:
+            m.step(1)
+        if new:   # the parts below run on other streams
+            core._torch().cuda.synchronize()
+        parts = {}
+        for i, m in active.items():
+            if m.done < m.spec.steps:
+                parts.setdefault(shape_key(m) + (_sgs_phase(m),), []).append(i)
+        keys = {tuple(ids) for ids in parts.values()}
+        for ids in [ids for ids in self._parts if ids not in keys]:
+            self._parts.pop(ids).close()
+        for k, ids in enumerate(sorted(keys, key=lambda t: -active[t[0]].points * len(t))):
+            ms = [active[i] for i in ids]
+            part = self._parts.get(ids)
+            if part is None:
+                part = self._parts[ids] = _ServedGroup(ms, self.streams[k % len(self.streams)],
+                                                       self.graphs)
+                self._formed += 1
+                self._largest = max(self._largest, len(ms))
+            part.step(min(self.round_steps, min(m.spec.steps - m.done for m in ms)))
 
     def estimate_bytes(self, spec):
         # workspace per grid point measured at plan creation (mixed ~560 B,
@@ -463,7 +557,7 @@ class Server:
                 # no tile autotuning at admission: its timed steps hold the
                 # request loop (and every queued request) for their duration
                 m = Member(requests[i], self.precision, self.streams[k % len(self.streams)],
-                           self.graphs, tune=False)
+                           self.graphs and not self.grouped, tune=False)
                 k += 1
                 m.t_admit = time.perf_counter() - t0
                 used += m.bytes
@@ -472,11 +566,16 @@ class Server:
                 time.sleep(max(0.0, min(0.05, requests[pending[0]].arrival - now)))
                 continue
             # one round: every admitted member advances up to round_steps
-            for i, m in active.items():
-                m.step(min(self.round_steps, m.spec.steps - m.done))
+            if self.grouped:
+                self._round_grouped(active)
+            else:
+                for i, m in active.items():
+                    m.step(min(self.round_steps, m.spec.steps - m.done))
             torch.cuda.synchronize()
             now = time.perf_counter() - t0
             for i in [i for i, m in active.items() if m.done >= m.spec.steps]:
+                for ids in [ids for ids in self._parts if i in ids]:
+                    self._parts.pop(ids).close()
                 m = active.pop(i)
                 lat[i] = now - requests[i].arrival
                 points += m.points * m.spec.steps
@@ -485,8 +584,13 @@ class Server:
                 used -= m.bytes
                 m.close()
         wall = time.perf_counter() - t0
+        for part in self._parts.values():
+            part.close()
+        self._parts.clear()
         v = np.array([lat[i] for i in sorted(lat)])
         return dict(requests=len(requests), wall_s=wall, p50_s=float(np.percentile(v, 50)),
                     p99_s=float(np.percentile(v, 99)), max_s=float(v.max()),
                     mpoint_steps_per_s=points / wall / 1e6, budget_bytes=self.budget,
-                    streams=len(self.streams), round_steps=self.round_steps)
+                    streams=len(self.streams), round_steps=self.round_steps,
+                    grouped=self.grouped, parts_formed=self._formed,
+                    largest_part=self._largest)

@@ -41,17 +41,18 @@ def test_graph_replay_matches_eager(case, prec):
     m.close()
 
 
-def test_served_result_is_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("grouped", [False, True])
+def test_served_result_is_the_same_alone_and_in_the_mix(grouped):
     reqs = [MemberSpec("stable", 32, 40, seed=1, arrival=0.0, cooling_K_per_h=0.3),
             MemberSpec("convective", 48, 30, seed=2, arrival=0.02),
             MemberSpec("neutral", 32, 50, seed=3, arrival=0.04),
             MemberSpec("convective", 32, 20, seed=4, arrival=0.06,
                        over=dict(SensibleHeatFlux=0.1, optSurfFlux=0))]
-    mix = Server(64 << 30, "mixed", nstreams=3, round_steps=7)
+    mix = Server(64 << 30, "mixed", nstreams=3, round_steps=7, grouped=grouped)
     rep = mix.serve(reqs, keep=range(len(reqs)))
     print("mix:", rep)
     for i, r in enumerate(reqs):
-        alone = Server(64 << 30, "mixed", nstreams=1, round_steps=11)
+        alone = Server(64 << 30, "mixed", nstreams=1, round_steps=11, grouped=False)
         solo = MemberSpec(r.case, r.n, r.steps, seed=r.seed, over=r.over,
                           cooling_K_per_h=r.cooling_K_per_h, arrival=0.0)
         alone.serve([solo], keep=[0])
@@ -59,6 +60,36 @@ def test_served_result_is_the_same_alone_and_in_the_mix():
             d = float(np.max(np.abs(mix.results[i][k] - alone.results[0][k])))
             print("request", i, k, "alone vs mix max abs diff", d)
             assert d == 0.0
+
+
+def test_grouped_serving_matches_runs_alone():
+    """Requests of one shape share launches once admitted: two stable and
+    two convective 32^3 requests arrive together and form parts of two (the
+    convective ones with the dynamic procedure every second step), a third of
+    each shape arrives later and joins or forms its own part by SGS phase,
+    and parts are rebuilt as requests finish.  Every result is bitwise the
+    request served alone with per-member graphs."""
+    reqs = [MemberSpec("stable", 32, 40, seed=11, arrival=0.0, cooling_K_per_h=0.2),
+            MemberSpec("stable", 32, 33, seed=12, arrival=0.0, cooling_K_per_h=0.5,
+                       over=dict(Ug2=6.0)),
+            MemberSpec("convective", 32, 36, seed=14, arrival=0.0,
+                       over=dict(dynamicSGS_call_time=2)),
+            MemberSpec("convective", 32, 29, seed=15, arrival=0.0,
+                       over=dict(dynamicSGS_call_time=2, SensibleHeatFlux=0.1)),
+            MemberSpec("stable", 32, 17, seed=13, arrival=0.05, cooling_K_per_h=0.3),
+            MemberSpec("convective", 32, 19, seed=16, arrival=0.05,
+                       over=dict(dynamicSGS_call_time=2))]
+    mix = Server(64 << 30, "mixed", nstreams=3, round_steps=4)
+    rep = mix.serve(reqs, keep=range(len(reqs)))
+    print("grouped mix:", rep)
+    assert rep["grouped"] and rep["largest_part"] >= 2
+    for i, r in enumerate(reqs):
+        alone = Server(64 << 30, "mixed", nstreams=1, round_steps=11, grouped=False)
+        solo = MemberSpec(r.case, r.n, r.steps, seed=r.seed, over=r.over,
+                          cooling_K_per_h=r.cooling_K_per_h, arrival=0.0)
+        alone.serve([solo], keep=[0])
+        for k in ("u", "v", "w", "TH"):
+            assert np.array_equal(mix.results[i][k], alone.results[0][k]), (i, r.case, k)
 
 
 def _solo(spec, steps, prec):
