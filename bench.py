@@ -63,6 +63,35 @@ def measured_traffic(kernel):
     return None if e is None else float(e["bytes_per_launch"])
 
 
+def measured_step_traffic(precision):
+    """HBM bytes per point-step of the whole step, from the committed counter
+    summary: every library kernel's bytes per launch times its dispatches,
+    over the steps of the profiled run and the points of its 256^3 grid; or
+    None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    sfx = "" if precision == "mixed" else "_fp64"
+    k, steps = d.get("kernels" + sfx), d.get("steps_in_run" + sfx)
+    if not k or not steps:
+        return None
+    return sum(e["bytes_per_launch"] * e["dispatches"] for e in k.values()) / steps / 256 ** 3
+
+
+def step_hbm(rate_mpts, precision):
+    """The step's measured HBM traffic at the measured rate, against peak."""
+    b = measured_step_traffic(precision)
+    if b is None:
+        return None
+    gbs = rate_mpts * 1e6 * b / 1e9
+    return {"bytes_per_point_step": round(b, 1), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of every library kernel of the 256^3 "
+                      "step (profiles/hbm_traffic.json)"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -379,6 +408,8 @@ def main():
         log(f"[bench] developed state: {developed}")
     copy_probe(args.copy_probe)
     cpu = None if args.no_cpu else cpu_baseline()
+    # the committed counter summary covers this workload only
+    profiled_workload = n == 256 and args.case == "stable" and not args.no_fusion
     out = {
         "metric": f"Mpoint-steps/s at {n}^3 (FP32-mixed vs FP64)",
         "value": round(mixed["mpts"], 2),
@@ -397,11 +428,21 @@ def main():
                                f"Thomas pressure, mixed precision",
                    "grid": [n, n, n], "optSgs": CASE_TEXT[args.case][2],
                    "FGR": CASE_TEXT[args.case][3], "precision": "mixed"},
-        "hbm_frac_step": round(mixed["mpts"] * 1e6 * B_ALG["mixed"] / (HBM_PEAK_GBS * 1e9), 4),
+        # the whole step's HBM traffic as measured (counter bytes), at the
+        # measured rate, against peak
+        "hbm_step": step_hbm(mixed["mpts"], "mixed") if profiled_workload else None,
+        # the reference's data flow (567 field passes per point-step, each
+        # FFT pass one read and one write) at the measured rate: an
+        # equivalent bandwidth, not traffic (it counts the passes that fusion
+        # and pruning remove, so it can exceed 1)
+        "reference_dataflow_frac_step": round(mixed["mpts"] * 1e6 * B_ALG["mixed"] /
+                                              (HBM_PEAK_GBS * 1e9), 4),
         "b_alg_bytes_per_point_step": B_ALG["mixed"],
         "fp64": None if fp64 is None else {
             "value": round(fp64["mpts"], 2), "ms_per_step": round(fp64["ms"], 4),
-            "hbm_frac_step": round(fp64["mpts"] * 1e6 * B_ALG["fp64"] / (HBM_PEAK_GBS * 1e9), 4)},
+            "hbm_step": step_hbm(fp64["mpts"], "fp64") if profiled_workload else None,
+            "reference_dataflow_frac_step": round(fp64["mpts"] * 1e6 * B_ALG["fp64"] /
+                                                  (HBM_PEAK_GBS * 1e9), 4)},
         "mixed_over_fp64": None if fp64 is None else round(mixed["mpts"] / fp64["mpts"], 3),
         # threads per workgroup of the y R2C, x and y C2R kernels, as the
         # plan's autotuner chose them after the warmup
