@@ -163,6 +163,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libalfa.so not built: {LIB_PATH} (run jax-alfa_amd/build.py)")
+    # torch first: torch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7), and the library must bind the one torch uses.
+    # Loaded the other way round, the library pulls in the system runtime
+    # ahead of torch's and the process later finds no device at plan creation.
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     for name, args in _SIGS.items():
         f = getattr(lib, name)
