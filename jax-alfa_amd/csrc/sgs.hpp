@@ -792,14 +792,16 @@ __device__ __forceinline__ void k_quintic(const double* __restrict__ m, int nz, 
   coef[3 * nz + k] = cc; coef[4 * nz + k] = bb; coef[5 * nz + k] = aa;
 }
 
-// Profiles: avg1 = mean(sqrt C)^2, avg2 = mean(C) from the box-clip sums.
+// Profiles: avg1 = mean(sqrt C)^2, avg2 = mean(C) from the box-clip sums;
+// with beta_out, the procedure's beta profile is copied out as well.
 __device__ __forceinline__ void k_profiles(const double* __restrict__ s, int nz, double inv_n, double* avg1,
-                           double* avg2) {
+                           double* avg2, const double* __restrict__ beta, double* beta_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nz) return;
   const double m1 = s[k] * inv_n;
   if (avg1) avg1[k] = m1 * m1;
   if (avg2) avg2[k] = s[nz + k] * inv_n;
+  if (beta_out) beta_out[k] = beta[k];
 }
 
 // device copy of n doubles (a kernel rather than a memcpy node, so that

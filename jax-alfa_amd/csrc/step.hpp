@@ -423,9 +423,8 @@ void stage_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, const T* const S
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cs2, s2, "box");
   if (live(P)) {
-    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, avg1, avg2);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, avg1, avg2, (const double*)beta, beta_out);
     launch_ok(P);
-    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -537,9 +536,8 @@ void stage_scalar_lasdd(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CsPr, s2, "sbox");
   if (live(P)) {
-    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, nullptr, prof);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, nullptr, prof, (const double*)beta, beta_out);
     launch_ok(P);
-    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -597,9 +595,8 @@ void stage_lasdd_wl(Ctx<T>& c, const T* u, const T* v, const T* w, const T* cons
   double* s2 = c.D("bsums", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, LM, MM, Cwl, s2, "box");
   if (live(P)) {
-    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, avg1, avg2);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, avg1, avg2, (const double*)beta, beta_out);
     launch_ok(P);
-    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
@@ -659,9 +656,8 @@ void stage_scalar_wl(Ctx<T>& c, const T* u, const T* v, const T* w, int w_uvp, c
   double* s2 = c.D("ssum2", 2 * c.nz);
   box_clip_sums<T>(P, c.s, c.nx, c.ny, c.nz, Tup, Tdn, CwlPr, s2, "sbox");
   if (live(P)) {
-    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, nullptr, prof);
+    launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_profiles), s2, c.nz, 1.0 / c.ncol, nullptr, prof, (const double*)beta, beta_out);
     launch_ok(P);
-    if (beta_out) launch_k<1024>(P, c.s, dim3((c.nz + 255) / 256), dim3(256), 0, KBODY(k_copy_d), c.nz, beta, beta_out);
   }
 }
 
